@@ -1,0 +1,1029 @@
+/*
+ * gfx950 kernels of the single-LP PDHG path and their extern "C" launchers
+ * (declared in include/dpdlp_dev.h).
+ *
+ * SpMV: "CSR-stream" row blocks in natural row order.  A workgroup of 256
+ * threads (4 wave64) owns a block of <= 256 consecutive rows holding <= CAP
+ * nonzeros; it streams the block's values and column indices with coalesced
+ * non-temporal loads, multiplies by the gathered vector, stages the products
+ * in LDS, and reduces each row with a power-of-two lane group sized to the
+ * block (rows/lanes fixed per block, so sums are deterministic).  A row longer
+ * than CAP owns a block and is reduced by the whole workgroup.  Because rows
+ * stay in natural order, the fused epilogues (projection, reflection, Halpern,
+ * residuals) read and write their row vectors coalesced.
+ *
+ * Every reduction writes per-workgroup partials that k_finalize combines in a
+ * fixed order, so results do not depend on scheduling.  Elementwise arithmetic
+ * is compiled without FMA contraction so it rounds exactly like the FP64
+ * oracle; only SpMV and reduction summation order differ from it.
+ */
+#include "dev_common.h"
+#include "dpdlp_dev.h"
+
+#define EW_GRID_MAX 2048
+
+static inline int ew_grid(long long n) {
+  long long g = (n + DPDLP_BS - 1) / DPDLP_BS;
+  if (g < 1) g = 1;
+  if (g > EW_GRID_MAX) g = EW_GRID_MAX;
+  return (int)g;
+}
+
+/* ------------------------------------------------------------------------ */
+/* CSR-stream SpMV with a fused epilogue                                     */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+__device__ __forceinline__ T ntload(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+
+template <typename VT, typename XT, class Epi, int CAP>
+__global__ __launch_bounds__(DPDLP_BS) void k_spmv(
+    const int *__restrict__ blk, const int *__restrict__ rp,
+    const int *__restrict__ ci, const VT *__restrict__ val,
+    const XT *__restrict__ x, Epi epi, double *__restrict__ part, int stride,
+    int slot0) {
+  constexpr int NS = Epi::NS, NM = Epi::NM;
+  constexpr int UNR = 4;
+  __shared__ double prod[CAP];
+  __shared__ int srp[DPDLP_BS + 1];
+  __shared__ double ssum[DPDLP_BS];
+  __shared__ double red[(NS + NM > 0 ? NS + NM : 1) * 4];
+
+  const int tid = threadIdx.x;
+  const int r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+  const int R = r1 - r0;
+  const int p0 = rp[r0], p1 = rp[r1];
+  const int nz = p1 - p0;
+  double s[NS > 0 ? NS : 1];
+  double mx[NM > 0 ? NM : 1];
+#pragma unroll
+  for (int k = 0; k < (NS > 0 ? NS : 1); ++k) s[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < (NM > 0 ? NM : 1); ++k) mx[k] = 0.0;
+
+  if (nz > CAP) {
+    /* one long row, whole workgroup */
+    double acc = 0.0;
+    for (int k = p0 + tid; k < p1; k += DPDLP_BS) {
+      double prodv = (double)ntload(val + k) * (double)x[ntload(ci + k)];
+      acc += prodv;
+    }
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) ssum[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      double tot = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
+      epi(r0, tot, s, mx);
+    }
+  } else {
+    for (int t = tid; t <= R; t += DPDLP_BS) srp[t] = rp[r0 + t] - p0;
+    for (int base = tid; base < nz; base += DPDLP_BS * UNR) {
+      int cidx[UNR];
+      VT vv[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        int k = base + u * DPDLP_BS;
+        if (k < nz) {
+          cidx[u] = ntload(ci + p0 + k);
+          vv[u] = ntload(val + p0 + k);
+        }
+      }
+      XT xx[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        int k = base + u * DPDLP_BS;
+        if (k < nz) xx[u] = x[cidx[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        int k = base + u * DPDLP_BS;
+        if (k < nz) prod[k] = (double)vv[u] * (double)xx[u];
+      }
+    }
+    __syncthreads();
+    int q = DPDLP_BS / R;
+    int g = 1 << (31 - __clz(q));
+    if (g > DPDLP_WAVE) g = DPDLP_WAVE;
+    const int grp = tid / g, lane = tid & (g - 1);
+    double acc = 0.0;
+    if (grp < R) {
+      const int e = srp[grp + 1];
+      for (int k = srp[grp] + lane; k < e; k += g) acc += prod[k];
+    }
+    for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, DPDLP_WAVE);
+    if (lane == 0 && grp < R) ssum[grp] = acc;
+    __syncthreads();
+    if (tid < R) epi(r0 + tid, ssum[tid], s, mx);
+  }
+  if (NS + NM > 0) {
+    if (epi.partials) block_partials<NS, NM>(s, mx, red, part, stride, slot0);
+  }
+}
+
+/* ---- epilogues ---- */
+
+struct EpiStore {
+  static constexpr int NS = 0, NM = 0;
+  double *y;
+  int partials;
+  __device__ void operator()(int r, double ax, double *, double *) const { y[r] = ax; }
+};
+
+/* Phase 1: pdlp_core_op.cu:393-418 then the x half of :445-462. */
+struct EpiPrimal {
+  static constexpr int NS = 3, NM = 0;
+  double *x_cur;
+  const double *x_init, *c, *lv, *uv;
+  double *x_ref, *x_pdhg, *z, *dx;
+  float *x_ref32;
+  double tau, w, rho;
+  int major, fpe, partials;
+  __device__ void operator()(int j, double aty, double *s, double *) const {
+    double xc = x_cur[j];
+    double t = xc - tau * (c[j] - aty);
+    double p = fmax(lv[j], fmin(t, uv[j]));
+    double xr;
+    if (major) {
+      x_pdhg[j] = p;
+      z[j] = (p - t) / tau;
+    }
+    xr = 2.0 * p - xc;
+    x_ref[j] = xr;
+    if (x_ref32) x_ref32[j] = (float)xr;
+    if (fpe) {
+      double d = xr - xc;
+      dx[j] = d;
+      blue_acc(d, s[0], s[1], s[2]);
+    }
+    double rr = rho * xr + (1.0 - rho) * xc;
+    x_cur[j] = w * rr + (1.0 - w) * x_init[j];
+  }
+};
+
+/* Phase 2: pdlp_core_op.cu:420-443 then the y half of :445-462. */
+struct EpiDual {
+  static constexpr int NS = 3, NM = 0;
+  double *y_cur;
+  const double *y_init, *lc, *uc;
+  double *y_ref, *y_pdhg, *dy;
+  float *y_cur32;
+  double sigma, w, rho;
+  int major, fpe, partials;
+  __device__ void operator()(int i, double ax, double *s, double *) const {
+    double yc = y_cur[i];
+    double t = yc / sigma - ax;
+    double p = fmax(-uc[i], fmin(t, -lc[i]));
+    double yr;
+    if (major) {
+      double yp = (t - p) * sigma;
+      y_pdhg[i] = yp;
+      yr = 2.0 * yp - yc;
+    } else {
+      yr = 2.0 * (t - p) * sigma - yc;
+    }
+    y_ref[i] = yr;
+    if (fpe) {
+      double d = yr - yc;
+      dy[i] = d;
+      blue_acc(d, s[0], s[1], s[2]);
+    }
+    double rr = rho * yr + (1.0 - rho) * yc;
+    double yn = w * rr + (1.0 - w) * y_init[i];
+    y_cur[i] = yn;
+    if (y_cur32) y_cur32[i] = (float)yn;
+  }
+};
+
+/* A' dy for the fixed-point error, pdlp_core_op.cu:666-692. */
+struct EpiFpe {
+  static constexpr int NS = 1, NM = 0;
+  double *dprod;
+  const double *dx;
+  int partials;
+  __device__ void operator()(int j, double ax, double *s, double *) const {
+    dprod[j] = ax;
+    s[0] += ax * dx[j];
+  }
+};
+
+/* Rows of compute_residual_kernel (utils.cu:503-524) + norms + restart dist. */
+struct EpiResP {
+  static constexpr int NS = 7, NM = 1;
+  double *pprod;
+  const double *lc, *uc, *D_r, *y, *y_init;
+  int dist, partials;
+  __device__ void operator()(int i, double ax, double *s, double *mx) const {
+    pprod[i] = ax;
+    double l = lc[i], u = uc[i];
+    double cl = fmax(l, fmin(ax, u));
+    double r = (ax - cl) * D_r[i];
+    blue_acc(r, s[0], s[1], s[2]);
+    mx[0] = fmax(mx[0], fabs(r));
+    double yv = y[i];
+    s[3] += fmax(yv, 0.0) * lfin(l) + fmin(yv, 0.0) * lfin(u);
+    if (dist) blue_acc(yv - y_init[i], s[4], s[5], s[6]);
+  }
+};
+
+/* Columns of compute_residual_kernel (utils.cu:525-530) + objectives. */
+struct EpiResD {
+  static constexpr int NS = 8, NM = 1;
+  double *dprod;
+  const double *c, *z, *D_c, *x_obj, *x_init;
+  int dist, partials;
+  __device__ void operator()(int j, double ax, double *s, double *mx) const {
+    dprod[j] = ax;
+    double cj = c[j], zj = z[j];
+    double r = (cj - ax - zj) * D_c[j];
+    blue_acc(r, s[0], s[1], s[2]);
+    mx[0] = fmax(mx[0], fabs(r));
+    double xv = x_obj[j];
+    s[3] += cj * xv;
+    s[4] += zj * xv;
+    if (dist) blue_acc(xv - x_init[j], s[5], s[6], s[7]);
+  }
+};
+
+/* A dx for the primal ray (utils.cu:750-793). */
+struct EpiInfP {
+  static constexpr int NS = 1, NM = 1;
+  double *pprod, *pslack;
+  const double *lc, *uc, *D_r, *dy;
+  int partials;
+  __device__ void operator()(int i, double ax, double *s, double *mx) const {
+    pprod[i] = ax;
+    double l = lc[i], u = uc[i], d = dy[i];
+    s[0] += fmax(d, 0.0) * lfin(l) + fmin(d, 0.0) * lfin(u);
+    double v = (fmax(0.0, -ax) * (double)(isfinite(l) ? 1 : 0) +
+                fmax(0.0, ax) * (double)(isfinite(u) ? 1 : 0)) * D_r[i];
+    pslack[i] = v;
+    mx[0] = fmax(mx[0], fabs(v));
+  }
+};
+
+/* A' dy for the dual ray (utils.cu:755-797). */
+struct EpiInfD {
+  static constexpr int NS = 2, NM = 1;
+  double *dprod, *dslack;
+  const double *lv, *uv, *D_c, *c, *dx;
+  int partials;
+  __device__ void operator()(int j, double ax, double *s, double *mx) const {
+    dprod[j] = ax;
+    double l = lv[j], u = uv[j], nd = -ax;
+    s[0] += fmax(nd, 0.0) * lfin(l) + fmin(nd, 0.0) * lfin(u);
+    s[1] += c[j] * dx[j];
+    double v = (fmax(0.0, nd) * (double)(isfinite(l) ? 0 : 1) -
+                fmin(0.0, nd) * (double)(isfinite(u) ? 0 : 1)) * D_c[j];
+    dslack[j] = v;
+    mx[0] = fmax(mx[0], fabs(v));
+  }
+};
+
+/* A (A' q) in the power iteration: store, <q, AA'q> and |AA'q| partials. */
+struct EpiPow {
+  static constexpr int NS = 4, NM = 0;
+  double *eig;
+  const double *next;
+  int partials;
+  __device__ void operator()(int i, double ax, double *s, double *) const {
+    eig[i] = ax;
+    s[0] += next[i] * ax;
+    blue_acc(ax, s[1], s[2], s[3]);
+  }
+};
+
+/* Reduced cost (pdlp_core_op.cu:31-40). */
+struct EpiRc {
+  static constexpr int NS = 0, NM = 0;
+  double *rc;
+  const double *c, *D_c;
+  double gamma;
+  int partials;
+  __device__ void operator()(int j, double ax, double *, double *) const {
+    rc[j] = (c[j] - ax) * D_c[j] / gamma;
+  }
+};
+
+/* Row max |a| (Ruiz) or sequential sum of |a|^p (Pock-Chambolle). */
+struct EpiStoreRow {
+  static constexpr int NS = 0, NM = 0;
+  double *y;
+  int partials;
+  __device__ void operator()(int r, double ax, double *, double *) const { y[r] = ax; }
+};
+
+template <class Epi>
+static hipError_t launch_spmv(const dpdlp_csr_t *A, const double *x, const Epi &epi,
+                              double *part, int stride, int slot0, hipStream_t s) {
+  if (A->nblk <= 0) return hipSuccess;
+  switch (A->cap) {
+  case 1024:
+    k_spmv<double, double, Epi, 1024><<<A->nblk, DPDLP_BS, 0, s>>>(
+        A->blk, A->rp, A->ci, A->v, x, epi, part, stride, slot0);
+    break;
+  case 4096:
+    k_spmv<double, double, Epi, 4096><<<A->nblk, DPDLP_BS, 0, s>>>(
+        A->blk, A->rp, A->ci, A->v, x, epi, part, stride, slot0);
+    break;
+  default:
+    k_spmv<double, double, Epi, 2048><<<A->nblk, DPDLP_BS, 0, s>>>(
+        A->blk, A->rp, A->ci, A->v, x, epi, part, stride, slot0);
+    break;
+  }
+  return hipGetLastError();
+}
+
+template <class Epi>
+static hipError_t launch_spmv32(const dpdlp_csr_t *A, const float *x, const Epi &epi,
+                                double *part, int stride, int slot0, hipStream_t s) {
+  if (A->nblk <= 0) return hipSuccess;
+  switch (A->cap) {
+  case 1024:
+    k_spmv<float, float, Epi, 1024><<<A->nblk, DPDLP_BS, 0, s>>>(
+        A->blk, A->rp, A->ci, A->v32, x, epi, part, stride, slot0);
+    break;
+  case 4096:
+    k_spmv<float, float, Epi, 4096><<<A->nblk, DPDLP_BS, 0, s>>>(
+        A->blk, A->rp, A->ci, A->v32, x, epi, part, stride, slot0);
+    break;
+  default:
+    k_spmv<float, float, Epi, 2048><<<A->nblk, DPDLP_BS, 0, s>>>(
+        A->blk, A->rp, A->ci, A->v32, x, epi, part, stride, slot0);
+    break;
+  }
+  return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------------ */
+/* Fixed-order finalize: one workgroup per slot                              */
+/* ------------------------------------------------------------------------ */
+
+/* maxmask bit k set: slot k is a maximum, else a sum.  Writes
+ * scal[dst0 + k] and, when dst1 >= 0 / dst2 >= 0, copies to those too. */
+__global__ __launch_bounds__(DPDLP_BS) void k_finalize(const double *__restrict__ part,
+                                                      int stride, int nparts, int slot0,
+                                                      unsigned maxmask, double *scal,
+                                                      int dst0, int dst1, int dst2) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  const bool ismax = (maxmask >> k) & 1u;
+  const double *p = part + (size_t)(slot0 + k) * stride;
+  double acc = 0.0;
+  if (ismax) {
+    for (int i = threadIdx.x; i < nparts; i += DPDLP_BS) acc = fmax(acc, p[i]);
+    acc = wave_max(acc);
+  } else {
+    for (int i = threadIdx.x; i < nparts; i += DPDLP_BS) acc += p[i];
+    acc = wave_sum(acc);
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = ismax ? fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))
+                     : (red[0] + red[1]) + (red[2] + red[3]);
+    scal[dst0 + k] = v;
+    if (dst1 >= 0) scal[dst1 + k] = v;
+    if (dst2 >= 0) scal[dst2 + k] = v;
+  }
+}
+
+static hipError_t finalize(const dpdlp_state_t *st, int nparts, int slot0, int count,
+                           unsigned maxmask, int dst0, int dst1, int dst2,
+                           hipStream_t s) {
+  k_finalize<<<count, DPDLP_BS, 0, s>>>(st->part, st->part_stride, nparts, slot0,
+                                        maxmask, st->scal, dst0, dst1, dst2);
+  return hipGetLastError();
+}
+
+/* ------------------------------------------------------------------------ */
+/* Elementwise kernels                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* restart copies (pdlp_core_op.cu:621-632 and :1290-1310) */
+__global__ void k_restart_copy(double *x_init, double *x_cur, const double *x_pdhg,
+                               double *y_init, double *y_cur, const double *y_pdhg,
+                               float *y_cur32, int n, int m, int part) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n + m;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < n) {
+      if (part & 1) {
+        double v = x_pdhg[i];
+        x_init[i] = v;
+        x_cur[i] = v;
+      }
+    } else {
+      int k = (int)(i - n);
+      if (part & 2) {
+        double v = y_pdhg[k];
+        y_init[k] = v;
+        y_cur[k] = v;
+        if (y_cur32) y_cur32[k] = (float)v;
+      }
+    }
+  }
+}
+
+/* ray projections (utils.cu:533-559) + max|dx|, max|dy| partials */
+__global__ __launch_bounds__(DPDLP_BS) void k_inf_project(
+    double *dx, const double *lv, const double *uv, int n, double *dy,
+    const double *lc, const double *uc, int m, double *part, int stride, int slot0) {
+  __shared__ double red[2 * 4];
+  double s[1] = {0.0};
+  double mx[2] = {0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n + m;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < n) {
+      double d = dx[i];
+      if (isfinite(lv[i])) d = fmax(d, 0.0);
+      if (isfinite(uv[i])) d = fmin(d, 0.0);
+      dx[i] = d;
+      mx[0] = fmax(mx[0], fabs(d));
+    } else {
+      int k = (int)(i - n);
+      double d = dy[k];
+      if (!isfinite(lc[k])) d = fmin(d, 0.0);
+      if (!isfinite(uc[k])) d = fmax(d, 0.0);
+      dy[k] = d;
+      mx[1] = fmax(mx[1], fabs(d));
+    }
+  }
+  block_partials<0, 2>(s, mx, red, part, stride, slot0);
+}
+
+/* dx *= 1/max|dx| when positive (utils.cu:731-737) */
+__global__ void k_scale_by_inv(double *v, int n, const double *scal, int idx) {
+  double a = scal[idx];
+  if (!(a > 0.0)) return;
+  double sc = 1.0 / a;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    v[i] *= sc;
+}
+
+/* dot(a, b) partial into one slot */
+__global__ __launch_bounds__(DPDLP_BS) void k_dot(const double *a, const double *b, int n,
+                                                 double *part, int stride, int slot0) {
+  __shared__ double red[4];
+  double s[1] = {0.0};
+  double mx[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    s[0] += a[i] * b[i];
+  block_partials<1, 0>(s, mx, red, part, stride, slot0);
+}
+
+/* sum of dual-objective contributions max(y,0) l_fin + min(y,0) u_fin */
+__global__ __launch_bounds__(DPDLP_BS) void k_dual_contrib_sum(const double *y, const double *lc,
+                                                              const double *uc, int m, double *part,
+                                                              int stride, int slot0) {
+  __shared__ double red[4];
+  double s[1] = {0.0};
+  double mx[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (long long)gridDim.x * blockDim.x) {
+    double yv = y[i];
+    s[0] += fmax(yv, 0.0) * lfin(lc[i]) + fmin(yv, 0.0) * lfin(uc[i]);
+  }
+  block_partials<1, 0>(s, mx, red, part, stride, slot0);
+}
+
+/* Blue partials of a vector (power iteration start) */
+__global__ __launch_bounds__(DPDLP_BS) void k_nrm_partials(const double *v, int n, double *part,
+                                                          int stride, int slot0) {
+  __shared__ double red[3 * 4];
+  double s[3] = {0.0, 0.0, 0.0};
+  double mx[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    blue_acc(v[i], s[0], s[1], s[2]);
+  block_partials<3, 0>(s, mx, red, part, stride, slot0);
+}
+
+/* next = eig / |eig|, utils.cu:124-132 */
+__global__ void k_pow_normalize(const double *eig, double *next, int m, const double *scal,
+                                int nrm_slot) {
+  double nrm = blue_combine(scal[nrm_slot], scal[nrm_slot + 1], scal[nrm_slot + 2]);
+  double inv = 1.0 / nrm;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (long long)gridDim.x * blockDim.x)
+    next[i] = eig[i] * inv;
+}
+
+/* residual z - sigma^2 q (utils.cu:145-153), then next = z / |z| for the
+ * following step (both read the old next first). */
+__global__ __launch_bounds__(DPDLP_BS) void k_pow_update(const double *eig, double *next, int m,
+                                                        const double *scal, int dot_slot,
+                                                        int nrm_slot, double *part, int stride,
+                                                        int slot0) {
+  __shared__ double red[3 * 4];
+  double s[3] = {0.0, 0.0, 0.0};
+  double mx[1] = {0.0};
+  double neg = -scal[dot_slot];
+  double nrm = blue_combine(scal[nrm_slot], scal[nrm_slot + 1], scal[nrm_slot + 2]);
+  double inv = 1.0 / nrm;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m;
+       i += (long long)gridDim.x * blockDim.x) {
+    double e = eig[i];
+    double r = next[i] * neg + e;
+    blue_acc(r, s[0], s[1], s[2]);
+    next[i] = e * inv;
+  }
+  block_partials<3, 0>(s, mx, red, part, stride, slot0);
+}
+
+__global__ void k_copy_scal(double *scal, int src, int dst, int count) {
+  if (threadIdx.x < count) scal[dst + threadIdx.x] = scal[src + threadIdx.x];
+}
+
+/* result unscale (pdlp_core_op.cu:464-480) */
+__global__ void k_unscale(const double *x, const double *y, const double *D_c, const double *D_r,
+                          double beta, double gamma, double *x_out, double *y_out, int n, int m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n + m;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < n) {
+      x_out[i] = x[i] / D_c[i] / beta;
+    } else {
+      int k = (int)(i - n);
+      y_out[k] = y[k] / D_r[k] / gamma;
+    }
+  }
+}
+
+/* ---- scaling (preconditioner.c) ---- */
+
+/* Per-row reduction of |a|: mode 0 = max, mode 1 = sequential sum of
+ * |a|^p in storage order (pow skipped when p == 1, which is exact). */
+__global__ void k_row_reduce(const int *rp, const double *v, int rows, int mode, double p,
+                             double *out) {
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+       r += (long long)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    int e = rp[r + 1];
+    if (mode == 0) {
+      for (int k = rp[r]; k < e; ++k) {
+        double a = fabs(v[k]);
+        if (a > acc) acc = a;
+      }
+    } else if (p == 1.0) {
+      for (int k = rp[r]; k < e; ++k) acc += fabs(v[k]);
+    } else {
+      for (int k = rp[r]; k < e; ++k) acc += pow(fabs(v[k]), p);
+    }
+    out[r] = acc;
+  }
+}
+
+/* rs/cs <- sqrt(.) or 1 below 1e-12; cumulative D *= factor */
+__global__ void k_scale_factors(double *f, double *D, int count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    double v = f[i];
+    v = (v < 1e-12) ? 1.0 : sqrt(v);
+    f[i] = v;
+    D[i] *= v;
+  }
+}
+
+/* a_k /= (rs[row] * cs[col]) for one CSR matrix; `rowf` scales rows, `colf`
+ * the column index space (pass them swapped for the transpose). */
+__global__ void k_scale_values(const int *rp, const int *ci, double *v, int rows,
+                               const double *rowf, const double *colf, int row_first) {
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+       r += (long long)gridDim.x * blockDim.x) {
+    double fr = rowf[r];
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      double fc = colf[ci[k]];
+      /* the product is formed as rs[i] * cs[j] in both matrices */
+      double d = row_first ? fr * fc : fc * fr;
+      v[k] /= d;
+    }
+  }
+}
+
+/* c /= cs, lv,uv *= cs, lc,uc /= rs (preconditioner.c:92-100) */
+__global__ void k_scale_vectors(double *c, double *lv, double *uv, const double *cs, int n,
+                                double *lc, double *uc, const double *rs, int m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n + m;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < n) {
+      double f = cs[i];
+      c[i] /= f;
+      uv[i] *= f;
+      lv[i] *= f;
+    } else {
+      int k = (int)(i - n);
+      double f = rs[k];
+      lc[k] /= f;
+      uc[k] /= f;
+    }
+  }
+}
+
+/* bound norm^2 and objective norm^2 partials (preconditioner.c:227-247) */
+__global__ __launch_bounds__(DPDLP_BS) void k_bound_obj_sums(const double *lc, const double *uc,
+                                                            int m, const double *c, int n,
+                                                            double *part, int stride, int slot0) {
+  __shared__ double red[2 * 4];
+  double s[2] = {0.0, 0.0};
+  double mx[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)m + n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < m) {
+      double l = lc[i], u = uc[i];
+      if (isfinite(l) && l != u) s[0] += l * l;
+      if (isfinite(u)) s[0] += u * u;
+    } else {
+      double v = c[i - m];
+      s[1] += v * v;
+    }
+  }
+  block_partials<2, 0>(s, mx, red, part, stride, slot0);
+}
+
+__global__ void k_bound_obj_apply(double *lc, double *uc, int m, double *lv, double *uv,
+                                  double *c, int n, double beta, double gamma) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)m + n;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < m) {
+      lc[i] *= beta;
+      uc[i] *= beta;
+    } else {
+      int k = (int)(i - m);
+      lv[k] *= beta;
+      uv[k] *= beta;
+      c[k] *= gamma;
+    }
+  }
+}
+
+__global__ void k_to_f32(const double *src, float *dst, long long count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (float)src[i];
+}
+
+__global__ void k_zero_finite(double *v, int count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    if (isfinite(v[i])) v[i] = 0.0;
+}
+
+__global__ void k_fill(double *v, double value, long long count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    v[i] = value;
+}
+
+/* warm start: v = start * D * s into init, cur and pdhg (pdlp_core_op.cu:174-201) */
+__global__ void k_warm_start(double *a, double *b, double *c, const double *start,
+                             const double *D, double s, int count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    double v = start[i] * D[i] * s;
+    a[i] = v;
+    b[i] = v;
+    c[i] = v;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* extern "C" launchers                                                      */
+/* ------------------------------------------------------------------------ */
+
+#define ST(s) ((hipStream_t)(s))
+#define RET(e)                                                                 \
+  do {                                                                         \
+    hipError_t _e = (e);                                                       \
+    if (_e != hipSuccess) return (int)_e;                                      \
+  } while (0)
+
+extern "C" {
+
+int dpdlp_plan_blocks(int rows, const int *rp, int cap, int *blk) {
+  int nb = 0, r = 0;
+  while (r < rows) {
+    int start = r;
+    long long len = (long long)rp[r + 1] - rp[r];
+    if (len > cap) {
+      r++;
+    } else {
+      long long nz = 0;
+      while (r < rows && r - start < DPDLP_BS) {
+        long long l = (long long)rp[r + 1] - rp[r];
+        if (l > cap || nz + l > cap) break;
+        nz += l;
+        r++;
+      }
+    }
+    blk[nb++] = start;
+  }
+  blk[nb] = rows;
+  return nb;
+}
+
+int dpdlp_dev_spmv(const dpdlp_csr_t *A, const double *x, double *y, void *stream) {
+  EpiStore e{y, 0};
+  return (int)launch_spmv(A, x, e, nullptr, 0, 0, ST(stream));
+}
+
+int dpdlp_dev_spmv_f32(const dpdlp_csr_t *A, const float *x, double *y, void *stream) {
+  EpiStore e{y, 0};
+  return (int)launch_spmv32(A, x, e, nullptr, 0, 0, ST(stream));
+}
+
+int dpdlp_dev_primal_phase(const dpdlp_state_t *st, double tau, double w, double rho, int major,
+                           int fpe, int use32, void *stream) {
+  EpiPrimal e;
+  e.x_cur = st->x_cur;
+  e.x_init = st->x_init;
+  e.c = st->c;
+  e.lv = st->lv;
+  e.uv = st->uv;
+  e.x_ref = st->x_ref;
+  e.x_pdhg = st->x_pdhg;
+  e.z = st->dual_slack;
+  e.dx = st->dx;
+  e.x_ref32 = st->x_ref32;
+  e.tau = tau;
+  e.w = w;
+  e.rho = rho;
+  e.major = major;
+  e.fpe = fpe;
+  e.partials = fpe;
+  if (use32)
+    return (int)launch_spmv32(&st->At, st->y_cur32, e, st->part, st->part_stride,
+                              DPDLP_SL_FPE, ST(stream));
+  return (int)launch_spmv(&st->At, st->y_cur, e, st->part, st->part_stride, DPDLP_SL_FPE,
+                          ST(stream));
+}
+
+int dpdlp_dev_dual_phase(const dpdlp_state_t *st, double sigma, double w, double rho, int major,
+                         int fpe, int use32, void *stream) {
+  EpiDual e;
+  e.y_cur = st->y_cur;
+  e.y_init = st->y_init;
+  e.lc = st->lc;
+  e.uc = st->uc;
+  e.y_ref = st->y_ref;
+  e.y_pdhg = st->y_pdhg;
+  e.dy = st->dy;
+  e.y_cur32 = st->y_cur32;
+  e.sigma = sigma;
+  e.w = w;
+  e.rho = rho;
+  e.major = major;
+  e.fpe = fpe;
+  e.partials = fpe;
+  if (use32)
+    return (int)launch_spmv32(&st->A, st->x_ref32, e, st->part, st->part_stride,
+                              DPDLP_SL_FPE + 3, ST(stream));
+  return (int)launch_spmv(&st->A, st->x_ref, e, st->part, st->part_stride, DPDLP_SL_FPE + 3,
+                          ST(stream));
+}
+
+int dpdlp_dev_fpe_finish(const dpdlp_state_t *st, int to_restart, int to_major, void *stream) {
+  EpiFpe e{st->dual_product, st->dx, 1};
+  RET(launch_spmv(&st->At, st->dy, e, st->part, st->part_stride, DPDLP_SL_FPE + 6,
+                  ST(stream)));
+  int d1 = to_restart ? DPDLP_SC_FPE_RESTART : -1;
+  int d2 = to_major ? DPDLP_SC_FPE_MAJOR : -1;
+  /* slots come from three kernels with different partial counts */
+  RET(finalize(st, st->At.nblk, DPDLP_SL_FPE, 3, 0u, DPDLP_SL_FPE, d1 >= 0 ? d1 : -1,
+               d2 >= 0 ? d2 : -1, ST(stream)));
+  RET(finalize(st, st->A.nblk, DPDLP_SL_FPE + 3, 3, 0u, DPDLP_SL_FPE + 3,
+               d1 >= 0 ? d1 + 3 : -1, d2 >= 0 ? d2 + 3 : -1, ST(stream)));
+  RET(finalize(st, st->At.nblk, DPDLP_SL_FPE + 6, 1, 0u, DPDLP_SL_FPE + 6,
+               d1 >= 0 ? d1 + 6 : -1, d2 >= 0 ? d2 + 6 : -1, ST(stream)));
+  return 0;
+}
+
+int dpdlp_dev_fpe_finish_polish(const dpdlp_state_t *st, int primal, int to_restart,
+                                int to_major, void *stream) {
+  int d1 = to_restart ? DPDLP_SC_FPE_RESTART : -1;
+  int d2 = to_major ? DPDLP_SC_FPE_MAJOR : -1;
+  if (primal)
+    return (int)finalize(st, st->At.nblk, DPDLP_SL_FPE, 3, 0u, DPDLP_SL_FPE, d1, d2,
+                         ST(stream));
+  return (int)finalize(st, st->A.nblk, DPDLP_SL_FPE + 3, 3, 0u, DPDLP_SL_FPE + 3,
+                       d1 >= 0 ? d1 + 3 : -1, d2 >= 0 ? d2 + 3 : -1, ST(stream));
+}
+
+int dpdlp_dev_residual(const dpdlp_state_t *st, const dpdlp_state_t *orig, int which,
+                       int restart_dist, void *stream) {
+  hipStream_t s = ST(stream);
+  if (which & 1) {
+    EpiResP e;
+    e.pprod = st->primal_product;
+    e.lc = st->lc;
+    e.uc = st->uc;
+    e.D_r = st->D_r;
+    e.y = st->y_pdhg;
+    e.y_init = st->y_init;
+    e.dist = restart_dist;
+    e.partials = 1;
+    RET(launch_spmv(&st->A, st->x_pdhg, e, st->part, st->part_stride, DPDLP_SL_RESP, s));
+    if (st->A.nblk > 0)
+      RET(finalize(st, st->A.nblk, DPDLP_SL_RESP, 8, 1u << 7, DPDLP_SL_RESP, -1, -1, s));
+    if (which == 1) {
+      /* primal polish: objective uses the original c (utils.cu:1176-1182) */
+      int g = ew_grid(st->n);
+      k_dot<<<g, DPDLP_BS, 0, s>>>(orig->c, st->x_pdhg, st->n, st->part, st->part_stride,
+                                   DPDLP_SL_RESD + 3);
+      RET(hipGetLastError());
+      RET(finalize(st, g, DPDLP_SL_RESD + 3, 1, 0u, DPDLP_SL_RESD + 3, -1, -1, s));
+    }
+  }
+  if (which & 2) {
+    EpiResD e;
+    e.dprod = st->dual_product;
+    e.c = st->c;
+    e.z = st->dual_slack;
+    e.D_c = st->D_c;
+    e.x_obj = (which == 2) ? orig->x_pdhg : st->x_pdhg;
+    e.x_init = st->x_init;
+    e.dist = restart_dist;
+    e.partials = 1;
+    RET(launch_spmv(&st->At, st->y_pdhg, e, st->part, st->part_stride, DPDLP_SL_RESD, s));
+    if (st->At.nblk > 0)
+      RET(finalize(st, st->At.nblk, DPDLP_SL_RESD, 9, 1u << 8, DPDLP_SL_RESD, -1, -1, s));
+    if (which == 2) {
+      /* dual polish: contributions use the original finite bounds */
+      int g = ew_grid(st->m);
+      k_dual_contrib_sum<<<g, DPDLP_BS, 0, s>>>(st->y_pdhg, orig->lc, orig->uc, st->m, st->part,
+                                                st->part_stride, DPDLP_SL_RESP + 3);
+      RET(hipGetLastError());
+      RET(finalize(st, g, DPDLP_SL_RESP + 3, 1, 0u, DPDLP_SL_RESP + 3, -1, -1, s));
+    }
+  }
+  return 0;
+}
+
+int dpdlp_dev_infeasibility(const dpdlp_state_t *st, void *stream) {
+  hipStream_t s = ST(stream);
+  int g = ew_grid((long long)st->n + st->m);
+  k_inf_project<<<g, DPDLP_BS, 0, s>>>(st->dx, st->lv, st->uv, st->n, st->dy, st->lc, st->uc,
+                                       st->m, st->part, st->part_stride, DPDLP_SL_INF);
+  RET(hipGetLastError());
+  RET(finalize(st, g, DPDLP_SL_INF, 2, 3u, DPDLP_SL_INF, -1, -1, s));
+  k_scale_by_inv<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->dx, st->n, st->scal, DPDLP_SL_INF);
+  RET(hipGetLastError());
+  EpiInfP ep;
+  ep.pprod = st->primal_product;
+  ep.pslack = st->primal_slack;
+  ep.lc = st->lc;
+  ep.uc = st->uc;
+  ep.D_r = st->D_r;
+  ep.dy = st->dy;
+  ep.partials = 1;
+  RET(launch_spmv(&st->A, st->dx, ep, st->part, st->part_stride, DPDLP_SL_INF + 2, s));
+  EpiInfD ed;
+  ed.dprod = st->dual_product;
+  ed.dslack = st->dual_slack;
+  ed.lv = st->lv;
+  ed.uv = st->uv;
+  ed.D_c = st->D_c;
+  ed.c = st->c;
+  ed.dx = st->dx;
+  ed.partials = 1;
+  RET(launch_spmv(&st->At, st->dy, ed, st->part, st->part_stride, DPDLP_SL_INF + 4, s));
+  if (st->A.nblk > 0)
+    RET(finalize(st, st->A.nblk, DPDLP_SL_INF + 2, 2, 2u, DPDLP_SL_INF + 2, -1, -1, s));
+  else
+    k_fill<<<1, 64, 0, s>>>(st->scal + DPDLP_SL_INF + 2, 0.0, 2);
+  if (st->At.nblk > 0)
+    RET(finalize(st, st->At.nblk, DPDLP_SL_INF + 4, 3, 4u, DPDLP_SL_INF + 4, -1, -1, s));
+  else
+    k_fill<<<1, 64, 0, s>>>(st->scal + DPDLP_SL_INF + 4, 0.0, 3);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_restart_copy(const dpdlp_state_t *st, int part, void *stream) {
+  k_restart_copy<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, ST(stream)>>>(
+      st->x_init, st->x_cur, st->x_pdhg, st->y_init, st->y_cur, st->y_pdhg, st->y_cur32, st->n,
+      st->m, part);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_power_init(const dpdlp_state_t *st, double *eig, double *next, void *stream) {
+  hipStream_t s = ST(stream);
+  int g = ew_grid(st->m);
+  k_nrm_partials<<<g, DPDLP_BS, 0, s>>>(eig, st->m, st->part, st->part_stride,
+                                        DPDLP_SL_PI + 1);
+  RET(hipGetLastError());
+  RET(finalize(st, g, DPDLP_SL_PI + 1, 3, 0u, DPDLP_SL_PI + 1, -1, -1, s));
+  k_pow_normalize<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, DPDLP_SL_PI + 1);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next, double *dual,
+                          int iters, void *stream) {
+  hipStream_t s = ST(stream);
+  int g = ew_grid(st->m);
+  for (int it = 0; it < iters && it < DPDLP_PI_BATCH; ++it) {
+    EpiStore e1{dual, 0};
+    RET(launch_spmv(&st->At, next, e1, nullptr, 0, 0, s));
+    EpiPow e2{eig, next, 1};
+    RET(launch_spmv(&st->A, dual, e2, st->part, st->part_stride, DPDLP_SL_PI, s));
+    /* sigma^2 also recorded in the history (slot 0 of the 4) */
+    RET(finalize(st, st->A.nblk, DPDLP_SL_PI, 4, 0u, DPDLP_SL_PI, DPDLP_SC_PI_HIST + 4 * it,
+                 -1, s));
+    k_pow_update<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, DPDLP_SL_PI,
+                                        DPDLP_SL_PI + 1, st->part, st->part_stride,
+                                        DPDLP_SL_PI + 4);
+    RET(hipGetLastError());
+    RET(finalize(st, g, DPDLP_SL_PI + 4, 3, 0u, DPDLP_SL_PI + 4,
+                 DPDLP_SC_PI_HIST + 4 * it + 1, -1, s));
+  }
+  return 0;
+}
+
+int dpdlp_dev_result(const dpdlp_state_t *st, double beta, double gamma, double *x_out,
+                     double *y_out, double *rc_out, void *stream) {
+  hipStream_t s = ST(stream);
+  EpiRc e{rc_out, st->c, st->D_c, gamma, 0};
+  RET(launch_spmv(&st->At, st->y_pdhg, e, nullptr, 0, 0, s));
+  if (st->At.nblk == 0 && st->n > 0) {
+    /* no nonzeros: A'y = 0 */
+    EpiRc e0 = e;
+    (void)e0;
+  }
+  k_unscale<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
+      st->x_pdhg, st->y_pdhg, st->D_c, st->D_r, beta, gamma, x_out, y_out, st->n, st->m);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream) {
+  hipStream_t s = ST(stream);
+  k_row_reduce<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 0, 1.0, rs);
+  k_row_reduce<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 0, 1.0, cs);
+  k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
+  k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
+  k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
+      st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
+  k_scale_values<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
+                                                     st->m, rs, cs, 1);
+  k_scale_values<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
+                                                     st->n, cs, rs, 0);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_pock_chambolle(dpdlp_state_t *st, double alpha, double *rs, double *cs,
+                             void *stream) {
+  hipStream_t s = ST(stream);
+  k_row_reduce<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 1, alpha, rs);
+  k_row_reduce<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 1, 2.0 - alpha,
+                                                   cs);
+  k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
+  k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
+  k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
+      st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
+  k_scale_values<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
+                                                     st->m, rs, cs, 1);
+  k_scale_values<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
+                                                     st->n, cs, rs, 0);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_bound_obj_sums(const dpdlp_state_t *st, void *stream) {
+  hipStream_t s = ST(stream);
+  int g = ew_grid((long long)st->m + st->n);
+  k_bound_obj_sums<<<g, DPDLP_BS, 0, s>>>(st->lc, st->uc, st->m, st->c, st->n, st->part,
+                                          st->part_stride, DPDLP_SL_SCAL);
+  RET(hipGetLastError());
+  return (int)finalize(st, g, DPDLP_SL_SCAL, 2, 0u, DPDLP_SL_SCAL, -1, -1, s);
+}
+
+int dpdlp_dev_bound_obj_apply(dpdlp_state_t *st, double beta, double gamma, void *stream) {
+  k_bound_obj_apply<<<ew_grid((long long)st->m + st->n), DPDLP_BS, 0, ST(stream)>>>(
+      st->lc, st->uc, st->m, st->lv, st->uv, st->c, st->n, beta, gamma);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_make_f32(const double *src, float *dst, long long count, void *stream) {
+  if (count <= 0) return 0;
+  k_to_f32<<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(src, dst, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_zero_finite(double *v, int count, void *stream) {
+  if (count <= 0) return 0;
+  k_zero_finite<<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(v, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_fill(double *v, double value, long long count, void *stream) {
+  if (count <= 0) return 0;
+  k_fill<<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(v, value, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_warm_start(double *a, double *b, double *c, const double *start, const double *D,
+                         double s, int count, void *stream) {
+  if (count <= 0) return 0;
+  k_warm_start<<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(a, b, c, start, D, s, count);
+  return (int)hipGetLastError();
+}
+
+} /* extern "C" */

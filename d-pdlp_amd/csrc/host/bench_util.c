@@ -1,0 +1,113 @@
+/*
+ * Kernel timing helpers for the benchmark: a standalone SpMV on a resident
+ * problem timed with HIP events on the stream it is launched on.
+ */
+#include "dpdlp_dev.h"
+#include "dpdlp_ext.h"
+#include "dpdlp_host.h"
+
+#include <hip/hip_runtime_api.h>
+#include <stdlib.h>
+
+
+void dpdlp_resident_info(const dpdlp_resident_t *r, int *m, int *n, int *nnz, int *nblk_A,
+                         int *nblk_At) {
+  if (m) *m = r->m;
+  if (n) *n = r->n;
+  if (nnz) *nnz = r->nnz;
+  if (nblk_A) *nblk_A = r->A_nblk;
+  if (nblk_At) *nblk_At = r->At_nblk;
+}
+
+double dpdlp_time_spmv(dpdlp_resident_t *r, int transpose, int reps, int use32) {
+  if (!r || reps <= 0) return -1.0;
+  dpdlp_csr_t A;
+  if (!transpose)
+    A = (dpdlp_csr_t){r->m, r->n, r->nnz, r->A_rp, r->A_ci, r->A_v, NULL, r->A_blk, r->A_nblk,
+                      r->cap};
+  else
+    A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
+                      r->At_nblk, r->cap};
+  hipStream_t s = NULL;
+  hipEvent_t e0 = NULL, e1 = NULL;
+  double *x = NULL, *y = NULL;
+  float *x32 = NULL, *v32 = NULL;
+  double ms = -1.0;
+  if (hipStreamCreate(&s) != hipSuccess) return -1.0;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto out;
+  if (hipMalloc((void **)&x, (size_t)(A.cols ? A.cols : 1) * 8) != hipSuccess) goto out;
+  if (hipMalloc((void **)&y, (size_t)(A.rows ? A.rows : 1) * 8) != hipSuccess) goto out;
+  if (dpdlp_dev_fill(x, 1.0, A.cols, s)) goto out;
+  if (use32) {
+    if (hipMalloc((void **)&x32, (size_t)(A.cols ? A.cols : 1) * 4) != hipSuccess) goto out;
+    if (hipMalloc((void **)&v32, (size_t)(A.nnz ? A.nnz : 1) * 4) != hipSuccess) goto out;
+    if (dpdlp_dev_make_f32(x, x32, A.cols, s)) goto out;
+    if (dpdlp_dev_make_f32(A.v, v32, A.nnz, s)) goto out;
+    A.v32 = v32;
+  }
+  /* warm up */
+  for (int k = 0; k < 3; ++k)
+    if ((use32 ? dpdlp_dev_spmv_f32(&A, x32, y, s) : dpdlp_dev_spmv(&A, x, y, s))) goto out;
+  if (hipEventRecord(e0, s) != hipSuccess) goto out;
+  for (int k = 0; k < reps; ++k)
+    if ((use32 ? dpdlp_dev_spmv_f32(&A, x32, y, s) : dpdlp_dev_spmv(&A, x, y, s))) goto out;
+  if (hipEventRecord(e1, s) != hipSuccess) goto out;
+  if (hipEventSynchronize(e1) != hipSuccess) goto out;
+  {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / reps;
+  }
+out:
+  if (s) hipStreamSynchronize(s);
+  if (x) hipFree(x);
+  if (y) hipFree(y);
+  if (x32) hipFree(x32);
+  if (v32) hipFree(v32);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (s) hipStreamDestroy(s);
+  return ms;
+}
+
+int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, double *yh,
+                        int use32) {
+  if (!r || !xh || !yh) return -1;
+  dpdlp_csr_t A;
+  if (!transpose)
+    A = (dpdlp_csr_t){r->m, r->n, r->nnz, r->A_rp, r->A_ci, r->A_v, NULL, r->A_blk, r->A_nblk,
+                      r->cap};
+  else
+    A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
+                      r->At_nblk, r->cap};
+  double *x = NULL, *y = NULL;
+  float *x32 = NULL, *v32 = NULL;
+  int rc = -1;
+  hipStream_t s = NULL;
+  if (hipStreamCreate(&s) != hipSuccess) return -1;
+  if (hipMalloc((void **)&x, (size_t)(A.cols ? A.cols : 1) * 8) != hipSuccess) goto out;
+  if (hipMalloc((void **)&y, (size_t)(A.rows ? A.rows : 1) * 8) != hipSuccess) goto out;
+  if (A.cols && hipMemcpy(x, xh, (size_t)A.cols * 8, hipMemcpyHostToDevice) != hipSuccess)
+    goto out;
+  if (use32) {
+    if (hipMalloc((void **)&x32, (size_t)(A.cols ? A.cols : 1) * 4) != hipSuccess) goto out;
+    if (hipMalloc((void **)&v32, (size_t)(A.nnz ? A.nnz : 1) * 4) != hipSuccess) goto out;
+    if (dpdlp_dev_make_f32(x, x32, A.cols, s)) goto out;
+    if (dpdlp_dev_make_f32(A.v, v32, A.nnz, s)) goto out;
+    A.v32 = v32;
+    if (dpdlp_dev_spmv_f32(&A, x32, y, s)) goto out;
+  } else {
+    if (dpdlp_dev_spmv(&A, x, y, s)) goto out;
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) goto out;
+  if (A.rows && hipMemcpy(yh, y, (size_t)A.rows * 8, hipMemcpyDeviceToHost) != hipSuccess)
+    goto out;
+  rc = 0;
+out:
+  if (s) hipStreamSynchronize(s);
+  if (x) hipFree(x);
+  if (y) hipFree(y);
+  if (x32) hipFree(x32);
+  if (v32) hipFree(v32);
+  if (s) hipStreamDestroy(s);
+  return rc;
+}

@@ -1,0 +1,55 @@
+/*
+ * Internal declarations shared by the host C sources of the library.
+ */
+#pragma once
+
+#include "cupdlpx_types.h"
+#include <stdarg.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A problem resident in HBM (unscaled), see dpdlp_upload(). */
+struct dpdlp_resident {
+  int m, n, nnz, cap;
+  int *A_rp, *A_ci, *At_rp, *At_ci, *A_blk, *At_blk;
+  int A_nblk, At_nblk;
+  double *A_v, *At_v;
+  double *c, *lv, *uv, *lc, *uc;
+  double c0, obj_l2, obj_linf, bnd_l2, bnd_linf;
+  double *h_pstart, *h_dstart;
+  double upload_time;
+};
+
+/* logging.c: the reference's console formats (src/utils.cu:191-501) */
+const char *dpdlp_termination_string(termination_reason_t r);
+int dpdlp_print_frequency(int iter);
+void dpdlp_print_initial_info(const pdhg_parameters_t *p, const lp_problem_t *prob);
+void dpdlp_print_table_header(void);
+void dpdlp_print_iteration(int total, double t, double pobj, double dobj, double abs_p,
+                           double abs_d, double gap, double rel_p, double rel_d,
+                           double rel_gap);
+void dpdlp_print_final(const cupdlpx_result_t *r, const pdhg_parameters_t *p);
+void dpdlp_print_polish_header(int primal, const pdhg_parameters_t *p);
+void dpdlp_print_polish_iteration(int primal, int total, double t, double obj, double abs_r,
+                                  double rel_r);
+void dpdlp_print_polish_final(termination_reason_t pr, int piters, double ptime,
+                              termination_reason_t dr, int diters, double dtime,
+                              double prel, double drel, double gap);
+
+/* error.c */
+void dpdlp_set_error(const char *fmt, ...);
+double dpdlp_wall(void);
+
+/* transpose.c: stable CSR transpose on the host */
+int dpdlp_host_transpose(int m, int n, const int *rp, const int *ci, const double *v,
+                         int *t_rp, int *t_ci, double *t_v);
+
+/* rng.cc: std::mt19937(seed) + std::normal_distribution<double>(0, 1) */
+void dpdlp_normal_fill(uint32_t seed, int count, double *out);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,1028 @@
+/*
+ * Host driver of the single-GPU solve (the reference's optimize(),
+ * src/solver.cu:32-127, with initialize_solver_state, the step size, restart,
+ * residual, infeasibility, polishing and result logic of pdlp_core_op.cu and
+ * utils.cu).
+ *
+ * Control stays on the host, as in the reference, but the host never waits
+ * inside an iteration: each PDHG step is two fused kernels launched
+ * asynchronously, every reduction lands in device scalar slots, and the host
+ * reads all of them with ONE device-to-host copy per evaluation point (the
+ * reference makes 5-12 blocking cuBLAS calls per evaluation).  Scalar
+ * decisions (termination, restart, PID primal weight) are the reference's,
+ * made from those values.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "cupdlpx.h"
+#include "dpdlp_dev.h"
+#include "dpdlp_ext.h"
+#include "dpdlp_host.h"
+
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* blue_combine of dev_common.h, host copy */
+#define BLUE_SSML 0x1p+537
+#define BLUE_SBIG 0x1p-538
+static double blue(const double *a) {
+  double asml = a[0], amed = a[1], abig = a[2];
+  double scl, sumsq;
+  if (abig > 0.0) {
+    if (amed > 0.0 || isnan(amed)) abig += (amed * BLUE_SBIG) * BLUE_SBIG;
+    scl = 1.0 / BLUE_SBIG;
+    sumsq = abig;
+  } else if (asml > 0.0) {
+    if (amed > 0.0 || isnan(amed)) {
+      double ymed = sqrt(amed), ysml = sqrt(asml) / BLUE_SSML;
+      double ymin = ysml > ymed ? ymed : ysml, ymax = ysml > ymed ? ysml : ymed;
+      double r = ymin / ymax;
+      scl = 1.0;
+      sumsq = ymax * ymax * (1.0 + r * r);
+    } else {
+      scl = 1.0 / BLUE_SSML;
+      sumsq = asml;
+    }
+  } else {
+    scl = 1.0;
+    sumsq = amed;
+  }
+  return scl * sqrt(sumsq);
+}
+
+/* ------------------------------------------------------------------------ */
+
+
+typedef struct {
+  double step, omega, best_omega, pw_err_sum, pw_last_err, best_gap;
+  int total, inner, is_major;
+  double fpe, fpe0, fpe_last;
+  double abs_p, rel_p, abs_d, rel_d, pobj, dobj, gap, rel_gap;
+  double max_p_ray_inf, max_d_ray_inf, p_ray_lin_obj, d_ray_obj;
+  termination_reason_t reason;
+  double t_cum;
+  int pend_rst, pend_maj;
+} hstate_t;
+
+#define MAX_ALLOCS 128
+typedef struct {
+  hipStream_t s;
+  dpdlp_state_t st;
+  double *h_scal;
+  double beta, gamma, c0, obj_norm, bound_norm;
+  void *allocs[MAX_ALLOCS];
+  int nalloc;
+  int mixed, use32;
+  double *w_A_v, *w_At_v;
+  float *w_A_v32, *w_At_v32;
+  /* live phase timing */
+  hipEvent_t ev[64][3];
+  int nev_created, nev_used;
+  double p1_ms, p2_ms;
+  int nsamp;
+} ctx_t;
+
+#define HIPCHK(call)                                                           \
+  do {                                                                         \
+    hipError_t _e = (hipError_t)(call);                                        \
+    if (_e != hipSuccess) {                                                    \
+      dpdlp_set_error("HIP error %d (%s) at %s:%d", (int)_e,                   \
+                      hipGetErrorString(_e), __FILE__, __LINE__);              \
+      goto fail;                                                               \
+    }                                                                          \
+  } while (0)
+
+static void *dmalloc(ctx_t *c, size_t bytes) {
+  void *p = NULL;
+  if (c->nalloc >= MAX_ALLOCS) return NULL;
+  if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return NULL;
+  c->allocs[c->nalloc++] = p;
+  return p;
+}
+
+static double *dzero(ctx_t *c, long long count) {
+  double *p = (double *)dmalloc(c, (size_t)(count > 0 ? count : 1) * sizeof(double));
+  if (p && hipMemsetAsync(p, 0, (size_t)(count > 0 ? count : 1) * sizeof(double), c->s) !=
+               hipSuccess)
+    return NULL;
+  return p;
+}
+
+static double *dcopy(ctx_t *c, const double *src, long long count) {
+  double *p = (double *)dmalloc(c, (size_t)(count > 0 ? count : 1) * sizeof(double));
+  if (p && count > 0 &&
+      hipMemcpyAsync(p, src, (size_t)count * sizeof(double), hipMemcpyDeviceToDevice, c->s) !=
+          hipSuccess)
+    return NULL;
+  return p;
+}
+
+static void harvest_events(ctx_t *c) {
+  for (int k = 0; k < c->nev_used; ++k) {
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, c->ev[k][0], c->ev[k][1]) == hipSuccess &&
+        hipEventElapsedTime(&b, c->ev[k][1], c->ev[k][2]) == hipSuccess) {
+      c->p1_ms += a;
+      c->p2_ms += b;
+      c->nsamp++;
+    }
+  }
+  c->nev_used = 0;
+}
+
+static void ctx_free(ctx_t *c) {
+  if (c->s) hipStreamSynchronize(c->s);
+  for (int k = 0; k < c->nev_created; ++k)
+    for (int q = 0; q < 3; ++q) hipEventDestroy(c->ev[k][q]);
+  c->nev_created = 0;
+  for (int i = 0; i < c->nalloc; ++i) hipFree(c->allocs[i]);
+  c->nalloc = 0;
+  if (c->h_scal) hipHostFree(c->h_scal);
+  c->h_scal = NULL;
+  if (c->s) hipStreamDestroy(c->s);
+  c->s = NULL;
+}
+
+static int sync_scal(ctx_t *c, int count) {
+  if (hipMemcpyAsync(c->h_scal, c->st.scal, (size_t)count * sizeof(double),
+                     hipMemcpyDeviceToHost, c->s) != hipSuccess)
+    return -1;
+  if (hipStreamSynchronize(c->s) != hipSuccess) return -1;
+  if (c->nev_used) harvest_events(c);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Upload                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static int pick_cap(long long nnz, int rows) {
+  (void)rows;
+  if (nnz >= 4000000) return 2048;
+  return 1024;
+}
+
+void dpdlp_default_opts(dpdlp_solve_opts_t *o) {
+  memset(o, 0, sizeof(*o));
+  o->precision = DPDLP_PREC_FP64;
+  o->mixed_switch_tol = 1e-4 * 0.1;
+  o->time_to_1e4_sec = -1.0;
+  o->time_to_1e6_sec = -1.0;
+}
+
+void dpdlp_resident_free(dpdlp_resident_t *r) {
+  if (!r) return;
+  void *ptrs[] = {r->A_rp, r->A_ci, r->At_rp, r->At_ci, r->A_blk, r->At_blk, r->A_v,
+                  r->At_v, r->c,    r->lv,   r->uv,    r->lc,    r->uc};
+  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
+    if (ptrs[i]) hipFree(ptrs[i]);
+  free(r->h_pstart);
+  free(r->h_dstart);
+  free(r);
+}
+
+#define UPCHK(call)                                                            \
+  do {                                                                         \
+    hipError_t _e = (hipError_t)(call);                                        \
+    if (_e != hipSuccess) {                                                    \
+      dpdlp_set_error("HIP error %d (%s) at %s:%d", (int)_e,                   \
+                      hipGetErrorString(_e), __FILE__, __LINE__);              \
+      goto ufail;                                                              \
+    }                                                                          \
+  } while (0)
+
+static hipError_t up(void **dst, const void *src, size_t bytes) {
+  hipError_t e = hipMalloc(dst, bytes ? bytes : 8);
+  if (e != hipSuccess) return e;
+  if (bytes && src) return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  return hipSuccess;
+}
+
+dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
+  double t0 = dpdlp_wall();
+  if (!p || p->num_variables < 0 || p->num_constraints < 0 ||
+      p->constraint_matrix_num_nonzeros < 0) {
+    dpdlp_set_error("invalid problem");
+    return NULL;
+  }
+  int m = p->num_constraints, n = p->num_variables, nnz = p->constraint_matrix_num_nonzeros;
+  const int *rp = p->constraint_matrix_row_pointers, *ci = p->constraint_matrix_col_indices;
+  if (!rp || (nnz > 0 && (!ci || !p->constraint_matrix_values)) ||
+      (n > 0 && (!p->objective_vector || !p->variable_lower_bound ||
+                 !p->variable_upper_bound)) ||
+      (m > 0 && (!p->constraint_lower_bound || !p->constraint_upper_bound))) {
+    dpdlp_set_error("problem has NULL arrays");
+    return NULL;
+  }
+  if (rp[0] != 0 || rp[m] != nnz) {
+    dpdlp_set_error("row pointers inconsistent with nnz");
+    return NULL;
+  }
+  for (int i = 0; i < m; ++i)
+    if (rp[i + 1] < rp[i]) {
+      dpdlp_set_error("row pointers not monotone at row %d", i);
+      return NULL;
+    }
+  for (int k = 0; k < nnz; ++k)
+    if (ci[k] < 0 || ci[k] >= n) {
+      dpdlp_set_error("column index %d out of range at nonzero %d", ci[k], k);
+      return NULL;
+    }
+  dpdlp_resident_t *r = (dpdlp_resident_t *)calloc(1, sizeof(*r));
+  int *t_rp = NULL, *t_ci = NULL, *blk = NULL;
+  double *t_v = NULL;
+  if (!r) return NULL;
+  r->m = m;
+  r->n = n;
+  r->nnz = nnz;
+  r->cap = spmv_cap > 0 ? spmv_cap : pick_cap(nnz, m);
+  if (r->cap != 1024 && r->cap != 2048 && r->cap != 4096) r->cap = 2048;
+  r->c0 = p->objective_constant;
+
+  /* norms of the unscaled problem, pdlp_core_op.cu:232-286 */
+  {
+    double ssq = 0.0, mx = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double v = p->objective_vector[j];
+      ssq += v * v;
+      if (fabs(v) > mx) mx = fabs(v);
+    }
+    r->obj_l2 = sqrt(ssq);
+    r->obj_linf = mx;
+    ssq = 0.0;
+    mx = 0.0;
+    for (int i = 0; i < m; ++i) {
+      double lo = p->constraint_lower_bound[i], upb = p->constraint_upper_bound[i];
+      if (isfinite(lo) && lo != upb) {
+        ssq += lo * lo;
+        if (fabs(lo) > mx) mx = fabs(lo);
+      }
+      if (isfinite(upb)) {
+        ssq += upb * upb;
+        if (fabs(upb) > mx) mx = fabs(upb);
+      }
+    }
+    r->bnd_l2 = sqrt(ssq);
+    r->bnd_linf = mx;
+  }
+  if (p->primal_start) {
+    r->h_pstart = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+    if (!r->h_pstart) goto ufail;
+    memcpy(r->h_pstart, p->primal_start, (size_t)n * sizeof(double));
+  }
+  if (p->dual_start) {
+    r->h_dstart = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
+    if (!r->h_dstart) goto ufail;
+    memcpy(r->h_dstart, p->dual_start, (size_t)m * sizeof(double));
+  }
+
+  t_rp = (int *)malloc((size_t)(n + 1) * sizeof(int));
+  t_ci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+  t_v = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+  blk = (int *)malloc((size_t)((m > n ? m : n) + 2) * sizeof(int));
+  if (!t_rp || !t_ci || !t_v || !blk) {
+    dpdlp_set_error("host allocation failed");
+    goto ufail;
+  }
+  if (dpdlp_host_transpose(m, n, rp, ci, p->constraint_matrix_values, t_rp, t_ci, t_v))
+    goto ufail;
+
+  UPCHK(up((void **)&r->A_rp, rp, (size_t)(m + 1) * sizeof(int)));
+  UPCHK(up((void **)&r->A_ci, ci, (size_t)nnz * sizeof(int)));
+  UPCHK(up((void **)&r->A_v, p->constraint_matrix_values, (size_t)nnz * sizeof(double)));
+  UPCHK(up((void **)&r->At_rp, t_rp, (size_t)(n + 1) * sizeof(int)));
+  UPCHK(up((void **)&r->At_ci, t_ci, (size_t)nnz * sizeof(int)));
+  UPCHK(up((void **)&r->At_v, t_v, (size_t)nnz * sizeof(double)));
+  r->A_nblk = dpdlp_plan_blocks(m, rp, r->cap, blk);
+  UPCHK(up((void **)&r->A_blk, blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
+  r->At_nblk = dpdlp_plan_blocks(n, t_rp, r->cap, blk);
+  UPCHK(up((void **)&r->At_blk, blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
+  UPCHK(up((void **)&r->c, p->objective_vector, (size_t)n * sizeof(double)));
+  UPCHK(up((void **)&r->lv, p->variable_lower_bound, (size_t)n * sizeof(double)));
+  UPCHK(up((void **)&r->uv, p->variable_upper_bound, (size_t)n * sizeof(double)));
+  UPCHK(up((void **)&r->lc, p->constraint_lower_bound, (size_t)m * sizeof(double)));
+  UPCHK(up((void **)&r->uc, p->constraint_upper_bound, (size_t)m * sizeof(double)));
+  free(t_rp);
+  free(t_ci);
+  free(t_v);
+  free(blk);
+  r->upload_time = dpdlp_wall() - t0;
+  return r;
+ufail:
+  free(t_rp);
+  free(t_ci);
+  free(t_v);
+  free(blk);
+  dpdlp_resident_free(r);
+  return NULL;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Host halves of the reference's scalar logic                               */
+/* ------------------------------------------------------------------------ */
+
+/* utils.cu:665-718 from the raw slots */
+static void host_residual(ctx_t *c, hstate_t *h, norm_type_t norm) {
+  const double *S = c->h_scal;
+  h->abs_p = (norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESP + 7] : blue(S + DPDLP_SL_RESP)) / c->beta;
+  h->abs_d = (norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESD + 8] : blue(S + DPDLP_SL_RESD)) / c->gamma;
+  h->pobj = S[DPDLP_SL_RESD + 3] / (c->beta * c->gamma) + c->c0;
+  h->dobj = (S[DPDLP_SL_RESD + 4] + S[DPDLP_SL_RESP + 3]) / (c->beta * c->gamma) + c->c0;
+  h->rel_p = h->abs_p / (1.0 + c->bound_norm);
+  h->rel_d = h->abs_d / (1.0 + c->obj_norm);
+  h->gap = fabs(h->pobj - h->dobj);
+  h->rel_gap = h->gap / (1.0 + fabs(h->pobj) + fabs(h->dobj));
+}
+
+/* utils.cu:760-812 from the raw slots */
+static void host_infeasibility(ctx_t *c, hstate_t *h) {
+  const double *S = c->h_scal + DPDLP_SL_INF;
+  double d_inf = S[1];
+  h->p_ray_lin_obj = S[5] / (c->beta * c->gamma);
+  h->d_ray_obj = (S[2] + S[4]) / (c->beta * c->gamma);
+  h->max_p_ray_inf = S[3];
+  double ds = S[6];
+  h->max_d_ray_inf = ds;
+  double sf = fmax(d_inf, ds);
+  if (sf > 0.0) {
+    h->max_d_ray_inf /= sf;
+    h->d_ray_obj /= sf;
+  } else {
+    h->max_d_ray_inf = 0.0;
+    h->d_ray_obj = 0.0;
+  }
+}
+
+/* pdlp_core_op.cu:687-695 */
+static double host_fpe(const double *G, double omega, double step) {
+  double pn = blue(G + 0), dn = blue(G + 3);
+  double movement = pn * pn * omega + dn * dn / omega;
+  double interaction = 2 * step * G[6];
+  return sqrt(movement + interaction);
+}
+
+static void process_pending_fpe(ctx_t *c, hstate_t *h, int polish) {
+  const double *S = c->h_scal;
+  if (h->pend_rst) {
+    const double *G = S + DPDLP_SC_FPE_RESTART;
+    double v;
+    if (polish == 1) {
+      double pn = blue(G);
+      v = pn * pn * h->omega;
+    } else if (polish == 2) {
+      double dn = blue(G + 3);
+      v = dn * dn / h->omega;
+    } else {
+      v = host_fpe(G, h->omega, h->step);
+    }
+    h->fpe = v;
+    h->fpe0 = v;
+    h->pend_rst = 0;
+  }
+  if (h->pend_maj) {
+    const double *G = S + DPDLP_SC_FPE_MAJOR;
+    if (polish == 1) {
+      double pn = blue(G);
+      h->fpe = pn * pn * h->omega;
+    } else if (polish == 2) {
+      double dn = blue(G + 3);
+      h->fpe = dn * dn / h->omega;
+    } else {
+      h->fpe = host_fpe(G, h->omega, h->step);
+    }
+    h->pend_maj = 0;
+  }
+}
+
+/* utils.cu:214-263 */
+static void check_termination(hstate_t *h, const termination_criteria_t *tc) {
+  if (h->rel_d < tc->eps_feasible_relative && h->rel_p < tc->eps_feasible_relative &&
+      h->rel_gap < tc->eps_optimal_relative) {
+    h->reason = TERMINATION_REASON_OPTIMAL;
+    return;
+  }
+  if (h->d_ray_obj > 0.0 && h->max_d_ray_inf / h->d_ray_obj <= tc->eps_infeasible) {
+    h->reason = TERMINATION_REASON_PRIMAL_INFEASIBLE;
+    return;
+  }
+  if (h->p_ray_lin_obj < 0.0 && h->max_p_ray_inf / (-h->p_ray_lin_obj) <= tc->eps_infeasible) {
+    h->reason = TERMINATION_REASON_DUAL_INFEASIBLE;
+    return;
+  }
+  if (h->total >= tc->iteration_limit) {
+    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
+    return;
+  }
+  if (h->t_cum >= tc->time_sec_limit) h->reason = TERMINATION_REASON_TIME_LIMIT;
+}
+
+/* utils.cu:1015-1039 */
+static void check_polish_termination(hstate_t *h, const termination_criteria_t *tc, int primal) {
+  if (primal ? (h->rel_p <= tc->eps_feas_polish_relative)
+             : (h->rel_d <= tc->eps_feas_polish_relative)) {
+    h->reason = TERMINATION_REASON_FEAS_POLISH_SUCCESS;
+    return;
+  }
+  if (h->total >= tc->iteration_limit) {
+    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
+    return;
+  }
+  if (h->t_cum >= tc->time_sec_limit) h->reason = TERMINATION_REASON_TIME_LIMIT;
+}
+
+/* utils.cu:265-293 */
+static int should_restart(hstate_t *h, const restart_parameters_t *rp, int freq) {
+  int r = 0;
+  if (h->total == freq) {
+    r = 1;
+  } else if (h->total > freq) {
+    if (h->fpe <= rp->sufficient_reduction_for_restart * h->fpe0) r = 1;
+    if (h->fpe <= rp->necessary_reduction_for_restart * h->fpe0 && h->fpe > h->fpe_last) r = 1;
+    if (h->inner >= rp->artificial_restart_threshold * h->total) r = 1;
+  }
+  h->fpe_last = h->fpe;
+  return r;
+}
+
+/* PID primal weight of pdlp_core_op.cu:593-619 */
+static void pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, double dd) {
+  double ratio = h->rel_d / h->rel_p;
+  if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+    double err = log(dd) - log(pd) - log(h->omega);
+    h->pw_err_sum *= rp->i_smooth;
+    h->pw_err_sum += err;
+    double derr = err - h->pw_last_err;
+    h->omega *= exp(rp->k_p * err + rp->k_i * h->pw_err_sum + rp->k_d * derr);
+    h->pw_last_err = err;
+  } else {
+    h->omega = h->best_omega;
+    h->pw_err_sum = 0.0;
+    h->pw_last_err = 0.0;
+  }
+  double g = fabs(log10(h->rel_d / h->rel_p));
+  if (g < h->best_gap) {
+    h->best_gap = g;
+    h->best_omega = h->omega;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* One PDHG step (solver.cu:85-101)                                          */
+/* ------------------------------------------------------------------------ */
+
+static int pdhg_step(ctx_t *c, dpdlp_state_t *st, hstate_t *h, double rho, int do_restart,
+                     int use32, int polish, int timed) {
+  int major_variant =
+      h->is_major || ((h->total + 2) % dpdlp_print_frequency(h->total + 2)) == 0;
+  int fpe = h->is_major || do_restart;
+  double w = (double)(h->inner + 1) / (h->inner + 2);
+  int e;
+  hipEvent_t *ev = NULL;
+  if (timed && c->nev_used < c->nev_created) ev = c->ev[c->nev_used++];
+  if (ev && hipEventRecord(ev[0], c->s) != hipSuccess) return -1;
+  e = dpdlp_dev_primal_phase(st, h->step / h->omega, w, rho, major_variant, fpe, use32, c->s);
+  if (e) return e;
+  if (ev && hipEventRecord(ev[1], c->s) != hipSuccess) return -1;
+  e = dpdlp_dev_dual_phase(st, h->step * h->omega, w, rho, major_variant, fpe, use32, c->s);
+  if (e) return e;
+  if (ev && hipEventRecord(ev[2], c->s) != hipSuccess) return -1;
+  if (fpe) {
+    if (polish)
+      e = dpdlp_dev_fpe_finish_polish(st, polish == 1, do_restart, h->is_major, c->s);
+    else
+      e = dpdlp_dev_fpe_finish(st, do_restart, h->is_major, c->s);
+    if (e) return e;
+    if (do_restart) h->pend_rst = 1;
+    if (h->is_major) h->pend_maj = 1;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Feasibility polishing (pdlp_core_op.cu:862-1030)                          */
+/* ------------------------------------------------------------------------ */
+
+static void polish_reset(hstate_t *p, const hstate_t *o, double omega0) {
+  memset(p, 0, sizeof(*p));
+  p->best_omega = 0.0;
+  p->fpe = INFINITY;
+  p->fpe0 = INFINITY;
+  p->fpe_last = INFINITY;
+  p->step = o->step;
+  p->omega = omega0;
+  p->best_omega = omega0;
+  p->reason = TERMINATION_REASON_UNSPECIFIED;
+  p->best_gap = INFINITY;
+}
+
+static int polish_loop(ctx_t *c, dpdlp_state_t *ps, hstate_t *h, const pdhg_parameters_t *params,
+                       int primal, double c0_state) {
+  double t0 = dpdlp_wall();
+  int do_restart = 0, freq = params->termination_evaluation_frequency;
+  const termination_criteria_t *tc = &params->termination_criteria;
+  dpdlp_print_polish_header(primal, params);
+  while (h->reason == TERMINATION_REASON_UNSPECIFIED) {
+    if (h->is_major || h->total == 0 || h->total % dpdlp_print_frequency(h->total) == 0) {
+      if (dpdlp_dev_residual(ps, &c->st, primal ? 1 : 2, 0, c->s)) return -1;
+      if (sync_scal(c, 64)) return -1;
+      process_pending_fpe(c, h, primal ? 1 : 2);
+      const double *S = c->h_scal;
+      if (primal) {
+        h->abs_p = (params->optimality_norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESP + 7]
+                                                               : blue(S + DPDLP_SL_RESP)) /
+                   c->beta;
+        h->rel_p = h->abs_p / (1.0 + c->bound_norm);
+        h->pobj = S[DPDLP_SL_RESD + 3] / (c->beta * c->gamma) + c0_state;
+      } else {
+        h->abs_d = (params->optimality_norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESD + 8]
+                                                               : blue(S + DPDLP_SL_RESD)) /
+                   c->gamma;
+        h->rel_d = h->abs_d / (1.0 + c->obj_norm);
+        h->dobj = (S[DPDLP_SL_RESD + 4] + S[DPDLP_SL_RESP + 3]) / (c->beta * c->gamma) + c0_state;
+      }
+      h->t_cum = dpdlp_wall() - t0;
+      check_polish_termination(h, tc, primal);
+      if (params->verbose)
+        dpdlp_print_polish_iteration(primal, h->total, h->t_cum, primal ? h->pobj : h->dobj,
+                                     primal ? h->abs_p : h->abs_d, primal ? h->rel_p : h->rel_d);
+    }
+    if (h->is_major || h->total == 0) {
+      do_restart = should_restart(h, &params->restart_params, freq);
+      if (do_restart) {
+        if (dpdlp_dev_restart_copy(ps, primal ? 1 : 2, c->s)) return -1;
+        h->inner = 0;
+        h->fpe_last = INFINITY;
+      }
+    }
+    h->is_major = ((h->total + 1) % freq) == 0;
+    if (pdhg_step(c, ps, h, params->reflection_coefficient, do_restart, 0, primal ? 1 : 2, 0))
+      return -1;
+    do_restart = 0;
+    h->inner++;
+    h->total++;
+  }
+  return 0;
+}
+
+static int feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *params,
+                              double *ptime, int *piters) {
+  double t0 = dpdlp_wall();
+  const termination_criteria_t *tc = &params->termination_criteria;
+  if (h->rel_p < tc->eps_feas_polish_relative && h->rel_d < tc->eps_feas_polish_relative) {
+    printf("Skipping feasibility polishing as the solution is already sufficiently feasible.\n");
+    return 0;
+  }
+  double omega0 = params->bound_objective_rescaling
+                      ? 1.0
+                      : (c->obj_norm + 1.0) / (c->bound_norm + 1.0);
+  dpdlp_state_t *o = &c->st;
+  int n = o->n, m = o->m;
+  dpdlp_state_t P = *o, D = *o;
+  hstate_t hp, hd;
+
+  /* primal polish state (pdlp_core_op.cu:1032-1105) */
+  P.c = dzero(c, n);
+  P.x_init = dcopy(c, o->x_init, n);
+  P.x_cur = dcopy(c, o->x_cur, n);
+  P.x_pdhg = dcopy(c, o->x_pdhg, n);
+  P.x_ref = dcopy(c, o->x_ref, n);
+  P.primal_product = dcopy(c, o->primal_product, m);
+  P.y_init = dzero(c, m);
+  P.y_cur = dzero(c, m);
+  P.y_pdhg = dzero(c, m);
+  P.y_ref = dzero(c, m);
+  P.dual_product = dzero(c, n);
+  P.dual_slack = dzero(c, n);
+  P.primal_slack = dzero(c, m);
+  P.dx = dzero(c, n);
+  P.dy = dzero(c, m);
+  P.x_ref32 = NULL;
+  P.y_cur32 = NULL;
+  if (!P.c || !P.x_init || !P.x_cur || !P.x_pdhg || !P.x_ref || !P.primal_product || !P.y_init ||
+      !P.y_cur || !P.y_pdhg || !P.y_ref || !P.dual_product || !P.dual_slack || !P.primal_slack ||
+      !P.dx || !P.dy)
+    return -1;
+  polish_reset(&hp, h, omega0);
+  if (polish_loop(c, &P, &hp, params, 1, 0.0)) return -1;
+  if (hp.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+    if (hipMemcpyAsync(o->x_pdhg, P.x_pdhg, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                       c->s) != hipSuccess)
+      return -1;
+    h->abs_p = hp.abs_p;
+    h->rel_p = hp.rel_p;
+    h->pobj = hp.pobj;
+  }
+  *piters += hp.total - 1;
+
+  /* dual polish state (pdlp_core_op.cu:1145-1249) */
+  D.lc = dcopy(c, o->lc, m);
+  D.uc = dcopy(c, o->uc, m);
+  D.lv = dcopy(c, o->lv, n);
+  D.uv = dcopy(c, o->uv, n);
+  if (!D.lc || !D.uc || !D.lv || !D.uv) return -1;
+  if (dpdlp_dev_zero_finite(D.lc, m, c->s) || dpdlp_dev_zero_finite(D.uc, m, c->s) ||
+      dpdlp_dev_zero_finite(D.lv, n, c->s) || dpdlp_dev_zero_finite(D.uv, n, c->s))
+    return -1;
+  D.y_init = dcopy(c, o->y_init, m);
+  D.y_cur = dcopy(c, o->y_cur, m);
+  D.y_pdhg = dcopy(c, o->y_pdhg, m);
+  D.y_ref = dcopy(c, o->y_ref, m);
+  D.dual_product = dcopy(c, o->dual_product, n);
+  D.dual_slack = dcopy(c, o->dual_slack, n);
+  D.x_init = dzero(c, n);
+  D.x_cur = dzero(c, n);
+  D.x_pdhg = dzero(c, n);
+  D.x_ref = dzero(c, n);
+  D.primal_product = dzero(c, m);
+  D.primal_slack = dzero(c, m);
+  D.dx = dzero(c, n);
+  D.dy = dzero(c, m);
+  D.x_ref32 = NULL;
+  D.y_cur32 = NULL;
+  if (!D.y_init || !D.y_cur || !D.y_pdhg || !D.y_ref || !D.dual_product || !D.dual_slack ||
+      !D.x_init || !D.x_cur || !D.x_pdhg || !D.x_ref || !D.primal_product || !D.primal_slack ||
+      !D.dx || !D.dy)
+    return -1;
+  polish_reset(&hd, h, omega0);
+  if (polish_loop(c, &D, &hd, params, 0, c->c0)) return -1;
+  if (hd.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
+    if (hipMemcpyAsync(o->y_pdhg, D.y_pdhg, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice,
+                       c->s) != hipSuccess)
+      return -1;
+    h->abs_d = hd.abs_d;
+    h->rel_d = hd.rel_d;
+    h->dobj = hd.dobj;
+  }
+  *piters += hd.total - 1;
+  h->gap = fabs(h->pobj - h->dobj);
+  h->rel_gap = h->gap / (1.0 + fabs(h->pobj) + fabs(h->dobj));
+  if (params->verbose) printf("---------------------------------------------------------------------------------------\n");
+  dpdlp_print_polish_final(hp.reason, hp.total - 1, hp.t_cum, hd.reason, hd.total - 1, hd.t_cum,
+                           hp.rel_p, hd.rel_d,
+                           fabs(hp.pobj - hd.dobj) / (1.0 + fabs(hp.pobj) + fabs(hd.dobj)));
+  *ptime = dpdlp_wall() - t0;
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solve                                                                     */
+/* ------------------------------------------------------------------------ */
+
+cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameters_t *params,
+                                       dpdlp_solve_opts_t *opts) {
+  double t_begin = dpdlp_wall();
+  ctx_t C;
+  memset(&C, 0, sizeof(C));
+  ctx_t *c = &C;
+  cupdlpx_result_t *res = NULL;
+  double *h_q = NULL, *d_start = NULL;
+  dpdlp_solve_opts_t local_opts;
+  if (!opts) {
+    dpdlp_default_opts(&local_opts);
+    opts = &local_opts;
+  }
+  opts->time_to_1e4_sec = -1.0;
+  opts->time_to_1e6_sec = -1.0;
+  opts->trace_recorded = 0;
+  opts->iterations_fp32 = 0;
+  if (!R || !params) {
+    dpdlp_set_error("NULL problem or parameters");
+    return NULL;
+  }
+  if (params->termination_evaluation_frequency <= 0) {
+    dpdlp_set_error("termination_evaluation_frequency must be positive");
+    return NULL;
+  }
+  int m = R->m, n = R->n, nnz = R->nnz;
+  dpdlp_state_t *st = &c->st;
+  hstate_t H;
+  memset(&H, 0, sizeof(H));
+  hstate_t *h = &H;
+  c->mixed = (opts->precision == DPDLP_PREC_MIXED);
+
+  HIPCHK(hipStreamCreate(&c->s));
+  if (opts->profile_phases) {
+    for (int k = 0; k < 64; ++k) {
+      for (int q = 0; q < 3; ++q) HIPCHK(hipEventCreate(&c->ev[k][q]));
+      c->nev_created = k + 1;
+    }
+  }
+  HIPCHK(hipHostMalloc((void **)&c->h_scal, DPDLP_SCAL_LEN * sizeof(double), 0));
+
+  /* ---- state (pdlp_core_op.cu:42-391) ---- */
+  st->m = m;
+  st->n = n;
+  c->w_A_v = dcopy(c, R->A_v, nnz);
+  c->w_At_v = dcopy(c, R->At_v, nnz);
+  st->A = (dpdlp_csr_t){m, n, nnz, R->A_rp, R->A_ci, c->w_A_v, NULL, R->A_blk, R->A_nblk, R->cap};
+  st->At =
+      (dpdlp_csr_t){n, m, nnz, R->At_rp, R->At_ci, c->w_At_v, NULL, R->At_blk, R->At_nblk, R->cap};
+  st->c = dcopy(c, R->c, n);
+  st->lv = dcopy(c, R->lv, n);
+  st->uv = dcopy(c, R->uv, n);
+  st->lc = dcopy(c, R->lc, m);
+  st->uc = dcopy(c, R->uc, m);
+  st->D_r = (double *)dmalloc(c, (size_t)(m ? m : 1) * sizeof(double));
+  st->D_c = (double *)dmalloc(c, (size_t)(n ? n : 1) * sizeof(double));
+  st->x_init = dzero(c, n);
+  st->x_cur = dzero(c, n);
+  st->x_pdhg = dzero(c, n);
+  st->x_ref = dzero(c, n);
+  st->dual_product = dzero(c, n);
+  st->dual_slack = dzero(c, n);
+  st->dx = dzero(c, n);
+  st->y_init = dzero(c, m);
+  st->y_cur = dzero(c, m);
+  st->y_pdhg = dzero(c, m);
+  st->y_ref = dzero(c, m);
+  st->primal_product = dzero(c, m);
+  st->primal_slack = dzero(c, m);
+  st->dy = dzero(c, m);
+  st->part_stride = R->A_nblk;
+  if (R->At_nblk > st->part_stride) st->part_stride = R->At_nblk;
+  if (st->part_stride < 2048) st->part_stride = 2048;
+  st->part = dzero(c, (long long)DPDLP_SLOTS * st->part_stride);
+  st->scal = dzero(c, DPDLP_SCAL_LEN);
+  double *rs = dzero(c, m), *cs = dzero(c, n);
+  double *pi_eig = dzero(c, m), *pi_next = dzero(c, m), *pi_dual = dzero(c, n);
+  if (!c->w_A_v || !c->w_At_v || !st->c || !st->lv || !st->uv || !st->lc || !st->uc ||
+      !st->D_r || !st->D_c || !st->x_init || !st->x_cur || !st->x_pdhg || !st->x_ref ||
+      !st->dual_product || !st->dual_slack || !st->dx || !st->y_init || !st->y_cur ||
+      !st->y_pdhg || !st->y_ref || !st->primal_product || !st->primal_slack || !st->dy ||
+      !st->part || !st->scal || !rs || !cs || !pi_eig || !pi_next || !pi_dual) {
+    dpdlp_set_error("device allocation failed");
+    goto fail;
+  }
+  c->c0 = R->c0;
+  c->obj_norm = params->optimality_norm == NORM_TYPE_L_INF ? R->obj_linf : R->obj_l2;
+  c->bound_norm = params->optimality_norm == NORM_TYPE_L_INF ? R->bnd_linf : R->bnd_l2;
+
+  /* ---- scaling on device (preconditioner.c:197-273) ---- */
+  double t_sc0 = dpdlp_wall();
+  HIPCHK(dpdlp_dev_fill(st->D_r, 1.0, m, c->s));
+  HIPCHK(dpdlp_dev_fill(st->D_c, 1.0, n, c->s));
+  for (int it = 0; it < params->l_inf_ruiz_iterations; ++it)
+    HIPCHK(dpdlp_dev_ruiz_step(st, rs, cs, c->s));
+  if (params->has_pock_chambolle_alpha)
+    HIPCHK(dpdlp_dev_pock_chambolle(st, params->pock_chambolle_alpha, rs, cs, c->s));
+  if (params->bound_objective_rescaling) {
+    HIPCHK(dpdlp_dev_bound_obj_sums(st, c->s));
+    if (sync_scal(c, 64)) {
+      dpdlp_set_error("device sync failed");
+      goto fail;
+    }
+    c->beta = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL]) + 1.0);
+    c->gamma = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL + 1]) + 1.0);
+    HIPCHK(dpdlp_dev_bound_obj_apply(st, c->beta, c->gamma, c->s));
+  } else {
+    c->beta = 1.0;
+    c->gamma = 1.0;
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  double rescale_time = dpdlp_wall() - t_sc0;
+
+  /* warm start, pdlp_core_op.cu:174-201 */
+  if (R->h_pstart && n > 0) {
+    d_start = NULL;
+    HIPCHK(hipMalloc((void **)&d_start, (size_t)n * sizeof(double)));
+    HIPCHK(hipMemcpy(d_start, R->h_pstart, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(dpdlp_dev_warm_start(st->x_init, st->x_cur, st->x_pdhg, d_start, st->D_c, c->beta, n,
+                                c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    hipFree(d_start);
+    d_start = NULL;
+  }
+  if (R->h_dstart && m > 0) {
+    HIPCHK(hipMalloc((void **)&d_start, (size_t)m * sizeof(double)));
+    HIPCHK(hipMemcpy(d_start, R->h_dstart, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(dpdlp_dev_warm_start(st->y_init, st->y_cur, st->y_pdhg, d_start, st->D_r, c->gamma, m,
+                                c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    hipFree(d_start);
+    d_start = NULL;
+  }
+
+  /* mixed precision: FP32 values and gather shadows */
+  if (c->mixed) {
+    c->w_A_v32 = (float *)dmalloc(c, (size_t)(nnz ? nnz : 1) * sizeof(float));
+    c->w_At_v32 = (float *)dmalloc(c, (size_t)(nnz ? nnz : 1) * sizeof(float));
+    st->x_ref32 = (float *)dmalloc(c, (size_t)(n ? n : 1) * sizeof(float));
+    st->y_cur32 = (float *)dmalloc(c, (size_t)(m ? m : 1) * sizeof(float));
+    if (!c->w_A_v32 || !c->w_At_v32 || !st->x_ref32 || !st->y_cur32) {
+      dpdlp_set_error("device allocation failed (FP32 copies)");
+      goto fail;
+    }
+    HIPCHK(dpdlp_dev_make_f32(c->w_A_v, c->w_A_v32, nnz, c->s));
+    HIPCHK(dpdlp_dev_make_f32(c->w_At_v, c->w_At_v32, nnz, c->s));
+    HIPCHK(dpdlp_dev_make_f32(st->y_cur, st->y_cur32, m, c->s));
+    st->A.v32 = c->w_A_v32;
+    st->At.v32 = c->w_At_v32;
+    c->use32 = 1;
+  }
+
+  /* ---- step size: power iteration (pdlp_core_op.cu:638-656) ---- */
+  double t_pi0 = dpdlp_wall();
+  int pi_iters = 0;
+  if (nnz == 0) {
+    h->step = 1.0;
+  } else {
+    h_q = (double *)malloc((size_t)m * sizeof(double));
+    if (!h_q) goto fail;
+    dpdlp_normal_fill(1u, m, h_q);
+    HIPCHK(hipMemcpy(pi_eig, h_q, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
+    double sigma_sq = 1.0;
+    int done = 0, batch = 8;
+    if (params->sv_max_iter > 0) HIPCHK(dpdlp_dev_power_init(st, pi_eig, pi_next, c->s));
+    while (!done && pi_iters < params->sv_max_iter) {
+      int k = params->sv_max_iter - pi_iters;
+      if (k > batch) k = batch;
+      HIPCHK(dpdlp_dev_power_steps(st, pi_eig, pi_next, pi_dual, k, c->s));
+      if (sync_scal(c, DPDLP_SC_PI_HIST + 4 * k)) goto fail;
+      for (int i = 0; i < k; ++i) {
+        const double *hh = c->h_scal + DPDLP_SC_PI_HIST + 4 * i;
+        sigma_sq = hh[0];
+        pi_iters++;
+        if (blue(hh + 1) < params->sv_tol) {
+          done = 1;
+          break;
+        }
+      }
+      batch = batch * 2 > DPDLP_PI_BATCH ? DPDLP_PI_BATCH : batch * 2;
+    }
+    h->step = 0.998 / sqrt(sigma_sq);
+  }
+  h->omega = params->bound_objective_rescaling ? 1.0 : (c->obj_norm + 1.0) / (c->bound_norm + 1.0);
+  h->best_omega = h->omega;
+  h->best_gap = INFINITY;
+  h->fpe_last = INFINITY;
+  h->reason = TERMINATION_REASON_UNSPECIFIED;
+  double power_time = dpdlp_wall() - t_pi0;
+
+  /* ---- main loop (solver.cu:56-102) ---- */
+  if (params->verbose) dpdlp_print_table_header();
+  double t0 = dpdlp_wall();
+  int do_restart = 0, freq = params->termination_evaluation_frequency;
+  const termination_criteria_t *tc = &params->termination_criteria;
+  int trace_k = 0;
+  while (h->total < tc->iteration_limit) {
+    if (opts->trace_points > 0 && trace_k < opts->trace_points &&
+        opts->trace_iters[trace_k] == h->total) {
+      if (n > 0)
+        HIPCHK(hipMemcpyAsync(opts->trace_x + (size_t)trace_k * n, st->x_cur,
+                              (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      if (m > 0)
+        HIPCHK(hipMemcpyAsync(opts->trace_y + (size_t)trace_k * m, st->y_cur,
+                              (size_t)m * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      trace_k++;
+      opts->trace_recorded = trace_k;
+    }
+    int restart_check = h->is_major || h->total == 0;
+    if (restart_check || h->total % dpdlp_print_frequency(h->total) == 0) {
+      HIPCHK(dpdlp_dev_residual(st, NULL, 3, restart_check, c->s));
+      int inf = h->is_major && h->total < 3 * freq;
+      if (inf) HIPCHK(dpdlp_dev_infeasibility(st, c->s));
+      if (sync_scal(c, 64)) {
+        dpdlp_set_error("device sync failed");
+        goto fail;
+      }
+      process_pending_fpe(c, h, 0);
+      host_residual(c, h, params->optimality_norm);
+      if (inf) host_infeasibility(c, h);
+      h->t_cum = dpdlp_wall() - t0;
+      double kkt = fmax(h->rel_p, fmax(h->rel_d, h->rel_gap));
+      if (opts->time_to_1e4_sec < 0 && kkt < 1e-4) opts->time_to_1e4_sec = h->t_cum;
+      if (opts->time_to_1e6_sec < 0 && kkt < 1e-6) opts->time_to_1e6_sec = h->t_cum;
+      if (c->use32 && kkt < opts->mixed_switch_tol) c->use32 = 0;
+      check_termination(h, tc);
+      if (params->verbose)
+        dpdlp_print_iteration(h->total, h->t_cum, h->pobj, h->dobj, h->abs_p, h->abs_d, h->gap,
+                              h->rel_p, h->rel_d, h->rel_gap);
+      if (h->reason != TERMINATION_REASON_UNSPECIFIED) break;
+    }
+    if (restart_check) {
+      do_restart = should_restart(h, &params->restart_params, freq);
+      if (do_restart) {
+        const double *S = c->h_scal;
+        pid_update(h, &params->restart_params, blue(S + DPDLP_SL_RESD + 5),
+                   blue(S + DPDLP_SL_RESP + 4));
+        HIPCHK(dpdlp_dev_restart_copy(st, 3, c->s));
+        h->inner = 0;
+        h->fpe_last = INFINITY;
+      }
+    }
+    h->is_major = ((h->total + 1) % freq) == 0;
+    HIPCHK(pdhg_step(c, st, h, params->reflection_coefficient, do_restart, c->use32, 0,
+                     opts->profile_phases && (h->total & 7) == 0));
+    if (c->use32) opts->iterations_fp32++;
+    do_restart = 0;
+    h->inner++;
+    h->total++;
+  }
+  if (h->reason == TERMINATION_REASON_UNSPECIFIED) {
+    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
+    HIPCHK(dpdlp_dev_residual(st, NULL, 3, 0, c->s));
+    if (sync_scal(c, 64)) goto fail;
+    process_pending_fpe(c, h, 0);
+    host_residual(c, h, params->optimality_norm);
+    if (params->verbose)
+      dpdlp_print_iteration(h->total, h->t_cum, h->pobj, h->dobj, h->abs_p, h->abs_d, h->gap,
+                            h->rel_p, h->rel_d, h->rel_gap);
+  }
+  double loop_time = dpdlp_wall() - t0;
+
+  double polish_time = 0.0;
+  int polish_iters = 0;
+  if (params->feasibility_polishing && h->reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
+      h->reason != TERMINATION_REASON_PRIMAL_INFEASIBLE) {
+    if (feasibility_polish(c, h, params, &polish_time, &polish_iters)) {
+      dpdlp_set_error("feasibility polishing failed on the device");
+      goto fail;
+    }
+  }
+
+  /* ---- result (pdlp_core_op.cu:789-859) ---- */
+  res = (cupdlpx_result_t *)calloc(1, sizeof(*res));
+  if (!res) goto fail;
+  res->primal_solution = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+  res->dual_solution = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
+  res->reduced_cost = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+  if (!res->primal_solution || !res->dual_solution || !res->reduced_cost) goto fail;
+  {
+    double *xo = pi_dual, *rco = st->dual_slack, *yo = pi_next; /* reuse work vectors */
+    if (n > 0 && st->At.nblk == 0) HIPCHK(hipMemsetAsync(rco, 0, (size_t)n * 8, c->s));
+    HIPCHK(dpdlp_dev_result(st, c->beta, c->gamma, xo, yo, rco, c->s));
+    if (n > 0) {
+      HIPCHK(hipMemcpyAsync(res->primal_solution, xo, (size_t)n * sizeof(double),
+                            hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipMemcpyAsync(res->reduced_cost, rco, (size_t)n * sizeof(double),
+                            hipMemcpyDeviceToHost, c->s));
+    }
+    if (m > 0)
+      HIPCHK(hipMemcpyAsync(res->dual_solution, yo, (size_t)m * sizeof(double),
+                            hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  res->num_variables = n;
+  res->num_constraints = m;
+  res->num_nonzeros = nnz;
+  res->total_count = h->total;
+  res->rescaling_time_sec = rescale_time;
+  res->cumulative_time_sec = h->t_cum;
+  res->absolute_primal_residual = h->abs_p;
+  res->relative_primal_residual = h->rel_p;
+  res->absolute_dual_residual = h->abs_d;
+  res->relative_dual_residual = h->rel_d;
+  res->primal_objective_value = h->pobj;
+  res->dual_objective_value = h->dobj;
+  res->objective_gap = h->gap;
+  res->relative_objective_gap = h->rel_gap;
+  res->max_primal_ray_infeasibility = h->max_p_ray_inf;
+  res->max_dual_ray_infeasibility = h->max_d_ray_inf;
+  res->primal_ray_linear_objective = h->p_ray_lin_obj;
+  res->dual_ray_objective = h->d_ray_obj;
+  res->termination_reason = h->reason;
+  res->feasibility_polishing_time = polish_time;
+  res->feasibility_iteration = polish_iters;
+
+  opts->setup_time_sec = rescale_time;
+  opts->power_time_sec = power_time;
+  opts->loop_time_sec = loop_time;
+  opts->power_iterations = pi_iters;
+  opts->step_size = h->step;
+  opts->phase_samples = c->nsamp;
+  opts->primal_phase_ms = c->nsamp ? c->p1_ms / c->nsamp : 0.0;
+  opts->dual_phase_ms = c->nsamp ? c->p2_ms / c->nsamp : 0.0;
+  free(h_q);
+  ctx_free(c);
+  opts->total_time_sec = dpdlp_wall() - t_begin;
+  if (!opts->quiet) dpdlp_print_final(res, params);
+  return res;
+
+fail:
+  free(h_q);
+  if (d_start) hipFree(d_start);
+  cupdlpx_result_free(res);
+  ctx_free(c);
+  return NULL;
+}
+
+cupdlpx_result_t *dpdlp_solve_ex(const lp_problem_t *prob, const pdhg_parameters_t *params,
+                                 dpdlp_solve_opts_t *opts) {
+  if (!prob) return NULL;
+  pdhg_parameters_t local;
+  if (params)
+    local = *params;
+  else
+    set_default_parameters(&local);
+  dpdlp_print_initial_info(&local, prob);
+  double t0 = dpdlp_wall();
+  dpdlp_resident_t *R = dpdlp_upload(prob, opts ? opts->spmv_cap : 0);
+  if (!R) return NULL;
+  cupdlpx_result_t *r = dpdlp_solve_resident(R, &local, opts);
+  dpdlp_resident_free(R);
+  if (r && opts) opts->total_time_sec = dpdlp_wall() - t0;
+  return r;
+}

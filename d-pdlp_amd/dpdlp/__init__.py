@@ -1,0 +1,349 @@
+"""Python (ctypes) binding of the MI355X PDLP library.
+
+Mirrors the reference's C API (include/cupdlpx.h: create_lp_problem,
+set_start_values, solve_lp_problem, set_default_parameters,
+cupdlpx_result_free, lp_problem_free; include/mps_parser.h: read_mps_file)
+over ``lib/libcupdlpx-dist.so`` plus the extension entry points of
+include/dpdlp_ext.h.  The binding only moves arguments; every computation runs
+in the native library on the GPU.  Importing fails loudly when the library has
+not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libcupdlpx-dist.so")
+CLI_PATH = os.path.join(PKG_ROOT, "lib", "cupdlpx-dist")
+
+from ._types import *  # noqa: F401,F403
+from ._types import (_dp, _ip, _Csr, _Dense, CupdlpxResult, GenSpec, LpProblem,
+                     MatrixDesc, PdhgParameters, SolveOpts, TERMINATION, MATRIX_COO,
+                     MATRIX_CSC, MATRIX_CSR, MATRIX_DENSE, PREC_FP64)
+
+# ---------------------------------------------------------------- library
+
+def _load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()' or make -C d-pdlp_amd)")
+    lib = C.CDLL(LIB_PATH)
+    P, R, O = C.POINTER(LpProblem), C.POINTER(CupdlpxResult), C.POINTER(SolveOpts)
+    sig = {
+        "create_lp_problem": (P, [_dp, C.POINTER(MatrixDesc), _dp, _dp, _dp, _dp, _dp]),
+        "set_start_values": (None, [P, _dp, _dp]),
+        "solve_lp_problem": (R, [P, C.POINTER(PdhgParameters)]),
+        "set_default_parameters": (None, [C.POINTER(PdhgParameters)]),
+        "cupdlpx_result_free": (None, [R]),
+        "lp_problem_free": (None, [P]),
+        "read_mps_file": (P, [C.c_char_p]),
+        "dpdlp_default_opts": (None, [O]),
+        "dpdlp_solve_ex": (R, [P, C.POINTER(PdhgParameters), O]),
+        "dpdlp_upload": (C.c_void_p, [P, C.c_int]),
+        "dpdlp_solve_resident": (R, [C.c_void_p, C.POINTER(PdhgParameters), O]),
+        "dpdlp_resident_free": (None, [C.c_void_p]),
+        "dpdlp_resident_info": (None, [C.c_void_p, _ip, _ip, _ip, _ip, _ip]),
+        "dpdlp_last_error": (C.c_char_p, []),
+        "dpdlp_time_spmv": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+        "dpdlp_generate": (P, [C.POINTER(GenSpec), _dp]),
+        "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),
+        "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a, ct=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_params(**kw) -> PdhgParameters:
+    """set_default_parameters() plus keyword overrides (nested names use '__',
+    e.g. termination_criteria__iteration_limit=100)."""
+    p = PdhgParameters()
+    lib.set_default_parameters(C.byref(p))
+    for k, v in kw.items():
+        obj, name = p, k
+        if "__" in k:
+            outer, name = k.split("__", 1)
+            obj = getattr(p, outer)
+        if not hasattr(obj, name):
+            raise AttributeError(k)
+        setattr(obj, name, v)
+    return p
+
+
+class Problem:
+    """Owns an lp_problem_t* from create_lp_problem / read_mps_file / dpdlp_generate."""
+
+    def __init__(self, ptr):
+        if not ptr:
+            raise ValueError("problem construction failed: " + last_error())
+        self.ptr = ptr
+
+    @classmethod
+    def from_csr(cls, m, n, row_ptr, col_ind, vals, c=None, con_lb=None, con_ub=None,
+                 var_lb=None, var_ub=None, obj_const=None):
+        rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        ci = np.ascontiguousarray(col_ind, dtype=np.int32)
+        v = _f64(vals)
+        d = MatrixDesc(m=m, n=n, fmt=MATRIX_CSR, zero_tolerance=0.0)
+        d.data.csr = _Csr(nnz=len(v), row_ptr=_ptr(rp, C.c_int), col_ind=_ptr(ci, C.c_int),
+                          vals=_ptr(v))
+        return cls._make(d, c, con_lb, con_ub, var_lb, var_ub, obj_const, keep=(rp, ci, v))
+
+    @classmethod
+    def from_dense(cls, A, c=None, con_lb=None, con_ub=None, var_lb=None, var_ub=None,
+                   obj_const=None, zero_tolerance=0.0):
+        A = _f64(np.atleast_2d(A))
+        m, n = A.shape
+        d = MatrixDesc(m=m, n=n, fmt=MATRIX_DENSE, zero_tolerance=zero_tolerance)
+        d.data.dense = _Dense(A=_ptr(A))
+        return cls._make(d, c, con_lb, con_ub, var_lb, var_ub, obj_const, keep=(A,))
+
+    @classmethod
+    def from_coo(cls, m, n, rows, cols, vals, fmt=MATRIX_COO, zero_tolerance=0.0, **kw):
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        cc = np.ascontiguousarray(cols, dtype=np.int32)
+        v = _f64(vals)
+        d = MatrixDesc(m=m, n=n, fmt=fmt, zero_tolerance=zero_tolerance)
+        d.data.coo = _Csr(nnz=len(v), row_ptr=_ptr(r, C.c_int), col_ind=_ptr(cc, C.c_int),
+                          vals=_ptr(v))
+        return cls._make(d, kw.get("c"), kw.get("con_lb"), kw.get("con_ub"), kw.get("var_lb"),
+                         kw.get("var_ub"), kw.get("obj_const"), keep=(r, cc, v))
+
+    @classmethod
+    def from_csc(cls, m, n, col_ptr, row_ind, vals, zero_tolerance=0.0, **kw):
+        cp = np.ascontiguousarray(col_ptr, dtype=np.int32)
+        ri = np.ascontiguousarray(row_ind, dtype=np.int32)
+        v = _f64(vals)
+        d = MatrixDesc(m=m, n=n, fmt=MATRIX_CSC, zero_tolerance=zero_tolerance)
+        d.data.csc = _Csr(nnz=len(v), row_ptr=_ptr(cp, C.c_int), col_ind=_ptr(ri, C.c_int),
+                          vals=_ptr(v))
+        return cls._make(d, kw.get("c"), kw.get("con_lb"), kw.get("con_ub"), kw.get("var_lb"),
+                         kw.get("var_ub"), kw.get("obj_const"), keep=(cp, ri, v))
+
+    @classmethod
+    def _make(cls, d, c, con_lb, con_ub, var_lb, var_ub, obj_const, keep):
+        c, con_lb, con_ub, var_lb, var_ub = map(_f64, (c, con_lb, con_ub, var_lb, var_ub))
+        oc = None if obj_const is None else C.byref(C.c_double(obj_const))
+        ptr = lib.create_lp_problem(_ptr(c), C.byref(d), _ptr(con_lb), _ptr(con_ub),
+                                    _ptr(var_lb), _ptr(var_ub), oc)
+        return cls(ptr)
+
+    @classmethod
+    def from_mps(cls, path):
+        return cls(lib.read_mps_file(os.fsencode(path)))
+
+    @property
+    def m(self):
+        return self.ptr.contents.num_constraints
+
+    @property
+    def n(self):
+        return self.ptr.contents.num_variables
+
+    @property
+    def nnz(self):
+        return self.ptr.contents.constraint_matrix_num_nonzeros
+
+    def arrays(self):
+        """Copies of the problem arrays (CSR, bounds, objective)."""
+        p = self.ptr.contents
+        m, n, nnz = p.num_constraints, p.num_variables, p.constraint_matrix_num_nonzeros
+
+        def g(ptr, k, dt=np.float64):
+            if k == 0 or not ptr:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(ptr, shape=(k,)).astype(dt, copy=True)
+
+        return dict(m=m, n=n, row_ptr=g(p.constraint_matrix_row_pointers, m + 1, np.int32),
+                    col_ind=g(p.constraint_matrix_col_indices, nnz, np.int32),
+                    vals=g(p.constraint_matrix_values, nnz), c=g(p.objective_vector, n),
+                    var_lb=g(p.variable_lower_bound, n), var_ub=g(p.variable_upper_bound, n),
+                    con_lb=g(p.constraint_lower_bound, m), con_ub=g(p.constraint_upper_bound, m),
+                    obj_const=p.objective_constant)
+
+    def set_start(self, primal=None, dual=None):
+        primal, dual = _f64(primal), _f64(dual)
+        lib.set_start_values(self.ptr, _ptr(primal), _ptr(dual))
+
+    def free(self):
+        if self.ptr:
+            lib.lp_problem_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclass
+class Result:
+    status: str
+    x: np.ndarray
+    y: np.ndarray
+    reduced_cost: np.ndarray
+    iterations: int
+    primal_obj: float
+    dual_obj: float
+    rel_primal: float
+    rel_dual: float
+    rel_gap: float
+    abs_primal: float
+    abs_dual: float
+    gap: float
+    solve_time: float
+    raw: dict
+
+
+def result_from_ptr(rp, free_fn=None) -> Result:
+    if not rp:
+        raise RuntimeError("solve failed: " + last_error())
+    r = rp.contents
+    n, m = r.num_variables, r.num_constraints
+
+    def g(ptr, k):
+        return np.ctypeslib.as_array(ptr, shape=(k,)).copy() if k > 0 else np.zeros(0)
+
+    raw = {name: getattr(r, name) for name, t in CupdlpxResult._fields_ if t not in (_dp,)}
+    out = Result(status=TERMINATION.get(r.termination_reason, "UNKNOWN"),
+                 x=g(r.primal_solution, n), y=g(r.dual_solution, m),
+                 reduced_cost=g(r.reduced_cost, n), iterations=r.total_count,
+                 primal_obj=r.primal_objective_value, dual_obj=r.dual_objective_value,
+                 rel_primal=r.relative_primal_residual, rel_dual=r.relative_dual_residual,
+                 rel_gap=r.relative_objective_gap, abs_primal=r.absolute_primal_residual,
+                 abs_dual=r.absolute_dual_residual, gap=r.objective_gap,
+                 solve_time=r.cumulative_time_sec, raw=raw)
+    (free_fn or lib.cupdlpx_result_free)(rp)
+    return out
+
+
+def make_opts(precision=PREC_FP64, quiet=True, spmv_cap=0, profile_phases=0,
+              trace_iters=None, n=0, m=0, mixed_switch_tol=None):
+    o = SolveOpts()
+    lib.dpdlp_default_opts(C.byref(o))
+    o.precision = precision
+    o.quiet = int(quiet)
+    o.spmv_cap = spmv_cap
+    o.profile_phases = profile_phases
+    if mixed_switch_tol is not None:
+        o.mixed_switch_tol = mixed_switch_tol
+    keep = []
+    if trace_iters is not None:
+        it = np.ascontiguousarray(trace_iters, dtype=np.int32)
+        tx = np.zeros((len(it), max(n, 1)))
+        ty = np.zeros((len(it), max(m, 1)))
+        o.trace_points = len(it)
+        o.trace_iters = _ptr(it, C.c_int)
+        o.trace_x = _ptr(tx)
+        o.trace_y = _ptr(ty)
+        keep = [it, tx, ty]
+    return o, keep
+
+
+def solve(problem: Problem, params: PdhgParameters | None = None, **opt_kw):
+    """solve_lp_problem with options; returns (Result, SolveOpts, trace arrays)."""
+    o, keep = make_opts(n=problem.n, m=problem.m, **opt_kw)
+    pp = None if params is None else C.byref(params)
+    rp = lib.dpdlp_solve_ex(problem.ptr, pp, C.byref(o))
+    res = result_from_ptr(rp)
+    trace = None
+    if keep:
+        trace = (keep[1][: o.trace_recorded, : problem.n], keep[2][: o.trace_recorded, : problem.m])
+    return res, o, trace
+
+
+def solve_plain(problem: Problem, params: PdhgParameters | None = None) -> Result:
+    """Exactly solve_lp_problem() of the C API."""
+    pp = None if params is None else C.byref(params)
+    return result_from_ptr(lib.solve_lp_problem(problem.ptr, pp))
+
+
+class Resident:
+    """A problem uploaded to HBM once (dpdlp_upload) and solved many times."""
+
+    def __init__(self, problem: Problem, spmv_cap=0):
+        self.h = lib.dpdlp_upload(problem.ptr, spmv_cap)
+        if not self.h:
+            raise RuntimeError("upload failed: " + last_error())
+        m, n, nnz, ba, bt = (C.c_int() for _ in range(5))
+        lib.dpdlp_resident_info(self.h, C.byref(m), C.byref(n), C.byref(nnz), C.byref(ba),
+                                C.byref(bt))
+        self.m, self.n, self.nnz, self.nblk_A, self.nblk_At = (
+            m.value, n.value, nnz.value, ba.value, bt.value)
+
+    def solve(self, params=None, **opt_kw):
+        o, _ = make_opts(n=self.n, m=self.m, **opt_kw)
+        pp = None if params is None else C.byref(params)
+        rp = lib.dpdlp_solve_resident(self.h, pp or C.byref(default_params()), C.byref(o))
+        return result_from_ptr(rp), o
+
+    def spmv(self, x, transpose=False, use32=False):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n if transpose else self.m)
+        rc = lib.dpdlp_resident_spmv(self.h, int(transpose), _ptr(x), _ptr(y), int(use32))
+        if rc != 0:
+            raise RuntimeError("spmv failed: " + last_error())
+        return y
+
+    def time_spmv(self, transpose=False, reps=20, use32=False):
+        return lib.dpdlp_time_spmv(self.h, int(transpose), reps, int(use32))
+
+    def free(self):
+        if self.h:
+            lib.dpdlp_resident_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def generate(m, n, nnz=0, row_rule=0, seed=0, zipf_s=1.5, zipf_max=50000,
+             poisson_lambda=8.0, pm1_frac=0.0):
+    """dpdlp_generate: (Problem, known optimal objective)."""
+    spec = GenSpec(m=m, n=n, nnz=nnz, row_rule=row_rule, zipf_s=zipf_s, zipf_max=zipf_max,
+                   poisson_lambda=poisson_lambda, pm1_frac=pm1_frac, seed=seed)
+    opt = C.c_double(0.0)
+    ptr = lib.dpdlp_generate(C.byref(spec), C.byref(opt))
+    return Problem(ptr), opt.value
+
+
+def last_error() -> str:
+    s = lib.dpdlp_last_error()
+    return s.decode() if s else ""
+
+
+def plan_blocks(row_ptr, cap):
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    rows = len(rp) - 1
+    out = np.zeros(rows + 2, dtype=np.int32)
+    nb = lib.dpdlp_plan_blocks(rows, _ptr(rp, C.c_int), cap, _ptr(out, C.c_int))
+    return out[: nb + 1]
+
+
+EXPORTED_SYMBOLS = [
+    "create_lp_problem", "set_start_values", "solve_lp_problem", "set_default_parameters",
+    "cupdlpx_result_free", "lp_problem_free", "read_mps_file",
+]

@@ -1,0 +1,132 @@
+/*
+ * Thin C-ABI between the host solver (C) and the hand-written gfx950 kernels.
+ *
+ * Every function takes device pointers held in plain C structs, sizes, and a
+ * HIP stream passed as void*; none allocates; each returns 0 on success or a
+ * hipError_t value.  Reductions write raw partial results into device scalar
+ * slots (dpdlp_state_t.scal) so that a whole evaluation needs one
+ * device-to-host copy.  The comment on each entry point names the reference
+ * call sites (cuSPARSE / cuBLAS / CUDA kernels) it replaces.
+ */
+#pragma once
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CSR matrix resident in HBM, with its row-block plan for the stream SpMV. */
+typedef struct {
+  int rows, cols, nnz;
+  const int *rp;     /* [rows+1] */
+  const int *ci;     /* [nnz] */
+  const double *v;   /* [nnz] FP64 values */
+  const float *v32;  /* [nnz] FP32 copy or NULL */
+  const int *blk;    /* [nblk+1] first row of each row block */
+  int nblk;
+  int cap;           /* nonzeros a multi-row block may hold (1024/2048/4096) */
+} dpdlp_csr_t;
+
+/* One LP's device state (scaled space).  Field meaning follows the
+ * reference's pdhg_solver_state_t (internal/internal_types.h:50-145). */
+typedef struct {
+  int m, n;
+  dpdlp_csr_t A, At;
+  double *lv, *uv, *c, *lc, *uc; /* scaled bounds and objective */
+  double *D_r, *D_c;             /* cumulative row / column scaling */
+  double *x_init, *x_cur, *x_pdhg, *x_ref, *dual_product, *dual_slack, *dx;
+  double *y_init, *y_cur, *y_pdhg, *y_ref, *primal_product, *primal_slack, *dy;
+  float *x_ref32, *y_cur32; /* FP32 gather shadows (mixed precision) or NULL */
+  double *part;   /* partial sums: DPDLP_SLOTS x part_stride */
+  int part_stride;
+  double *scal;   /* DPDLP_SCAL_LEN doubles of raw reduction results */
+} dpdlp_state_t;
+
+/* Scalar / partial slot map (raw values; the host combines them). */
+enum {
+  DPDLP_SL_RESP = 0,   /* rows: rp blue[3], psum, ydist blue[3], rp max  (8) */
+  DPDLP_SL_RESD = 8,   /* cols: rd blue[3], c.x, z.x, xdist blue[3], rd max (9) */
+  DPDLP_SL_FPE = 17,   /* dx blue[3], dy blue[3], <A'dy,dx>               (7) */
+  DPDLP_SL_INF = 24,   /* max|dx|, max|dy|, psum, pinf, dsum, c.dx, dinf   (7) */
+  DPDLP_SL_PI = 31,    /* <q,AA'q>, |AA'q| blue[3], residual blue[3]       (7) */
+  DPDLP_SL_SCAL = 38,  /* scaling: bound blue[3], obj blue[3]              (6) */
+  DPDLP_SLOTS = 44,
+  DPDLP_SC_FPE_RESTART = 48, /* copy of the 7 FPE slots from a restart step */
+  DPDLP_SC_FPE_MAJOR = 56,   /* copy of the 7 FPE slots from a major step */
+  DPDLP_SC_PI_HIST = 64,     /* power iteration history, 4 per iteration */
+  DPDLP_PI_BATCH = 256,
+  DPDLP_SCAL_LEN = 64 + 4 * 256
+};
+
+/* Row-block plan on the host: rows with more than `cap` nonzeros get a block
+ * of their own, other blocks hold <= 256 rows and <= cap nonzeros.  Writes
+ * blk[0..nblk] and returns nblk. */
+int dpdlp_plan_blocks(int rows, const int *rp_host, int cap, int *blk_out);
+
+/* y = A x (FP64), replaces cusparseSpMV CSR_ALG2 at pdlp_core_op.cu:501,534
+ * and utils.cu:134,138,645,650,750,755. */
+int dpdlp_dev_spmv(const dpdlp_csr_t *A, const double *x, double *y, void *stream);
+/* y = A x with FP32 values and FP32 x, FP64 accumulation. */
+int dpdlp_dev_spmv_f32(const dpdlp_csr_t *A, const float *x, double *y, void *stream);
+
+/* Phase 1 of one PDHG step: A'y_cur, primal step with projection and
+ * reflection, Halpern update of x_cur.  Replaces the SpMV at
+ * pdlp_core_op.cu:501 + kernels :393-418 + the x half of :445-462.
+ * major: also write x_pdhg and dual_slack; fpe: also write dx = x_ref - x_cur
+ * and its Blue partials.  use32: gather the FP32 shadow of y with FP32 values. */
+int dpdlp_dev_primal_phase(const dpdlp_state_t *st, double tau, double w,
+                           double rho, int major, int fpe, int use32,
+                           void *stream);
+/* Phase 2: A x_ref, dual step, Halpern update of y_cur.  Replaces
+ * pdlp_core_op.cu:534 + kernels :420-443 + the y half of :445-462. */
+int dpdlp_dev_dual_phase(const dpdlp_state_t *st, double sigma, double w,
+                         double rho, int major, int fpe, int use32,
+                         void *stream);
+/* Finishes the fixed-point error of pdlp_core_op.cu:658-696: A'dy into
+ * dual_product with <A'dy, dx> partials, then reduces the 7 FPE slots and
+ * copies them to DPDLP_SC_FPE_RESTART and/or DPDLP_SC_FPE_MAJOR. */
+int dpdlp_dev_fpe_finish(const dpdlp_state_t *st, int to_restart, int to_major,
+                         void *stream);
+/* Primal-only / dual-only FPE finish for feasibility polishing
+ * (pdlp_core_op.cu:1332-1353): reduces the dx (primal) or dy slots. */
+int dpdlp_dev_fpe_finish_polish(const dpdlp_state_t *st, int primal,
+                                int to_restart, int to_major, void *stream);
+/* KKT residual of utils.cu:639-719 (2 SpMV + residual kernel + norms/dots)
+ * into DPDLP_SL_RESP / DPDLP_SL_RESD; restart_dist adds the restart distances
+ * of pdlp_core_op.cu:579-591.  which: 3 = both, 1 = primal rows only
+ * (utils.cu:1145-1183), 2 = dual columns only (utils.cu:1185-1229).  For
+ * which==2, lc_fin/uc_fin/x_obj come from the original state `orig`. */
+int dpdlp_dev_residual(const dpdlp_state_t *st, const dpdlp_state_t *orig,
+                       int which, int restart_dist, void *stream);
+/* Infeasibility rays of utils.cu:721-813 into DPDLP_SL_INF. */
+int dpdlp_dev_infeasibility(const dpdlp_state_t *st, void *stream);
+/* Restart copies of pdlp_core_op.cu:621-632 (part: 3 both, 1 x, 2 y). */
+int dpdlp_dev_restart_copy(const dpdlp_state_t *st, int part, void *stream);
+/* Power iteration (utils.cu:66-171), `iters` steps starting at history index
+ * 0; q0 must already be in `eig` (m).  Workspace: next (m), dual (n). */
+int dpdlp_dev_power_init(const dpdlp_state_t *st, double *eig, double *next,
+                         void *stream);
+int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next,
+                          double *dual, int iters, void *stream);
+/* Result of pdlp_core_op.cu:789-859: reduced cost into rc, unscaled x, y. */
+int dpdlp_dev_result(const dpdlp_state_t *st, double beta, double gamma,
+                     double *x_out, double *y_out, double *rc_out, void *stream);
+
+/* Scaling on device (preconditioner.c:89-273).  rs/cs are work vectors. */
+int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream);
+int dpdlp_dev_pock_chambolle(dpdlp_state_t *st, double alpha, double *rs,
+                             double *cs, void *stream);
+int dpdlp_dev_bound_obj_sums(const dpdlp_state_t *st, void *stream);
+int dpdlp_dev_bound_obj_apply(dpdlp_state_t *st, double beta, double gamma,
+                              void *stream);
+/* FP32 copies of the scaled values (mixed precision). */
+int dpdlp_dev_make_f32(const double *src, float *dst, long long count, void *stream);
+/* Polishing helpers: v <- 0 where finite (pdlp_core_op.cu:1136-1143). */
+int dpdlp_dev_zero_finite(double *v, int count, void *stream);
+int dpdlp_dev_fill(double *v, double value, long long count, void *stream);
+/* Warm start (pdlp_core_op.cu:174-201): a = b = c = start * D * s. */
+int dpdlp_dev_warm_start(double *a, double *b, double *c, const double *start,
+                         const double *D, double s, int count, void *stream);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,110 @@
+"""GPU parity tests of the core path: SpMV, full solves vs the oracle."""
+import numpy as np
+import pytest
+
+from helpers import capi_example, csr_matvec, random_csr, rel
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("cap", [1024, 2048, 4096])
+@pytest.mark.parametrize("shape", ["uniform", "powerlaw", "empty_rows"])
+def test_spmv_matches_brute_force(dp, cap, shape):
+    rng = np.random.default_rng(5)
+    m, n = 3000, 2000
+    if shape == "uniform":
+        lens = rng.poisson(10, m)
+        extra = ()
+    elif shape == "powerlaw":
+        lens = np.minimum((rng.pareto(1.5, m) * 3).astype(int) + 1, 1500)
+        extra = ((7, 1999), (2500, 1800), (2999, 4096 if cap < 4096 else 1024))
+    else:
+        lens = rng.integers(0, 3, m) * (rng.random(m) < 0.3)
+        extra = ((100, 600),)
+    rp, ci, v = random_csr(m, n, lens, seed=11, long_rows=extra)
+    c = np.zeros(n)
+    prob = dp.Problem.from_csr(m, n, rp, ci, v, c=c)
+    R = dp.Resident(prob, spmv_cap=cap)
+    x = rng.standard_normal(n)
+    y = R.spmv(x)
+    ref = csr_matvec(rp, ci, v, x)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    yv = rng.standard_normal(m)
+    z = R.spmv(yv, transpose=True)
+    # adjointness y'(Ax) = (A'y)'x
+    assert abs(yv @ y - z @ x) <= 1e-10 * (np.abs(yv) @ np.abs(ref) + 1)
+    refz = np.zeros(n)
+    np.add.at(refz, ci, v * np.repeat(yv, np.diff(rp)))
+    assert np.max(np.abs(z - refz)) <= 1e-12 * max(1.0, np.max(np.abs(refz)))
+
+
+def test_capi_example(dp):
+    A, c, l, u = capi_example()
+    prob = dp.Problem.from_dense(A, c=c, con_lb=l, con_ub=u)
+    res = dp.solve_plain(prob, dp.default_params(presolve=False))
+    assert res.status == "OPTIMAL"
+    assert np.allclose(res.x, [1.0, 2.0], atol=1e-3)
+    assert np.allclose(res.y, [1.0, -1.0, 0.0], atol=1e-3)
+    assert abs(res.primal_obj - 3.0) < 1e-3
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_parity_trace_vs_oracle(dp, orc, seed):
+    """Iterates within 1e-8 relative of the oracle over the first 1000
+    iterations, same status, objective within 1e-6 relative."""
+    prob, opt = dp.generate(2000, 5000, 50000, row_rule=0, seed=seed)
+    iters = [0, 1, 10, 50, 100, 200, 201, 400, 600, 800, 1000]
+    params = dp.default_params(presolve=False)
+    res, o, (tx, ty) = dp.solve(prob, params, trace_iters=iters)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(presolve=False), trace_iters=iters,
+                                 n=prob.n, m=prob.m)
+    assert res.status == ref["status"]
+    k = min(len(tx), len(ox))
+    assert k == len(ox)
+    for i in range(k):
+        assert rel(tx[i], ox[i]) < 1e-8, (iters[i], rel(tx[i], ox[i]))
+        assert rel(ty[i], oy[i]) < 1e-8, (iters[i], rel(ty[i], oy[i]))
+    assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
+        1.0, abs(ref["primal_objective_value"]))
+    assert res.rel_primal < 1e-4 and res.rel_dual < 1e-4 and res.rel_gap < 1e-4
+    assert abs(res.primal_obj - opt) <= 1e-3 * max(1.0, abs(opt))
+
+
+CAP = dict(presolve=False, termination_criteria__iteration_limit=5000)
+
+
+def test_primal_infeasible(dp, orc):
+    # x + y >= 2 and x + y <= 1 over the unit box
+    A = np.array([[1.0, 1.0], [1.0, 1.0]])
+    prob = dp.Problem.from_dense(A, c=np.zeros(2), con_lb=np.array([2.0, -np.inf]),
+                                 con_ub=np.array([np.inf, 1.0]), var_lb=np.zeros(2),
+                                 var_ub=np.ones(2))
+    res = dp.solve_plain(prob, dp.default_params(**CAP))
+    ref, _ = orc.solve(prob.ptr, orc.default_params(**CAP))
+    assert res.status == ref["status"] == "PRIMAL_INFEASIBLE"
+    assert res.iterations == ref["total_count"] == 200
+
+
+def test_dual_infeasible(dp, orc):
+    # min -x s.t. x - y >= 0 (x, y >= 0): unbounded along x = y
+    A = np.array([[1.0, -1.0]])
+    prob = dp.Problem.from_dense(A, c=np.array([-1.0, 0.0]), con_lb=np.array([0.0]),
+                                 con_ub=np.array([np.inf]), var_lb=np.zeros(2),
+                                 var_ub=np.full(2, np.inf))
+    res = dp.solve_plain(prob, dp.default_params(**CAP))
+    ref, _ = orc.solve(prob.ptr, orc.default_params(**CAP))
+    assert res.status == ref["status"] == "DUAL_INFEASIBLE"
+    assert abs(res.raw["primal_ray_linear_objective"] - ref["primal_ray_linear_objective"]) < 1e-9
+
+
+def test_iteration_and_time_limits(dp):
+    prob, _ = dp.generate(500, 1200, 8000, row_rule=0, seed=3)
+    r0 = dp.solve_plain(prob, dp.default_params(presolve=False,
+                                                termination_criteria__iteration_limit=0))
+    assert r0.status == "ITERATION_LIMIT" and r0.iterations == 0
+    r1 = dp.solve_plain(prob, dp.default_params(presolve=False,
+                                                termination_criteria__iteration_limit=37))
+    assert r1.status == "ITERATION_LIMIT" and r1.iterations == 37
+    r2 = dp.solve_plain(prob, dp.default_params(presolve=False,
+                                                termination_criteria__time_sec_limit=0.0))
+    assert r2.status == "TIME_LIMIT" and r2.iterations == 0
