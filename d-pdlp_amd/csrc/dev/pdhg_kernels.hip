@@ -22,6 +22,14 @@
 
 #define EW_GRID_MAX 2048
 
+/* grid for one wavefront per row (4 rows per workgroup), capped */
+static inline int wave_grid(long long rows) {
+  long long g = (rows + 3) / 4;
+  if (g < 1) g = 1;
+  if (g > 16384) g = 16384;
+  return (int)g;
+}
+
 static inline int ew_grid(long long n) {
   long long g = (n + DPDLP_BS - 1) / DPDLP_BS;
   if (g < 1) g = 1;
@@ -38,6 +46,90 @@ __device__ __forceinline__ T ntload(const T *p) {
   return __builtin_nontemporal_load(p);
 }
 
+/* One thread's share of a long row [p0, p1): four independent gathers in
+ * flight, four accumulators combined in a fixed order. */
+template <typename VT, typename XT>
+__device__ __forceinline__ double row_sum_wg(const int *__restrict__ ci,
+                                             const VT *__restrict__ val,
+                                             const XT *__restrict__ x, int p0, int p1) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int k = p0 + (int)threadIdx.x;
+  for (; k + 3 * DPDLP_BS < p1; k += 4 * DPDLP_BS) {
+    int c0 = ntload(ci + k), c1 = ntload(ci + k + DPDLP_BS);
+    int c2 = ntload(ci + k + 2 * DPDLP_BS), c3 = ntload(ci + k + 3 * DPDLP_BS);
+    VT v0 = ntload(val + k), v1 = ntload(val + k + DPDLP_BS);
+    VT v2 = ntload(val + k + 2 * DPDLP_BS), v3 = ntload(val + k + 3 * DPDLP_BS);
+    XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+    a0 += (double)v0 * (double)x0;
+    a1 += (double)v1 * (double)x1;
+    a2 += (double)v2 * (double)x2;
+    a3 += (double)v3 * (double)x3;
+  }
+  for (; k < p1; k += DPDLP_BS) a0 += (double)ntload(val + k) * (double)x[ntload(ci + k)];
+  return (a0 + a1) + (a2 + a3);
+}
+
+
+/* Deterministic row sums of the staged products prod[srp[r] .. srp[r+1]) for
+ * the R rows of a block.  Phase A: rows of at most 16 entries per lane are
+ * summed by a group of g lanes (g = 256/R rounded down to a power of two, at
+ * most 64; one lane per row when R > 256).  Phase B: every longer row is
+ * summed by a whole wavefront (lanes strided, then a fixed shuffle tree), long
+ * rows dealt round-robin to the 4 wavefronts.  emit(r, sum) is called once per
+ * row by exactly one lane. */
+template <class Emit>
+__device__ __forceinline__ void rows_reduce(const double *__restrict__ prod,
+                                            const int *__restrict__ srp, int R, Emit emit) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int g = 1;
+  if (R <= DPDLP_BS) {
+    g = 1 << (31 - __clz(DPDLP_BS / R));
+    if (g > DPDLP_WAVE) g = DPDLP_WAVE;
+  }
+  const int lim = g * 16;
+  if (g == 1) {
+    for (int r = tid; r < R; r += DPDLP_BS) {
+      const int a = srp[r], e = srp[r + 1];
+      if (e - a <= lim) {
+        double acc = 0.0;
+        for (int q = a; q < e; ++q) acc += prod[q];
+        emit(r, acc);
+      }
+    }
+  } else {
+    const int grp = tid / g, ln = tid & (g - 1);
+    double acc = 0.0;
+    bool mine = false;
+    if (grp < R) {
+      const int a = srp[grp], e = srp[grp + 1];
+      mine = (e - a) <= lim;
+      if (mine)
+        for (int q = a + ln; q < e; q += g) acc += prod[q];
+    }
+    for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, DPDLP_WAVE);
+    if (mine && ln == 0) emit(grp, acc);
+  }
+  int cnt = 0;
+  for (int base = 0; base < R; base += DPDLP_WAVE) {
+    const int r = base + lane;
+    bool lng = false;
+    if (r < R) lng = (srp[r + 1] - srp[r]) > lim;
+    unsigned long long mask = __ballot(lng);
+    while (mask) {
+      const int j = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      if ((cnt & 3) == wid) {
+        const int rr = base + j, a = srp[rr], e = srp[rr + 1];
+        double acc = 0.0;
+        for (int q = a + lane; q < e; q += DPDLP_WAVE) acc += prod[q];
+        acc = wave_sum(acc);
+        if (lane == 0) emit(rr, acc);
+      }
+      ++cnt;
+    }
+  }
+}
+
 template <typename VT, typename XT, class Epi, int CAP>
 __global__ __launch_bounds__(DPDLP_BS) void k_spmv(
     const int *__restrict__ blk, const int *__restrict__ rp,
@@ -45,7 +137,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_spmv(
     const XT *__restrict__ x, Epi epi, double *__restrict__ part, int stride,
     int slot0) {
   constexpr int NS = Epi::NS, NM = Epi::NM;
-  constexpr int UNR = 4;
+  constexpr int UNR = 8;
   __shared__ double prod[CAP];
   __shared__ int srp[DPDLP_BS + 1];
   __shared__ double ssum[DPDLP_BS];
@@ -65,11 +157,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_spmv(
 
   if (nz > CAP) {
     /* one long row, whole workgroup */
-    double acc = 0.0;
-    for (int k = p0 + tid; k < p1; k += DPDLP_BS) {
-      double prodv = (double)ntload(val + k) * (double)x[ntload(ci + k)];
-      acc += prodv;
-    }
+    double acc = row_sum_wg<VT, XT>(ci, val, x, p0, p1);
     acc = wave_sum(acc);
     if ((tid & 63) == 0) ssum[tid >> 6] = acc;
     __syncthreads();
@@ -103,23 +191,104 @@ __global__ __launch_bounds__(DPDLP_BS) void k_spmv(
       }
     }
     __syncthreads();
-    int q = DPDLP_BS / R;
-    int g = 1 << (31 - __clz(q));
-    if (g > DPDLP_WAVE) g = DPDLP_WAVE;
-    const int grp = tid / g, lane = tid & (g - 1);
-    double acc = 0.0;
-    if (grp < R) {
-      const int e = srp[grp + 1];
-      for (int k = srp[grp] + lane; k < e; k += g) acc += prod[k];
-    }
-    for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, DPDLP_WAVE);
-    if (lane == 0 && grp < R) ssum[grp] = acc;
+    rows_reduce(prod, srp, R, [&](int r, double v) { ssum[r] = v; });
     __syncthreads();
     if (tid < R) epi(r0 + tid, ssum[tid], s, mx);
   }
   if (NS + NM > 0) {
     if (epi.partials) block_partials<NS, NM>(s, mx, red, part, stride, slot0);
   }
+}
+
+/* Column-panel tile SpMV, pass 1: workgroup w = b*P + k sums the entries of
+ * row block b that lie in column panel k and writes one partial per row to
+ * xpart[k*rows + row].  Same staging and row reduction as k_spmv. */
+template <typename VT, typename XT, int CAP>
+__global__ __launch_bounds__(DPDLP_BS) void k_xp_partial(
+    const int *__restrict__ blk, const int *__restrict__ toff,
+    const unsigned short *__restrict__ rofs, const int *__restrict__ pci,
+    const VT *__restrict__ pv, const XT *__restrict__ x, double *__restrict__ xpart,
+    int rows, int P) {
+  constexpr int UNR = 8;
+  __shared__ double prod[CAP];
+  __shared__ int srp[DPDLP_PANEL_RMAX + 1];
+  __shared__ double ssum[DPDLP_BS];
+  const int tid = threadIdx.x;
+  const int w = blockIdx.x, b = w / P, k = w - b * P;
+  const int r0 = blk[b], r1 = blk[b + 1];
+  const int R = r1 - r0;
+  const int t0 = toff[w], t1 = toff[w + 1];
+  const int nz = t1 - t0;
+  double *out = xpart + (size_t)k * rows;
+  if (R == 1) {
+    double acc = row_sum_wg<VT, XT>(pci, pv, x, t0, t1);
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) ssum[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) out[r0] = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
+    return;
+  }
+  const unsigned short *ro = rofs + (size_t)P * (r0 + b) + (size_t)k * (R + 1);
+  for (int t = tid; t <= R; t += DPDLP_BS) srp[t] = ro[t];
+  for (int base = tid; base < nz; base += DPDLP_BS * UNR) {
+    int cidx[UNR];
+    VT vv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      int q = base + u * DPDLP_BS;
+      if (q < nz) {
+        cidx[u] = ntload(pci + t0 + q);
+        vv[u] = ntload(pv + t0 + q);
+      }
+    }
+    XT xx[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      int q = base + u * DPDLP_BS;
+      if (q < nz) xx[u] = x[cidx[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      int q = base + u * DPDLP_BS;
+      if (q < nz) prod[q] = (double)vv[u] * (double)xx[u];
+    }
+  }
+  __syncthreads();
+  rows_reduce(prod, srp, R, [&](int r, double v) { out[r0 + r] = v; });
+}
+
+/* Column-panel tile SpMV, pass 2: row sums in panel order, then the fused
+ * epilogue and its per-workgroup partial reductions. */
+template <class Epi>
+__global__ __launch_bounds__(DPDLP_BS) void k_xp_epi(const double *__restrict__ xpart, int rows,
+                                                    int P, Epi epi, double *__restrict__ part,
+                                                    int stride, int slot0) {
+  constexpr int NS = Epi::NS, NM = Epi::NM;
+  __shared__ double red[(NS + NM > 0 ? NS + NM : 1) * 4];
+  double s[NS > 0 ? NS : 1];
+  double mx[NM > 0 ? NM : 1];
+#pragma unroll
+  for (int q = 0; q < (NS > 0 ? NS : 1); ++q) s[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < (NM > 0 ? NM : 1); ++q) mx[q] = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < rows;
+       i += (long long)gridDim.x * blockDim.x) {
+    double a = xpart[i];
+    for (int k = 1; k < P; ++k) a += xpart[(size_t)k * rows + i];
+    epi((int)i, a, s, mx);
+  }
+  if (NS + NM > 0) {
+    if (epi.partials) block_partials<NS, NM>(s, mx, red, part, stride, slot0);
+  }
+}
+
+/* dst[t] = src[perm[t]] */
+template <typename T>
+__global__ void k_permute(const T *__restrict__ src, const int *__restrict__ perm,
+                          T *__restrict__ dst, long long count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[perm[i]];
 }
 
 /* ---- epilogues ---- */
@@ -314,46 +483,67 @@ struct EpiStoreRow {
   __device__ void operator()(int r, double ax, double *, double *) const { y[r] = ax; }
 };
 
-template <class Epi>
-static hipError_t launch_spmv(const dpdlp_csr_t *A, const double *x, const Epi &epi,
-                              double *part, int stride, int slot0, hipStream_t s) {
-  if (A->nblk <= 0) return hipSuccess;
+/* partial-sum count written by an SpMV with a reducing epilogue */
+static inline int spmv_nparts(const dpdlp_csr_t *A) {
+  return A->P > 1 ? ew_grid(A->rows) : A->nblk;
+}
+
+template <typename VT, typename XT, class Epi>
+static hipError_t launch_spmv_t(const dpdlp_csr_t *A, const int *ci, const VT *v, const VT *pv,
+                                const XT *x, const Epi &epi, double *part, int stride,
+                                int slot0, hipStream_t s) {
+  if (A->rows <= 0) return hipSuccess;
+  if (A->P > 1) {
+    int grid = A->nblk * A->P;
+    switch (A->cap) {
+    case 1024:
+      k_xp_partial<VT, XT, 1024><<<grid, DPDLP_BS, 0, s>>>(A->blk, A->toff, A->rofs, A->pci, pv,
+                                                          x, A->xpart, A->rows, A->P);
+      break;
+    case 4096:
+      k_xp_partial<VT, XT, 4096><<<grid, DPDLP_BS, 0, s>>>(A->blk, A->toff, A->rofs, A->pci, pv,
+                                                          x, A->xpart, A->rows, A->P);
+      break;
+    default:
+      k_xp_partial<VT, XT, 2048><<<grid, DPDLP_BS, 0, s>>>(A->blk, A->toff, A->rofs, A->pci, pv,
+                                                          x, A->xpart, A->rows, A->P);
+      break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    k_xp_epi<Epi><<<ew_grid(A->rows), DPDLP_BS, 0, s>>>(A->xpart, A->rows, A->P, epi, part,
+                                                        stride, slot0);
+    return hipGetLastError();
+  }
   switch (A->cap) {
   case 1024:
-    k_spmv<double, double, Epi, 1024><<<A->nblk, DPDLP_BS, 0, s>>>(
-        A->blk, A->rp, A->ci, A->v, x, epi, part, stride, slot0);
+    k_spmv<VT, XT, Epi, 1024><<<A->nblk, DPDLP_BS, 0, s>>>(A->blk, A->rp, ci, v, x, epi, part,
+                                                          stride, slot0);
     break;
   case 4096:
-    k_spmv<double, double, Epi, 4096><<<A->nblk, DPDLP_BS, 0, s>>>(
-        A->blk, A->rp, A->ci, A->v, x, epi, part, stride, slot0);
+    k_spmv<VT, XT, Epi, 4096><<<A->nblk, DPDLP_BS, 0, s>>>(A->blk, A->rp, ci, v, x, epi, part,
+                                                          stride, slot0);
     break;
   default:
-    k_spmv<double, double, Epi, 2048><<<A->nblk, DPDLP_BS, 0, s>>>(
-        A->blk, A->rp, A->ci, A->v, x, epi, part, stride, slot0);
+    k_spmv<VT, XT, Epi, 2048><<<A->nblk, DPDLP_BS, 0, s>>>(A->blk, A->rp, ci, v, x, epi, part,
+                                                          stride, slot0);
     break;
   }
   return hipGetLastError();
 }
 
 template <class Epi>
+static hipError_t launch_spmv(const dpdlp_csr_t *A, const double *x, const Epi &epi,
+                              double *part, int stride, int slot0, hipStream_t s) {
+  return launch_spmv_t<double, double, Epi>(A, A->ci, A->v, A->pv, x, epi, part, stride, slot0,
+                                            s);
+}
+
+template <class Epi>
 static hipError_t launch_spmv32(const dpdlp_csr_t *A, const float *x, const Epi &epi,
                                 double *part, int stride, int slot0, hipStream_t s) {
-  if (A->nblk <= 0) return hipSuccess;
-  switch (A->cap) {
-  case 1024:
-    k_spmv<float, float, Epi, 1024><<<A->nblk, DPDLP_BS, 0, s>>>(
-        A->blk, A->rp, A->ci, A->v32, x, epi, part, stride, slot0);
-    break;
-  case 4096:
-    k_spmv<float, float, Epi, 4096><<<A->nblk, DPDLP_BS, 0, s>>>(
-        A->blk, A->rp, A->ci, A->v32, x, epi, part, stride, slot0);
-    break;
-  default:
-    k_spmv<float, float, Epi, 2048><<<A->nblk, DPDLP_BS, 0, s>>>(
-        A->blk, A->rp, A->ci, A->v32, x, epi, part, stride, slot0);
-    break;
-  }
-  return hipGetLastError();
+  return launch_spmv_t<float, float, Epi>(A, A->ci, A->v32, A->pv32, x, epi, part, stride,
+                                          slot0, s);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -553,25 +743,32 @@ __global__ void k_unscale(const double *x, const double *y, const double *D_c, c
 
 /* ---- scaling (preconditioner.c) ---- */
 
-/* Per-row reduction of |a|: mode 0 = max, mode 1 = sequential sum of
- * |a|^p in storage order (pow skipped when p == 1, which is exact). */
-__global__ void k_row_reduce(const int *rp, const double *v, int rows, int mode, double p,
-                             double *out) {
-  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
-       r += (long long)gridDim.x * blockDim.x) {
+/* Per-row reduction of |a|, one wavefront per row with coalesced loads:
+ * mode 0 = max (exact in any order); mode 1 = sum of |a|^p accumulated
+ * strictly in storage order by lane 0 (so column sums over the transpose
+ * match a row-major host accumulation bit for bit); pow is skipped for
+ * p == 1, which is exact. */
+__global__ __launch_bounds__(DPDLP_BS) void k_row_reduce(const int *rp, const double *v, int rows,
+                                                        int mode, double p, double *out) {
+  const int lane = threadIdx.x & 63;
+  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwave = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long r = wave0; r < rows; r += nwave) {
+    const int a = rp[r], e = rp[r + 1];
     double acc = 0.0;
-    int e = rp[r + 1];
     if (mode == 0) {
-      for (int k = rp[r]; k < e; ++k) {
-        double a = fabs(v[k]);
-        if (a > acc) acc = a;
-      }
-    } else if (p == 1.0) {
-      for (int k = rp[r]; k < e; ++k) acc += fabs(v[k]);
+      for (int q = a + lane; q < e; q += 64) acc = fmax(acc, fabs(v[q]));
+      acc = wave_max(acc);
     } else {
-      for (int k = rp[r]; k < e; ++k) acc += pow(fabs(v[k]), p);
+      for (int base = a; base < e; base += 64) {
+        const int q = base + lane;
+        double t = 0.0;
+        if (q < e) t = (p == 1.0) ? fabs(v[q]) : pow(fabs(v[q]), p);
+        const int cnt = (e - base) < 64 ? (e - base) : 64;
+        for (int j = 0; j < cnt; ++j) acc += __shfl(t, j, 64);
+      }
     }
-    out[r] = acc;
+    if (lane == 0) out[r] = acc;
   }
 }
 
@@ -586,18 +783,22 @@ __global__ void k_scale_factors(double *f, double *D, int count) {
   }
 }
 
-/* a_k /= (rs[row] * cs[col]) for one CSR matrix; `rowf` scales rows, `colf`
- * the column index space (pass them swapped for the transpose). */
-__global__ void k_scale_values(const int *rp, const int *ci, double *v, int rows,
-                               const double *rowf, const double *colf, int row_first) {
-  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
-       r += (long long)gridDim.x * blockDim.x) {
-    double fr = rowf[r];
-    for (int k = rp[r]; k < rp[r + 1]; ++k) {
-      double fc = colf[ci[k]];
-      /* the product is formed as rs[i] * cs[j] in both matrices */
-      double d = row_first ? fr * fc : fc * fr;
-      v[k] /= d;
+/* a_k /= (rs[row] * cs[col]) for one CSR matrix, one wavefront per row;
+ * `rowf` scales rows, `colf` the column index space (pass them swapped for
+ * the transpose: the product is formed as rs[i] * cs[j] in both). */
+__global__ __launch_bounds__(DPDLP_BS) void k_scale_values(const int *rp, const int *ci, double *v,
+                                                          int rows, const double *rowf,
+                                                          const double *colf, int row_first) {
+  const int lane = threadIdx.x & 63;
+  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwave = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long r = wave0; r < rows; r += nwave) {
+    const double fr = rowf[r];
+    const int e = rp[r + 1];
+    for (int q = rp[r] + lane; q < e; q += 64) {
+      const double fc = colf[ci[q]];
+      const double d = row_first ? fr * fc : fc * fr;
+      v[q] /= d;
     }
   }
 }
@@ -790,11 +991,11 @@ int dpdlp_dev_fpe_finish(const dpdlp_state_t *st, int to_restart, int to_major, 
   int d1 = to_restart ? DPDLP_SC_FPE_RESTART : -1;
   int d2 = to_major ? DPDLP_SC_FPE_MAJOR : -1;
   /* slots come from three kernels with different partial counts */
-  RET(finalize(st, st->At.nblk, DPDLP_SL_FPE, 3, 0u, DPDLP_SL_FPE, d1 >= 0 ? d1 : -1,
-               d2 >= 0 ? d2 : -1, ST(stream)));
-  RET(finalize(st, st->A.nblk, DPDLP_SL_FPE + 3, 3, 0u, DPDLP_SL_FPE + 3,
+  RET(finalize(st, spmv_nparts(&st->At), DPDLP_SL_FPE, 3, 0u, DPDLP_SL_FPE,
+               d1 >= 0 ? d1 : -1, d2 >= 0 ? d2 : -1, ST(stream)));
+  RET(finalize(st, spmv_nparts(&st->A), DPDLP_SL_FPE + 3, 3, 0u, DPDLP_SL_FPE + 3,
                d1 >= 0 ? d1 + 3 : -1, d2 >= 0 ? d2 + 3 : -1, ST(stream)));
-  RET(finalize(st, st->At.nblk, DPDLP_SL_FPE + 6, 1, 0u, DPDLP_SL_FPE + 6,
+  RET(finalize(st, spmv_nparts(&st->At), DPDLP_SL_FPE + 6, 1, 0u, DPDLP_SL_FPE + 6,
                d1 >= 0 ? d1 + 6 : -1, d2 >= 0 ? d2 + 6 : -1, ST(stream)));
   return 0;
 }
@@ -804,9 +1005,9 @@ int dpdlp_dev_fpe_finish_polish(const dpdlp_state_t *st, int primal, int to_rest
   int d1 = to_restart ? DPDLP_SC_FPE_RESTART : -1;
   int d2 = to_major ? DPDLP_SC_FPE_MAJOR : -1;
   if (primal)
-    return (int)finalize(st, st->At.nblk, DPDLP_SL_FPE, 3, 0u, DPDLP_SL_FPE, d1, d2,
+    return (int)finalize(st, spmv_nparts(&st->At), DPDLP_SL_FPE, 3, 0u, DPDLP_SL_FPE, d1, d2,
                          ST(stream));
-  return (int)finalize(st, st->A.nblk, DPDLP_SL_FPE + 3, 3, 0u, DPDLP_SL_FPE + 3,
+  return (int)finalize(st, spmv_nparts(&st->A), DPDLP_SL_FPE + 3, 3, 0u, DPDLP_SL_FPE + 3,
                        d1 >= 0 ? d1 + 3 : -1, d2 >= 0 ? d2 + 3 : -1, ST(stream));
 }
 
@@ -824,8 +1025,7 @@ int dpdlp_dev_residual(const dpdlp_state_t *st, const dpdlp_state_t *orig, int w
     e.dist = restart_dist;
     e.partials = 1;
     RET(launch_spmv(&st->A, st->x_pdhg, e, st->part, st->part_stride, DPDLP_SL_RESP, s));
-    if (st->A.nblk > 0)
-      RET(finalize(st, st->A.nblk, DPDLP_SL_RESP, 8, 1u << 7, DPDLP_SL_RESP, -1, -1, s));
+    RET(finalize(st, spmv_nparts(&st->A), DPDLP_SL_RESP, 8, 1u << 7, DPDLP_SL_RESP, -1, -1, s));
     if (which == 1) {
       /* primal polish: objective uses the original c (utils.cu:1176-1182) */
       int g = ew_grid(st->n);
@@ -846,8 +1046,7 @@ int dpdlp_dev_residual(const dpdlp_state_t *st, const dpdlp_state_t *orig, int w
     e.dist = restart_dist;
     e.partials = 1;
     RET(launch_spmv(&st->At, st->y_pdhg, e, st->part, st->part_stride, DPDLP_SL_RESD, s));
-    if (st->At.nblk > 0)
-      RET(finalize(st, st->At.nblk, DPDLP_SL_RESD, 9, 1u << 8, DPDLP_SL_RESD, -1, -1, s));
+    RET(finalize(st, spmv_nparts(&st->At), DPDLP_SL_RESD, 9, 1u << 8, DPDLP_SL_RESD, -1, -1, s));
     if (which == 2) {
       /* dual polish: contributions use the original finite bounds */
       int g = ew_grid(st->m);
@@ -888,15 +1087,9 @@ int dpdlp_dev_infeasibility(const dpdlp_state_t *st, void *stream) {
   ed.dx = st->dx;
   ed.partials = 1;
   RET(launch_spmv(&st->At, st->dy, ed, st->part, st->part_stride, DPDLP_SL_INF + 4, s));
-  if (st->A.nblk > 0)
-    RET(finalize(st, st->A.nblk, DPDLP_SL_INF + 2, 2, 2u, DPDLP_SL_INF + 2, -1, -1, s));
-  else
-    k_fill<<<1, 64, 0, s>>>(st->scal + DPDLP_SL_INF + 2, 0.0, 2);
-  if (st->At.nblk > 0)
-    RET(finalize(st, st->At.nblk, DPDLP_SL_INF + 4, 3, 4u, DPDLP_SL_INF + 4, -1, -1, s));
-  else
-    k_fill<<<1, 64, 0, s>>>(st->scal + DPDLP_SL_INF + 4, 0.0, 3);
-  return (int)hipGetLastError();
+  RET(finalize(st, spmv_nparts(&st->A), DPDLP_SL_INF + 2, 2, 2u, DPDLP_SL_INF + 2, -1, -1, s));
+  RET(finalize(st, spmv_nparts(&st->At), DPDLP_SL_INF + 4, 3, 4u, DPDLP_SL_INF + 4, -1, -1, s));
+  return 0;
 }
 
 int dpdlp_dev_restart_copy(const dpdlp_state_t *st, int part, void *stream) {
@@ -927,7 +1120,7 @@ int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next, do
     EpiPow e2{eig, next, 1};
     RET(launch_spmv(&st->A, dual, e2, st->part, st->part_stride, DPDLP_SL_PI, s));
     /* sigma^2 also recorded in the history (slot 0 of the 4) */
-    RET(finalize(st, st->A.nblk, DPDLP_SL_PI, 4, 0u, DPDLP_SL_PI, DPDLP_SC_PI_HIST + 4 * it,
+    RET(finalize(st, spmv_nparts(&st->A), DPDLP_SL_PI, 4, 0u, DPDLP_SL_PI, DPDLP_SC_PI_HIST + 4 * it,
                  -1, s));
     k_pow_update<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, DPDLP_SL_PI,
                                         DPDLP_SL_PI + 1, st->part, st->part_stride,
@@ -944,11 +1137,6 @@ int dpdlp_dev_result(const dpdlp_state_t *st, double beta, double gamma, double 
   hipStream_t s = ST(stream);
   EpiRc e{rc_out, st->c, st->D_c, gamma, 0};
   RET(launch_spmv(&st->At, st->y_pdhg, e, nullptr, 0, 0, s));
-  if (st->At.nblk == 0 && st->n > 0) {
-    /* no nonzeros: A'y = 0 */
-    EpiRc e0 = e;
-    (void)e0;
-  }
   k_unscale<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
       st->x_pdhg, st->y_pdhg, st->D_c, st->D_r, beta, gamma, x_out, y_out, st->n, st->m);
   return (int)hipGetLastError();
@@ -956,15 +1144,15 @@ int dpdlp_dev_result(const dpdlp_state_t *st, double beta, double gamma, double 
 
 int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream) {
   hipStream_t s = ST(stream);
-  k_row_reduce<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 0, 1.0, rs);
-  k_row_reduce<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 0, 1.0, cs);
+  k_row_reduce<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 0, 1.0, rs);
+  k_row_reduce<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 0, 1.0, cs);
   k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
   k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
   k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
       st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
-  k_scale_values<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
+  k_scale_values<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
                                                      st->m, rs, cs, 1);
-  k_scale_values<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
+  k_scale_values<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
                                                      st->n, cs, rs, 0);
   return (int)hipGetLastError();
 }
@@ -972,16 +1160,16 @@ int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream)
 int dpdlp_dev_pock_chambolle(dpdlp_state_t *st, double alpha, double *rs, double *cs,
                              void *stream) {
   hipStream_t s = ST(stream);
-  k_row_reduce<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 1, alpha, rs);
-  k_row_reduce<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 1, 2.0 - alpha,
-                                                   cs);
+  k_row_reduce<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 1, alpha, rs);
+  k_row_reduce<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 1, 2.0 - alpha,
+                                                     cs);
   k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
   k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
   k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
       st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
-  k_scale_values<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
+  k_scale_values<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
                                                      st->m, rs, cs, 1);
-  k_scale_values<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
+  k_scale_values<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
                                                      st->n, cs, rs, 0);
   return (int)hipGetLastError();
 }
@@ -1016,6 +1204,20 @@ int dpdlp_dev_zero_finite(double *v, int count, void *stream) {
 int dpdlp_dev_fill(double *v, double value, long long count, void *stream) {
   if (count <= 0) return 0;
   k_fill<<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(v, value, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_permute(const double *src, const int *perm, double *dst, long long count,
+                      void *stream) {
+  if (count <= 0) return 0;
+  k_permute<double><<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(src, perm, dst, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_permute_f32(const float *src, const int *perm, float *dst, long long count,
+                          void *stream) {
+  if (count <= 0) return 0;
+  k_permute<float><<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(src, perm, dst, count);
   return (int)hipGetLastError();
 }
 

@@ -22,16 +22,14 @@ void dpdlp_resident_info(const dpdlp_resident_t *r, int *m, int *n, int *nnz, in
 double dpdlp_time_spmv(dpdlp_resident_t *r, int transpose, int reps, int use32) {
   if (!r || reps <= 0) return -1.0;
   dpdlp_csr_t A;
-  if (!transpose)
-    A = (dpdlp_csr_t){r->m, r->n, r->nnz, r->A_rp, r->A_ci, r->A_v, NULL, r->A_blk, r->A_nblk,
-                      r->cap};
-  else
-    A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
-                      r->At_nblk, r->cap};
+  double *xpart = NULL;
+  long long xl = dpdlp_resident_xpart_len(r);
+  if (xl > 0 && hipMalloc((void **)&xpart, (size_t)xl * sizeof(double)) != hipSuccess) return -1;
+  dpdlp_resident_csr(r, transpose, &A, xpart);
   hipStream_t s = NULL;
   hipEvent_t e0 = NULL, e1 = NULL;
   double *x = NULL, *y = NULL;
-  float *x32 = NULL, *v32 = NULL;
+  float *x32 = NULL, *v32 = NULL, *pv32 = NULL;
   double ms = -1.0;
   if (hipStreamCreate(&s) != hipSuccess) return -1.0;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto out;
@@ -44,6 +42,11 @@ double dpdlp_time_spmv(dpdlp_resident_t *r, int transpose, int reps, int use32) 
     if (dpdlp_dev_make_f32(x, x32, A.cols, s)) goto out;
     if (dpdlp_dev_make_f32(A.v, v32, A.nnz, s)) goto out;
     A.v32 = v32;
+    if (A.P > 1) {
+      if (hipMalloc((void **)&pv32, (size_t)(A.nnz ? A.nnz : 1) * 4) != hipSuccess) goto out;
+      if (dpdlp_dev_make_f32(A.pv, pv32, A.nnz, s)) goto out;
+      A.pv32 = pv32;
+    }
   }
   /* warm up */
   for (int k = 0; k < 3; ++k)
@@ -63,6 +66,8 @@ out:
   if (y) hipFree(y);
   if (x32) hipFree(x32);
   if (v32) hipFree(v32);
+  if (pv32) hipFree(pv32);
+  if (xpart) hipFree(xpart);
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   if (s) hipStreamDestroy(s);
@@ -73,14 +78,12 @@ int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, do
                         int use32) {
   if (!r || !xh || !yh) return -1;
   dpdlp_csr_t A;
-  if (!transpose)
-    A = (dpdlp_csr_t){r->m, r->n, r->nnz, r->A_rp, r->A_ci, r->A_v, NULL, r->A_blk, r->A_nblk,
-                      r->cap};
-  else
-    A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
-                      r->At_nblk, r->cap};
+  double *xpart = NULL;
+  long long xl = dpdlp_resident_xpart_len(r);
+  if (xl > 0 && hipMalloc((void **)&xpart, (size_t)xl * sizeof(double)) != hipSuccess) return -1;
+  dpdlp_resident_csr(r, transpose, &A, xpart);
   double *x = NULL, *y = NULL;
-  float *x32 = NULL, *v32 = NULL;
+  float *x32 = NULL, *v32 = NULL, *pv32 = NULL;
   int rc = -1;
   hipStream_t s = NULL;
   if (hipStreamCreate(&s) != hipSuccess) return -1;
@@ -94,6 +97,11 @@ int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, do
     if (dpdlp_dev_make_f32(x, x32, A.cols, s)) goto out;
     if (dpdlp_dev_make_f32(A.v, v32, A.nnz, s)) goto out;
     A.v32 = v32;
+    if (A.P > 1) {
+      if (hipMalloc((void **)&pv32, (size_t)(A.nnz ? A.nnz : 1) * 4) != hipSuccess) goto out;
+      if (dpdlp_dev_make_f32(A.pv, pv32, A.nnz, s)) goto out;
+      A.pv32 = pv32;
+    }
     if (dpdlp_dev_spmv_f32(&A, x32, y, s)) goto out;
   } else {
     if (dpdlp_dev_spmv(&A, x, y, s)) goto out;
@@ -108,6 +116,8 @@ out:
   if (y) hipFree(y);
   if (x32) hipFree(x32);
   if (v32) hipFree(v32);
+  if (pv32) hipFree(pv32);
+  if (xpart) hipFree(xpart);
   if (s) hipStreamDestroy(s);
   return rc;
 }

@@ -17,11 +17,23 @@ struct dpdlp_resident {
   int *A_rp, *A_ci, *At_rp, *At_ci, *A_blk, *At_blk;
   int A_nblk, At_nblk;
   double *A_v, *At_v;
+  /* column-panel tiles (P == 1: none) */
+  int A_P, At_P;
+  int *A_toff, *At_toff, *A_perm, *At_perm, *A_pci, *At_pci;
+  unsigned short *A_rofs, *At_rofs;
+  double *A_pv, *At_pv; /* unscaled tiled values */
   double *c, *lv, *uv, *lc, *uc;
   double c0, obj_l2, obj_linf, bnd_l2, bnd_linf;
   double *h_pstart, *h_dstart;
   double upload_time;
 };
+
+/* Device descriptors of a resident problem's unscaled matrices (xpart must be
+ * provided by the caller when P > 1). */
+void dpdlp_resident_csr(const struct dpdlp_resident *r, int transpose, void *csr_out,
+                        double *xpart);
+/* Doubles of partial-sum scratch the column-panel SpMVs of r need. */
+long long dpdlp_resident_xpart_len(const struct dpdlp_resident *r);
 
 /* logging.c: the reference's console formats (src/utils.cu:191-501) */
 const char *dpdlp_termination_string(termination_reason_t r);

@@ -158,6 +158,34 @@ static int sync_scal(ctx_t *c, int count) {
 /* Upload                                                                    */
 /* ------------------------------------------------------------------------ */
 
+/* Column panels so that each XCD's slice of the gathered FP64 vector is at
+ * most 2 MB (half the 4 MB L2 of an XCD); 1, 2, 4 or 8. */
+static int pick_panels(int cols) {
+  long long bytes = (long long)cols * 8;
+  int P = 1;
+  while (P < 8 && bytes / P > (2LL << 20)) P *= 2;
+  return P;
+}
+
+void dpdlp_resident_csr(const dpdlp_resident_t *r, int transpose, void *out, double *xpart) {
+  dpdlp_csr_t *A = (dpdlp_csr_t *)out;
+  memset(A, 0, sizeof(*A));
+  if (!transpose) {
+    *A = (dpdlp_csr_t){r->m, r->n, r->nnz, r->A_rp, r->A_ci, r->A_v, NULL, r->A_blk, r->A_nblk,
+                       r->cap, r->A_P, r->A_toff, r->A_rofs, r->A_pci, r->A_pv, NULL, xpart};
+  } else {
+    *A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
+                       r->At_nblk, r->cap, r->At_P, r->At_toff, r->At_rofs, r->At_pci, r->At_pv,
+                       NULL, xpart};
+  }
+}
+
+long long dpdlp_resident_xpart_len(const dpdlp_resident_t *r) {
+  long long a = r->A_P > 1 ? (long long)r->A_P * r->m : 0;
+  long long b = r->At_P > 1 ? (long long)r->At_P * r->n : 0;
+  return a > b ? a : b;
+}
+
 static int pick_cap(long long nnz, int rows) {
   (void)rows;
   if (nnz >= 4000000) return 2048;
@@ -174,8 +202,10 @@ void dpdlp_default_opts(dpdlp_solve_opts_t *o) {
 
 void dpdlp_resident_free(dpdlp_resident_t *r) {
   if (!r) return;
-  void *ptrs[] = {r->A_rp, r->A_ci, r->At_rp, r->At_ci, r->A_blk, r->At_blk, r->A_v,
-                  r->At_v, r->c,    r->lv,   r->uv,    r->lc,    r->uc};
+  void *ptrs[] = {r->A_rp,    r->A_ci,    r->At_rp,   r->At_ci,   r->A_blk,   r->At_blk,
+                  r->A_v,     r->At_v,    r->c,       r->lv,      r->uv,      r->lc,
+                  r->uc,      r->A_toff,  r->At_toff, r->A_perm,  r->At_perm, r->A_pci,
+                  r->At_pci,  r->A_rofs,  r->At_rofs, r->A_pv,    r->At_pv};
   for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     if (ptrs[i]) hipFree(ptrs[i]);
   free(r->h_pstart);
@@ -200,7 +230,47 @@ static hipError_t up(void **dst, const void *src, size_t bytes) {
   return hipSuccess;
 }
 
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt);
+
 dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
+  return upload_with(p, spmv_cap, 0, 0);
+}
+
+/* Times both SpMVs of a resident problem (mean of `reps`, ms). */
+static void time_both(dpdlp_resident_t *r, double *ta, double *tt) {
+  *ta = dpdlp_time_spmv(r, 0, 10, 0);
+  *tt = dpdlp_time_spmv(r, 1, 10, 0);
+}
+
+dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *chosen) {
+  /* candidates P in {1,2,4,8} for A and A' independently; each candidate
+   * upload is timed with the plain SpMV and freed */
+  static const int cand[4] = {1, 2, 4, 8};
+  double bestA = 1e30, bestT = 1e30;
+  int PA = 1, PT = 1;
+  for (int i = 0; i < 4; ++i) {
+    dpdlp_resident_t *r = upload_with(p, spmv_cap, cand[i], cand[i]);
+    if (!r) continue;
+    double ta, tt;
+    time_both(r, &ta, &tt);
+    if (ta > 0 && ta < bestA) {
+      bestA = ta;
+      PA = cand[i];
+    }
+    if (tt > 0 && tt < bestT) {
+      bestT = tt;
+      PT = cand[i];
+    }
+    dpdlp_resident_free(r);
+  }
+  if (chosen) {
+    chosen[0] = PA;
+    chosen[1] = PT;
+  }
+  return upload_with(p, spmv_cap, PA, PT);
+}
+
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt) {
   double t0 = dpdlp_wall();
   if (!p || p->num_variables < 0 || p->num_constraints < 0 ||
       p->constraint_matrix_num_nonzeros < 0) {
@@ -295,10 +365,73 @@ dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
   UPCHK(up((void **)&r->At_rp, t_rp, (size_t)(n + 1) * sizeof(int)));
   UPCHK(up((void **)&r->At_ci, t_ci, (size_t)nnz * sizeof(int)));
   UPCHK(up((void **)&r->At_v, t_v, (size_t)nnz * sizeof(double)));
-  r->A_nblk = dpdlp_plan_blocks(m, rp, r->cap, blk);
+  /* XCD column-panel tiles */
+  r->A_P = PA > 0 ? PA : pick_panels(n);
+  r->At_P = PAt > 0 ? PAt : pick_panels(m);
+  if (PA <= 0) {
+    const char *env = getenv("DPDLP_PANELS");
+    if (env) {
+      int a = 0, b = 0;
+      if (sscanf(env, "%d,%d", &a, &b) == 2) {
+        if (a == 1 || a == 2 || a == 4 || a == 8) r->A_P = a;
+        if (b == 1 || b == 2 || b == 4 || b == 8) r->At_P = b;
+      }
+    }
+  }
+  r->A_nblk = r->A_P > 1 ? dpdlp_plan_blocks_panels(m, n, rp, ci, r->cap, r->A_P,
+                                                    DPDLP_PANEL_RMAX, blk)
+                         : dpdlp_plan_blocks(m, rp, r->cap, blk);
   UPCHK(up((void **)&r->A_blk, blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
-  r->At_nblk = dpdlp_plan_blocks(n, t_rp, r->cap, blk);
+  r->At_nblk = r->At_P > 1 ? dpdlp_plan_blocks_panels(n, m, t_rp, t_ci, r->cap, r->At_P,
+                                                      DPDLP_PANEL_RMAX, blk)
+                           : dpdlp_plan_blocks(n, t_rp, r->cap, blk);
   UPCHK(up((void **)&r->At_blk, blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
+
+  for (int which = 0; which < 2; ++which) {
+    int P = which ? r->At_P : r->A_P;
+    if (P <= 1) continue;
+    int rows = which ? n : m, cols = which ? m : n, nb = which ? r->At_nblk : r->A_nblk;
+    const int *hrp = which ? t_rp : rp, *hci = which ? t_ci : ci;
+    const double *hv = which ? t_v : p->constraint_matrix_values;
+    int *blkh = (int *)malloc((size_t)(nb + 1) * sizeof(int));
+    int *toff = (int *)malloc(((size_t)nb * P + 1) * sizeof(int));
+    unsigned short *rofs =
+        (unsigned short *)malloc((size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
+    int *perm = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+    int *pci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+    double *pv = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    int ok = blkh && toff && rofs && perm && pci && pv;
+    if (ok) {
+      dpdlp_plan_blocks_panels(rows, cols, hrp, hci, r->cap, P, DPDLP_PANEL_RMAX, blkh);
+      ok = dpdlp_build_panels(rows, cols, hrp, hci, blkh, nb, P, toff, rofs, perm, pci) == 0;
+    }
+    if (ok) {
+      for (int k = 0; k < nnz; ++k) pv[k] = hv[perm[k]];
+      hipError_t e = hipSuccess;
+      int **dt = which ? &r->At_toff : &r->A_toff, **dp = which ? &r->At_perm : &r->A_perm;
+      int **dc = which ? &r->At_pci : &r->A_pci;
+      unsigned short **dr = which ? &r->At_rofs : &r->A_rofs;
+      double **dv = which ? &r->At_pv : &r->A_pv;
+      if (e == hipSuccess) e = up((void **)dt, toff, ((size_t)nb * P + 1) * sizeof(int));
+      if (e == hipSuccess)
+        e = up((void **)dr, rofs, (size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
+      if (e == hipSuccess) e = up((void **)dp, perm, (size_t)nnz * sizeof(int));
+      if (e == hipSuccess) e = up((void **)dc, pci, (size_t)nnz * sizeof(int));
+      if (e == hipSuccess) e = up((void **)dv, pv, (size_t)nnz * sizeof(double));
+      ok = e == hipSuccess;
+      if (!ok) dpdlp_set_error("HIP error uploading panel tiles: %s", hipGetErrorString(e));
+    }
+    free(blkh);
+    free(toff);
+    free(rofs);
+    free(perm);
+    free(pci);
+    free(pv);
+    if (!ok) {
+      dpdlp_set_error("building column-panel tiles failed");
+      goto ufail;
+    }
+  }
   UPCHK(up((void **)&r->c, p->objective_vector, (size_t)n * sizeof(double)));
   UPCHK(up((void **)&r->lv, p->variable_lower_bound, (size_t)n * sizeof(double)));
   UPCHK(up((void **)&r->uv, p->variable_upper_bound, (size_t)n * sizeof(double)));
@@ -716,9 +849,24 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
   st->n = n;
   c->w_A_v = dcopy(c, R->A_v, nnz);
   c->w_At_v = dcopy(c, R->At_v, nnz);
-  st->A = (dpdlp_csr_t){m, n, nnz, R->A_rp, R->A_ci, c->w_A_v, NULL, R->A_blk, R->A_nblk, R->cap};
-  st->At =
-      (dpdlp_csr_t){n, m, nnz, R->At_rp, R->At_ci, c->w_At_v, NULL, R->At_blk, R->At_nblk, R->cap};
+  {
+    long long xl = dpdlp_resident_xpart_len(R);
+    double *xpart = xl > 0 ? (double *)dmalloc(c, (size_t)xl * sizeof(double)) : NULL;
+    if (xl > 0 && !xpart) {
+      dpdlp_set_error("device allocation failed (panel partials)");
+      goto fail;
+    }
+    dpdlp_resident_csr(R, 0, &st->A, xpart);
+    dpdlp_resident_csr(R, 1, &st->At, xpart);
+    st->A.v = c->w_A_v;
+    st->At.v = c->w_At_v;
+    st->A.pv = st->A.P > 1 ? (double *)dmalloc(c, (size_t)(nnz ? nnz : 1) * sizeof(double)) : NULL;
+    st->At.pv = st->At.P > 1 ? (double *)dmalloc(c, (size_t)(nnz ? nnz : 1) * sizeof(double)) : NULL;
+    if ((st->A.P > 1 && !st->A.pv) || (st->At.P > 1 && !st->At.pv)) {
+      dpdlp_set_error("device allocation failed (tiled values)");
+      goto fail;
+    }
+  }
   st->c = dcopy(c, R->c, n);
   st->lv = dcopy(c, R->lv, n);
   st->uv = dcopy(c, R->uv, n);
@@ -780,6 +928,10 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     c->beta = 1.0;
     c->gamma = 1.0;
   }
+  /* scaled values in tile order */
+  if (st->A.P > 1) HIPCHK(dpdlp_dev_permute(c->w_A_v, R->A_perm, (double *)st->A.pv, nnz, c->s));
+  if (st->At.P > 1)
+    HIPCHK(dpdlp_dev_permute(c->w_At_v, R->At_perm, (double *)st->At.pv, nnz, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   double rescale_time = dpdlp_wall() - t_sc0;
 
@@ -816,6 +968,18 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     }
     HIPCHK(dpdlp_dev_make_f32(c->w_A_v, c->w_A_v32, nnz, c->s));
     HIPCHK(dpdlp_dev_make_f32(c->w_At_v, c->w_At_v32, nnz, c->s));
+    if (st->A.P > 1) {
+      float *t = (float *)dmalloc(c, (size_t)(nnz ? nnz : 1) * sizeof(float));
+      if (!t) goto fail;
+      HIPCHK(dpdlp_dev_make_f32(st->A.pv, t, nnz, c->s));
+      st->A.pv32 = t;
+    }
+    if (st->At.P > 1) {
+      float *t = (float *)dmalloc(c, (size_t)(nnz ? nnz : 1) * sizeof(float));
+      if (!t) goto fail;
+      HIPCHK(dpdlp_dev_make_f32(st->At.pv, t, nnz, c->s));
+      st->At.pv32 = t;
+    }
     HIPCHK(dpdlp_dev_make_f32(st->y_cur, st->y_cur32, m, c->s));
     st->A.v32 = c->w_A_v32;
     st->At.v32 = c->w_At_v32;
@@ -849,7 +1013,7 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
           break;
         }
       }
-      batch = batch * 2 > DPDLP_PI_BATCH ? DPDLP_PI_BATCH : batch * 2;
+      batch = batch * 2 > 32 ? 32 : batch * 2;
     }
     h->step = 0.998 / sqrt(sigma_sq);
   }

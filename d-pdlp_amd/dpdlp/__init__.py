@@ -46,6 +46,7 @@ def _load() -> C.CDLL:
         "dpdlp_default_opts": (None, [O]),
         "dpdlp_solve_ex": (R, [P, C.POINTER(PdhgParameters), O]),
         "dpdlp_upload": (C.c_void_p, [P, C.c_int]),
+        "dpdlp_upload_tuned": (C.c_void_p, [P, C.c_int, _ip]),
         "dpdlp_solve_resident": (R, [C.c_void_p, C.POINTER(PdhgParameters), O]),
         "dpdlp_resident_free": (None, [C.c_void_p]),
         "dpdlp_resident_info": (None, [C.c_void_p, _ip, _ip, _ip, _ip, _ip]),
@@ -281,8 +282,14 @@ def solve_plain(problem: Problem, params: PdhgParameters | None = None) -> Resul
 class Resident:
     """A problem uploaded to HBM once (dpdlp_upload) and solved many times."""
 
-    def __init__(self, problem: Problem, spmv_cap=0):
-        self.h = lib.dpdlp_upload(problem.ptr, spmv_cap)
+    def __init__(self, problem: Problem, spmv_cap=0, tune=False):
+        self.panels = None
+        if tune:
+            ch = (C.c_int * 2)()
+            self.h = lib.dpdlp_upload_tuned(problem.ptr, spmv_cap, ch)
+            self.panels = (ch[0], ch[1])
+        else:
+            self.h = lib.dpdlp_upload(problem.ptr, spmv_cap)
         if not self.h:
             raise RuntimeError("upload failed: " + last_error())
         m, n, nnz, ba, bt = (C.c_int() for _ in range(5))

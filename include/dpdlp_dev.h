@@ -14,7 +14,16 @@
 extern "C" {
 #endif
 
-/* CSR matrix resident in HBM, with its row-block plan for the stream SpMV. */
+/* CSR matrix resident in HBM, with its row-block plan for the stream SpMV.
+ *
+ * When P > 1 the nonzeros are also stored as XCD column-panel tiles: the
+ * columns are cut into P equal panels, and the entries of row block b that
+ * fall in panel k form tile b*P + k (entries in row order, columns global).
+ * Workgroup w of the tiled SpMV handles tile w; the hardware dispatches
+ * workgroups round-robin over the 8 XCDs, so every XCD gathers from only one
+ * panel of the vector, which stays resident in its 4 MB L2.  Per-panel row
+ * sums go to xpart[k*rows + row] and a second kernel adds the P partials in
+ * panel order and applies the epilogue. */
 typedef struct {
   int rows, cols, nnz;
   const int *rp;     /* [rows+1] */
@@ -24,6 +33,14 @@ typedef struct {
   const int *blk;    /* [nblk+1] first row of each row block */
   int nblk;
   int cap;           /* nonzeros a multi-row block may hold (1024/2048/4096) */
+  /* column-panel tiles (P == 1: not used) */
+  int P;
+  const int *toff;            /* [nblk*P+1] first entry of each tile */
+  const unsigned short *rofs; /* tile-local row offsets, P*(R+1) per block */
+  const int *pci;             /* [nnz] column of each tiled entry */
+  const double *pv;           /* [nnz] tiled FP64 values */
+  const float *pv32;          /* [nnz] tiled FP32 values or NULL */
+  double *xpart;              /* [P*rows] partial row sums (scratch) */
 } dpdlp_csr_t;
 
 /* One LP's device state (scaled space).  Field meaning follows the
@@ -61,6 +78,23 @@ enum {
  * of their own, other blocks hold <= 256 rows and <= cap nonzeros.  Writes
  * blk[0..nblk] and returns nblk. */
 int dpdlp_plan_blocks(int rows, const int *rp_host, int cap, int *blk_out);
+
+/* Host construction of the column-panel tiles of a CSR matrix with a given
+ * block plan: toff[nblk*P+1], rofs[P*(rows+nblk)], perm[nnz] (tiled position
+ * -> CSR position) and pci[nnz].  Returns 0, or -1 if a multi-row tile would
+ * exceed 65535 entries. */
+int dpdlp_build_panels(int rows, int cols, const int *rp, const int *ci, const int *blk,
+                       int nblk, int P, int *toff, unsigned short *rofs, int *perm, int *pci);
+/* Row blocks for the tiled SpMV (<= rmax rows, every tile <= cap entries,
+ * rows that do not fit alone).  rmax <= DPDLP_PANEL_RMAX.  Returns nblk. */
+int dpdlp_plan_blocks_panels(int rows, int cols, const int *rp, const int *ci, int cap, int P,
+                             int rmax, int *blk);
+#define DPDLP_PANEL_RMAX 1024
+/* dst[t] = src[perm[t]] (tiled copies of values after scaling). */
+int dpdlp_dev_permute(const double *src, const int *perm, double *dst, long long count,
+                      void *stream);
+int dpdlp_dev_permute_f32(const float *src, const int *perm, float *dst, long long count,
+                          void *stream);
 
 /* y = A x (FP64), replaces cusparseSpMV CSR_ALG2 at pdlp_core_op.cu:501,534
  * and utils.cu:134,138,645,650,750,755. */

@@ -59,6 +59,9 @@ cupdlpx_result_t *dpdlp_solve_ex(const lp_problem_t *prob,
  * solved many times without host-to-device traffic. */
 typedef struct dpdlp_resident dpdlp_resident_t;
 dpdlp_resident_t *dpdlp_upload(const lp_problem_t *prob, int spmv_cap);
+/* Same, choosing the XCD column-panel count of A and A' (1, 2, 4 or 8) by
+ * timing each candidate SpMV on the device; chosen[0..1] receives them. */
+dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *prob, int spmv_cap, int *chosen);
 cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *res,
                                        const pdhg_parameters_t *params,
                                        dpdlp_solve_opts_t *opts);

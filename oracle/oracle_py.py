@@ -35,6 +35,7 @@ def _load(path):
     sig = {
         "orc_solve": (R, [P, C.POINTER(PdhgParameters), C.POINTER(OrcTrace)]),
         "orc_result_free": (None, [R]),
+        "orc_last_timings": (None, [_dp]),
         "orc_default_parameters": (None, [C.POINTER(PdhgParameters)]),
         "orc_num_threads": (C.c_int, []),
         "orc_get_print_frequency": (C.c_int, [C.c_int]),
@@ -138,3 +139,10 @@ def build_problem(m, n, row_ptr, col_ind, vals, c, con_lb, con_ub, var_lb, var_u
                      primal_start=None, dual_start=None)
     prob._keep = arrs
     return prob
+
+
+def last_timings(omp=False):
+    """(rescale, transpose, power, loop, polish) wall seconds of the last solve."""
+    out = np.zeros(5)
+    lib(omp).orc_last_timings(p(out))
+    return dict(zip(("rescale", "transpose", "power", "loop", "polish"), out.tolist()))

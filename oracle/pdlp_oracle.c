@@ -968,6 +968,10 @@ static void feasibility_polish(const pdhg_parameters_t *params, ost *s,
 /* Driver (solver.cu:32-127)                                                 */
 /* ------------------------------------------------------------------------ */
 
+/* wall times of the last orc_solve: rescale, transpose, power, loop, polish */
+static double g_timings[5];
+void orc_last_timings(double *out) { memcpy(out, g_timings, sizeof(g_timings)); }
+
 cupdlpx_result_t *orc_solve(const lp_problem_t *prob,
                             const pdhg_parameters_t *params,
                             orc_trace_t *trace) {
@@ -994,6 +998,7 @@ cupdlpx_result_t *orc_solve(const lp_problem_t *prob,
   s.c0 = prob->objective_constant;
 
   /* pdlp_core_op.cu:93-130 */
+  double t_tr0 = wall_now();
   int *At_rp = (int *)xmalloc((size_t)(n + 1) * sizeof(int));
   int *At_ci = (int *)xmalloc((size_t)(nnz ? nnz : 1) * sizeof(int));
   double *At_v = dalloc(nnz);
@@ -1052,6 +1057,9 @@ cupdlpx_result_t *orc_solve(const lp_problem_t *prob,
   s.fpe_last = INFINITY;
   s.reason = TERMINATION_REASON_UNSPECIFIED;
 
+  double t_pi0 = wall_now();
+  g_timings[0] = t_rescale;
+  g_timings[1] = t_pi0 - t_tr0;
   /* pdlp_core_op.cu:638-656 */
   if (nnz == 0) {
     s.step = 1.0;
@@ -1065,6 +1073,7 @@ cupdlpx_result_t *orc_solve(const lp_problem_t *prob,
 
   /* main loop, solver.cu:56-102 */
   double t0 = wall_now();
+  g_timings[2] = t0 - t_pi0;
   int do_restart = 0;
   int freq = params->termination_evaluation_frequency;
   while (s.total < params->termination_criteria.iteration_limit) {
@@ -1098,11 +1107,13 @@ cupdlpx_result_t *orc_solve(const lp_problem_t *prob,
     s.reason = TERMINATION_REASON_ITERATION_LIMIT;
     compute_residual(&s, params->optimality_norm);
   }
+  g_timings[3] = wall_now() - t0;
   double polish_time = 0.0;
   int polish_iters = 0;
   if (params->feasibility_polishing && s.reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
       s.reason != TERMINATION_REASON_PRIMAL_INFEASIBLE)
     feasibility_polish(params, &s, &polish_time, &polish_iters);
+  g_timings[4] = polish_time;
 
   /* create_result_from_state, pdlp_core_op.cu:789-859 */
   cupdlpx_result_t *r = (cupdlpx_result_t *)calloc(1, sizeof(cupdlpx_result_t));

@@ -51,6 +51,9 @@ cupdlpx_result_t *orc_solve(const lp_problem_t *prob,
                             const pdhg_parameters_t *params,
                             orc_trace_t *trace);
 void orc_result_free(cupdlpx_result_t *r);
+/* Wall times (s) of the last orc_solve in this process: rescale, transpose,
+ * power iteration, main loop, polish. */
+void orc_last_timings(double *out5);
 void orc_default_parameters(pdhg_parameters_t *p);
 int orc_num_threads(void);
 

@@ -108,3 +108,53 @@ def test_iteration_and_time_limits(dp):
     r2 = dp.solve_plain(prob, dp.default_params(presolve=False,
                                                 termination_criteria__time_sec_limit=0.0))
     assert r2.status == "TIME_LIMIT" and r2.iterations == 0
+
+
+@pytest.fixture
+def panels(request):
+    import os
+    old = os.environ.get("DPDLP_PANELS")
+    os.environ["DPDLP_PANELS"] = request.param
+    yield request.param
+    if old is None:
+        del os.environ["DPDLP_PANELS"]
+    else:
+        os.environ["DPDLP_PANELS"] = old
+
+
+@pytest.mark.parametrize("panels", ["2,2", "4,8", "8,4"], indirect=True)
+@pytest.mark.parametrize("cap", [1024, 2048])
+def test_spmv_column_panels(dp, panels, cap):
+    """XCD column-panel tiles: same products as brute force, long rows split
+    across panels, empty tiles."""
+    rng = np.random.default_rng(17)
+    m, n = 2500, 3000
+    lens = np.minimum((rng.pareto(1.3, m) * 3).astype(int), 2900)
+    rp, ci, v = random_csr(m, n, lens, seed=23, long_rows=((0, 2999), (1234, 2500)))
+    prob = dp.Problem.from_csr(m, n, rp, ci, v, c=np.zeros(n))
+    R = dp.Resident(prob, spmv_cap=cap)
+    x = rng.standard_normal(n)
+    ref = csr_matvec(rp, ci, v, x)
+    y = R.spmv(x)
+    assert np.max(np.abs(y - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    yv = rng.standard_normal(m)
+    z = R.spmv(yv, transpose=True)
+    refz = np.zeros(n)
+    np.add.at(refz, ci, v * np.repeat(yv, np.diff(rp)))
+    assert np.max(np.abs(z - refz)) <= 1e-12 * max(1.0, np.max(np.abs(refz)))
+    y32 = R.spmv(x, use32=True)
+    assert np.max(np.abs(y32 - ref)) <= 1e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("panels", ["8,4"], indirect=True)
+def test_parity_trace_with_panels(dp, orc, panels):
+    prob, opt = dp.generate(2000, 5000, 50000, row_rule=0, seed=2)
+    iters = [0, 10, 100, 400, 1000]
+    res, o, (tx, ty) = dp.solve(prob, dp.default_params(presolve=False), trace_iters=iters)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(presolve=False), trace_iters=iters,
+                                 n=prob.n, m=prob.m)
+    assert res.status == ref["status"] == "OPTIMAL"
+    for i in range(len(ox)):
+        assert rel(tx[i], ox[i]) < 1e-8 and rel(ty[i], oy[i]) < 1e-8, iters[i]
+    assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
+        1.0, abs(ref["primal_objective_value"]))
