@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--spmv-cap", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="panel counts from the size rule")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 256 MiB plain copy kernel (counter calibration)")
     ap.add_argument("--mixed", action="store_true", help="FP32 matrix / FP64 state")
     ap.add_argument("--verbose", action="store_true", help="print the solver's iteration log")
     args = ap.parse_args()
@@ -155,6 +157,17 @@ def main():
                 A_GBps=spmv_bytes(m, n, nnz) / (ta * 1e-3) / 1e9,
                 At_GBps=spmv_bytes(n, m, nnz) / (tt * 1e-3) / 1e9)
     iter_GBps = (b1 + b2) / ((p1 + p2) * 1e-3) / 1e9
+    copy_GBps = None
+    if args.calib_copy:
+        nbytes = 256 << 20
+        cms = dpdlp.lib.dpdlp_copy_calibration(nbytes, 5)
+        copy_GBps = 2 * nbytes / (cms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "cfg1_fp64_hbm_traffic.json")
+    if args.workload == "cfg1" and not args.mixed and os.path.exists(tfile):
+        with open(tfile) as f:
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch", {}).get(dom[0])
     cpu = None
     if not args.no_cpu:
         print("# cpu baseline sample ...", file=sys.stderr, flush=True)
@@ -189,7 +202,8 @@ def main():
         "spmv": spmv,
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": None, "algorithmic_bytes_per_launch": dom[1]},
+                     "traffic": traffic, "algorithmic_bytes_per_launch": dom[1]},
+        "copy_kernel_GBps": copy_GBps,
         "cpu_baseline": cpu,
         "xcd_panels": R.panels,
         "generate_s": gen_s,

@@ -877,6 +877,14 @@ __global__ void k_fill(double *v, double value, long long count) {
     v[i] = value;
 }
 
+/* plain copy of known size, used to calibrate the HBM byte counters */
+__global__ __launch_bounds__(DPDLP_BS) void k_copy_calib(const double4 *__restrict__ src,
+                                                        double4 *__restrict__ dst, long long n4) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 /* warm start: v = start * D * s into init, cur and pdhg (pdlp_core_op.cu:174-201) */
 __global__ void k_warm_start(double *a, double *b, double *c, const double *start,
                              const double *D, double s, int count) {
@@ -1218,6 +1226,15 @@ int dpdlp_dev_permute_f32(const float *src, const int *perm, float *dst, long lo
                           void *stream) {
   if (count <= 0) return 0;
   k_permute<float><<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(src, perm, dst, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_copy_calib(const void *src, void *dst, long long bytes, void *stream) {
+  long long n4 = bytes / 32;
+  if (n4 <= 0) return 0;
+  long long g = (n4 + DPDLP_BS - 1) / DPDLP_BS;
+  if (g > 65536) g = 65536;
+  k_copy_calib<<<(int)g, DPDLP_BS, 0, ST(stream)>>>((const double4 *)src, (double4 *)dst, n4);
   return (int)hipGetLastError();
 }
 

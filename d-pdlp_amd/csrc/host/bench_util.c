@@ -121,3 +121,29 @@ out:
   if (s) hipStreamDestroy(s);
   return rc;
 }
+
+double dpdlp_copy_calibration(long long bytes, int reps) {
+  void *a = NULL, *b = NULL;
+  hipEvent_t e0 = NULL, e1 = NULL;
+  double ms = -1.0;
+  if (hipMalloc(&a, (size_t)bytes) != hipSuccess) return -1.0;
+  if (hipMalloc(&b, (size_t)bytes) != hipSuccess) goto out;
+  if (hipMemset(a, 0, (size_t)bytes) != hipSuccess) goto out;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto out;
+  if (dpdlp_dev_copy_calib(a, b, bytes, NULL)) goto out;
+  if (hipEventRecord(e0, NULL) != hipSuccess) goto out;
+  for (int k = 0; k < reps; ++k)
+    if (dpdlp_dev_copy_calib(a, b, bytes, NULL)) goto out;
+  if (hipEventRecord(e1, NULL) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) goto out;
+  {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / reps;
+  }
+out:
+  hipDeviceSynchronize();
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  return ms;
+}

@@ -157,6 +157,9 @@ int dpdlp_dev_make_f32(const double *src, float *dst, long long count, void *str
 /* Polishing helpers: v <- 0 where finite (pdlp_core_op.cu:1136-1143). */
 int dpdlp_dev_zero_finite(double *v, int count, void *stream);
 int dpdlp_dev_fill(double *v, double value, long long count, void *stream);
+/* Plain device copy of `bytes` (multiple of 32): the reference point that
+ * scales the FETCH_SIZE / WRITE_SIZE counters to bytes. */
+int dpdlp_dev_copy_calib(const void *src, void *dst, long long bytes, void *stream);
 /* Warm start (pdlp_core_op.cu:174-201): a = b = c = start * D * s. */
 int dpdlp_dev_warm_start(double *a, double *b, double *c, const double *start,
                          const double *D, double s, int count, void *stream);

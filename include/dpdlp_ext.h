@@ -78,6 +78,10 @@ double dpdlp_time_spmv(dpdlp_resident_t *res, int transpose, int reps, int use32
 int dpdlp_resident_spmv(dpdlp_resident_t *res, int transpose, const double *x, double *y,
                         int use32);
 
+/* Copies `bytes` device-to-device with a plain kernel `reps` times; returns the
+ * mean kernel time in ms (reference kernel for counter calibration). */
+double dpdlp_copy_calibration(long long bytes, int reps);
+
 /* Problem sizes of a resident problem (m, n, nnz, row blocks of A and A'). */
 void dpdlp_resident_info(const dpdlp_resident_t *res, int *m, int *n, int *nnz,
                          int *nblk_A, int *nblk_At);

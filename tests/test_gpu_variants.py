@@ -1,0 +1,104 @@
+"""GPU parity against the oracle across parameter variants, feasibility
+polishing, warm starts, and the CLI end to end."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+VARIANTS = {
+    "linf": dict(optimality_norm=1),
+    "no_bound_obj": dict(bound_objective_rescaling=False),
+    "no_ruiz_no_pc": dict(l_inf_ruiz_iterations=0, has_pock_chambolle_alpha=False),
+    "pc_alpha_half": dict(pock_chambolle_alpha=0.5),
+    "eval_freq_50": dict(termination_evaluation_frequency=50),
+    "reflection_0.8": dict(reflection_coefficient=0.8),
+    "tight_eps": dict(termination_criteria__eps_optimal_relative=1e-6,
+                      termination_criteria__eps_feasible_relative=1e-6),
+}
+
+
+@pytest.mark.parametrize("name", sorted(VARIANTS))
+def test_variant_parity(dp, orc, name):
+    kw = dict(VARIANTS[name], presolve=False)
+    prob, opt = dp.generate(800, 2000, 16000, row_rule=0, seed=11)
+    iters = [0, 10, 100, 300, 600]
+    res, o, (tx, ty) = dp.solve(prob, dp.default_params(**kw), trace_iters=iters)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(**kw), trace_iters=iters,
+                                 n=prob.n, m=prob.m)
+    assert res.status == ref["status"]
+    assert len(tx) == len(ox)
+    tol = 1e-7 if name == "pc_alpha_half" else 1e-8  # device pow vs libm pow: ulp level
+    for i in range(len(ox)):
+        assert rel(tx[i], ox[i]) < tol and rel(ty[i], oy[i]) < tol, (name, iters[i])
+    assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
+        1.0, abs(ref["primal_objective_value"]))
+    assert abs(res.iterations - ref["total_count"]) <= 10
+
+
+def test_warm_start_parity(dp, orc):
+    prob, opt = dp.generate(600, 1500, 12000, row_rule=0, seed=21)
+    rng = np.random.default_rng(0)
+    x0 = rng.uniform(0, 5, prob.n)
+    y0 = rng.standard_normal(prob.m)
+    prob.set_start(x0, y0)
+    iters = [0, 1, 50, 400]
+    res, o, (tx, ty) = dp.solve(prob, dp.default_params(presolve=False), trace_iters=iters)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(presolve=False), trace_iters=iters,
+                                 n=prob.n, m=prob.m)
+    assert res.status == ref["status"]
+    for i in range(len(ox)):
+        assert rel(tx[i], ox[i]) < 1e-8 and rel(ty[i], oy[i]) < 1e-8
+
+
+def test_feasibility_polish_parity(dp, orc):
+    prob, opt = dp.generate(700, 1800, 14000, row_rule=0, seed=31)
+    kw = dict(presolve=False, feasibility_polishing=True)
+    res = dp.solve_plain(prob, dp.default_params(**kw))
+    ref, _ = orc.solve(prob.ptr, orc.default_params(**kw))
+    assert res.status == ref["status"] == "OPTIMAL"
+    assert res.raw["feasibility_polishing_time"] > 0
+    assert abs(res.raw["feasibility_iteration"] - ref["feasibility_iteration"]) <= 20
+    assert res.rel_primal <= 1e-6 + 1e-12 and res.rel_dual <= 1e-6 + 1e-12
+    assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
+        1.0, abs(ref["primal_objective_value"]))
+    assert rel(res.x, ref["x"]) < 1e-5 and rel(res.y, ref["y"]) < 1e-5
+
+
+def test_result_vectors_match_oracle(dp, orc):
+    prob, opt = dp.generate(900, 2100, 17000, row_rule=2, seed=41, poisson_lambda=8.0)
+    params = dict(presolve=False, termination_criteria__iteration_limit=1000)
+    res = dp.solve_plain(prob, dp.default_params(**params))
+    ref, _ = orc.solve(prob.ptr, orc.default_params(**params))
+    assert res.status == ref["status"] == "ITERATION_LIMIT" or res.status == ref["status"]
+    assert rel(res.x, ref["x"]) < 1e-8 and rel(res.y, ref["y"]) < 1e-8
+    assert rel(res.reduced_cost, ref["rc"]) < 1e-8
+    for k in ("relative_primal_residual", "relative_dual_residual", "relative_objective_gap"):
+        assert abs(res.raw[k] - ref[k]) <= 1e-8 * max(1e-12, abs(ref[k])) + 1e-14
+
+
+def test_cli_end_to_end(dp, tmp_path):
+    mps = os.path.join(ROOT, "tests", "golden", "capi_example.mps")
+    r = subprocess.run([dp.CLI_PATH, "--no_presolve", mps, str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Solution Summary" in r.stdout and "OPTIMAL" in r.stdout
+    summ = (tmp_path / "capi_example_summary.txt").read_text().splitlines()
+    keys = [l.split(":")[0] for l in summ]
+    assert keys == ["Termination Reason", "Runtime (sec)", "Iterations Count",
+                    "Primal Objective Value", "Dual Objective Value", "Relative Primal Residual",
+                    "Relative Dual Residual", "Absolute Objective Gap", "Relative Objective Gap",
+                    "Rows", "Columns", "Nonzeros"]
+    assert summ[0] == "Termination Reason: OPTIMAL"
+    assert summ[9:] == ["Rows: 3", "Columns: 2", "Nonzeros: 5"]
+    x = [float(v) for v in (tmp_path / "capi_example_primal_solution.txt").read_text().split()]
+    y = [float(v) for v in (tmp_path / "capi_example_dual_solution.txt").read_text().split()]
+    assert np.allclose(x, [1, 2], atol=1e-3) and np.allclose(y, [1, -1, 0], atol=1e-3)
+    r = subprocess.run([dp.CLI_PATH, "-f", "--eps_feas_polish", "1e-7", mps, str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0

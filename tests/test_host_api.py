@@ -1,0 +1,265 @@
+"""Host-side boundary tests (CPU only): exported C ABI, struct layouts,
+create_lp_problem converters, MPS reader rules, CLI argument handling, and the
+SpMV block / column-panel plans.  No GPU compute is called."""
+import ctypes as C
+import gzip
+import json
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import random_csr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _decls(header):
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    txt = re.sub(r"//.*", "", txt)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", txt)
+    return [n for n in names if n not in ("if", "for", "while", "sizeof", "return")]
+
+
+def test_library_exports_every_declared_symbol(dp):
+    lib = C.CDLL(dp.LIB_PATH)
+    for h in ("cupdlpx.h", "mps_parser.h", "dpdlp_ext.h", "dpdlp_dev.h"):
+        for name in _decls(os.path.join(ROOT, "include", h)):
+            assert hasattr(lib, name), f"{name} ({h}) not exported"
+    for name in dp.EXPORTED_SYMBOLS:
+        assert hasattr(lib, name)
+
+
+def test_struct_sizes_match_reference_abi(dp):
+    assert C.sizeof(dp.LpProblem) == 104
+    assert C.sizeof(dp.PdhgParameters) == 192
+    assert C.sizeof(dp.CupdlpxResult) == 208
+    assert C.sizeof(dp.MatrixDesc) == 56
+    assert dp.PdhgParameters.termination_criteria.offset == 40
+    assert dp.PdhgParameters.restart_params.offset == 88
+    assert dp.PdhgParameters.permute_block_size.offset == 184
+    assert dp.CupdlpxResult.termination_reason.offset == 184
+    assert dp.CupdlpxResult.feasibility_iteration.offset == 200
+
+
+def test_default_parameters(dp):
+    p = dp.default_params()
+    assert p.l_inf_ruiz_iterations == 10 and p.has_pock_chambolle_alpha
+    assert p.pock_chambolle_alpha == 1.0 and p.bound_objective_rescaling
+    assert p.termination_evaluation_frequency == 200 and p.sv_max_iter == 5000
+    assert p.sv_tol == 1e-4 and p.reflection_coefficient == 1.0
+    tc = p.termination_criteria
+    assert (tc.eps_optimal_relative, tc.eps_feasible_relative, tc.eps_infeasible) == (1e-4, 1e-4, 1e-10)
+    assert tc.time_sec_limit == 3600.0 and tc.iteration_limit == 2**31 - 1
+    assert tc.eps_feas_polish_relative == 1e-6
+    rp = p.restart_params
+    assert (rp.artificial_restart_threshold, rp.sufficient_reduction_for_restart,
+            rp.necessary_reduction_for_restart) == (0.36, 0.2, 0.5)
+    assert (rp.k_p, rp.k_i, rp.k_d, rp.i_smooth) == (0.99, 0.01, 0.0, 0.3)
+    assert p.optimality_norm == dp.NORM_L2 and p.presolve and not p.feasibility_polishing
+
+
+def _csr_of(dense, tol=0.0):
+    m, n = dense.shape
+    rp, ci, v = [0], [], []
+    for i in range(m):
+        for j in range(n):
+            if abs(dense[i, j]) > tol:
+                ci.append(j)
+                v.append(dense[i, j])
+        rp.append(len(ci))
+    return np.array(rp), np.array(ci), np.array(v)
+
+
+def test_converters_agree(dp):
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((7, 9)) * (rng.random((7, 9)) < 0.4)
+    A[2, 3] = 1e-13  # below the dense default tolerance
+    m, n = A.shape
+    rp, ci, v = _csr_of(A, 1e-12)
+    ref = dict(row_ptr=rp, col_ind=ci, vals=v)
+    pd = dp.Problem.from_dense(A)
+    pr = dp.Problem.from_csr(m, n, rp, ci, v)
+    rows, cols = np.nonzero(np.abs(A) > 1e-12)
+    order = rng.permutation(len(rows))  # COO in arbitrary order
+    pc = dp.Problem.from_coo(m, n, rows[order], cols[order], A[rows, cols][order])
+    ccp, cri, cv = _csr_of(A.T.copy(), 1e-12)
+    pcs = dp.Problem.from_csc(m, n, ccp, cri, cv)
+    for p in (pd, pr, pcs):
+        a = p.arrays()
+        for k in ref:
+            assert np.array_equal(a[k], ref[k]), k
+    a = pc.arrays()  # COO keeps input order within rows: same rows, same sets
+    assert np.array_equal(a["row_ptr"], rp)
+    for i in range(m):
+        s = slice(rp[i], rp[i + 1])
+        assert sorted(zip(a["col_ind"][s], a["vals"][s])) == sorted(zip(ci[s], v[s]))
+
+
+def test_create_defaults_and_errors(dp):
+    A = np.array([[1.0, 0.0], [2.0, 3.0]])
+    p = dp.Problem.from_dense(A)
+    a = p.arrays()
+    assert np.all(a["c"] == 0) and np.all(a["var_lb"] == -np.inf) and np.all(a["var_ub"] == np.inf)
+    assert np.all(a["con_lb"] == -np.inf) and np.all(a["con_ub"] == np.inf)
+    assert a["obj_const"] == 0.0
+    with pytest.raises(ValueError):
+        dp.Problem.from_coo(2, 2, [0, 5], [0, 1], [1.0, 2.0])   # row out of range
+    with pytest.raises(ValueError):
+        dp.Problem.from_coo(2, 2, [0, 1], [0, 7], [1.0, 2.0])   # column out of range
+    with pytest.raises(ValueError):
+        dp.Problem.from_csc(2, 2, [0, 1, 2], [0, 9], [1.0, 2.0])
+    d = dp.MatrixDesc(m=1, n=1, fmt=7, zero_tolerance=0.0)
+    assert not dp.lib.create_lp_problem(None, C.byref(d), None, None, None, None, None)
+    # tolerance filters CSC / COO only when positive
+    p = dp.Problem.from_coo(1, 3, [0, 0, 0], [0, 1, 2], [1e-9, 0.5, 0.0], zero_tolerance=1e-6)
+    assert p.nnz == 1
+    p = dp.Problem.from_coo(1, 3, [0, 0, 0], [0, 1, 2], [1e-9, 0.5, 0.0])
+    assert p.nnz == 3
+
+
+def test_set_start_values_copies(dp):
+    p = dp.Problem.from_dense(np.eye(2))
+    x = np.array([1.0, 2.0])
+    p.set_start(primal=x)
+    x[0] = 99.0
+    got = np.ctypeslib.as_array(p.ptr.contents.primal_start, shape=(2,))
+    assert list(got) == [1.0, 2.0]
+    assert not p.ptr.contents.dual_start
+    p.set_start(dual=[3.0, 4.0])
+    assert not p.ptr.contents.primal_start
+
+
+def _expect(path):
+    e = json.load(open(path))
+    f = lambda xs: np.array([float(v) for v in xs])
+    return e, f
+
+
+@pytest.mark.parametrize("gz", [False, True])
+def test_mps_rules_golden(dp, tmp_path, gz):
+    src = os.path.join(GOLD, "rules.mps")
+    path = src
+    if gz:
+        path = str(tmp_path / "rules.mps.gz")
+        with open(src, "rb") as fi, gzip.open(path, "wb") as fo:
+            shutil.copyfileobj(fi, fo)
+    p = dp.Problem.from_mps(path)
+    a = p.arrays()
+    e, f = _expect(os.path.join(GOLD, "rules_expected.json"))
+    assert (a["m"], a["n"]) == (e["m"], e["n"])
+    assert list(a["row_ptr"]) == e["row_ptr"] and list(a["col_ind"]) == e["col_ind"]
+    assert np.array_equal(a["vals"], f(e["vals"])) and np.array_equal(a["c"], f(e["c"]))
+    assert a["obj_const"] == e["obj_const"]
+    for k in ("con_lb", "con_ub", "var_lb", "var_ub"):
+        assert np.array_equal(a[k], f(e[k])), k
+
+
+def test_mps_objsense_quirk_and_missing_file(dp):
+    p = dp.Problem.from_mps(os.path.join(GOLD, "objsense_quirk.mps"))
+    # a lone "MAX" line is taken as a section header (src/mps_parser.c:397-434)
+    assert list(p.arrays()["c"]) == [1.0]
+    with pytest.raises(ValueError):
+        dp.Problem.from_mps("/nonexistent/file.mps")
+
+
+def test_mps_capi_example(dp):
+    a = dp.Problem.from_mps(os.path.join(GOLD, "capi_example.mps")).arrays()
+    dense = np.zeros((3, 2))
+    for i in range(3):
+        dense[i, a["col_ind"][a["row_ptr"][i]:a["row_ptr"][i + 1]]] = a["vals"][a["row_ptr"][i]:a["row_ptr"][i + 1]]
+    assert np.array_equal(dense, [[1, 2], [0, 1], [3, 2]])
+    assert list(a["con_lb"]) == [5.0, -np.inf, -np.inf] and list(a["con_ub"]) == [5.0, 2.0, 8.0]
+    assert np.all(np.isinf(a["var_lb"])) and np.all(np.isinf(a["var_ub"]))
+
+
+def test_cli_arguments(dp, tmp_path):
+    cli = dp.CLI_PATH
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    assert subprocess.run([cli, "-h"], capture_output=True).returncode == 0
+    r = subprocess.run([cli, "--opt_norm", "l3", "a.mps", str(tmp_path)], capture_output=True,
+                       text=True)
+    assert r.returncode == 1 and "opt_norm" in r.stderr
+    r = subprocess.run([cli, str(tmp_path / "missing.mps"), str(tmp_path)], capture_output=True,
+                       text=True)
+    assert r.returncode == 1
+
+
+@pytest.mark.parametrize("cap", [1024, 2048, 4096])
+def test_block_plan_invariants(dp, cap):
+    rng = np.random.default_rng(1)
+    lens = np.minimum((rng.pareto(1.1, 4000) * 5).astype(int), 9000)
+    lens[::97] = 0
+    rp = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    blk = dp.plan_blocks(rp, cap)
+    assert blk[0] == 0 and blk[-1] == 4000 and np.all(np.diff(blk) >= 1)
+    for b in range(len(blk) - 1):
+        r0, r1 = blk[b], blk[b + 1]
+        nz = rp[r1] - rp[r0]
+        assert r1 - r0 <= 256
+        assert nz <= cap or r1 - r0 == 1
+
+
+@pytest.mark.parametrize("P", [2, 4, 8])
+def test_panel_tiles_reconstruct_spmv(dp, P):
+    L = dp.lib
+    rng = np.random.default_rng(P)
+    m, n = 1500, 2200
+    lens = np.minimum((rng.pareto(1.2, m) * 4).astype(int) + 1, 2100)
+    rp, ci, v = random_csr(m, n, lens, 5)
+    Pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    blk = np.zeros(m + 2, np.int32)
+    nb = L.dpdlp_plan_blocks_panels(m, n, Pi(rp), Pi(ci), 1024, P, 1024, Pi(blk))
+    blk = blk[:nb + 1]
+    toff = np.zeros(nb * P + 1, np.int32)
+    rofs = np.zeros(P * (m + nb), np.uint16)
+    perm = np.zeros(len(v), np.int32)
+    pci = np.zeros(len(v), np.int32)
+    assert L.dpdlp_build_panels(m, n, Pi(rp), Pi(ci), Pi(blk), nb, P, Pi(toff),
+                                rofs.ctypes.data_as(C.POINTER(C.c_ushort)), Pi(perm), Pi(pci)) == 0
+    assert sorted(perm) == list(range(len(v))) and np.array_equal(pci, ci[perm])
+    x = rng.standard_normal(n)
+    y = np.zeros(m)
+    W = -(-n // P)
+    for b in range(nb):
+        r0, r1 = blk[b], blk[b + 1]
+        R = r1 - r0
+        assert R <= 1024
+        for k in range(P):
+            t0, t1 = toff[b * P + k], toff[b * P + k + 1]
+            assert np.all(pci[t0:t1] // W == k)
+            if R == 1:
+                y[r0] += np.dot(v[perm[t0:t1]], x[pci[t0:t1]])
+                continue
+            assert t1 - t0 <= 1024
+            ro = rofs[P * (r0 + b) + k * (R + 1): P * (r0 + b) + k * (R + 1) + R + 1].astype(int)
+            for i in range(R):
+                s = slice(t0 + ro[i], t0 + ro[i + 1])
+                y[r0 + i] += np.dot(v[perm[s]], x[pci[s]])
+    ref = np.zeros(m)
+    np.add.at(ref, np.repeat(np.arange(m), np.diff(rp)), v * x[ci])
+    assert np.allclose(y, ref, rtol=1e-13, atol=1e-12)
+
+
+def test_generator_rules(dp):
+    p, opt = dp.generate(3000, 7000, 45000, row_rule=1, seed=5)
+    assert p.nnz == 45000
+    a = p.arrays()
+    lens = np.diff(a["row_ptr"])
+    assert lens.min() >= 1 and lens.max() > 10 * lens.mean()  # heavy tail
+    for i in range(0, 3000, 97):  # distinct sorted columns
+        s = a["col_ind"][a["row_ptr"][i]:a["row_ptr"][i + 1]]
+        assert np.all(np.diff(s) > 0)
+    q, _ = dp.generate(2000, 4000, 0, row_rule=2, seed=6, poisson_lambda=8.0)
+    ql = np.diff(q.arrays()["row_ptr"])
+    assert ql.min() >= 1 and abs(ql.mean() - 9.0) < 0.3
+    # feasible by construction: bounds consistent, c'x* finite
+    assert np.all(a["con_lb"] <= a["con_ub"]) and np.isfinite(opt)
+    p2, opt2 = dp.generate(3000, 7000, 45000, row_rule=1, seed=5)
+    assert opt2 == opt and np.array_equal(p2.arrays()["vals"], a["vals"])
