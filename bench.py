@@ -34,7 +34,40 @@ WORKLOADS = {
                  desc="m=1e6 n=2e6 nnz=2e7 Zipf(1.5) rows, FP64, eps 1e-4"),
     "cfg0": dict(m=2_000, n=5_000, nnz=50_000, row_rule=0, pm1_frac=0.0, seed=0,
                  desc="m=2e3 n=5e3 nnz=5e4 uniform positions, FP64, eps 1e-4"),
+    "cfg2": dict(m=4_000_000, n=8_000_000, nnz=120_000_000, row_rule=1, pm1_frac=0.0,
+                 seed=2000, desc="m=4e6 n=8e6 nnz=1.2e8 Zipf(1.5) rows"),
 }
+
+
+def precision_compare(R, dpdlp, opt, eps):
+    """Solve the same resident LP to eps with FP64 and with the mixed mode;
+    report time-to-1e-4 / time-to-eps and the objective error of each."""
+    out = {}
+    params = dpdlp.default_params(presolve=False,
+                                  termination_criteria__eps_optimal_relative=eps,
+                                  termination_criteria__eps_feasible_relative=eps)
+    sols = {}
+    for name, prec in (("fp64", dpdlp.PREC_FP64), ("mixed", dpdlp.PREC_MIXED)):
+        t = time.perf_counter()
+        r, o = R.solve(params, precision=prec, profile_phases=1)
+        wall = time.perf_counter() - t
+        sols[name] = r
+        out[name] = dict(status=r.status, iterations=r.iterations, fp32_iterations=o.iterations_fp32,
+                         wall_s=wall, loop_s=o.loop_time_sec, time_to_1e4_s=o.time_to_1e4_sec,
+                         time_to_1e6_s=o.time_to_1e6_sec, objective=r.primal_obj,
+                         rel_err_vs_known_opt=abs(r.primal_obj - opt) / max(1.0, abs(opt)),
+                         rel_kkt=[r.rel_primal, r.rel_dual, r.rel_gap],
+                         phase_ms=[o.primal_phase_ms, o.dual_phase_ms])
+        print(f"# {name}: {r.status} iters={r.iterations} (fp32 {o.iterations_fp32}) "
+              f"wall={wall:.2f}s t1e-4={o.time_to_1e4_sec:.2f}s t1e-6={o.time_to_1e6_sec:.2f}s",
+              file=sys.stderr, flush=True)
+    import numpy as np
+    a, b = sols["fp64"], sols["mixed"]
+    out["mixed_vs_fp64"] = dict(
+        objective_rel=abs(a.primal_obj - b.primal_obj) / max(1.0, abs(a.primal_obj)),
+        x_rel=float(np.linalg.norm(a.x - b.x) / max(np.linalg.norm(a.x), 1e-300)),
+        y_rel=float(np.linalg.norm(a.y - b.y) / max(np.linalg.norm(a.y), 1e-300)))
+    return out
 
 
 def spmv_bytes(rows, cols, nnz, sv=8, sx=8):
@@ -92,6 +125,8 @@ def main():
                     help="also run a 256 MiB plain copy kernel (counter calibration)")
     ap.add_argument("--mixed", action="store_true", help="FP32 matrix / FP64 state")
     ap.add_argument("--verbose", action="store_true", help="print the solver's iteration log")
+    ap.add_argument("--compare-precision", type=float, default=0.0, metavar="EPS",
+                    help="solve to EPS in FP64 and in mixed precision and report both")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this solver runs one LP on one GPU; multi-GPU runs are not "
@@ -118,6 +153,12 @@ def main():
     up_s = time.perf_counter() - t
     print(f"# uploaded in {up_s:.1f}s, row blocks A={R.nblk_A} A'={R.nblk_At} "
           f"panels={R.panels}", file=sys.stderr, flush=True)
+    if args.compare_precision > 0:
+        res = precision_compare(R, dpdlp, opt, args.compare_precision)
+        print(json.dumps({"workload": f"{args.workload}: {w['desc']}", "m": prob.m, "n": prob.n,
+                          "nnz": prob.nnz, "eps": args.compare_precision, "known_optimum": opt,
+                          "panels": R.panels, **res}))
+        return
     params = dpdlp.default_params(presolve=False, verbose=args.verbose)
     prec = dpdlp.PREC_MIXED if args.mixed else dpdlp.PREC_FP64
     for k in range(args.warmup):

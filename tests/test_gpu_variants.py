@@ -102,3 +102,21 @@ def test_cli_end_to_end(dp, tmp_path):
     r = subprocess.run([dp.CLI_PATH, "-f", "--eps_feas_polish", "1e-7", mps, str(tmp_path)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
+
+
+@pytest.mark.parametrize("panels", ["1,1", "8,4"])
+def test_mixed_precision_reaches_fp64_accuracy(dp, orc, panels, monkeypatch):
+    """FP32 matrix / gathers with FP64 state: the FP64 finish reaches the same
+    tolerance, and the solution matches the FP64 oracle to the tolerance."""
+    monkeypatch.setenv("DPDLP_PANELS", panels)
+    prob, opt = dp.generate(1500, 3500, 30000, row_rule=0, seed=51)
+    kw = dict(presolve=False, termination_criteria__eps_optimal_relative=1e-6,
+              termination_criteria__eps_feasible_relative=1e-6)
+    res, o, _ = dp.solve(prob, dp.default_params(**kw), precision=dp.PREC_MIXED)
+    ref, _ = orc.solve(prob.ptr, orc.default_params(**kw))
+    assert res.status == ref["status"] == "OPTIMAL"
+    assert o.iterations_fp32 > 0
+    assert res.rel_primal < 1e-6 and res.rel_dual < 1e-6 and res.rel_gap < 1e-6
+    assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-5 * max(
+        1.0, abs(ref["primal_objective_value"]))
+    assert rel(res.x, ref["x"]) < 1e-3
