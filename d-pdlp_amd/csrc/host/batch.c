@@ -1,0 +1,819 @@
+/*
+ * Batched solving: many LPs advance together through the single-LP algorithm
+ * of solver.c (the reference's optimize(), src/solver.cu:32-127), each with
+ * its own step size, primal weight, restart state and termination state.
+ *
+ * Time is cut into windows of W = gcd(10, termination_evaluation_frequency)
+ * iterations.  Every evaluation point of the reference loop (T % f(T) == 0
+ * with f(T) = 10, 100, ..., and the major points T % eval_freq == 0) is a
+ * multiple of W, and an LP always starts its loop at a window boundary, so
+ * all evaluations fall on window starts:
+ *
+ *   window start  one residual (+ infeasibility) launch for every LP that
+ *                 evaluates, one 2-D device-to-host copy of every LP's scalar
+ *                 slots (and power-iteration history), one synchronisation;
+ *                 the host then runs the reference's per-LP decisions
+ *                 (termination, restart, PID weight) from those values;
+ *   W rounds      per round at most: restart copy, primal phase, dual phase,
+ *                 fixed-point-error finish, power-iteration step, each ONE
+ *                 launch over the row blocks of every LP in flight, driven by
+ *                 a per-LP control record written for the whole window.
+ *
+ * Finished LPs leave at the window start where they terminate (result copied
+ * out, state freed); new LPs join there too, so slots are backfilled.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "control.h"
+#include "cupdlpx.h"
+#include "dpdlp_dev.h"
+#include "dpdlp_ext.h"
+#include "dpdlp_host.h"
+#include "solver_internal.h"
+
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define BCHK(call)                                                             \
+  do {                                                                         \
+    hipError_t _e = (hipError_t)(call);                                        \
+    if (_e != hipSuccess) {                                                    \
+      dpdlp_set_error("HIP error %d (%s) at %s:%d", (int)_e,                   \
+                      hipGetErrorString(_e), __FILE__, __LINE__);              \
+      return -1;                                                               \
+    }                                                                          \
+  } while (0)
+
+enum { S_FREE = 0, S_PI, S_PDHG };
+enum { NLIST = 5 };
+
+typedef struct {
+  int stage;
+  long long tag;
+  dpdlp_resident_t *R;
+  int own_R;
+  ctx_t c;
+  hstate_t h;
+  double t_add, t_pi0, t_loop0, rescale_time, power_time;
+  int pi_iters, pi_issued;
+  double sigma_sq;
+  int do_restart;
+  int ev_on, ev_rc, ev_inf, ev_final, fresh;
+} bslot_t;
+
+typedef struct {
+  long long tag;
+  cupdlpx_result_t *res;
+  double latency;
+} bdone_t;
+
+struct dpdlp_batch {
+  pdhg_parameters_t prm;
+  dpdlp_batch_opts_t o;
+  int W, freq, rbw;
+  hipStream_t s;
+  int cap;
+  bslot_t *sl;
+  int nhw, in_flight, dirty;
+  dpdlp_blp_t *d_lps, *h_lps;
+  dpdlp_bctl_t *d_ctl_eval, *h_ctl_eval;
+  dpdlp_bctl_t *d_ctl_win, *h_ctl_win[2];
+  hipEvent_t ev_win[2];
+  int win_buf;
+  double *d_scal, *h_scal, *h_rb;
+  dpdlp_bblk_t *d_list, *h_list;
+  long long list_cap;
+  int *d_rng, *h_rng;
+  double *d_part;
+  long long part_len;
+  double *d_normals;
+  int normals_len;
+  dpdlp_batch_dev_t bd;
+  bdone_t *done;
+  int ndone, done_head, done_cap;
+  long long windows, syncs, launches;
+  size_t state_bytes;
+};
+
+void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o) {
+  memset(o, 0, sizeof(*o));
+  o->max_lps = 1024;
+  o->precision = DPDLP_PREC_FP64;
+  o->mixed_switch_tol = 1e-4 * 0.1;
+  o->spmv_cap = 1024;
+}
+
+static int gcd(int a, int b) {
+  while (b) {
+    int t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+void dpdlp_batch_free(dpdlp_batch_t *b) {
+  if (!b) return;
+  if (b->s) hipStreamSynchronize(b->s);
+  for (int i = 0; i < b->cap && b->sl; ++i) {
+    bslot_t *q = &b->sl[i];
+    if (q->stage == S_FREE) continue;
+    dpdlp_i_ctx_free(&q->c);
+    if (q->own_R) dpdlp_resident_free(q->R);
+  }
+  for (int i = b->done_head; i < b->ndone; ++i) cupdlpx_result_free(b->done[i].res);
+  free(b->done);
+  free(b->sl);
+  void *dev[] = {b->d_lps, b->d_ctl_eval, b->d_ctl_win, b->d_scal,
+                 b->d_list, b->d_rng,     b->d_part,    b->d_normals};
+  for (size_t k = 0; k < sizeof(dev) / sizeof(dev[0]); ++k)
+    if (dev[k]) hipFree(dev[k]);
+  void *host[] = {b->h_lps,  b->h_ctl_eval, b->h_ctl_win[0], b->h_ctl_win[1],
+                  b->h_scal, b->h_rb,       b->h_list,       b->h_rng};
+  for (size_t k = 0; k < sizeof(host) / sizeof(host[0]); ++k)
+    if (host[k]) hipHostFree(host[k]);
+  for (int k = 0; k < 2; ++k)
+    if (b->ev_win[k]) hipEventDestroy(b->ev_win[k]);
+  if (b->s) hipStreamDestroy(b->s);
+  free(b);
+}
+
+#define CCHK(call)                                                             \
+  do {                                                                         \
+    hipError_t _e = (hipError_t)(call);                                        \
+    if (_e != hipSuccess) {                                                    \
+      dpdlp_set_error("HIP error %d (%s) creating a batch", (int)_e,           \
+                      hipGetErrorString(_e));                                  \
+      dpdlp_batch_free(b);                                                     \
+      return NULL;                                                             \
+    }                                                                          \
+  } while (0)
+
+dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
+                                  const dpdlp_batch_opts_t *o) {
+  dpdlp_batch_t *b = (dpdlp_batch_t *)calloc(1, sizeof(*b));
+  if (!b) return NULL;
+  if (params)
+    b->prm = *params;
+  else
+    set_default_parameters(&b->prm);
+  if (o)
+    b->o = *o;
+  else
+    dpdlp_batch_default_opts(&b->o);
+  if (b->o.max_lps <= 0) b->o.max_lps = 1024;
+  if (b->o.spmv_cap != 1024 && b->o.spmv_cap != 2048 && b->o.spmv_cap != 4096)
+    b->o.spmv_cap = 1024;
+  if (b->prm.termination_evaluation_frequency <= 0) {
+    dpdlp_set_error("termination_evaluation_frequency must be positive");
+    free(b);
+    return NULL;
+  }
+  b->freq = b->prm.termination_evaluation_frequency;
+  b->W = gcd(10, b->freq);
+  b->rbw = DPDLP_SC_PI_HIST + 4 * b->W;
+  b->cap = b->o.max_lps;
+  b->sl = (bslot_t *)calloc((size_t)b->cap, sizeof(bslot_t));
+  b->done_cap = 64;
+  b->done = (bdone_t *)malloc((size_t)b->done_cap * sizeof(bdone_t));
+  if (!b->sl || !b->done) {
+    dpdlp_batch_free(b);
+    return NULL;
+  }
+  size_t C = (size_t)b->cap;
+  CCHK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+  CCHK(hipMalloc((void **)&b->d_lps, C * sizeof(dpdlp_blp_t)));
+  CCHK(hipHostMalloc((void **)&b->h_lps, C * sizeof(dpdlp_blp_t), 0));
+  CCHK(hipMalloc((void **)&b->d_ctl_eval, C * sizeof(dpdlp_bctl_t)));
+  CCHK(hipHostMalloc((void **)&b->h_ctl_eval, C * sizeof(dpdlp_bctl_t), 0));
+  CCHK(hipMalloc((void **)&b->d_ctl_win, C * b->W * sizeof(dpdlp_bctl_t)));
+  for (int k = 0; k < 2; ++k) {
+    CCHK(hipHostMalloc((void **)&b->h_ctl_win[k], C * b->W * sizeof(dpdlp_bctl_t), 0));
+    CCHK(hipEventCreateWithFlags(&b->ev_win[k], hipEventDisableTiming));
+  }
+  CCHK(hipMalloc((void **)&b->d_scal, C * DPDLP_SCAL_LEN * sizeof(double)));
+  CCHK(hipMemset(b->d_scal, 0, C * DPDLP_SCAL_LEN * sizeof(double)));
+  CCHK(hipHostMalloc((void **)&b->h_scal, C * DPDLP_SCAL_LEN * sizeof(double), 0));
+  CCHK(hipHostMalloc((void **)&b->h_rb, C * b->rbw * sizeof(double), 0));
+  CCHK(hipMalloc((void **)&b->d_rng, C * 2 * NLIST * sizeof(int)));
+  CCHK(hipHostMalloc((void **)&b->h_rng, C * 2 * NLIST * sizeof(int), 0));
+  memset(b->h_lps, 0, C * sizeof(dpdlp_blp_t));
+  b->state_bytes = C * (sizeof(dpdlp_blp_t) + sizeof(dpdlp_bctl_t) * (1 + b->W) +
+                        DPDLP_SCAL_LEN * sizeof(double) + 2 * NLIST * sizeof(int));
+  return b;
+}
+
+int dpdlp_batch_in_flight(const dpdlp_batch_t *b) { return b ? b->in_flight : 0; }
+
+size_t dpdlp_batch_bytes(const dpdlp_batch_t *b) {
+  if (!b) return 0;
+  size_t s = b->state_bytes + (size_t)b->list_cap * sizeof(dpdlp_bblk_t) +
+             (size_t)b->part_len * sizeof(double) + (size_t)b->normals_len * sizeof(double);
+  for (int i = 0; i < b->nhw; ++i)
+    if (b->sl[i].stage != S_FREE) s += b->sl[i].c.arena_len;
+  return s;
+}
+
+void dpdlp_batch_counters(const dpdlp_batch_t *b, long long *windows, long long *syncs,
+                          long long *launches) {
+  if (windows) *windows = b ? b->windows : 0;
+  if (syncs) *syncs = b ? b->syncs : 0;
+  if (launches) *launches = b ? b->launches : 0;
+}
+
+/* the power-iteration start vector: draws of std::mt19937(1) + normal
+ * distribution; a solve of m rows uses the first m (utils.cu:80-92) */
+static int ensure_normals(dpdlp_batch_t *b, int m) {
+  if (m <= b->normals_len) return 0;
+  int len = m > 2 * b->normals_len ? m : 2 * b->normals_len;
+  double *h = (double *)malloc((size_t)len * sizeof(double));
+  if (!h) return -1;
+  dpdlp_normal_fill(1u, len, h);
+  BCHK(hipStreamSynchronize(b->s));
+  if (b->d_normals) hipFree(b->d_normals);
+  b->d_normals = NULL;
+  b->normals_len = 0;
+  if (hipMalloc((void **)&b->d_normals, (size_t)len * sizeof(double)) != hipSuccess ||
+      hipMemcpy(b->d_normals, h, (size_t)len * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    free(h);
+    dpdlp_set_error("device allocation failed (power-iteration start vector)");
+    return -1;
+  }
+  free(h);
+  b->normals_len = len;
+  return 0;
+}
+
+static void fill_blp(dpdlp_blp_t *L, const ctx_t *c) {
+  const dpdlp_state_t *st = &c->st;
+  memset(L, 0, sizeof(*L));
+  L->m = st->m;
+  L->n = st->n;
+  L->A_rp = st->A.rp;
+  L->A_ci = st->A.ci;
+  L->At_rp = st->At.rp;
+  L->At_ci = st->At.ci;
+  L->A_v = st->A.v;
+  L->At_v = st->At.v;
+  L->A_v32 = st->A.v32;
+  L->At_v32 = st->At.v32;
+  L->lv = st->lv;
+  L->uv = st->uv;
+  L->c = st->c;
+  L->lc = st->lc;
+  L->uc = st->uc;
+  L->D_r = st->D_r;
+  L->D_c = st->D_c;
+  L->x_init = st->x_init;
+  L->x_cur = st->x_cur;
+  L->x_pdhg = st->x_pdhg;
+  L->x_ref = st->x_ref;
+  L->dual_product = st->dual_product;
+  L->dual_slack = st->dual_slack;
+  L->dx = st->dx;
+  L->y_init = st->y_init;
+  L->y_cur = st->y_cur;
+  L->y_pdhg = st->y_pdhg;
+  L->y_ref = st->y_ref;
+  L->primal_product = st->primal_product;
+  L->primal_slack = st->primal_slack;
+  L->dy = st->dy;
+  L->x_ref32 = st->x_ref32;
+  L->y_cur32 = st->y_cur32;
+  L->pi_eig = c->pi_eig;
+  L->pi_next = c->pi_next;
+  L->pi_dual = c->pi_dual;
+  L->scal = st->scal;
+}
+
+static void start_loop(dpdlp_batch_t *b, bslot_t *q) {
+  double step = q->R->nnz == 0 ? 1.0 : 0.998 / sqrt(q->sigma_sq);
+  dpdlp_i_init_hstate(&q->h, &q->c, &b->prm, step);
+  q->power_time = dpdlp_wall() - q->t_pi0;
+  q->t_loop0 = dpdlp_wall();
+  q->stage = S_PDHG;
+  q->do_restart = 0;
+  q->fresh = 1;
+}
+
+int dpdlp_batch_add(dpdlp_batch_t *b, dpdlp_resident_t *R, int take_ownership, long long tag) {
+  if (!b || !R) {
+    dpdlp_set_error("NULL batch or problem");
+    return -1;
+  }
+  if (R->cap != b->o.spmv_cap || R->A_P != 1 || R->At_P != 1) {
+    dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
+                    b->o.spmv_cap);
+    return -1;
+  }
+  int i = 0;
+  while (i < b->cap && b->sl[i].stage != S_FREE) ++i;
+  if (i == b->cap) {
+    dpdlp_set_error("batch full (%d LPs)", b->cap);
+    return -1;
+  }
+  bslot_t *q = &b->sl[i];
+  memset(q, 0, sizeof(*q));
+  q->t_add = dpdlp_wall();
+  q->tag = tag;
+  q->R = R;
+  if (dpdlp_i_ctx_setup(&q->c, R, &b->prm, b->o.precision == DPDLP_PREC_MIXED, b->s,
+                        b->d_scal + (size_t)i * DPDLP_SCAL_LEN,
+                        b->h_scal + (size_t)i * DPDLP_SCAL_LEN, &q->rescale_time)) {
+    dpdlp_i_ctx_free(&q->c);
+    memset(q, 0, sizeof(*q));
+    return -1;
+  }
+  fill_blp(&b->h_lps[i], &q->c);
+  q->t_pi0 = dpdlp_wall();
+  q->sigma_sq = 1.0;
+  if (R->nnz > 0 && b->prm.sv_max_iter > 0) {
+    if (ensure_normals(b, R->m)) goto fail;
+    if (hipMemcpyAsync(q->c.pi_eig, b->d_normals, (size_t)R->m * sizeof(double),
+                       hipMemcpyDeviceToDevice, b->s) != hipSuccess ||
+        dpdlp_dev_power_init(&q->c.st, q->c.pi_eig, q->c.pi_next, b->s)) {
+      dpdlp_set_error("power-iteration start failed on the device");
+      goto fail;
+    }
+    q->stage = S_PI;
+  } else {
+    start_loop(b, q);
+    q->fresh = 0; /* its first evaluation comes with the next window */
+  }
+  q->own_R = take_ownership;
+  if (i + 1 > b->nhw) b->nhw = i + 1;
+  b->in_flight++;
+  b->dirty = 1;
+  return 0;
+fail:
+  dpdlp_i_ctx_free(&q->c);
+  memset(q, 0, sizeof(*q));
+  return -1;
+}
+
+/* ---- block lists ---- */
+
+static int rebuild_lists(dpdlp_batch_t *b) {
+  long long cnt[NLIST] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < b->nhw; ++i) {
+    bslot_t *q = &b->sl[i];
+    if (q->stage == S_FREE) continue;
+    const dpdlp_resident_t *R = q->R;
+    cnt[0] += R->A_nblk;
+    cnt[1] += R->At_nblk;
+    cnt[2] += dpdlp_ew_grid(R->n);
+    cnt[3] += dpdlp_ew_grid(R->m);
+    cnt[4] += dpdlp_ew_grid((long long)R->n + R->m);
+  }
+  long long total = cnt[0] + cnt[1] + cnt[2] + cnt[3] + cnt[4];
+  long long need_stride = 1;
+  for (int k = 0; k < NLIST; ++k)
+    if (cnt[k] > need_stride) need_stride = cnt[k];
+  if (need_stride > 0x7fffffffLL) {
+    dpdlp_set_error("batch too large (%lld work items in one list)", need_stride);
+    return -1;
+  }
+  /* pinned buffers and the part array are rewritten: drain the stream */
+  BCHK(hipStreamSynchronize(b->s));
+  b->syncs++;
+  if (total > b->list_cap) {
+    long long nc = total + total / 4 + 1024;
+    if (b->d_list) hipFree(b->d_list);
+    if (b->h_list) hipHostFree(b->h_list);
+    b->d_list = NULL;
+    b->h_list = NULL;
+    b->list_cap = 0;
+    BCHK(hipMalloc((void **)&b->d_list, (size_t)nc * sizeof(dpdlp_bblk_t)));
+    BCHK(hipHostMalloc((void **)&b->h_list, (size_t)nc * sizeof(dpdlp_bblk_t), 0));
+    b->list_cap = nc;
+  }
+  long long plen = (long long)DPDLP_SLOTS * need_stride;
+  if (plen > b->part_len) {
+    long long nl = plen + plen / 4;
+    if (b->d_part) hipFree(b->d_part);
+    b->d_part = NULL;
+    b->part_len = 0;
+    BCHK(hipMalloc((void **)&b->d_part, (size_t)nl * sizeof(double)));
+    b->part_len = nl;
+  }
+  long long off[NLIST];
+  off[0] = 0;
+  for (int k = 1; k < NLIST; ++k) off[k] = off[k - 1] + cnt[k - 1];
+  long long pos[NLIST];
+  for (int k = 0; k < NLIST; ++k) pos[k] = 0;
+  int *rng = b->h_rng;
+  memset(rng, 0, (size_t)b->cap * 2 * NLIST * sizeof(int));
+  for (int i = 0; i < b->nhw; ++i) {
+    bslot_t *q = &b->sl[i];
+    if (q->stage == S_FREE) continue;
+    const dpdlp_resident_t *R = q->R;
+    for (int k = 0; k < NLIST; ++k) rng[(size_t)k * 2 * b->cap + 2 * i] = (int)pos[k];
+    dpdlp_bblk_t *L = b->h_list;
+    for (int j = 0; j < R->A_nblk; ++j) {
+      dpdlp_bblk_t *e = &L[off[0] + pos[0]++];
+      e->lp = i;
+      e->a = R->h_A_blk[j];
+      e->b = R->h_A_blk[j + 1];
+      e->pad = 0;
+    }
+    for (int j = 0; j < R->At_nblk; ++j) {
+      dpdlp_bblk_t *e = &L[off[1] + pos[1]++];
+      e->lp = i;
+      e->a = R->h_At_blk[j];
+      e->b = R->h_At_blk[j + 1];
+      e->pad = 0;
+    }
+    long long cnts[3] = {R->n, R->m, (long long)R->n + R->m};
+    for (int k = 0; k < 3; ++k) {
+      int G = dpdlp_ew_grid(cnts[k]);
+      for (int j = 0; j < G; ++j) {
+        dpdlp_bblk_t *e = &L[off[2 + k] + pos[2 + k]++];
+        e->lp = i;
+        e->a = j;
+        e->b = G;
+        e->pad = 0;
+      }
+    }
+    for (int k = 0; k < NLIST; ++k) rng[(size_t)k * 2 * b->cap + 2 * i + 1] = (int)pos[k];
+  }
+  if (total > 0)
+    BCHK(hipMemcpyAsync(b->d_list, b->h_list, (size_t)total * sizeof(dpdlp_bblk_t),
+                        hipMemcpyHostToDevice, b->s));
+  BCHK(hipMemcpyAsync(b->d_rng, b->h_rng, (size_t)b->cap * 2 * NLIST * sizeof(int),
+                      hipMemcpyHostToDevice, b->s));
+  BCHK(hipMemcpyAsync(b->d_lps, b->h_lps, (size_t)b->nhw * sizeof(dpdlp_blp_t),
+                      hipMemcpyHostToDevice, b->s));
+  dpdlp_batch_dev_t *d = &b->bd;
+  d->lps = b->d_lps;
+  d->bA = b->d_list + off[0];
+  d->bAt = b->d_list + off[1];
+  d->eN = b->d_list + off[2];
+  d->eM = b->d_list + off[3];
+  d->eNM = b->d_list + off[4];
+  d->nbA = (int)cnt[0];
+  d->nbAt = (int)cnt[1];
+  d->neN = (int)cnt[2];
+  d->neM = (int)cnt[3];
+  d->neNM = (int)cnt[4];
+  d->rA = b->d_rng;
+  d->rAt = b->d_rng + 2 * b->cap;
+  d->rN = b->d_rng + 4 * b->cap;
+  d->rM = b->d_rng + 6 * b->cap;
+  d->rNM = b->d_rng + 8 * b->cap;
+  d->part = b->d_part;
+  d->stride = (int)need_stride;
+  d->cap = b->o.spmv_cap;
+  b->dirty = 0;
+  return 0;
+}
+
+/* ---- evaluation points ---- */
+
+/* marks what slot q evaluates at the top of its iteration T = h.total */
+static void mark_eval(dpdlp_batch_t *b, bslot_t *q) {
+  hstate_t *h = &q->h;
+  const termination_criteria_t *tc = &b->prm.termination_criteria;
+  q->ev_on = q->ev_rc = q->ev_inf = q->ev_final = 0;
+  if (h->total >= tc->iteration_limit) {
+    q->ev_on = 1;
+    q->ev_final = 1;
+    return;
+  }
+  int rc = h->is_major || h->total == 0;
+  if (rc || h->total % dpdlp_print_frequency(h->total) == 0) {
+    q->ev_on = 1;
+    q->ev_rc = rc;
+    q->ev_inf = h->is_major && h->total < 3 * b->freq;
+  }
+}
+
+/* launches the residual (and infeasibility) kernels of the marked slots and
+ * reads every slot's scalar slots and power-iteration history back: one
+ * synchronisation */
+static int eval_and_read(dpdlp_batch_t *b, int any_eval, int any_inf) {
+  if (b->dirty && rebuild_lists(b)) return -1;
+  if (any_eval) {
+    memset(b->h_ctl_eval, 0, (size_t)b->nhw * sizeof(dpdlp_bctl_t));
+    for (int i = 0; i < b->nhw; ++i) {
+      bslot_t *q = &b->sl[i];
+      if (q->stage != S_PDHG || !q->ev_on) continue;
+      dpdlp_bctl_t *c = &b->h_ctl_eval[i];
+      c->eval = 1;
+      c->restart_dist = q->ev_rc && !q->ev_final;
+      c->inf = q->ev_inf;
+    }
+    BCHK(hipMemcpyAsync(b->d_ctl_eval, b->h_ctl_eval, (size_t)b->nhw * sizeof(dpdlp_bctl_t),
+                        hipMemcpyHostToDevice, b->s));
+    b->bd.ctl = b->d_ctl_eval;
+    b->bd.nlp = b->nhw;
+    if (dpdlp_bdev_residual(&b->bd, b->s)) return -1;
+    b->launches += 4;
+    if (any_inf) {
+      if (dpdlp_bdev_infeasibility(&b->bd, b->s)) return -1;
+      b->launches += 7;
+    }
+  }
+  BCHK(hipMemcpy2DAsync(b->h_rb, (size_t)b->rbw * sizeof(double), b->d_scal,
+                        (size_t)DPDLP_SCAL_LEN * sizeof(double), (size_t)b->rbw * sizeof(double),
+                        (size_t)b->nhw, hipMemcpyDeviceToHost, b->s));
+  BCHK(hipStreamSynchronize(b->s));
+  b->syncs++;
+  return 0;
+}
+
+static int push_done(dpdlp_batch_t *b, long long tag, cupdlpx_result_t *res, double lat) {
+  if (b->done_head > 0 && b->done_head == b->ndone) b->done_head = b->ndone = 0;
+  if (b->ndone == b->done_cap) {
+    int nc = b->done_cap * 2;
+    bdone_t *d = (bdone_t *)realloc(b->done, (size_t)nc * sizeof(bdone_t));
+    if (!d) return -1;
+    b->done = d;
+    b->done_cap = nc;
+  }
+  b->done[b->ndone].tag = tag;
+  b->done[b->ndone].res = res;
+  b->done[b->ndone].latency = lat;
+  b->ndone++;
+  return 0;
+}
+
+/* an LP terminated: polish (as the single-LP path does), copy its result out,
+ * free its slot */
+static int finish(dpdlp_batch_t *b, int i) {
+  bslot_t *q = &b->sl[i];
+  hstate_t *h = &q->h;
+  double polish_time = 0.0;
+  int polish_iters = 0;
+  if (b->prm.feasibility_polishing && h->reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
+      h->reason != TERMINATION_REASON_PRIMAL_INFEASIBLE) {
+    if (dpdlp_i_feasibility_polish(&q->c, h, &b->prm, &polish_time, &polish_iters)) {
+      dpdlp_set_error("feasibility polishing failed on the device");
+      return -1;
+    }
+  }
+  cupdlpx_result_t *res = dpdlp_i_ctx_result(&q->c, h, q->rescale_time, polish_time,
+                                             polish_iters);
+  if (!res) return -1;
+  double lat = dpdlp_wall() - q->t_add;
+  if (push_done(b, q->tag, res, lat)) {
+    cupdlpx_result_free(res);
+    return -1;
+  }
+  dpdlp_i_ctx_free(&q->c);
+  if (q->own_R) dpdlp_resident_free(q->R);
+  memset(q, 0, sizeof(*q));
+  memset(&b->h_lps[i], 0, sizeof(dpdlp_blp_t));
+  b->in_flight--;
+  b->dirty = 1;
+  while (b->nhw > 0 && b->sl[b->nhw - 1].stage == S_FREE) b->nhw--;
+  return 0;
+}
+
+/* the reference loop's decisions at an evaluation point (solver.cu:60-84) */
+static int process_eval(dpdlp_batch_t *b, int i) {
+  bslot_t *q = &b->sl[i];
+  hstate_t *h = &q->h;
+  const double *S = b->h_rb + (size_t)i * b->rbw;
+  const pdhg_parameters_t *p = &b->prm;
+  q->ev_on = 0;
+  ctl_pending_fpe(h, S, 0);
+  ctl_residual(h, &q->c.k, S, p->optimality_norm);
+  if (q->ev_final) {
+    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
+    return finish(b, i);
+  }
+  if (q->ev_inf) ctl_infeasibility(h, &q->c.k, S);
+  h->t_cum = dpdlp_wall() - q->t_loop0;
+  double kkt = fmax(h->rel_p, fmax(h->rel_d, h->rel_gap));
+  if (q->c.use32 && kkt < b->o.mixed_switch_tol) q->c.use32 = 0;
+  ctl_check_termination(h, &p->termination_criteria);
+  if (h->reason != TERMINATION_REASON_UNSPECIFIED) return finish(b, i);
+  if (q->ev_rc) {
+    q->do_restart = ctl_should_restart(h, &p->restart_params, b->freq);
+    if (q->do_restart) {
+      ctl_pid_update(h, &p->restart_params, dpdlp_blue(S + DPDLP_SL_RESD + 5),
+                     dpdlp_blue(S + DPDLP_SL_RESP + 4));
+      h->inner = 0;
+      h->fpe_last = INFINITY;
+    }
+  }
+  return 0;
+}
+
+/* power-iteration history of the last window (utils.cu:66-171) */
+static void process_pi(dpdlp_batch_t *b, int i) {
+  bslot_t *q = &b->sl[i];
+  const double *S = b->h_rb + (size_t)i * b->rbw;
+  int done = 0;
+  for (int k = 0; k < q->pi_issued; ++k) {
+    const double *hh = S + DPDLP_SC_PI_HIST + 4 * k;
+    q->sigma_sq = hh[0];
+    q->pi_iters++;
+    if (dpdlp_blue(hh + 1) < b->prm.sv_tol) {
+      done = 1;
+      break;
+    }
+  }
+  q->pi_issued = 0;
+  if (done || q->pi_iters >= b->prm.sv_max_iter) start_loop(b, q);
+}
+
+/* ---- one window ---- */
+
+static int launch_window(dpdlp_batch_t *b) {
+  const int W = b->W, nl = b->nhw;
+  const pdhg_parameters_t *p = &b->prm;
+  const int limit = p->termination_criteria.iteration_limit;
+  int k = b->win_buf;
+  BCHK(hipEventSynchronize(b->ev_win[k]));
+  dpdlp_bctl_t *ctl = b->h_ctl_win[k];
+  memset(ctl, 0, (size_t)W * nl * sizeof(dpdlp_bctl_t));
+  int any_phase[10] = {0}, any_fpe[10] = {0}, any_pi[10] = {0}, any_restart = 0;
+  for (int i = 0; i < nl; ++i) {
+    bslot_t *q = &b->sl[i];
+    if (q->stage == S_PDHG) {
+      hstate_t *h = &q->h;
+      for (int r = 0; r < W && h->total < limit; ++r) {
+        dpdlp_bctl_t *c = &ctl[(size_t)r * nl + i];
+        int is_major = ((h->total + 1) % b->freq) == 0;
+        int dr = (r == 0) && q->do_restart;
+        int major_variant =
+            is_major || ((h->total + 2) % dpdlp_print_frequency(h->total + 2)) == 0;
+        int fpe = is_major || dr;
+        c->tau = h->step / h->omega;
+        c->sigma = h->step * h->omega;
+        c->w = (double)(h->inner + 1) / (h->inner + 2);
+        c->rho = p->reflection_coefficient;
+        c->phase = 1;
+        c->major = major_variant;
+        c->fpe = fpe;
+        c->fpe_to = (dr ? 1 : 0) | (is_major ? 2 : 0);
+        c->restart = dr ? 3 : 0;
+        c->use32 = q->c.use32;
+        if (fpe) {
+          if (dr) h->pend_rst = 1;
+          if (is_major) h->pend_maj = 1;
+        }
+        h->is_major = is_major;
+        h->inner++;
+        h->total++;
+        any_phase[r] = 1;
+        any_fpe[r] |= fpe;
+        any_restart |= dr;
+      }
+      q->do_restart = 0;
+    } else if (q->stage == S_PI) {
+      int left = p->sv_max_iter - q->pi_iters;
+      int kk = left < W ? left : W;
+      for (int r = 0; r < kk; ++r) {
+        dpdlp_bctl_t *c = &ctl[(size_t)r * nl + i];
+        c->pi = 1;
+        c->pi_slot = r;
+        any_pi[r] = 1;
+      }
+      q->pi_issued = kk;
+    }
+  }
+  BCHK(hipMemcpyAsync(b->d_ctl_win, ctl, (size_t)W * nl * sizeof(dpdlp_bctl_t),
+                      hipMemcpyHostToDevice, b->s));
+  BCHK(hipEventRecord(b->ev_win[k], b->s));
+  b->win_buf ^= 1;
+  for (int r = 0; r < W; ++r) {
+    b->bd.ctl = b->d_ctl_win + (size_t)r * nl;
+    b->bd.nlp = nl;
+    if (r == 0 && any_restart) {
+      if (dpdlp_bdev_restart_copy(&b->bd, b->s)) return -1;
+      b->launches += 1;
+    }
+    if (any_phase[r]) {
+      if (dpdlp_bdev_phases(&b->bd, b->s)) return -1;
+      b->launches += 2;
+    }
+    if (any_fpe[r]) {
+      if (dpdlp_bdev_fpe(&b->bd, b->s)) return -1;
+      b->launches += 5;
+    }
+    if (any_pi[r]) {
+      if (dpdlp_bdev_power_step(&b->bd, b->s)) return -1;
+      b->launches += 5;
+    }
+  }
+  b->windows++;
+  return 0;
+}
+
+int dpdlp_batch_step(dpdlp_batch_t *b) {
+  if (!b) return -1;
+  if (b->in_flight == 0) return 0;
+  /* evaluation points and power-iteration reads at this window start */
+  int any_eval = 0, any_inf = 0, any_read = 0;
+  for (int i = 0; i < b->nhw; ++i) {
+    bslot_t *q = &b->sl[i];
+    if (q->stage == S_PDHG) {
+      mark_eval(b, q);
+      any_eval |= q->ev_on;
+      any_inf |= q->ev_inf;
+    } else if (q->stage == S_PI && q->pi_issued > 0) {
+      any_read = 1;
+    }
+  }
+  if (any_eval || any_read) {
+    if (eval_and_read(b, any_eval, any_inf)) return -1;
+    int any_fresh = 0;
+    for (int i = 0; i < b->nhw; ++i) {
+      bslot_t *q = &b->sl[i];
+      if (q->stage == S_PI && q->pi_issued > 0) {
+        process_pi(b, i);
+        any_fresh |= q->fresh;
+      }
+    }
+    for (int i = 0; i < b->nhw; ++i) {
+      bslot_t *q = &b->sl[i];
+      if (q->stage == S_PDHG && q->ev_on && !q->fresh)
+        if (process_eval(b, i)) return -1;
+    }
+    if (any_fresh) {
+      /* LPs whose step size was just fixed evaluate their iteration 0 now */
+      any_inf = 0;
+      for (int i = 0; i < b->nhw; ++i) {
+        bslot_t *q = &b->sl[i];
+        if (q->stage == S_PDHG) {
+          if (q->fresh)
+            mark_eval(b, q);
+          else
+            q->ev_on = 0;
+          any_inf |= q->ev_on && q->ev_inf;
+        }
+      }
+      if (eval_and_read(b, 1, any_inf)) return -1;
+      for (int i = 0; i < b->nhw; ++i) {
+        bslot_t *q = &b->sl[i];
+        if (q->stage == S_PDHG && q->fresh) {
+          q->fresh = 0;
+          if (q->ev_on && process_eval(b, i)) return -1;
+        }
+      }
+    }
+  }
+  if (b->in_flight == 0) return 0;
+  if (b->dirty && rebuild_lists(b)) return -1;
+  if (launch_window(b)) return -1;
+  return b->in_flight;
+}
+
+int dpdlp_batch_take(dpdlp_batch_t *b, long long *tag, cupdlpx_result_t **res,
+                     double *latency_sec) {
+  if (!b || b->done_head >= b->ndone) return 0;
+  bdone_t *d = &b->done[b->done_head++];
+  if (tag) *tag = d->tag;
+  if (res)
+    *res = d->res;
+  else
+    cupdlpx_result_free(d->res);
+  if (latency_sec) *latency_sec = d->latency;
+  return 1;
+}
+
+int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parameters_t *params,
+                      const dpdlp_batch_opts_t *o, cupdlpx_result_t **results) {
+  if (count < 0 || (count > 0 && (!res || !results))) {
+    dpdlp_set_error("invalid batch arguments");
+    return -1;
+  }
+  dpdlp_batch_opts_t local;
+  if (o)
+    local = *o;
+  else
+    dpdlp_batch_default_opts(&local);
+  if (local.max_lps < count) local.max_lps = count;
+  for (int i = 0; i < count; ++i) results[i] = NULL;
+  dpdlp_batch_t *b = dpdlp_batch_create(params, &local);
+  if (!b) return -1;
+  int got = 0;
+  for (int i = 0; i < count; ++i)
+    if (dpdlp_batch_add(b, res[i], 0, i)) goto fail;
+  while (dpdlp_batch_in_flight(b) > 0)
+    if (dpdlp_batch_step(b) < 0) goto fail;
+  long long tag;
+  cupdlpx_result_t *r;
+  while (dpdlp_batch_take(b, &tag, &r, NULL)) {
+    if (tag >= 0 && tag < count && !results[tag]) {
+      results[tag] = r;
+      got++;
+    } else {
+      cupdlpx_result_free(r);
+    }
+  }
+  dpdlp_batch_free(b);
+  return got;
+fail:
+  dpdlp_batch_free(b);
+  for (int i = 0; i < count; ++i) {
+    cupdlpx_result_free(results[i]);
+    results[i] = NULL;
+  }
+  return -1;
+}

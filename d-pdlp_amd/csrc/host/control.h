@@ -1,0 +1,45 @@
+/*
+ * The reference's per-LP scalar decisions, computed on the host from the raw
+ * device reduction slots: residuals and objectives (utils.cu:639-719),
+ * infeasibility rays (utils.cu:721-813), fixed-point error
+ * (pdlp_core_op.cu:658-696), termination (utils.cu:214-263), the restart test
+ * (utils.cu:265-293) and the PID primal weight (pdlp_core_op.cu:593-619).
+ * Shared by the single-LP driver and the batched driver.
+ */
+#pragma once
+
+#include "cupdlpx_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  double step, omega, best_omega, pw_err_sum, pw_last_err, best_gap;
+  int total, inner, is_major;
+  double fpe, fpe0, fpe_last;
+  double abs_p, rel_p, abs_d, rel_d, pobj, dobj, gap, rel_gap;
+  double max_p_ray_inf, max_d_ray_inf, p_ray_lin_obj, d_ray_obj;
+  termination_reason_t reason;
+  double t_cum;
+  int pend_rst, pend_maj;
+} hstate_t;
+
+
+/* per-LP constants: scaling factors, objective constant, unscaled norms */
+typedef struct {
+  double beta, gamma, c0, obj_norm, bound_norm;
+} lpk_t;
+
+double dpdlp_blue(const double *three_sums);
+void ctl_residual(hstate_t *h, const lpk_t *k, const double *S, norm_type_t norm);
+void ctl_infeasibility(hstate_t *h, const lpk_t *k, const double *S);
+void ctl_pending_fpe(hstate_t *h, const double *S, int polish);
+void ctl_check_termination(hstate_t *h, const termination_criteria_t *tc);
+void ctl_check_polish_termination(hstate_t *h, const termination_criteria_t *tc, int primal);
+int ctl_should_restart(hstate_t *h, const restart_parameters_t *rp, int freq);
+void ctl_pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, double dd);
+
+#ifdef __cplusplus
+}
+#endif

@@ -25,6 +25,7 @@ struct dpdlp_resident {
   double *c, *lv, *uv, *lc, *uc;
   double c0, obj_l2, obj_linf, bnd_l2, bnd_linf;
   double *h_pstart, *h_dstart;
+  int *h_A_blk, *h_At_blk; /* host copies of the row-block plans */
   double upload_time;
 };
 

@@ -17,72 +17,14 @@
 #include "dpdlp_dev.h"
 #include "dpdlp_ext.h"
 #include "dpdlp_host.h"
+#include "control.h"
+#include "solver_internal.h"
 
 #include <hip/hip_runtime_api.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
-
-/* blue_combine of dev_common.h, host copy */
-#define BLUE_SSML 0x1p+537
-#define BLUE_SBIG 0x1p-538
-static double blue(const double *a) {
-  double asml = a[0], amed = a[1], abig = a[2];
-  double scl, sumsq;
-  if (abig > 0.0) {
-    if (amed > 0.0 || isnan(amed)) abig += (amed * BLUE_SBIG) * BLUE_SBIG;
-    scl = 1.0 / BLUE_SBIG;
-    sumsq = abig;
-  } else if (asml > 0.0) {
-    if (amed > 0.0 || isnan(amed)) {
-      double ymed = sqrt(amed), ysml = sqrt(asml) / BLUE_SSML;
-      double ymin = ysml > ymed ? ymed : ysml, ymax = ysml > ymed ? ysml : ymed;
-      double r = ymin / ymax;
-      scl = 1.0;
-      sumsq = ymax * ymax * (1.0 + r * r);
-    } else {
-      scl = 1.0 / BLUE_SSML;
-      sumsq = asml;
-    }
-  } else {
-    scl = 1.0;
-    sumsq = amed;
-  }
-  return scl * sqrt(sumsq);
-}
-
-/* ------------------------------------------------------------------------ */
-
-
-typedef struct {
-  double step, omega, best_omega, pw_err_sum, pw_last_err, best_gap;
-  int total, inner, is_major;
-  double fpe, fpe0, fpe_last;
-  double abs_p, rel_p, abs_d, rel_d, pobj, dobj, gap, rel_gap;
-  double max_p_ray_inf, max_d_ray_inf, p_ray_lin_obj, d_ray_obj;
-  termination_reason_t reason;
-  double t_cum;
-  int pend_rst, pend_maj;
-} hstate_t;
-
-#define MAX_ALLOCS 128
-typedef struct {
-  hipStream_t s;
-  dpdlp_state_t st;
-  double *h_scal;
-  double beta, gamma, c0, obj_norm, bound_norm;
-  void *allocs[MAX_ALLOCS];
-  int nalloc;
-  int mixed, use32;
-  double *w_A_v, *w_At_v;
-  float *w_A_v32, *w_At_v32;
-  /* live phase timing */
-  hipEvent_t ev[64][3];
-  int nev_created, nev_used;
-  double p1_ms, p2_ms;
-  int nsamp;
-} ctx_t;
 
 #define HIPCHK(call)                                                           \
   do {                                                                         \
@@ -96,17 +38,25 @@ typedef struct {
 
 static void *dmalloc(ctx_t *c, size_t bytes) {
   void *p = NULL;
+  size_t need = (bytes ? bytes : 8);
+  need = (need + 255) & ~(size_t)255;
+  if (c->arena && c->arena_off + need <= c->arena_len) {
+    p = c->arena + c->arena_off;
+    c->arena_off += need;
+    return p;
+  }
   if (c->nalloc >= MAX_ALLOCS) return NULL;
-  if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return NULL;
+  if (hipMalloc(&p, need) != hipSuccess) return NULL;
   c->allocs[c->nalloc++] = p;
   return p;
 }
 
+/* zero-filled: arena memory is cleared once when the arena is made */
 static double *dzero(ctx_t *c, long long count) {
-  double *p = (double *)dmalloc(c, (size_t)(count > 0 ? count : 1) * sizeof(double));
-  if (p && hipMemsetAsync(p, 0, (size_t)(count > 0 ? count : 1) * sizeof(double), c->s) !=
-               hipSuccess)
-    return NULL;
+  size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(double);
+  int from_arena = c->arena && c->arena_off + ((bytes + 255) & ~(size_t)255) <= c->arena_len;
+  double *p = (double *)dmalloc(c, bytes);
+  if (p && !from_arena && hipMemsetAsync(p, 0, bytes, c->s) != hipSuccess) return NULL;
   return p;
 }
 
@@ -132,16 +82,19 @@ static void harvest_events(ctx_t *c) {
   c->nev_used = 0;
 }
 
-static void ctx_free(ctx_t *c) {
+void dpdlp_i_ctx_free(ctx_t *c) {
   if (c->s) hipStreamSynchronize(c->s);
   for (int k = 0; k < c->nev_created; ++k)
     for (int q = 0; q < 3; ++q) hipEventDestroy(c->ev[k][q]);
   c->nev_created = 0;
   for (int i = 0; i < c->nalloc; ++i) hipFree(c->allocs[i]);
   c->nalloc = 0;
-  if (c->h_scal) hipHostFree(c->h_scal);
+  if (c->arena) hipFree(c->arena);
+  c->arena = NULL;
+  c->arena_off = c->arena_len = 0;
+  if (c->h_scal && c->owns_scal) hipHostFree(c->h_scal);
   c->h_scal = NULL;
-  if (c->s) hipStreamDestroy(c->s);
+  if (c->s && c->owns_stream) hipStreamDestroy(c->s);
   c->s = NULL;
 }
 
@@ -210,6 +163,8 @@ void dpdlp_resident_free(dpdlp_resident_t *r) {
     if (ptrs[i]) hipFree(ptrs[i]);
   free(r->h_pstart);
   free(r->h_dstart);
+  free(r->h_A_blk);
+  free(r->h_At_blk);
   free(r);
 }
 
@@ -234,6 +189,26 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
 
 dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
   return upload_with(p, spmv_cap, 0, 0);
+}
+
+dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panels_A,
+                                  int panels_At) {
+  if ((panels_A && panels_A != 1 && panels_A != 2 && panels_A != 4 && panels_A != 8) ||
+      (panels_At && panels_At != 1 && panels_At != 2 && panels_At != 4 && panels_At != 8)) {
+    dpdlp_set_error("panel counts must be 0 (automatic), 1, 2, 4 or 8");
+    return NULL;
+  }
+  return upload_with(p, spmv_cap, panels_A, panels_At);
+}
+
+size_t dpdlp_resident_bytes(const dpdlp_resident_t *r) {
+  if (!r) return 0;
+  size_t nnz = (size_t)r->nnz, m = (size_t)r->m, n = (size_t)r->n;
+  size_t b = nnz * 24 + (m + n + 2) * 4 + (size_t)(r->A_nblk + r->At_nblk + 2) * 4 +
+             (3 * n + 2 * m) * 8;
+  if (r->A_P > 1) b += nnz * 16 + (size_t)r->A_P * (m + (size_t)r->A_nblk) * 2;
+  if (r->At_P > 1) b += nnz * 16 + (size_t)r->At_P * (n + (size_t)r->At_nblk) * 2;
+  return b;
 }
 
 /* Times both SpMVs of a resident problem (mean of `reps`, ms). */
@@ -382,10 +357,16 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
                                                     DPDLP_PANEL_RMAX, blk)
                          : dpdlp_plan_blocks(m, rp, r->cap, blk);
   UPCHK(up((void **)&r->A_blk, blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
+  r->h_A_blk = (int *)malloc((size_t)(r->A_nblk + 1) * sizeof(int));
+  if (!r->h_A_blk) goto ufail;
+  memcpy(r->h_A_blk, blk, (size_t)(r->A_nblk + 1) * sizeof(int));
   r->At_nblk = r->At_P > 1 ? dpdlp_plan_blocks_panels(n, m, t_rp, t_ci, r->cap, r->At_P,
                                                       DPDLP_PANEL_RMAX, blk)
                            : dpdlp_plan_blocks(n, t_rp, r->cap, blk);
   UPCHK(up((void **)&r->At_blk, blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
+  r->h_At_blk = (int *)malloc((size_t)(r->At_nblk + 1) * sizeof(int));
+  if (!r->h_At_blk) goto ufail;
+  memcpy(r->h_At_blk, blk, (size_t)(r->At_nblk + 1) * sizeof(int));
 
   for (int which = 0; which < 2; ++which) {
     int P = which ? r->At_P : r->A_P;
@@ -453,155 +434,6 @@ ufail:
 }
 
 /* ------------------------------------------------------------------------ */
-/* Host halves of the reference's scalar logic                               */
-/* ------------------------------------------------------------------------ */
-
-/* utils.cu:665-718 from the raw slots */
-static void host_residual(ctx_t *c, hstate_t *h, norm_type_t norm) {
-  const double *S = c->h_scal;
-  h->abs_p = (norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESP + 7] : blue(S + DPDLP_SL_RESP)) / c->beta;
-  h->abs_d = (norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESD + 8] : blue(S + DPDLP_SL_RESD)) / c->gamma;
-  h->pobj = S[DPDLP_SL_RESD + 3] / (c->beta * c->gamma) + c->c0;
-  h->dobj = (S[DPDLP_SL_RESD + 4] + S[DPDLP_SL_RESP + 3]) / (c->beta * c->gamma) + c->c0;
-  h->rel_p = h->abs_p / (1.0 + c->bound_norm);
-  h->rel_d = h->abs_d / (1.0 + c->obj_norm);
-  h->gap = fabs(h->pobj - h->dobj);
-  h->rel_gap = h->gap / (1.0 + fabs(h->pobj) + fabs(h->dobj));
-}
-
-/* utils.cu:760-812 from the raw slots */
-static void host_infeasibility(ctx_t *c, hstate_t *h) {
-  const double *S = c->h_scal + DPDLP_SL_INF;
-  double d_inf = S[1];
-  h->p_ray_lin_obj = S[5] / (c->beta * c->gamma);
-  h->d_ray_obj = (S[2] + S[4]) / (c->beta * c->gamma);
-  h->max_p_ray_inf = S[3];
-  double ds = S[6];
-  h->max_d_ray_inf = ds;
-  double sf = fmax(d_inf, ds);
-  if (sf > 0.0) {
-    h->max_d_ray_inf /= sf;
-    h->d_ray_obj /= sf;
-  } else {
-    h->max_d_ray_inf = 0.0;
-    h->d_ray_obj = 0.0;
-  }
-}
-
-/* pdlp_core_op.cu:687-695 */
-static double host_fpe(const double *G, double omega, double step) {
-  double pn = blue(G + 0), dn = blue(G + 3);
-  double movement = pn * pn * omega + dn * dn / omega;
-  double interaction = 2 * step * G[6];
-  return sqrt(movement + interaction);
-}
-
-static void process_pending_fpe(ctx_t *c, hstate_t *h, int polish) {
-  const double *S = c->h_scal;
-  if (h->pend_rst) {
-    const double *G = S + DPDLP_SC_FPE_RESTART;
-    double v;
-    if (polish == 1) {
-      double pn = blue(G);
-      v = pn * pn * h->omega;
-    } else if (polish == 2) {
-      double dn = blue(G + 3);
-      v = dn * dn / h->omega;
-    } else {
-      v = host_fpe(G, h->omega, h->step);
-    }
-    h->fpe = v;
-    h->fpe0 = v;
-    h->pend_rst = 0;
-  }
-  if (h->pend_maj) {
-    const double *G = S + DPDLP_SC_FPE_MAJOR;
-    if (polish == 1) {
-      double pn = blue(G);
-      h->fpe = pn * pn * h->omega;
-    } else if (polish == 2) {
-      double dn = blue(G + 3);
-      h->fpe = dn * dn / h->omega;
-    } else {
-      h->fpe = host_fpe(G, h->omega, h->step);
-    }
-    h->pend_maj = 0;
-  }
-}
-
-/* utils.cu:214-263 */
-static void check_termination(hstate_t *h, const termination_criteria_t *tc) {
-  if (h->rel_d < tc->eps_feasible_relative && h->rel_p < tc->eps_feasible_relative &&
-      h->rel_gap < tc->eps_optimal_relative) {
-    h->reason = TERMINATION_REASON_OPTIMAL;
-    return;
-  }
-  if (h->d_ray_obj > 0.0 && h->max_d_ray_inf / h->d_ray_obj <= tc->eps_infeasible) {
-    h->reason = TERMINATION_REASON_PRIMAL_INFEASIBLE;
-    return;
-  }
-  if (h->p_ray_lin_obj < 0.0 && h->max_p_ray_inf / (-h->p_ray_lin_obj) <= tc->eps_infeasible) {
-    h->reason = TERMINATION_REASON_DUAL_INFEASIBLE;
-    return;
-  }
-  if (h->total >= tc->iteration_limit) {
-    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
-    return;
-  }
-  if (h->t_cum >= tc->time_sec_limit) h->reason = TERMINATION_REASON_TIME_LIMIT;
-}
-
-/* utils.cu:1015-1039 */
-static void check_polish_termination(hstate_t *h, const termination_criteria_t *tc, int primal) {
-  if (primal ? (h->rel_p <= tc->eps_feas_polish_relative)
-             : (h->rel_d <= tc->eps_feas_polish_relative)) {
-    h->reason = TERMINATION_REASON_FEAS_POLISH_SUCCESS;
-    return;
-  }
-  if (h->total >= tc->iteration_limit) {
-    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
-    return;
-  }
-  if (h->t_cum >= tc->time_sec_limit) h->reason = TERMINATION_REASON_TIME_LIMIT;
-}
-
-/* utils.cu:265-293 */
-static int should_restart(hstate_t *h, const restart_parameters_t *rp, int freq) {
-  int r = 0;
-  if (h->total == freq) {
-    r = 1;
-  } else if (h->total > freq) {
-    if (h->fpe <= rp->sufficient_reduction_for_restart * h->fpe0) r = 1;
-    if (h->fpe <= rp->necessary_reduction_for_restart * h->fpe0 && h->fpe > h->fpe_last) r = 1;
-    if (h->inner >= rp->artificial_restart_threshold * h->total) r = 1;
-  }
-  h->fpe_last = h->fpe;
-  return r;
-}
-
-/* PID primal weight of pdlp_core_op.cu:593-619 */
-static void pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, double dd) {
-  double ratio = h->rel_d / h->rel_p;
-  if (pd > 1e-16 && dd > 1e-16 && pd < 1e12 && dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
-    double err = log(dd) - log(pd) - log(h->omega);
-    h->pw_err_sum *= rp->i_smooth;
-    h->pw_err_sum += err;
-    double derr = err - h->pw_last_err;
-    h->omega *= exp(rp->k_p * err + rp->k_i * h->pw_err_sum + rp->k_d * derr);
-    h->pw_last_err = err;
-  } else {
-    h->omega = h->best_omega;
-    h->pw_err_sum = 0.0;
-    h->pw_last_err = 0.0;
-  }
-  double g = fabs(log10(h->rel_d / h->rel_p));
-  if (g < h->best_gap) {
-    h->best_gap = g;
-    h->best_omega = h->omega;
-  }
-}
-
-/* ------------------------------------------------------------------------ */
 /* One PDHG step (solver.cu:85-101)                                          */
 /* ------------------------------------------------------------------------ */
 
@@ -660,29 +492,29 @@ static int polish_loop(ctx_t *c, dpdlp_state_t *ps, hstate_t *h, const pdhg_para
     if (h->is_major || h->total == 0 || h->total % dpdlp_print_frequency(h->total) == 0) {
       if (dpdlp_dev_residual(ps, &c->st, primal ? 1 : 2, 0, c->s)) return -1;
       if (sync_scal(c, 64)) return -1;
-      process_pending_fpe(c, h, primal ? 1 : 2);
+      ctl_pending_fpe(h, c->h_scal, primal ? 1 : 2);
       const double *S = c->h_scal;
       if (primal) {
         h->abs_p = (params->optimality_norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESP + 7]
-                                                               : blue(S + DPDLP_SL_RESP)) /
-                   c->beta;
-        h->rel_p = h->abs_p / (1.0 + c->bound_norm);
-        h->pobj = S[DPDLP_SL_RESD + 3] / (c->beta * c->gamma) + c0_state;
+                                                               : dpdlp_blue(S + DPDLP_SL_RESP)) /
+                   c->k.beta;
+        h->rel_p = h->abs_p / (1.0 + c->k.bound_norm);
+        h->pobj = S[DPDLP_SL_RESD + 3] / (c->k.beta * c->k.gamma) + c0_state;
       } else {
         h->abs_d = (params->optimality_norm == NORM_TYPE_L_INF ? S[DPDLP_SL_RESD + 8]
-                                                               : blue(S + DPDLP_SL_RESD)) /
-                   c->gamma;
-        h->rel_d = h->abs_d / (1.0 + c->obj_norm);
-        h->dobj = (S[DPDLP_SL_RESD + 4] + S[DPDLP_SL_RESP + 3]) / (c->beta * c->gamma) + c0_state;
+                                                               : dpdlp_blue(S + DPDLP_SL_RESD)) /
+                   c->k.gamma;
+        h->rel_d = h->abs_d / (1.0 + c->k.obj_norm);
+        h->dobj = (S[DPDLP_SL_RESD + 4] + S[DPDLP_SL_RESP + 3]) / (c->k.beta * c->k.gamma) + c0_state;
       }
       h->t_cum = dpdlp_wall() - t0;
-      check_polish_termination(h, tc, primal);
+      ctl_check_polish_termination(h, tc, primal);
       if (params->verbose)
         dpdlp_print_polish_iteration(primal, h->total, h->t_cum, primal ? h->pobj : h->dobj,
                                      primal ? h->abs_p : h->abs_d, primal ? h->rel_p : h->rel_d);
     }
     if (h->is_major || h->total == 0) {
-      do_restart = should_restart(h, &params->restart_params, freq);
+      do_restart = ctl_should_restart(h, &params->restart_params, freq);
       if (do_restart) {
         if (dpdlp_dev_restart_copy(ps, primal ? 1 : 2, c->s)) return -1;
         h->inner = 0;
@@ -699,7 +531,7 @@ static int polish_loop(ctx_t *c, dpdlp_state_t *ps, hstate_t *h, const pdhg_para
   return 0;
 }
 
-static int feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *params,
+int dpdlp_i_feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *params,
                               double *ptime, int *piters) {
   double t0 = dpdlp_wall();
   const termination_criteria_t *tc = &params->termination_criteria;
@@ -709,7 +541,7 @@ static int feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *pa
   }
   double omega0 = params->bound_objective_rescaling
                       ? 1.0
-                      : (c->obj_norm + 1.0) / (c->bound_norm + 1.0);
+                      : (c->k.obj_norm + 1.0) / (c->k.bound_norm + 1.0);
   dpdlp_state_t *o = &c->st;
   int n = o->n, m = o->m;
   dpdlp_state_t P = *o, D = *o;
@@ -779,7 +611,7 @@ static int feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *pa
       !D.dx || !D.dy)
     return -1;
   polish_reset(&hd, h, omega0);
-  if (polish_loop(c, &D, &hd, params, 0, c->c0)) return -1;
+  if (polish_loop(c, &D, &hd, params, 0, c->k.c0)) return -1;
   if (hd.reason == TERMINATION_REASON_FEAS_POLISH_SUCCESS) {
     if (hipMemcpyAsync(o->y_pdhg, D.y_pdhg, (size_t)m * sizeof(double), hipMemcpyDeviceToDevice,
                        c->s) != hipSuccess)
@@ -803,46 +635,56 @@ static int feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *pa
 /* Solve                                                                     */
 /* ------------------------------------------------------------------------ */
 
-cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameters_t *params,
-                                       dpdlp_solve_opts_t *opts) {
-  double t_begin = dpdlp_wall();
-  ctx_t C;
-  memset(&C, 0, sizeof(C));
-  ctx_t *c = &C;
-  cupdlpx_result_t *res = NULL;
-  double *h_q = NULL, *d_start = NULL;
-  dpdlp_solve_opts_t local_opts;
-  if (!opts) {
-    dpdlp_default_opts(&local_opts);
-    opts = &local_opts;
-  }
-  opts->time_to_1e4_sec = -1.0;
-  opts->time_to_1e6_sec = -1.0;
-  opts->trace_recorded = 0;
-  opts->iterations_fp32 = 0;
-  if (!R || !params) {
-    dpdlp_set_error("NULL problem or parameters");
-    return NULL;
-  }
-  if (params->termination_evaluation_frequency <= 0) {
-    dpdlp_set_error("termination_evaluation_frequency must be positive");
-    return NULL;
-  }
+static size_t piece(long long count, size_t elem) {
+  size_t b = (size_t)(count > 0 ? count : 1) * elem;
+  return (b + 255) & ~(size_t)255;
+}
+
+size_t dpdlp_i_ctx_footprint(const dpdlp_resident_t *R, int mixed, int own_scal) {
+  long long m = R->m, n = R->n, nnz = R->nnz;
+  long long stride = R->A_nblk > R->At_nblk ? R->A_nblk : R->At_nblk;
+  if (stride < 2048) stride = 2048;
+  long long xl = dpdlp_resident_xpart_len(R);
+  size_t b = 2 * piece(nnz, 8);
+  if (xl > 0) b += piece(xl, 8);
+  if (R->A_P > 1) b += piece(nnz, 8) + (mixed ? piece(nnz, 4) : 0);
+  if (R->At_P > 1) b += piece(nnz, 8) + (mixed ? piece(nnz, 4) : 0);
+  b += 13 * piece(n, 8) + 12 * piece(m, 8); /* vectors, scaling, work */
+  b += piece((long long)DPDLP_SLOTS * stride, 8);
+  if (own_scal) b += piece(DPDLP_SCAL_LEN, 8);
+  if (mixed) b += 2 * piece(nnz, 4) + piece(n, 4) + piece(m, 4);
+  return b;
+}
+
+int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params, int mixed,
+                      hipStream_t stream, double *scal_dev, double *scal_host,
+                      double *rescale_time) {
+  double *d_start = NULL;
   int m = R->m, n = R->n, nnz = R->nnz;
   dpdlp_state_t *st = &c->st;
-  hstate_t H;
-  memset(&H, 0, sizeof(H));
-  hstate_t *h = &H;
-  c->mixed = (opts->precision == DPDLP_PREC_MIXED);
-
-  HIPCHK(hipStreamCreate(&c->s));
-  if (opts->profile_phases) {
-    for (int k = 0; k < 64; ++k) {
-      for (int q = 0; q < 3; ++q) HIPCHK(hipEventCreate(&c->ev[k][q]));
-      c->nev_created = k + 1;
-    }
+  c->mixed = mixed;
+  if (stream) {
+    c->s = stream;
+  } else {
+    HIPCHK(hipStreamCreate(&c->s));
+    c->owns_stream = 1;
   }
-  HIPCHK(hipHostMalloc((void **)&c->h_scal, DPDLP_SCAL_LEN * sizeof(double), 0));
+  if (scal_host) {
+    c->h_scal = scal_host;
+  } else {
+    HIPCHK(hipHostMalloc((void **)&c->h_scal, DPDLP_SCAL_LEN * sizeof(double), 0));
+    c->owns_scal = 1;
+  }
+  /* one device arena for the LP's state, cleared once */
+  c->arena_len = dpdlp_i_ctx_footprint(R, mixed, scal_dev == NULL);
+  if (hipMalloc((void **)&c->arena, c->arena_len) != hipSuccess) {
+    dpdlp_set_error("device allocation failed (%zu bytes of LP state)", c->arena_len);
+    c->arena = NULL;
+    c->arena_len = 0;
+    goto fail;
+  }
+  c->arena_off = 0;
+  HIPCHK(hipMemsetAsync(c->arena, 0, c->arena_len, c->s));
 
   /* ---- state (pdlp_core_op.cu:42-391) ---- */
   st->m = m;
@@ -892,9 +734,14 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
   if (R->At_nblk > st->part_stride) st->part_stride = R->At_nblk;
   if (st->part_stride < 2048) st->part_stride = 2048;
   st->part = dzero(c, (long long)DPDLP_SLOTS * st->part_stride);
-  st->scal = dzero(c, DPDLP_SCAL_LEN);
+  st->scal = scal_dev ? scal_dev : dzero(c, DPDLP_SCAL_LEN);
   double *rs = dzero(c, m), *cs = dzero(c, n);
   double *pi_eig = dzero(c, m), *pi_next = dzero(c, m), *pi_dual = dzero(c, n);
+  c->rs = rs;
+  c->cs = cs;
+  c->pi_eig = pi_eig;
+  c->pi_next = pi_next;
+  c->pi_dual = pi_dual;
   if (!c->w_A_v || !c->w_At_v || !st->c || !st->lv || !st->uv || !st->lc || !st->uc ||
       !st->D_r || !st->D_c || !st->x_init || !st->x_cur || !st->x_pdhg || !st->x_ref ||
       !st->dual_product || !st->dual_slack || !st->dx || !st->y_init || !st->y_cur ||
@@ -903,9 +750,9 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     dpdlp_set_error("device allocation failed");
     goto fail;
   }
-  c->c0 = R->c0;
-  c->obj_norm = params->optimality_norm == NORM_TYPE_L_INF ? R->obj_linf : R->obj_l2;
-  c->bound_norm = params->optimality_norm == NORM_TYPE_L_INF ? R->bnd_linf : R->bnd_l2;
+  c->k.c0 = R->c0;
+  c->k.obj_norm = params->optimality_norm == NORM_TYPE_L_INF ? R->obj_linf : R->obj_l2;
+  c->k.bound_norm = params->optimality_norm == NORM_TYPE_L_INF ? R->bnd_linf : R->bnd_l2;
 
   /* ---- scaling on device (preconditioner.c:197-273) ---- */
   double t_sc0 = dpdlp_wall();
@@ -921,26 +768,26 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
       dpdlp_set_error("device sync failed");
       goto fail;
     }
-    c->beta = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL]) + 1.0);
-    c->gamma = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL + 1]) + 1.0);
-    HIPCHK(dpdlp_dev_bound_obj_apply(st, c->beta, c->gamma, c->s));
+    c->k.beta = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL]) + 1.0);
+    c->k.gamma = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL + 1]) + 1.0);
+    HIPCHK(dpdlp_dev_bound_obj_apply(st, c->k.beta, c->k.gamma, c->s));
   } else {
-    c->beta = 1.0;
-    c->gamma = 1.0;
+    c->k.beta = 1.0;
+    c->k.gamma = 1.0;
   }
   /* scaled values in tile order */
   if (st->A.P > 1) HIPCHK(dpdlp_dev_permute(c->w_A_v, R->A_perm, (double *)st->A.pv, nnz, c->s));
   if (st->At.P > 1)
     HIPCHK(dpdlp_dev_permute(c->w_At_v, R->At_perm, (double *)st->At.pv, nnz, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
-  double rescale_time = dpdlp_wall() - t_sc0;
+  *rescale_time = dpdlp_wall() - t_sc0;
 
   /* warm start, pdlp_core_op.cu:174-201 */
   if (R->h_pstart && n > 0) {
     d_start = NULL;
     HIPCHK(hipMalloc((void **)&d_start, (size_t)n * sizeof(double)));
     HIPCHK(hipMemcpy(d_start, R->h_pstart, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(dpdlp_dev_warm_start(st->x_init, st->x_cur, st->x_pdhg, d_start, st->D_c, c->beta, n,
+    HIPCHK(dpdlp_dev_warm_start(st->x_init, st->x_cur, st->x_pdhg, d_start, st->D_c, c->k.beta, n,
                                 c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     hipFree(d_start);
@@ -949,7 +796,7 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
   if (R->h_dstart && m > 0) {
     HIPCHK(hipMalloc((void **)&d_start, (size_t)m * sizeof(double)));
     HIPCHK(hipMemcpy(d_start, R->h_dstart, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(dpdlp_dev_warm_start(st->y_init, st->y_cur, st->y_pdhg, d_start, st->D_r, c->gamma, m,
+    HIPCHK(dpdlp_dev_warm_start(st->y_init, st->y_cur, st->y_pdhg, d_start, st->D_r, c->k.gamma, m,
                                 c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     hipFree(d_start);
@@ -986,138 +833,38 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     c->use32 = 1;
   }
 
-  /* ---- step size: power iteration (pdlp_core_op.cu:638-656) ---- */
-  double t_pi0 = dpdlp_wall();
-  int pi_iters = 0;
-  if (nnz == 0) {
-    h->step = 1.0;
-  } else {
-    h_q = (double *)malloc((size_t)m * sizeof(double));
-    if (!h_q) goto fail;
-    dpdlp_normal_fill(1u, m, h_q);
-    HIPCHK(hipMemcpy(pi_eig, h_q, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
-    double sigma_sq = 1.0;
-    int done = 0, batch = 8;
-    if (params->sv_max_iter > 0) HIPCHK(dpdlp_dev_power_init(st, pi_eig, pi_next, c->s));
-    while (!done && pi_iters < params->sv_max_iter) {
-      int k = params->sv_max_iter - pi_iters;
-      if (k > batch) k = batch;
-      HIPCHK(dpdlp_dev_power_steps(st, pi_eig, pi_next, pi_dual, k, c->s));
-      if (sync_scal(c, DPDLP_SC_PI_HIST + 4 * k)) goto fail;
-      for (int i = 0; i < k; ++i) {
-        const double *hh = c->h_scal + DPDLP_SC_PI_HIST + 4 * i;
-        sigma_sq = hh[0];
-        pi_iters++;
-        if (blue(hh + 1) < params->sv_tol) {
-          done = 1;
-          break;
-        }
-      }
-      batch = batch * 2 > 32 ? 32 : batch * 2;
-    }
-    h->step = 0.998 / sqrt(sigma_sq);
-  }
-  h->omega = params->bound_objective_rescaling ? 1.0 : (c->obj_norm + 1.0) / (c->bound_norm + 1.0);
+  return 0;
+fail:
+  if (d_start) hipFree(d_start);
+  return -1;
+}
+
+void dpdlp_i_init_hstate(hstate_t *h, const ctx_t *c, const pdhg_parameters_t *params,
+                         double step) {
+  memset(h, 0, sizeof(*h));
+  h->step = step;
+  h->omega = params->bound_objective_rescaling ? 1.0 : (c->k.obj_norm + 1.0) / (c->k.bound_norm + 1.0);
   h->best_omega = h->omega;
   h->best_gap = INFINITY;
   h->fpe_last = INFINITY;
   h->reason = TERMINATION_REASON_UNSPECIFIED;
-  double power_time = dpdlp_wall() - t_pi0;
+}
 
-  /* ---- main loop (solver.cu:56-102) ---- */
-  if (params->verbose) dpdlp_print_table_header();
-  double t0 = dpdlp_wall();
-  int do_restart = 0, freq = params->termination_evaluation_frequency;
-  const termination_criteria_t *tc = &params->termination_criteria;
-  int trace_k = 0;
-  while (h->total < tc->iteration_limit) {
-    if (opts->trace_points > 0 && trace_k < opts->trace_points &&
-        opts->trace_iters[trace_k] == h->total) {
-      if (n > 0)
-        HIPCHK(hipMemcpyAsync(opts->trace_x + (size_t)trace_k * n, st->x_cur,
-                              (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->s));
-      if (m > 0)
-        HIPCHK(hipMemcpyAsync(opts->trace_y + (size_t)trace_k * m, st->y_cur,
-                              (size_t)m * sizeof(double), hipMemcpyDeviceToHost, c->s));
-      HIPCHK(hipStreamSynchronize(c->s));
-      trace_k++;
-      opts->trace_recorded = trace_k;
-    }
-    int restart_check = h->is_major || h->total == 0;
-    if (restart_check || h->total % dpdlp_print_frequency(h->total) == 0) {
-      HIPCHK(dpdlp_dev_residual(st, NULL, 3, restart_check, c->s));
-      int inf = h->is_major && h->total < 3 * freq;
-      if (inf) HIPCHK(dpdlp_dev_infeasibility(st, c->s));
-      if (sync_scal(c, 64)) {
-        dpdlp_set_error("device sync failed");
-        goto fail;
-      }
-      process_pending_fpe(c, h, 0);
-      host_residual(c, h, params->optimality_norm);
-      if (inf) host_infeasibility(c, h);
-      h->t_cum = dpdlp_wall() - t0;
-      double kkt = fmax(h->rel_p, fmax(h->rel_d, h->rel_gap));
-      if (opts->time_to_1e4_sec < 0 && kkt < 1e-4) opts->time_to_1e4_sec = h->t_cum;
-      if (opts->time_to_1e6_sec < 0 && kkt < 1e-6) opts->time_to_1e6_sec = h->t_cum;
-      if (c->use32 && kkt < opts->mixed_switch_tol) c->use32 = 0;
-      check_termination(h, tc);
-      if (params->verbose)
-        dpdlp_print_iteration(h->total, h->t_cum, h->pobj, h->dobj, h->abs_p, h->abs_d, h->gap,
-                              h->rel_p, h->rel_d, h->rel_gap);
-      if (h->reason != TERMINATION_REASON_UNSPECIFIED) break;
-    }
-    if (restart_check) {
-      do_restart = should_restart(h, &params->restart_params, freq);
-      if (do_restart) {
-        const double *S = c->h_scal;
-        pid_update(h, &params->restart_params, blue(S + DPDLP_SL_RESD + 5),
-                   blue(S + DPDLP_SL_RESP + 4));
-        HIPCHK(dpdlp_dev_restart_copy(st, 3, c->s));
-        h->inner = 0;
-        h->fpe_last = INFINITY;
-      }
-    }
-    h->is_major = ((h->total + 1) % freq) == 0;
-    HIPCHK(pdhg_step(c, st, h, params->reflection_coefficient, do_restart, c->use32, 0,
-                     opts->profile_phases && (h->total & 7) == 0));
-    if (c->use32) opts->iterations_fp32++;
-    do_restart = 0;
-    h->inner++;
-    h->total++;
-  }
-  if (h->reason == TERMINATION_REASON_UNSPECIFIED) {
-    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
-    HIPCHK(dpdlp_dev_residual(st, NULL, 3, 0, c->s));
-    if (sync_scal(c, 64)) goto fail;
-    process_pending_fpe(c, h, 0);
-    host_residual(c, h, params->optimality_norm);
-    if (params->verbose)
-      dpdlp_print_iteration(h->total, h->t_cum, h->pobj, h->dobj, h->abs_p, h->abs_d, h->gap,
-                            h->rel_p, h->rel_d, h->rel_gap);
-  }
-  double loop_time = dpdlp_wall() - t0;
-
-  double polish_time = 0.0;
-  int polish_iters = 0;
-  if (params->feasibility_polishing && h->reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
-      h->reason != TERMINATION_REASON_PRIMAL_INFEASIBLE) {
-    if (feasibility_polish(c, h, params, &polish_time, &polish_iters)) {
-      dpdlp_set_error("feasibility polishing failed on the device");
-      goto fail;
-    }
-  }
-
+cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale_time,
+                                     double polish_time, int polish_iters) {
+  dpdlp_state_t *st = &c->st;
+  int m = st->m, n = st->n, nnz = st->A.nnz;
   /* ---- result (pdlp_core_op.cu:789-859) ---- */
-  res = (cupdlpx_result_t *)calloc(1, sizeof(*res));
+  cupdlpx_result_t *res = (cupdlpx_result_t *)calloc(1, sizeof(*res));
   if (!res) goto fail;
   res->primal_solution = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
   res->dual_solution = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
   res->reduced_cost = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
   if (!res->primal_solution || !res->dual_solution || !res->reduced_cost) goto fail;
   {
-    double *xo = pi_dual, *rco = st->dual_slack, *yo = pi_next; /* reuse work vectors */
+    double *xo = c->pi_dual, *rco = st->dual_slack, *yo = c->pi_next; /* reuse work vectors */
     if (n > 0 && st->At.nblk == 0) HIPCHK(hipMemsetAsync(rco, 0, (size_t)n * 8, c->s));
-    HIPCHK(dpdlp_dev_result(st, c->beta, c->gamma, xo, yo, rco, c->s));
+    HIPCHK(dpdlp_dev_result(st, c->k.beta, c->k.gamma, xo, yo, rco, c->s));
     if (n > 0) {
       HIPCHK(hipMemcpyAsync(res->primal_solution, xo, (size_t)n * sizeof(double),
                             hipMemcpyDeviceToHost, c->s));
@@ -1151,6 +898,178 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
   res->feasibility_polishing_time = polish_time;
   res->feasibility_iteration = polish_iters;
 
+  return res;
+fail:
+  cupdlpx_result_free(res);
+  return NULL;
+}
+
+cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameters_t *params,
+                                       dpdlp_solve_opts_t *opts) {
+  double t_begin = dpdlp_wall();
+  ctx_t C;
+  memset(&C, 0, sizeof(C));
+  ctx_t *c = &C;
+  cupdlpx_result_t *res = NULL;
+  double *h_q = NULL, *d_start = NULL;
+  dpdlp_solve_opts_t local_opts;
+  if (!opts) {
+    dpdlp_default_opts(&local_opts);
+    opts = &local_opts;
+  }
+  opts->time_to_1e4_sec = -1.0;
+  opts->time_to_1e6_sec = -1.0;
+  opts->trace_recorded = 0;
+  opts->iterations_fp32 = 0;
+  if (!R || !params) {
+    dpdlp_set_error("NULL problem or parameters");
+    return NULL;
+  }
+  if (params->termination_evaluation_frequency <= 0) {
+    dpdlp_set_error("termination_evaluation_frequency must be positive");
+    return NULL;
+  }
+  int m = R->m, n = R->n, nnz = R->nnz;
+  dpdlp_state_t *st = &c->st;
+  hstate_t H;
+  memset(&H, 0, sizeof(H));
+  hstate_t *h = &H;
+  c->mixed = (opts->precision == DPDLP_PREC_MIXED);
+
+  if (opts->profile_phases) {
+    HIPCHK(hipStreamCreate(&c->s));
+    c->owns_stream = 1;
+    for (int k = 0; k < 64; ++k) {
+      for (int q = 0; q < 3; ++q) HIPCHK(hipEventCreate(&c->ev[k][q]));
+      c->nev_created = k + 1;
+    }
+  }
+  double rescale_time = 0.0;
+  if (dpdlp_i_ctx_setup(c, R, params, opts->precision == DPDLP_PREC_MIXED, c->s, NULL, NULL,
+                        &rescale_time))
+    goto fail;
+  double *pi_eig = c->pi_eig, *pi_next = c->pi_next, *pi_dual = c->pi_dual;
+
+  /* ---- step size: power iteration (pdlp_core_op.cu:638-656) ---- */
+  double t_pi0 = dpdlp_wall();
+  int pi_iters = 0;
+  if (nnz == 0) {
+    h->step = 1.0;
+  } else {
+    h_q = (double *)malloc((size_t)m * sizeof(double));
+    if (!h_q) goto fail;
+    dpdlp_normal_fill(1u, m, h_q);
+    HIPCHK(hipMemcpy(pi_eig, h_q, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
+    double sigma_sq = 1.0;
+    int done = 0, batch = 8;
+    if (params->sv_max_iter > 0) HIPCHK(dpdlp_dev_power_init(st, pi_eig, pi_next, c->s));
+    while (!done && pi_iters < params->sv_max_iter) {
+      int k = params->sv_max_iter - pi_iters;
+      if (k > batch) k = batch;
+      HIPCHK(dpdlp_dev_power_steps(st, pi_eig, pi_next, pi_dual, k, c->s));
+      if (sync_scal(c, DPDLP_SC_PI_HIST + 4 * k)) goto fail;
+      for (int i = 0; i < k; ++i) {
+        const double *hh = c->h_scal + DPDLP_SC_PI_HIST + 4 * i;
+        sigma_sq = hh[0];
+        pi_iters++;
+        if (dpdlp_blue(hh + 1) < params->sv_tol) {
+          done = 1;
+          break;
+        }
+      }
+      batch = batch * 2 > 32 ? 32 : batch * 2;
+    }
+    h->step = 0.998 / sqrt(sigma_sq);
+  }
+  dpdlp_i_init_hstate(h, c, params, h->step);
+  double power_time = dpdlp_wall() - t_pi0;
+
+  /* ---- main loop (solver.cu:56-102) ---- */
+  if (params->verbose) dpdlp_print_table_header();
+  double t0 = dpdlp_wall();
+  int do_restart = 0, freq = params->termination_evaluation_frequency;
+  const termination_criteria_t *tc = &params->termination_criteria;
+  int trace_k = 0;
+  while (h->total < tc->iteration_limit) {
+    if (opts->trace_points > 0 && trace_k < opts->trace_points &&
+        opts->trace_iters[trace_k] == h->total) {
+      if (n > 0)
+        HIPCHK(hipMemcpyAsync(opts->trace_x + (size_t)trace_k * n, st->x_cur,
+                              (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      if (m > 0)
+        HIPCHK(hipMemcpyAsync(opts->trace_y + (size_t)trace_k * m, st->y_cur,
+                              (size_t)m * sizeof(double), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      trace_k++;
+      opts->trace_recorded = trace_k;
+    }
+    int restart_check = h->is_major || h->total == 0;
+    if (restart_check || h->total % dpdlp_print_frequency(h->total) == 0) {
+      HIPCHK(dpdlp_dev_residual(st, NULL, 3, restart_check, c->s));
+      int inf = h->is_major && h->total < 3 * freq;
+      if (inf) HIPCHK(dpdlp_dev_infeasibility(st, c->s));
+      if (sync_scal(c, 64)) {
+        dpdlp_set_error("device sync failed");
+        goto fail;
+      }
+      ctl_pending_fpe(h, c->h_scal, 0);
+      ctl_residual(h, &c->k, c->h_scal, params->optimality_norm);
+      if (inf) ctl_infeasibility(h, &c->k, c->h_scal);
+      h->t_cum = dpdlp_wall() - t0;
+      double kkt = fmax(h->rel_p, fmax(h->rel_d, h->rel_gap));
+      if (opts->time_to_1e4_sec < 0 && kkt < 1e-4) opts->time_to_1e4_sec = h->t_cum;
+      if (opts->time_to_1e6_sec < 0 && kkt < 1e-6) opts->time_to_1e6_sec = h->t_cum;
+      if (c->use32 && kkt < opts->mixed_switch_tol) c->use32 = 0;
+      ctl_check_termination(h, tc);
+      if (params->verbose)
+        dpdlp_print_iteration(h->total, h->t_cum, h->pobj, h->dobj, h->abs_p, h->abs_d, h->gap,
+                              h->rel_p, h->rel_d, h->rel_gap);
+      if (h->reason != TERMINATION_REASON_UNSPECIFIED) break;
+    }
+    if (restart_check) {
+      do_restart = ctl_should_restart(h, &params->restart_params, freq);
+      if (do_restart) {
+        const double *S = c->h_scal;
+        ctl_pid_update(h, &params->restart_params, dpdlp_blue(S + DPDLP_SL_RESD + 5),
+                       dpdlp_blue(S + DPDLP_SL_RESP + 4));
+        HIPCHK(dpdlp_dev_restart_copy(st, 3, c->s));
+        h->inner = 0;
+        h->fpe_last = INFINITY;
+      }
+    }
+    h->is_major = ((h->total + 1) % freq) == 0;
+    HIPCHK(pdhg_step(c, st, h, params->reflection_coefficient, do_restart, c->use32, 0,
+                     opts->profile_phases && (h->total & 7) == 0));
+    if (c->use32) opts->iterations_fp32++;
+    do_restart = 0;
+    h->inner++;
+    h->total++;
+  }
+  if (h->reason == TERMINATION_REASON_UNSPECIFIED) {
+    h->reason = TERMINATION_REASON_ITERATION_LIMIT;
+    HIPCHK(dpdlp_dev_residual(st, NULL, 3, 0, c->s));
+    if (sync_scal(c, 64)) goto fail;
+    ctl_pending_fpe(h, c->h_scal, 0);
+    ctl_residual(h, &c->k, c->h_scal, params->optimality_norm);
+    if (params->verbose)
+      dpdlp_print_iteration(h->total, h->t_cum, h->pobj, h->dobj, h->abs_p, h->abs_d, h->gap,
+                            h->rel_p, h->rel_d, h->rel_gap);
+  }
+  double loop_time = dpdlp_wall() - t0;
+
+  double polish_time = 0.0;
+  int polish_iters = 0;
+  if (params->feasibility_polishing && h->reason != TERMINATION_REASON_DUAL_INFEASIBLE &&
+      h->reason != TERMINATION_REASON_PRIMAL_INFEASIBLE) {
+    if (dpdlp_i_feasibility_polish(c, h, params, &polish_time, &polish_iters)) {
+      dpdlp_set_error("feasibility polishing failed on the device");
+      goto fail;
+    }
+  }
+
+  res = dpdlp_i_ctx_result(c, h, rescale_time, polish_time, polish_iters);
+  if (!res) goto fail;
+
   opts->setup_time_sec = rescale_time;
   opts->power_time_sec = power_time;
   opts->loop_time_sec = loop_time;
@@ -1160,7 +1079,7 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
   opts->primal_phase_ms = c->nsamp ? c->p1_ms / c->nsamp : 0.0;
   opts->dual_phase_ms = c->nsamp ? c->p2_ms / c->nsamp : 0.0;
   free(h_q);
-  ctx_free(c);
+  dpdlp_i_ctx_free(c);
   opts->total_time_sec = dpdlp_wall() - t_begin;
   if (!opts->quiet) dpdlp_print_final(res, params);
   return res;
@@ -1169,7 +1088,7 @@ fail:
   free(h_q);
   if (d_start) hipFree(d_start);
   cupdlpx_result_free(res);
-  ctx_free(c);
+  dpdlp_i_ctx_free(c);
   return NULL;
 }
 

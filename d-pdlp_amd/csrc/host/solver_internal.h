@@ -1,0 +1,54 @@
+/*
+ * Internal pieces of the solve driver shared by the single-LP path
+ * (solver.c) and the batched / served path (batch.c).
+ */
+#pragma once
+
+#include "control.h"
+#include "dpdlp_dev.h"
+#include "dpdlp_ext.h"
+#include "dpdlp_host.h"
+
+#include <hip/hip_runtime_api.h>
+
+#define MAX_ALLOCS 128
+typedef struct {
+  hipStream_t s;
+  dpdlp_state_t st;
+  double *h_scal;
+  lpk_t k;
+  void *allocs[MAX_ALLOCS];
+  int nalloc;
+  int mixed, use32;
+  int owns_stream, owns_scal;
+  char *arena; /* one device allocation for the LP state (zeroed) */
+  size_t arena_off, arena_len;
+  double *pi_eig, *pi_next, *pi_dual, *rs, *cs;
+  double *w_A_v, *w_At_v;
+  float *w_A_v32, *w_At_v32;
+  /* live phase timing */
+  hipEvent_t ev[64][3];
+  int nev_created, nev_used;
+  double p1_ms, p2_ms;
+  int nsamp;
+} ctx_t;
+
+
+/* Allocates one LP's device state from a resident problem and runs the
+ * device scaling, warm start and FP32 copies.  stream: NULL = own stream;
+ * scal_dev/scal_host: external slot arrays (NULL = own).  Returns 0 or -1. */
+int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params, int mixed,
+                      hipStream_t stream, double *scal_dev, double *scal_host,
+                      double *rescale_time);
+/* Unscaled result of a finished LP (D2H, synchronous on c->s). */
+cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale_time,
+                                     double polish_time, int polish_iters);
+void dpdlp_i_ctx_free(ctx_t *c);
+/* Initial control state after the step size is known. */
+void dpdlp_i_init_hstate(hstate_t *h, const ctx_t *c, const pdhg_parameters_t *params,
+                         double step);
+/* Device bytes of one LP's solver state (the arena dpdlp_i_ctx_setup makes). */
+size_t dpdlp_i_ctx_footprint(const dpdlp_resident_t *R, int mixed, int own_scal);
+/* Feasibility polishing after the main loop (pdlp_core_op.cu:862-1030). */
+int dpdlp_i_feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *params,
+                               double *ptime, int *piters);

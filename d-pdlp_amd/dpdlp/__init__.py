@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libcupdlpx-dist.so")
 CLI_PATH = os.path.join(PKG_ROOT, "lib", "cupdlpx-dist")
 
 from ._types import *  # noqa: F401,F403
-from ._types import (_dp, _ip, _Csr, _Dense, CupdlpxResult, GenSpec, LpProblem,
+from ._types import (_dp, _ip, _Csr, _Dense, BatchOpts, CupdlpxResult, GenSpec, LpProblem,
                      MatrixDesc, PdhgParameters, SolveOpts, TERMINATION, MATRIX_COO,
                      MATRIX_CSC, MATRIX_CSR, MATRIX_DENSE, PREC_FP64)
 
@@ -56,6 +56,22 @@ def _load() -> C.CDLL:
         "dpdlp_copy_calibration": (C.c_double, [C.c_longlong, C.c_int]),
         "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
+        "dpdlp_upload_ex": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int]),
+        "dpdlp_resident_bytes": (C.c_size_t, [C.c_void_p]),
+        "dpdlp_batch_default_opts": (None, [C.POINTER(BatchOpts)]),
+        "dpdlp_batch_create": (C.c_void_p, [C.POINTER(PdhgParameters), C.POINTER(BatchOpts)]),
+        "dpdlp_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_longlong]),
+        "dpdlp_batch_step": (C.c_int, [C.c_void_p]),
+        "dpdlp_batch_take": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(R),
+                                       C.POINTER(C.c_double)]),
+        "dpdlp_batch_in_flight": (C.c_int, [C.c_void_p]),
+        "dpdlp_batch_bytes": (C.c_size_t, [C.c_void_p]),
+        "dpdlp_batch_counters": (None, [C.c_void_p, C.POINTER(C.c_longlong),
+                                        C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+        "dpdlp_batch_free": (None, [C.c_void_p]),
+        "dpdlp_solve_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int,
+                                        C.POINTER(PdhgParameters), C.POINTER(BatchOpts),
+                                        C.POINTER(R)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -283,9 +299,12 @@ def solve_plain(problem: Problem, params: PdhgParameters | None = None) -> Resul
 class Resident:
     """A problem uploaded to HBM once (dpdlp_upload) and solved many times."""
 
-    def __init__(self, problem: Problem, spmv_cap=0, tune=False):
+    def __init__(self, problem: Problem, spmv_cap=0, tune=False, panels=None):
         self.panels = None
-        if tune:
+        if panels is not None:
+            self.h = lib.dpdlp_upload_ex(problem.ptr, spmv_cap, panels[0], panels[1])
+            self.panels = tuple(panels)
+        elif tune:
             ch = (C.c_int * 2)()
             self.h = lib.dpdlp_upload_tuned(problem.ptr, spmv_cap, ch)
             self.panels = (ch[0], ch[1])
@@ -326,6 +345,94 @@ class Resident:
             self.free()
         except Exception:
             pass
+
+
+def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=1024, mixed_switch_tol=None):
+    o = BatchOpts()
+    lib.dpdlp_batch_default_opts(C.byref(o))
+    o.max_lps = max_lps
+    o.precision = precision
+    o.spmv_cap = spmv_cap
+    if mixed_switch_tol is not None:
+        o.mixed_switch_tol = mixed_switch_tol
+    return o
+
+
+class Batch:
+    """dpdlp_batch_*: LPs added at any time advance together, one window of
+    iterations per step(); finished LPs come out of take()."""
+
+    def __init__(self, params=None, **opt_kw):
+        self.opts = batch_opts(**opt_kw)
+        self.params = params if params is not None else default_params()
+        self.h = lib.dpdlp_batch_create(C.byref(self.params), C.byref(self.opts))
+        if not self.h:
+            raise RuntimeError("batch create failed: " + last_error())
+        self._keep = {}
+
+    def add(self, resident: "Resident", tag: int):
+        """Adds an LP uploaded with Resident(problem, spmv_cap=cap, panels=(1, 1))."""
+        if lib.dpdlp_batch_add(self.h, resident.h, 0, tag) != 0:
+            raise RuntimeError("batch add failed: " + last_error())
+        self._keep[tag] = resident
+
+    def step(self) -> int:
+        k = lib.dpdlp_batch_step(self.h)
+        if k < 0:
+            raise RuntimeError("batch step failed: " + last_error())
+        return k
+
+    def take(self):
+        """[(tag, Result, latency_sec)] of the LPs finished so far."""
+        out = []
+        tag, lat = C.c_longlong(), C.c_double()
+        rp = C.POINTER(CupdlpxResult)()
+        while lib.dpdlp_batch_take(self.h, C.byref(tag), C.byref(rp), C.byref(lat)) == 1:
+            out.append((tag.value, result_from_ptr(rp), lat.value))
+            self._keep.pop(tag.value, None)
+        return out
+
+    def in_flight(self) -> int:
+        return lib.dpdlp_batch_in_flight(self.h)
+
+    def counters(self):
+        w, s_, l_ = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        lib.dpdlp_batch_counters(self.h, C.byref(w), C.byref(s_), C.byref(l_))
+        return {"windows": w.value, "syncs": s_.value, "launches": l_.value}
+
+    def run(self):
+        """Steps until every LP finished; returns {tag: Result}."""
+        res = {}
+        while self.in_flight() > 0:
+            self.step()
+            for tag, r, _ in self.take():
+                res[tag] = r
+        for tag, r, _ in self.take():
+            res[tag] = r
+        return res
+
+    def free(self):
+        if self.h:
+            lib.dpdlp_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def solve_batch(residents, params=None, **opt_kw):
+    """dpdlp_solve_batch over Resident objects; list of Results in input order."""
+    o = batch_opts(**opt_kw)
+    pp = params if params is not None else default_params()
+    arr = (C.c_void_p * len(residents))(*[r.h for r in residents])
+    out = (C.POINTER(CupdlpxResult) * len(residents))()
+    k = lib.dpdlp_solve_batch(arr, len(residents), C.byref(pp), C.byref(o), out)
+    if k != len(residents):
+        raise RuntimeError("batch solve failed: " + last_error())
+    return [result_from_ptr(out[i]) for i in range(len(residents))]
 
 
 def generate(m, n, nnz=0, row_rule=0, seed=0, zipf_s=1.5, zipf_max=50000,

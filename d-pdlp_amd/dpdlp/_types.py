@@ -122,3 +122,9 @@ class GenSpec(C.Structure):
     ]
 
 
+
+
+class BatchOpts(C.Structure):
+    """dpdlp_batch_opts_t (include/dpdlp_ext.h)."""
+    _fields_ = [("max_lps", C.c_int), ("precision", C.c_int), ("mixed_switch_tol", C.c_double),
+                ("spmv_cap", C.c_int)]

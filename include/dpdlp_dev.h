@@ -58,6 +58,75 @@ typedef struct {
   double *scal;   /* DPDLP_SCAL_LEN doubles of raw reduction results */
 } dpdlp_state_t;
 
+/* Workgroups (of 256 lanes) of every grid-stride element-wise launch
+ * over `count` items, at most 2048; also the number of partial sums such a
+ * launch's reduction writes. */
+static inline int dpdlp_ew_grid(long long count) {
+  long long g = (count + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 2048) g = 2048;
+  return (int)g;
+}
+
+/* ---- batched solving: many LPs per launch ---- */
+
+/* One LP of a batch (device pointers; scal is that LP's slot array). */
+typedef struct {
+  int m, n;
+  const int *A_rp, *A_ci, *At_rp, *At_ci;
+  const double *A_v, *At_v;
+  const float *A_v32, *At_v32;
+  double *lv, *uv, *c, *lc, *uc, *D_r, *D_c;
+  double *x_init, *x_cur, *x_pdhg, *x_ref, *dual_product, *dual_slack, *dx;
+  double *y_init, *y_cur, *y_pdhg, *y_ref, *primal_product, *primal_slack, *dy;
+  float *x_ref32, *y_cur32;
+  double *pi_eig, *pi_next, *pi_dual;
+  double *scal;
+} dpdlp_blp_t;
+
+/* Per-LP control of one batched step, written by the host. */
+typedef struct {
+  double tau, sigma, w, rho;
+  int phase;        /* run the PDHG phases */
+  int major;        /* major variant: write x_pdhg, dual_slack, y_pdhg */
+  int fpe;          /* write dx, dy and finish the fixed-point error */
+  int fpe_to;       /* bit 0: copy FPE slots to the restart group, bit 1: major group */
+  int eval;         /* KKT residual */
+  int restart_dist; /* add restart distances to the residual */
+  int inf;          /* infeasibility rays */
+  int restart;      /* restart copy mask: 1 x, 2 y */
+  int use32;        /* FP32 matrix and gathers */
+  int pi;           /* power-iteration step */
+  int pi_slot;      /* history index of this power-iteration step */
+  int pad;
+} dpdlp_bctl_t;
+
+/* Work item: a row block (lp, r0, r1) of one LP's matrix, or an element
+ * block (lp, j, G): block j of the G grid-stride blocks of that LP. */
+typedef struct {
+  int lp, a, b, pad;
+} dpdlp_bblk_t;
+
+typedef struct {
+  int nlp;
+  const dpdlp_blp_t *lps;
+  const dpdlp_bctl_t *ctl;
+  const dpdlp_bblk_t *bA, *bAt, *eN, *eM, *eNM;
+  int nbA, nbAt, neN, neM, neNM;
+  const int *rA, *rAt, *rN, *rM, *rNM; /* per LP [lo, hi) into the lists */
+  double *part;
+  int stride;
+  int cap;
+} dpdlp_batch_dev_t;
+
+/* Batched steps: every launch covers all LPs whose control flag is set. */
+int dpdlp_bdev_restart_copy(const dpdlp_batch_dev_t *b, void *stream);
+int dpdlp_bdev_phases(const dpdlp_batch_dev_t *b, void *stream);
+int dpdlp_bdev_fpe(const dpdlp_batch_dev_t *b, void *stream);
+int dpdlp_bdev_residual(const dpdlp_batch_dev_t *b, void *stream);
+int dpdlp_bdev_infeasibility(const dpdlp_batch_dev_t *b, void *stream);
+int dpdlp_bdev_power_step(const dpdlp_batch_dev_t *b, void *stream);
+
 /* Scalar / partial slot map (raw values; the host combines them). */
 enum {
   DPDLP_SL_RESP = 0,   /* rows: rp blue[3], psum, ydist blue[3], rp max  (8) */

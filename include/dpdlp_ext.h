@@ -9,6 +9,7 @@
 #pragma once
 
 #include "cupdlpx_types.h"
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -67,6 +68,14 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *res,
                                        dpdlp_solve_opts_t *opts);
 void dpdlp_resident_free(dpdlp_resident_t *res);
 
+/* Upload with a fixed row-block capacity (1024, 2048, 4096; 0 = by size) and
+ * column-panel counts for A and A' (1, 2, 4, 8; 0 = by size).  Batches need
+ * panels 1, 1 and the batch's capacity. */
+dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *prob, int spmv_cap, int panels_A,
+                                  int panels_At);
+/* Device bytes a resident problem holds (matrix, transpose, plans, vectors). */
+size_t dpdlp_resident_bytes(const dpdlp_resident_t *res);
+
 /* Last error message of this thread's most recent failing call ("" if none). */
 const char *dpdlp_last_error(void);
 
@@ -85,6 +94,52 @@ double dpdlp_copy_calibration(long long bytes, int reps);
 /* Problem sizes of a resident problem (m, n, nnz, row blocks of A and A'). */
 void dpdlp_resident_info(const dpdlp_resident_t *res, int *m, int *n, int *nnz,
                          int *nblk_A, int *nblk_At);
+
+/* ---- batched solving: many LPs advancing together ---- */
+
+/* Every LP of a batch runs the single-LP algorithm unchanged (same scaling,
+ * step size, restart, termination and polishing; same iterates bit for bit
+ * whichever other LPs share the batch); the batch launches each kernel kind
+ * once per iteration for all LPs in flight.  LPs join and leave at window
+ * boundaries (every gcd(10, termination_evaluation_frequency) iterations). */
+typedef struct {
+  int max_lps;             /* slot capacity (default 1024) */
+  int precision;           /* DPDLP_PREC_FP64 (default) or DPDLP_PREC_MIXED */
+  double mixed_switch_tol; /* mixed: FP64 from this max relative KKT on */
+  int spmv_cap;            /* row-block capacity of every LP (default 1024) */
+} dpdlp_batch_opts_t;
+
+void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o);
+
+typedef struct dpdlp_batch dpdlp_batch_t;
+
+/* params: shared by every LP of the batch (copied). */
+dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params, const dpdlp_batch_opts_t *o);
+/* Sets up an LP (uploaded by dpdlp_upload_ex(prob, cap, 1, 1)) in a free slot;
+ * it starts at the next dpdlp_batch_step.  take_ownership: the batch frees
+ * res when the LP finishes.  Returns 0, or -1 (batch full, wrong layout, no
+ * memory). */
+int dpdlp_batch_add(dpdlp_batch_t *b, dpdlp_resident_t *res, int take_ownership, long long tag);
+/* Runs one window of iterations for every LP in flight; finished LPs' results
+ * become available to dpdlp_batch_take.  Returns the LPs still in flight, or
+ * -1 on a device error. */
+int dpdlp_batch_step(dpdlp_batch_t *b);
+/* Pops one finished LP: returns 1 and sets *tag, *res (caller frees with
+ * cupdlpx_result_free) and *latency_sec (add to result), 0 when none waits. */
+int dpdlp_batch_take(dpdlp_batch_t *b, long long *tag, cupdlpx_result_t **res,
+                     double *latency_sec);
+int dpdlp_batch_in_flight(const dpdlp_batch_t *b);
+/* Device bytes the batch holds for LP state beyond the resident problems. */
+size_t dpdlp_batch_bytes(const dpdlp_batch_t *b);
+/* Counters: windows run, host-device synchronisations, kernel launches. */
+void dpdlp_batch_counters(const dpdlp_batch_t *b, long long *windows, long long *syncs,
+                          long long *launches);
+void dpdlp_batch_free(dpdlp_batch_t *b);
+
+/* Solves `count` resident problems in one batch; results[i] belongs to
+ * res[i].  Returns the number of results (count) or -1. */
+int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parameters_t *params,
+                      const dpdlp_batch_opts_t *o, cupdlpx_result_t **results);
 
 /* ---- seeded synthetic LPs (feasible by construction) ---- */
 typedef struct {

@@ -1,0 +1,160 @@
+"""Batched solving: every LP of a batch gives the same result alone and in a
+mix (bit for bit), the same as the single-LP path on the same row-block
+layout, and matches the oracle; LPs joining a running batch (continuous
+batching) keep that property."""
+import numpy as np
+import pytest
+
+from helpers import rel
+
+pytestmark = pytest.mark.gpu
+
+CAP = 1024
+SIZES = [(300, 700), (1200, 2500), (2500, 6000), (500, 1400), (4000, 7000), (150, 400)]
+
+
+def uneven_lps(dp, sizes=SIZES, seed0=3000):
+    """Poisson(8)+1 nonzeros per row, as the batch workload generator."""
+    out = []
+    for k, (m, n) in enumerate(sizes):
+        prob, opt = dp.generate(m, n, 0, row_rule=2, poisson_lambda=8.0, seed=seed0 + k)
+        out.append((prob, opt))
+    return out
+
+
+def same(a, b):
+    return (a.status == b.status and a.iterations == b.iterations
+            and np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+            and np.array_equal(a.reduced_cost, b.reduced_cost)
+            and a.primal_obj == b.primal_obj and a.dual_obj == b.dual_obj)
+
+
+def residents(dp, lps):
+    return [dp.Resident(p, spmv_cap=CAP, panels=(1, 1)) for p, _ in lps]
+
+
+def alone(dp, res, params, **kw):
+    return [dp.solve_batch([r], params, spmv_cap=CAP, **kw)[0] for r in res]
+
+
+def test_batch_alone_equals_mix_equals_single(dp):
+    lps = uneven_lps(dp)
+    res = residents(dp, lps)
+    params = dp.default_params(presolve=False)
+    mix = dp.solve_batch(res, params, spmv_cap=CAP)
+    solo_batch = alone(dp, res, params)
+    for k, r in enumerate(res):
+        single, _ = r.solve(params)
+        assert same(mix[k], solo_batch[k]), k
+        assert same(mix[k], single), k
+        assert mix[k].status == "OPTIMAL"
+
+
+def test_batch_matches_oracle(dp, orc):
+    lps = uneven_lps(dp, SIZES[:4], seed0=3100)
+    res = residents(dp, lps)
+    params = dp.default_params(presolve=False)
+    mix = dp.solve_batch(res, params, spmv_cap=CAP)
+    for k, (prob, opt) in enumerate(lps):
+        ref, _ = orc.solve(prob.ptr, orc.default_params(presolve=False))
+        assert mix[k].status == ref["status"]
+        assert abs(mix[k].primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
+            1.0, abs(ref["primal_objective_value"]))
+        assert abs(mix[k].iterations - ref["total_count"]) <= 10
+        assert abs(mix[k].primal_obj - opt) <= 1e-3 * max(1.0, abs(opt))
+
+
+def test_continuous_batching_join_while_running(dp):
+    """LPs added while others are mid-solve, and into slots freed by
+    finished ones, give the same result as alone."""
+    lps = uneven_lps(dp, SIZES + [(800, 1600), (2000, 3000)], seed0=3200)
+    res = residents(dp, lps)
+    params = dp.default_params(presolve=False)
+    ref = alone(dp, res, params)
+    b = dp.Batch(params, max_lps=3, spmv_cap=CAP)
+    pending = list(range(len(res)))
+    got = {}
+    steps = 0
+    while pending or b.in_flight() > 0:
+        # join at staggered windows: one new LP every 7 windows while a slot is free
+        if pending and b.in_flight() < 3 and steps % 7 == 0:
+            k = pending.pop(0)
+            b.add(res[k], k)
+        if b.in_flight() > 0:
+            b.step()
+        steps += 1
+        for tag, r, lat in b.take():
+            assert lat > 0
+            got[tag] = r
+    assert sorted(got) == list(range(len(res)))
+    for k in range(len(res)):
+        assert same(got[k], ref[k]), k
+    assert b.counters()["windows"] > 0
+    b.free()
+
+
+def test_batch_mixed_precision_alone_equals_mix(dp):
+    lps = uneven_lps(dp, SIZES[:5], seed0=3300)
+    res = residents(dp, lps)
+    params = dp.default_params(presolve=False)
+    mix = dp.solve_batch(res, params, spmv_cap=CAP, precision=dp.PREC_MIXED)
+    solo = alone(dp, res, params, precision=dp.PREC_MIXED)
+    fp64 = dp.solve_batch(res, params, spmv_cap=CAP)
+    for k in range(len(res)):
+        assert same(mix[k], solo[k]), k
+        assert mix[k].status == "OPTIMAL"
+        # same LP, same accuracy target: objectives agree to the tolerance
+        assert abs(mix[k].primal_obj - fp64[k].primal_obj) <= 1e-3 * max(
+            1.0, abs(fp64[k].primal_obj))
+
+
+def test_batch_edge_cases_match_single(dp):
+    """Infeasible LPs, an iteration limit that is not a window multiple, an
+    eval frequency that is not a multiple of 10, and polishing."""
+    A = np.array([[1.0, 1.0], [1.0, 1.0]])
+    infeas = dp.Problem.from_dense(A, c=np.zeros(2), con_lb=np.array([2.0, -np.inf]),
+                                   con_ub=np.array([np.inf, 1.0]), var_lb=np.zeros(2),
+                                   var_ub=np.ones(2))
+    A2 = np.array([[1.0, -1.0]])
+    unbdd = dp.Problem.from_dense(A2, c=np.array([-1.0, 0.0]), con_lb=np.array([0.0]),
+                                  con_ub=np.array([np.inf]), var_lb=np.zeros(2),
+                                  var_ub=np.full(2, np.inf))
+    lps = uneven_lps(dp, SIZES[:3], seed0=3400)
+    probs = [infeas, unbdd] + [p for p, _ in lps]
+    res = [dp.Resident(p, spmv_cap=CAP, panels=(1, 1)) for p in probs]
+    cases = [
+        dict(presolve=False, termination_criteria__iteration_limit=5000),
+        dict(presolve=False, termination_criteria__iteration_limit=37),
+        dict(presolve=False, termination_evaluation_frequency=25,
+             termination_criteria__iteration_limit=5000),
+        dict(presolve=False, feasibility_polishing=True,
+             termination_criteria__iteration_limit=5000),
+    ]
+    for kw in cases:
+        params = dp.default_params(**kw)
+        mix = dp.solve_batch(res, params, spmv_cap=CAP)
+        for k, r in enumerate(res):
+            single, _ = r.solve(params)
+            assert same(mix[k], single), (kw, k)
+    params = dp.default_params(**cases[0])
+    mix = dp.solve_batch(res, params, spmv_cap=CAP)
+    assert mix[0].status == "PRIMAL_INFEASIBLE"
+    assert mix[1].status == "DUAL_INFEASIBLE"
+
+
+def test_batch_empty_and_zero_limit(dp):
+    params = dp.default_params(presolve=False, termination_criteria__iteration_limit=0)
+    lps = uneven_lps(dp, SIZES[:2], seed0=3500)
+    res = residents(dp, lps)
+    out = dp.solve_batch(res, params, spmv_cap=CAP)
+    assert all(r.status == "ITERATION_LIMIT" and r.iterations == 0 for r in out)
+    assert dp.solve_batch([], params, spmv_cap=CAP) == []
+
+
+def test_batch_rejects_tiled_layout(dp):
+    prob, _ = dp.generate(300, 700, 0, row_rule=2, poisson_lambda=8.0, seed=1)
+    r = dp.Resident(prob, spmv_cap=CAP, panels=(2, 1))
+    b = dp.Batch(dp.default_params(presolve=False), spmv_cap=CAP)
+    with pytest.raises(RuntimeError):
+        b.add(r, 0)
+    b.free()
