@@ -32,6 +32,8 @@
 
 #include <hip/hip_runtime_api.h>
 #include <math.h>
+#include <pthread.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -46,11 +48,13 @@
     }                                                                          \
   } while (0)
 
-enum { S_FREE = 0, S_PI, S_PDHG };
+/* slot stages: FREE -> RESERVED (being set up, possibly on another thread)
+ * -> READY (set up, waiting for a window start) -> PI -> PDHG -> FREE */
+enum { S_FREE = 0, S_RESERVED, S_READY, S_PI, S_PDHG };
 enum { NLIST = 5 };
 
 typedef struct {
-  int stage;
+  int stage; /* read and written with atomics: set-up threads move it */
   long long tag;
   dpdlp_resident_t *R;
   int own_R;
@@ -70,6 +74,7 @@ typedef struct {
 } bdone_t;
 
 struct dpdlp_batch {
+  pthread_mutex_t mu; /* slot reservation and READY hand-over */
   pdhg_parameters_t prm;
   dpdlp_batch_opts_t o;
   int W, freq, rbw;
@@ -97,6 +102,13 @@ struct dpdlp_batch {
   size_t state_bytes;
 };
 
+static int stage_of(const bslot_t *q) { return __atomic_load_n(&q->stage, __ATOMIC_ACQUIRE); }
+static void set_stage(bslot_t *q, int st) { __atomic_store_n(&q->stage, st, __ATOMIC_RELEASE); }
+static int active(const bslot_t *q) {
+  int st = stage_of(q);
+  return st == S_PI || st == S_PDHG;
+}
+
 void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o) {
   memset(o, 0, sizeof(*o));
   o->max_lps = 1024;
@@ -119,7 +131,7 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   if (b->s) hipStreamSynchronize(b->s);
   for (int i = 0; i < b->cap && b->sl; ++i) {
     bslot_t *q = &b->sl[i];
-    if (q->stage == S_FREE) continue;
+    if (stage_of(q) == S_FREE) continue;
     dpdlp_i_ctx_free(&q->c);
     if (q->own_R) dpdlp_resident_free(q->R);
   }
@@ -137,6 +149,7 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   for (int k = 0; k < 2; ++k)
     if (b->ev_win[k]) hipEventDestroy(b->ev_win[k]);
   if (b->s) hipStreamDestroy(b->s);
+  pthread_mutex_destroy(&b->mu);
   free(b);
 }
 
@@ -155,6 +168,7 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
                                   const dpdlp_batch_opts_t *o) {
   dpdlp_batch_t *b = (dpdlp_batch_t *)calloc(1, sizeof(*b));
   if (!b) return NULL;
+  pthread_mutex_init(&b->mu, NULL);
   if (params)
     b->prm = *params;
   else
@@ -168,6 +182,7 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
     b->o.spmv_cap = 1024;
   if (b->prm.termination_evaluation_frequency <= 0) {
     dpdlp_set_error("termination_evaluation_frequency must be positive");
+    pthread_mutex_destroy(&b->mu);
     free(b);
     return NULL;
   }
@@ -205,14 +220,20 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
   return b;
 }
 
-int dpdlp_batch_in_flight(const dpdlp_batch_t *b) { return b ? b->in_flight : 0; }
+int dpdlp_batch_in_flight(const dpdlp_batch_t *b) {
+  if (!b) return 0;
+  pthread_mutex_lock((pthread_mutex_t *)&b->mu);
+  int k = b->in_flight;
+  pthread_mutex_unlock((pthread_mutex_t *)&b->mu);
+  return k;
+}
 
 size_t dpdlp_batch_bytes(const dpdlp_batch_t *b) {
   if (!b) return 0;
   size_t s = b->state_bytes + (size_t)b->list_cap * sizeof(dpdlp_bblk_t) +
              (size_t)b->part_len * sizeof(double) + (size_t)b->normals_len * sizeof(double);
   for (int i = 0; i < b->nhw; ++i)
-    if (b->sl[i].stage != S_FREE) s += b->sl[i].c.arena_len;
+    if (active(&b->sl[i])) s += b->sl[i].c.arena_len;
   return s;
 }
 
@@ -294,9 +315,87 @@ static void start_loop(dpdlp_batch_t *b, bslot_t *q) {
   dpdlp_i_init_hstate(&q->h, &q->c, &b->prm, step);
   q->power_time = dpdlp_wall() - q->t_pi0;
   q->t_loop0 = dpdlp_wall();
-  q->stage = S_PDHG;
+  set_stage(q, S_PDHG);
   q->do_restart = 0;
   q->fresh = 1;
+}
+
+int dpdlp_i_batch_reserve(dpdlp_batch_t *b) {
+  pthread_mutex_lock(&b->mu);
+  int i = 0;
+  while (i < b->cap && stage_of(&b->sl[i]) != S_FREE) ++i;
+  if (i < b->cap) set_stage(&b->sl[i], S_RESERVED);
+  pthread_mutex_unlock(&b->mu);
+  return i < b->cap ? i : -1;
+}
+
+void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int i) {
+  pthread_mutex_lock(&b->mu);
+  set_stage(&b->sl[i], S_FREE);
+  pthread_mutex_unlock(&b->mu);
+}
+
+/* Sets up a reserved slot (any thread, on `stream`; NULL = the batch's
+ * stream): the same scaled problem a single-LP solve would build.  The slot
+ * joins at the next window start. */
+int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take_ownership,
+                          long long tag, void *stream) {
+  if (R->cap != b->o.spmv_cap || R->A_P != 1 || R->At_P != 1) {
+    dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
+                    b->o.spmv_cap);
+    dpdlp_i_batch_unreserve(b, i);
+    return -1;
+  }
+  bslot_t *q = &b->sl[i];
+  memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
+  q->t_add = dpdlp_wall();
+  q->tag = tag;
+  q->R = R;
+  hipStream_t s = stream ? (hipStream_t)stream : b->s;
+  if (dpdlp_i_ctx_setup(&q->c, R, &b->prm, b->o.precision == DPDLP_PREC_MIXED, s,
+                        b->d_scal + (size_t)i * DPDLP_SCAL_LEN,
+                        b->h_scal + (size_t)i * DPDLP_SCAL_LEN, &q->rescale_time)) {
+    dpdlp_i_ctx_free(&q->c);
+    memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
+    dpdlp_i_batch_unreserve(b, i);
+    return -1;
+  }
+  q->own_R = take_ownership;
+  pthread_mutex_lock(&b->mu);
+  set_stage(q, S_READY);
+  b->in_flight++;
+  pthread_mutex_unlock(&b->mu);
+  return 0;
+}
+
+/* READY slots join (main thread, at a window start): power iteration start
+ * (utils.cu:80-135) or, without nonzeros, the loop directly */
+static int activate_ready(dpdlp_batch_t *b) {
+  for (int i = 0; i < b->cap; ++i) {
+    bslot_t *q = &b->sl[i];
+    if (stage_of(q) != S_READY) continue;
+    dpdlp_resident_t *R = q->R;
+    q->c.s = b->s; /* it may have been set up on another stream */
+    fill_blp(&b->h_lps[i], &q->c);
+    q->t_pi0 = dpdlp_wall();
+    q->sigma_sq = 1.0;
+    if (i + 1 > b->nhw) b->nhw = i + 1;
+    b->dirty = 1;
+    if (R->nnz > 0 && b->prm.sv_max_iter > 0) {
+      if (ensure_normals(b, R->m)) return -1;
+      if (hipMemcpyAsync(q->c.pi_eig, b->d_normals, (size_t)R->m * sizeof(double),
+                         hipMemcpyDeviceToDevice, b->s) != hipSuccess ||
+          dpdlp_dev_power_init(&q->c.st, q->c.pi_eig, q->c.pi_next, b->s)) {
+        dpdlp_set_error("power-iteration start failed on the device");
+        return -1;
+      }
+      set_stage(q, S_PI);
+    } else {
+      start_loop(b, q);
+      q->fresh = 0; /* iteration 0 is evaluated with the others below */
+    }
+  }
+  return 0;
 }
 
 int dpdlp_batch_add(dpdlp_batch_t *b, dpdlp_resident_t *R, int take_ownership, long long tag) {
@@ -304,54 +403,12 @@ int dpdlp_batch_add(dpdlp_batch_t *b, dpdlp_resident_t *R, int take_ownership, l
     dpdlp_set_error("NULL batch or problem");
     return -1;
   }
-  if (R->cap != b->o.spmv_cap || R->A_P != 1 || R->At_P != 1) {
-    dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
-                    b->o.spmv_cap);
-    return -1;
-  }
-  int i = 0;
-  while (i < b->cap && b->sl[i].stage != S_FREE) ++i;
-  if (i == b->cap) {
+  int i = dpdlp_i_batch_reserve(b);
+  if (i < 0) {
     dpdlp_set_error("batch full (%d LPs)", b->cap);
     return -1;
   }
-  bslot_t *q = &b->sl[i];
-  memset(q, 0, sizeof(*q));
-  q->t_add = dpdlp_wall();
-  q->tag = tag;
-  q->R = R;
-  if (dpdlp_i_ctx_setup(&q->c, R, &b->prm, b->o.precision == DPDLP_PREC_MIXED, b->s,
-                        b->d_scal + (size_t)i * DPDLP_SCAL_LEN,
-                        b->h_scal + (size_t)i * DPDLP_SCAL_LEN, &q->rescale_time)) {
-    dpdlp_i_ctx_free(&q->c);
-    memset(q, 0, sizeof(*q));
-    return -1;
-  }
-  fill_blp(&b->h_lps[i], &q->c);
-  q->t_pi0 = dpdlp_wall();
-  q->sigma_sq = 1.0;
-  if (R->nnz > 0 && b->prm.sv_max_iter > 0) {
-    if (ensure_normals(b, R->m)) goto fail;
-    if (hipMemcpyAsync(q->c.pi_eig, b->d_normals, (size_t)R->m * sizeof(double),
-                       hipMemcpyDeviceToDevice, b->s) != hipSuccess ||
-        dpdlp_dev_power_init(&q->c.st, q->c.pi_eig, q->c.pi_next, b->s)) {
-      dpdlp_set_error("power-iteration start failed on the device");
-      goto fail;
-    }
-    q->stage = S_PI;
-  } else {
-    start_loop(b, q);
-    q->fresh = 0; /* its first evaluation comes with the next window */
-  }
-  q->own_R = take_ownership;
-  if (i + 1 > b->nhw) b->nhw = i + 1;
-  b->in_flight++;
-  b->dirty = 1;
-  return 0;
-fail:
-  dpdlp_i_ctx_free(&q->c);
-  memset(q, 0, sizeof(*q));
-  return -1;
+  return dpdlp_i_batch_prepare(b, i, R, take_ownership, tag, NULL);
 }
 
 /* ---- block lists ---- */
@@ -360,7 +417,7 @@ static int rebuild_lists(dpdlp_batch_t *b) {
   long long cnt[NLIST] = {0, 0, 0, 0, 0};
   for (int i = 0; i < b->nhw; ++i) {
     bslot_t *q = &b->sl[i];
-    if (q->stage == S_FREE) continue;
+    if (!active(q)) continue;
     const dpdlp_resident_t *R = q->R;
     cnt[0] += R->A_nblk;
     cnt[1] += R->At_nblk;
@@ -408,7 +465,7 @@ static int rebuild_lists(dpdlp_batch_t *b) {
   memset(rng, 0, (size_t)b->cap * 2 * NLIST * sizeof(int));
   for (int i = 0; i < b->nhw; ++i) {
     bslot_t *q = &b->sl[i];
-    if (q->stage == S_FREE) continue;
+    if (!active(q)) continue;
     const dpdlp_resident_t *R = q->R;
     for (int k = 0; k < NLIST; ++k) rng[(size_t)k * 2 * b->cap + 2 * i] = (int)pos[k];
     dpdlp_bblk_t *L = b->h_list;
@@ -499,7 +556,7 @@ static int eval_and_read(dpdlp_batch_t *b, int any_eval, int any_inf) {
     memset(b->h_ctl_eval, 0, (size_t)b->nhw * sizeof(dpdlp_bctl_t));
     for (int i = 0; i < b->nhw; ++i) {
       bslot_t *q = &b->sl[i];
-      if (q->stage != S_PDHG || !q->ev_on) continue;
+      if (stage_of(q) != S_PDHG || !q->ev_on) continue;
       dpdlp_bctl_t *c = &b->h_ctl_eval[i];
       c->eval = 1;
       c->restart_dist = q->ev_rc && !q->ev_final;
@@ -564,11 +621,14 @@ static int finish(dpdlp_batch_t *b, int i) {
   }
   dpdlp_i_ctx_free(&q->c);
   if (q->own_R) dpdlp_resident_free(q->R);
-  memset(q, 0, sizeof(*q));
+  memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
   memset(&b->h_lps[i], 0, sizeof(dpdlp_blp_t));
-  b->in_flight--;
   b->dirty = 1;
-  while (b->nhw > 0 && b->sl[b->nhw - 1].stage == S_FREE) b->nhw--;
+  pthread_mutex_lock(&b->mu);
+  set_stage(q, S_FREE);
+  b->in_flight--;
+  pthread_mutex_unlock(&b->mu);
+  while (b->nhw > 0 && !active(&b->sl[b->nhw - 1])) b->nhw--;
   return 0;
 }
 
@@ -588,7 +648,11 @@ static int process_eval(dpdlp_batch_t *b, int i) {
   if (q->ev_inf) ctl_infeasibility(h, &q->c.k, S);
   h->t_cum = dpdlp_wall() - q->t_loop0;
   double kkt = fmax(h->rel_p, fmax(h->rel_d, h->rel_gap));
-  if (q->c.use32 && kkt < b->o.mixed_switch_tol) q->c.use32 = 0;
+  if (q->c.use32 &&
+      !ctl_mixed_keep_fp32(h, kkt, b->o.mixed_switch_tol,
+                           fmax(p->termination_criteria.eps_optimal_relative,
+                                p->termination_criteria.eps_feasible_relative)))
+    q->c.use32 = 0;
   ctl_check_termination(h, &p->termination_criteria);
   if (h->reason != TERMINATION_REASON_UNSPECIFIED) return finish(b, i);
   if (q->ev_rc) {
@@ -634,7 +698,8 @@ static int launch_window(dpdlp_batch_t *b) {
   int any_phase[10] = {0}, any_fpe[10] = {0}, any_pi[10] = {0}, any_restart = 0;
   for (int i = 0; i < nl; ++i) {
     bslot_t *q = &b->sl[i];
-    if (q->stage == S_PDHG) {
+    int st = stage_of(q);
+    if (st == S_PDHG) {
       hstate_t *h = &q->h;
       for (int r = 0; r < W && h->total < limit; ++r) {
         dpdlp_bctl_t *c = &ctl[(size_t)r * nl + i];
@@ -665,7 +730,7 @@ static int launch_window(dpdlp_batch_t *b) {
         any_restart |= dr;
       }
       q->do_restart = 0;
-    } else if (q->stage == S_PI) {
+    } else if (st == S_PI) {
       int left = p->sv_max_iter - q->pi_iters;
       int kk = left < W ? left : W;
       for (int r = 0; r < kk; ++r) {
@@ -707,16 +772,17 @@ static int launch_window(dpdlp_batch_t *b) {
 
 int dpdlp_batch_step(dpdlp_batch_t *b) {
   if (!b) return -1;
-  if (b->in_flight == 0) return 0;
+  if (activate_ready(b)) return -1;
+  if (b->nhw == 0) return dpdlp_batch_in_flight(b);
   /* evaluation points and power-iteration reads at this window start */
   int any_eval = 0, any_inf = 0, any_read = 0;
   for (int i = 0; i < b->nhw; ++i) {
     bslot_t *q = &b->sl[i];
-    if (q->stage == S_PDHG) {
+    if (stage_of(q) == S_PDHG) {
       mark_eval(b, q);
       any_eval |= q->ev_on;
       any_inf |= q->ev_inf;
-    } else if (q->stage == S_PI && q->pi_issued > 0) {
+    } else if (stage_of(q) == S_PI && q->pi_issued > 0) {
       any_read = 1;
     }
   }
@@ -725,14 +791,14 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
     int any_fresh = 0;
     for (int i = 0; i < b->nhw; ++i) {
       bslot_t *q = &b->sl[i];
-      if (q->stage == S_PI && q->pi_issued > 0) {
+      if (stage_of(q) == S_PI && q->pi_issued > 0) {
         process_pi(b, i);
         any_fresh |= q->fresh;
       }
     }
     for (int i = 0; i < b->nhw; ++i) {
       bslot_t *q = &b->sl[i];
-      if (q->stage == S_PDHG && q->ev_on && !q->fresh)
+      if (stage_of(q) == S_PDHG && q->ev_on && !q->fresh)
         if (process_eval(b, i)) return -1;
     }
     if (any_fresh) {
@@ -740,7 +806,7 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
       any_inf = 0;
       for (int i = 0; i < b->nhw; ++i) {
         bslot_t *q = &b->sl[i];
-        if (q->stage == S_PDHG) {
+        if (stage_of(q) == S_PDHG) {
           if (q->fresh)
             mark_eval(b, q);
           else
@@ -751,17 +817,18 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
       if (eval_and_read(b, 1, any_inf)) return -1;
       for (int i = 0; i < b->nhw; ++i) {
         bslot_t *q = &b->sl[i];
-        if (q->stage == S_PDHG && q->fresh) {
+        if (stage_of(q) == S_PDHG && q->fresh) {
           q->fresh = 0;
           if (q->ev_on && process_eval(b, i)) return -1;
         }
       }
     }
   }
-  if (b->in_flight == 0) return 0;
-  if (b->dirty && rebuild_lists(b)) return -1;
-  if (launch_window(b)) return -1;
-  return b->in_flight;
+  if (b->nhw > 0) {
+    if (b->dirty && rebuild_lists(b)) return -1;
+    if (launch_window(b)) return -1;
+  }
+  return dpdlp_batch_in_flight(b);
 }
 
 int dpdlp_batch_take(dpdlp_batch_t *b, long long *tag, cupdlpx_result_t **res,

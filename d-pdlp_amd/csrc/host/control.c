@@ -185,3 +185,14 @@ void ctl_pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, doub
   }
 }
 
+
+int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps) {
+  if (kkt < switch_tol) return 0;
+  if (kkt <= 0.9 * h->m32_best) {
+    h->m32_best = kkt;
+    h->m32_best_T = h->total;
+    return 1;
+  }
+  int window = h->total / 4 > 200 ? h->total / 4 : 200;
+  return !(h->m32_best < 100.0 * eps && h->total - h->m32_best_T >= window);
+}

@@ -23,6 +23,8 @@ typedef struct {
   termination_reason_t reason;
   double t_cum;
   int pend_rst, pend_maj;
+  double m32_best; /* mixed precision: best max relative KKT seen in FP32 */
+  int m32_best_T;  /* iteration at which it was reached */
 } hstate_t;
 
 
@@ -39,6 +41,12 @@ void ctl_check_termination(hstate_t *h, const termination_criteria_t *tc);
 void ctl_check_polish_termination(hstate_t *h, const termination_criteria_t *tc, int primal);
 int ctl_should_restart(hstate_t *h, const restart_parameters_t *rp, int freq);
 void ctl_pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, double dd);
+/* Mixed precision: 1 while FP32 iterations should continue.  FP32 ends once
+ * the max relative KKT error is below switch_tol, or once it is within 100x
+ * of the target eps and the best value seen in FP32 has not improved by 10%
+ * for max(200, T/4) iterations (the FP32 rounding floor of the residuals is
+ * near the target). */
+int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps);
 
 #ifdef __cplusplus
 }

@@ -331,3 +331,28 @@ lp_problem_t *dpdlp_generate(const dpdlp_gen_spec_t *g, double *opt_obj) {
   free(ax);
   return p;
 }
+
+/* Batch workload member k: m log-uniform in [m_lo, m_hi], n = m * U[1.5, 3],
+ * Poisson(8)+1 nonzeros per row, LP seed base_seed + k.  The sizes come
+ * from their own xoshiro stream seeded with (base_seed + k) ^ a constant, so
+ * member k is the same LP whichever members are drawn with it. */
+void dpdlp_batch_member_spec(int k, uint64_t base_seed, int m_lo, int m_hi,
+                             dpdlp_gen_spec_t *out) {
+  xr_t R;
+  xr_seed(&R, (base_seed + (uint64_t)k) ^ 0x5eed5eed5eed5eedull);
+  double lo = log((double)m_lo), hi = log((double)m_hi);
+  int m = (int)llround(exp(lo + (hi - lo) * xr_u01(&R)));
+  if (m < 1) m = 1;
+  int n = (int)llround((double)m * (1.5 + 1.5 * xr_u01(&R)));
+  if (n < 1) n = 1;
+  memset(out, 0, sizeof(*out));
+  out->m = m;
+  out->n = n;
+  out->nnz = 0;
+  out->row_rule = 2;
+  out->poisson_lambda = 8.0;
+  out->zipf_s = 1.5;
+  out->zipf_max = 50000;
+  out->pm1_frac = 0.0;
+  out->seed = base_seed + (uint64_t)k;
+}

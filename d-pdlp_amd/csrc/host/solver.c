@@ -848,6 +848,7 @@ void dpdlp_i_init_hstate(hstate_t *h, const ctx_t *c, const pdhg_parameters_t *p
   h->best_gap = INFINITY;
   h->fpe_last = INFINITY;
   h->reason = TERMINATION_REASON_UNSPECIFIED;
+  h->m32_best = INFINITY;
 }
 
 cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale_time,
@@ -1019,7 +1020,10 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
       double kkt = fmax(h->rel_p, fmax(h->rel_d, h->rel_gap));
       if (opts->time_to_1e4_sec < 0 && kkt < 1e-4) opts->time_to_1e4_sec = h->t_cum;
       if (opts->time_to_1e6_sec < 0 && kkt < 1e-6) opts->time_to_1e6_sec = h->t_cum;
-      if (c->use32 && kkt < opts->mixed_switch_tol) c->use32 = 0;
+      if (c->use32 && !ctl_mixed_keep_fp32(h, kkt, opts->mixed_switch_tol,
+                                           fmax(tc->eps_optimal_relative,
+                                                tc->eps_feasible_relative)))
+        c->use32 = 0;
       ctl_check_termination(h, tc);
       if (params->verbose)
         dpdlp_print_iteration(h->total, h->t_cum, h->pobj, h->dobj, h->abs_p, h->abs_d, h->gap,

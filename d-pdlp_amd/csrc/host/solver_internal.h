@@ -52,3 +52,10 @@ size_t dpdlp_i_ctx_footprint(const dpdlp_resident_t *R, int mixed, int own_scal)
 /* Feasibility polishing after the main loop (pdlp_core_op.cu:862-1030). */
 int dpdlp_i_feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *params,
                                double *ptime, int *piters);
+
+/* batch.c: slot hand-over for set-up on another thread (the serving loop) */
+typedef struct dpdlp_batch dpdlp_batch_t;
+int dpdlp_i_batch_reserve(dpdlp_batch_t *b);
+void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int slot);
+int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int slot, dpdlp_resident_t *R, int take_ownership,
+                          long long tag, void *stream);

@@ -23,6 +23,7 @@ CLI_PATH = os.path.join(PKG_ROOT, "lib", "cupdlpx-dist")
 
 from ._types import *  # noqa: F401,F403
 from ._types import (_dp, _ip, _Csr, _Dense, BatchOpts, CupdlpxResult, GenSpec, LpProblem,
+                     ServeReport, ServeSpec,
                      MatrixDesc, PdhgParameters, SolveOpts, TERMINATION, MATRIX_COO,
                      MATRIX_CSC, MATRIX_CSR, MATRIX_DENSE, PREC_FP64)
 
@@ -58,6 +59,12 @@ def _load() -> C.CDLL:
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
         "dpdlp_upload_ex": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int]),
         "dpdlp_resident_bytes": (C.c_size_t, [C.c_void_p]),
+        "dpdlp_batch_member_spec": (None, [C.c_int, C.c_uint64, C.c_int, C.c_int,
+                                           C.POINTER(GenSpec)]),
+        "dpdlp_serve_default_spec": (None, [C.POINTER(ServeSpec)]),
+        "dpdlp_serve_run": (C.c_int, [C.POINTER(ServeSpec), C.POINTER(PdhgParameters),
+                                      C.POINTER(ServeReport), _dp, _ip, C.POINTER(R),
+                                      C.c_int]),
         "dpdlp_batch_default_opts": (None, [C.POINTER(BatchOpts)]),
         "dpdlp_batch_create": (C.c_void_p, [C.POINTER(PdhgParameters), C.POINTER(BatchOpts)]),
         "dpdlp_batch_add": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_longlong]),
@@ -440,6 +447,41 @@ def generate(m, n, nnz=0, row_rule=0, seed=0, zipf_s=1.5, zipf_max=50000,
     """dpdlp_generate: (Problem, known optimal objective)."""
     spec = GenSpec(m=m, n=n, nnz=nnz, row_rule=row_rule, zipf_s=zipf_s, zipf_max=zipf_max,
                    poisson_lambda=poisson_lambda, pm1_frac=pm1_frac, seed=seed)
+    opt = C.c_double(0.0)
+    ptr = lib.dpdlp_generate(C.byref(spec), C.byref(opt))
+    return Problem(ptr), opt.value
+
+
+def serve(params=None, max_requests=0, want_results=False, **spec_kw):
+    """dpdlp_serve_run: returns (report dict, latencies, pool indices, results or None)."""
+    sp = ServeSpec()
+    lib.dpdlp_serve_default_spec(C.byref(sp))
+    for k, v in spec_kw.items():
+        if not hasattr(sp, k):
+            raise KeyError(k)
+        setattr(sp, k, v)
+    rep = ServeReport()
+    cap = max_requests if max_requests > 0 else int(sp.rate * sp.duration_sec * 1.5 + 64)
+    lat = np.full(cap, np.nan)
+    idx = np.full(cap, -1, dtype=np.int32)
+    res = (C.POINTER(CupdlpxResult) * cap)() if want_results else None
+    pp = params if params is not None else default_params()
+    rc = lib.dpdlp_serve_run(C.byref(sp), C.byref(pp), C.byref(rep), _ptr(lat),
+                             _ptr(idx, C.c_int), res, cap)
+    if rc != 0:
+        raise RuntimeError("serve run failed: " + last_error())
+    n = rep.arrived
+    report = {name: getattr(rep, name) for name, _ in ServeReport._fields_}
+    results = None
+    if want_results:
+        results = [result_from_ptr(res[i]) if res[i] else None for i in range(n)]
+    return report, lat[:n], idx[:n], results
+
+
+def batch_member(k, base_seed=3000, m_lo=500, m_hi=200_000):
+    """Member k of the uneven batch workload: (Problem, known optimum)."""
+    spec = GenSpec()
+    lib.dpdlp_batch_member_spec(k, base_seed, m_lo, m_hi, C.byref(spec))
     opt = C.c_double(0.0)
     ptr = lib.dpdlp_generate(C.byref(spec), C.byref(opt))
     return Problem(ptr), opt.value

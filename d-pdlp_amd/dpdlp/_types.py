@@ -128,3 +128,24 @@ class BatchOpts(C.Structure):
     """dpdlp_batch_opts_t (include/dpdlp_ext.h)."""
     _fields_ = [("max_lps", C.c_int), ("precision", C.c_int), ("mixed_switch_tol", C.c_double),
                 ("spmv_cap", C.c_int)]
+
+
+class ServeSpec(C.Structure):
+    """dpdlp_serve_spec_t (include/dpdlp_ext.h)."""
+    _fields_ = [("rate", C.c_double), ("duration_sec", C.c_double), ("drain_sec", C.c_double),
+                ("seed", C.c_uint64), ("pool_small", C.c_int), ("pool_large", C.c_int),
+                ("large_frac", C.c_double), ("large_m", C.c_int), ("large_nnz", C.c_longlong),
+                ("m_lo", C.c_int), ("m_hi", C.c_int), ("member_seed", C.c_uint64),
+                ("hbm_budget_gb", C.c_double), ("max_lps", C.c_int), ("precision", C.c_int),
+                ("mode", C.c_int), ("spmv_cap", C.c_int)]
+
+
+class ServeReport(C.Structure):
+    """dpdlp_serve_report_t (include/dpdlp_ext.h)."""
+    _fields_ = [("arrived", C.c_int), ("completed", C.c_int), ("optimal", C.c_int),
+                ("unfinished", C.c_int), ("makespan_sec", C.c_double),
+                ("lps_per_sec", C.c_double), ("p50_sec", C.c_double), ("p90_sec", C.c_double),
+                ("p99_sec", C.c_double), ("mean_sec", C.c_double), ("max_sec", C.c_double),
+                ("max_in_flight", C.c_int), ("peak_gb", C.c_double),
+                ("pool_gen_sec", C.c_double), ("windows", C.c_longlong),
+                ("syncs", C.c_longlong)]

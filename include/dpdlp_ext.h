@@ -141,6 +141,47 @@ void dpdlp_batch_free(dpdlp_batch_t *b);
 int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parameters_t *params,
                       const dpdlp_batch_opts_t *o, cupdlpx_result_t **results);
 
+/* ---- serving: a Poisson request stream with continuous batching ---- */
+
+typedef struct {
+  double rate;          /* arrivals per second (Poisson process) */
+  double duration_sec;  /* requests arrive during [0, duration) */
+  double drain_sec;     /* the run stops at duration + drain at the latest */
+  uint64_t seed;        /* arrival gaps and request draws */
+  int pool_small;       /* distinct uneven LPs: batch workload members 0.. */
+  int pool_large;       /* distinct large LPs */
+  double large_frac;    /* share of requests drawn from the large pool */
+  int large_m;          /* large LP rows (columns m * U[1.5, 3]) */
+  long long large_nnz;  /* large LP nonzeros (Poisson rows of that mean) */
+  int m_lo, m_hi;       /* uneven LPs: m log-uniform in [m_lo, m_hi] */
+  uint64_t member_seed; /* LP seeds */
+  double hbm_budget_gb; /* admission: device bytes of the LPs in flight */
+  int max_lps;          /* admission: LPs in flight */
+  int precision;        /* batch precision (DPDLP_PREC_MIXED default) */
+  int mode;             /* 0 continuous batching, 1 plain loop (one at a time, FP64) */
+  int spmv_cap;         /* row-block capacity (1024) */
+} dpdlp_serve_spec_t;
+
+typedef struct {
+  int arrived, completed, optimal, unfinished;
+  double makespan_sec;  /* clock start to last completion */
+  double lps_per_sec;   /* completed / makespan */
+  double p50_sec, p90_sec, p99_sec, mean_sec, max_sec; /* completion - arrival;
+                           unfinished requests count as +inf in the percentiles */
+  int max_in_flight;
+  double peak_gb;       /* admitted device bytes, peak */
+  double pool_gen_sec;  /* host generation of the request pool (not timed) */
+  long long windows, syncs;
+} dpdlp_serve_report_t;
+
+void dpdlp_serve_default_spec(dpdlp_serve_spec_t *s);
+/* Runs one request stream.  Optional outputs, one entry per arrived request
+ * (at most max_requests when > 0): latency, pool index, result (caller frees
+ * each with cupdlpx_result_free; NULL when unfinished).  Returns 0 or -1. */
+int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *params,
+                    dpdlp_serve_report_t *rep, double *latency_out, int *pool_index_out,
+                    cupdlpx_result_t **results_out, int max_requests);
+
 /* ---- seeded synthetic LPs (feasible by construction) ---- */
 typedef struct {
   int m, n;
@@ -156,6 +197,10 @@ typedef struct {
 /* Generates an lp_problem_t (free with lp_problem_free) and, optionally, the
  * known optimal objective c'x*. */
 lp_problem_t *dpdlp_generate(const dpdlp_gen_spec_t *spec, double *opt_obj);
+/* Member k of the uneven batch workload: m log-uniform in [m_lo, m_hi],
+ * n = m * U[1.5, 3], Poisson(8)+1 nonzeros per row, LP seed base_seed + k. */
+void dpdlp_batch_member_spec(int k, uint64_t base_seed, int m_lo, int m_hi,
+                             dpdlp_gen_spec_t *out);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,74 @@
+"""Serving with continuous batching: every served request's result equals the
+same LP solved alone, requests queue for slots and memory and are backfilled
+as others retire, and the plain one-at-a-time loop serves the same stream."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+POOL = dict(pool_small=6, pool_large=0, large_frac=0.0, m_lo=300, m_hi=3000,
+            member_seed=3600)
+
+
+def same(a, b):
+    return (a.status == b.status and a.iterations == b.iterations
+            and np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+            and a.primal_obj == b.primal_obj and a.dual_obj == b.dual_obj)
+
+
+def alone_results(dp, params, precision):
+    out = []
+    for k in range(POOL["pool_small"]):
+        prob, _ = dp.batch_member(k, POOL["member_seed"], POOL["m_lo"], POOL["m_hi"])
+        r = dp.Resident(prob, spmv_cap=1024, panels=(1, 1))
+        out.append(dp.solve_batch([r], params, spmv_cap=1024, precision=precision)[0])
+    return out
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_served_results_equal_alone(dp, precision):
+    params = dp.default_params(presolve=False)
+    ref = alone_results(dp, params, precision)
+    # ~60 requests in 0.2 s into at most 5 slots: requests queue, join running
+    # batches and backfill retired slots
+    rep, lat, idx, res = dp.serve(params, want_results=True, rate=300.0, duration_sec=0.2,
+                                  drain_sec=60.0, seed=11, max_lps=5, precision=precision,
+                                  **POOL)
+    assert rep["arrived"] > 20
+    assert rep["completed"] == rep["arrived"] and rep["unfinished"] == 0
+    assert rep["optimal"] == rep["arrived"]
+    assert rep["max_in_flight"] <= 5
+    assert np.all(lat > 0) and rep["p99_sec"] >= rep["p50_sec"] > 0
+    for k in range(rep["arrived"]):
+        assert same(res[k], ref[idx[k]]), (k, idx[k])
+
+
+def test_plain_loop_serves_same_results(dp):
+    params = dp.default_params(presolve=False)
+    kw = dict(rate=100.0, duration_sec=0.2, drain_sec=60.0, seed=5, precision=0, **POOL)
+    rb, _, ib, resb = dp.serve(params, want_results=True, mode=0, max_lps=8, **kw)
+    rp, _, ip, resp = dp.serve(params, want_results=True, mode=1, **kw)
+    assert rb["arrived"] == rp["arrived"] > 5 and np.array_equal(ib, ip)
+    assert rp["max_in_flight"] == 1 and rp["completed"] == rp["arrived"]
+    # small LPs: the single-LP path uses the same row-block layout, so the
+    # FP64 results agree bit for bit
+    for k in range(rb["arrived"]):
+        assert same(resb[k], resp[k]), k
+
+
+def test_memory_admission_limits_in_flight(dp):
+    params = dp.default_params(presolve=False)
+    rep, lat, idx, _ = dp.serve(params, rate=200.0, duration_sec=0.1, drain_sec=60.0, seed=3,
+                                max_lps=64, hbm_budget_gb=1e-6, precision=0, **POOL)
+    # a budget below one LP admits them one at a time (never zero in flight)
+    assert rep["max_in_flight"] == 1
+    assert rep["completed"] == rep["arrived"] > 3
+
+
+def test_drain_cap_reports_unfinished(dp):
+    params = dp.default_params(presolve=False, termination_criteria__iteration_limit=10**9)
+    rep, lat, idx, _ = dp.serve(params, rate=400.0, duration_sec=0.05, drain_sec=0.0, seed=9,
+                                max_lps=4, precision=0, **POOL)
+    assert rep["arrived"] > 5
+    assert rep["completed"] + rep["unfinished"] == rep["arrived"]
+    assert np.isinf(lat).sum() == rep["unfinished"]

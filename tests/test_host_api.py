@@ -263,3 +263,33 @@ def test_generator_rules(dp):
     assert np.all(a["con_lb"] <= a["con_ub"]) and np.isfinite(opt)
     p2, opt2 = dp.generate(3000, 7000, 45000, row_rule=1, seed=5)
     assert opt2 == opt and np.array_equal(p2.arrays()["vals"], a["vals"])
+
+
+def test_extension_struct_layouts_match_c(dp, tmp_path):
+    """ctypes mirrors of the extension structs agree with the C compiler's
+    sizes and offsets (include/dpdlp_ext.h)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    mirrors = {"dpdlp_solve_opts_t": dp.SolveOpts, "dpdlp_gen_spec_t": dp.GenSpec,
+               "dpdlp_batch_opts_t": dp.BatchOpts, "dpdlp_serve_spec_t": dp.ServeSpec,
+               "dpdlp_serve_report_t": dp.ServeReport}
+    lines = ['#include "dpdlp_ext.h"', "#include <stddef.h>", "#include <stdio.h>",
+             "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, val = line.split()
+        py = mirrors[cname]
+        want = C.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert int(val) == want, (cname, field, int(val), want)
