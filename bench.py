@@ -36,7 +36,177 @@ WORKLOADS = {
                  desc="m=2e3 n=5e3 nnz=5e4 uniform positions, FP64, eps 1e-4"),
     "cfg2": dict(m=4_000_000, n=8_000_000, nnz=120_000_000, row_rule=1, pm1_frac=0.0,
                  seed=2000, desc="m=4e6 n=8e6 nnz=1.2e8 Zipf(1.5) rows"),
+    "cfg3": dict(count=512, base_seed=3000, m_lo=500, m_hi=200_000,
+                 desc="batch of 512 LPs: m log-uniform [500, 2e5], n = m*U[1.5,3], "
+                      "Poisson(8)+1 nnz/row, seeds 3000..3511, eps 1e-4"),
+    "cfg4": dict(rates=(50.0, 200.0, 800.0), large_frac=0.05, large_m=1_000_000,
+                 large_nnz=10_000_000, budget_gb=200.0,
+                 desc="serving: Poisson arrivals, LPs from the batch generator + 5% "
+                      "m=1e6 nnz=1e7, 200 GB admission, eps 1e-4"),
 }
+
+
+def finite(v):
+    """JSON-safe number (None for inf / nan)."""
+    import math
+    if isinstance(v, float) and not math.isfinite(v):
+        return None
+    return v
+
+
+def clean(o):
+    if isinstance(o, dict):
+        return {k: clean(v) for k, v in o.items()}
+    if isinstance(o, (list, tuple)):
+        return [clean(v) for v in o]
+    return finite(o)
+
+
+def cpu_batch_baseline(probs, budget_s):
+    """OpenMP oracle solving batch members in index order for about budget_s."""
+    import oracle_py as orc
+    L = orc.lib(omp=True)
+    threads = L.orc_num_threads()
+    prm = orc.default_params(presolve=False)
+    t0 = time.perf_counter()
+    done = 0
+    for prob, _ in probs:
+        orc.solve(prob.ptr, prm, omp=True)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=done / el, unit="LPs/s", cores=threads, kind="port",
+                sample=f"OpenMP FP64 oracle ({threads} threads) solving batch members 0..{done - 1} "
+                       f"to eps 1e-4 one after another: {done} LPs in {el:.1f} s")
+
+
+def bench_batch(args, dpdlp):
+    w = WORKLOADS["cfg3"]
+    count = args.batch_count or w["count"]
+    t = time.perf_counter()
+    probs = [dpdlp.batch_member(k, w["base_seed"], w["m_lo"], w["m_hi"]) for k in range(count)]
+    gen_s = time.perf_counter() - t
+    nnz = sum(p.nnz for p, _ in probs)
+    print(f"# generated {count} LPs, nnz={nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    plain_res = [dpdlp.Resident(p) for p, _ in probs]
+    batch_res = [dpdlp.Resident(p, spmv_cap=1024, panels=(1, 1)) for p, _ in probs]
+    up_s = time.perf_counter() - t
+    params = dpdlp.default_params(presolve=False)
+    for k in range(args.warmup):
+        plain_res[0].solve(params)
+        dpdlp.solve_batch(batch_res[:4], params, spmv_cap=1024, precision=dpdlp.PREC_MIXED)
+    # plain loop: the single-LP FP64 path, one LP after another
+    t = time.perf_counter()
+    plain = [r.solve(params)[0] for r in plain_res]
+    plain_s = time.perf_counter() - t
+    print(f"# plain loop: {plain_s:.2f}s {count / plain_s:.1f} LPs/s", file=sys.stderr, flush=True)
+
+    def run_batch(prec, profile):
+        st = {}
+        t0 = time.perf_counter()
+        got = dpdlp.solve_batch(batch_res, params, stats=st, max_lps=count, spmv_cap=1024,
+                                precision=prec, profile_phases=profile)
+        el = time.perf_counter() - t0
+        cnt = {k: st[k] for k in ("windows", "syncs", "launches")}
+        ph = {"samples": st["phase_samples"], "primal_ms": st["primal_ms"],
+              "primal_bytes": st["primal_bytes"], "dual_ms": st["dual_ms"],
+              "dual_bytes": st["dual_bytes"]}
+        return got, el, cnt, ph
+
+    runs = []
+    for k in range(args.steps):
+        res, el, cnt, ph = run_batch(dpdlp.PREC_MIXED, 1)
+        runs.append(el)
+        print(f"# batch mixed step {k}: {el:.2f}s {count / el:.1f} LPs/s {cnt}", file=sys.stderr,
+              flush=True)
+    mixed, mixed_ph, mixed_cnt = res, ph, cnt
+    fp64, fp64_s, fp64_cnt, _ = run_batch(dpdlp.PREC_FP64, 0)
+    print(f"# batch fp64: {fp64_s:.2f}s {count / fp64_s:.1f} LPs/s", file=sys.stderr, flush=True)
+    step_s = sum(runs) / len(runs)
+
+    def obj_err(a, b):
+        return max(abs(x.primal_obj - y.primal_obj) / max(1.0, abs(y.primal_obj))
+                   for x, y in zip(a, b))
+    statuses = {}
+    for r in mixed:
+        statuses[r.status] = statuses.get(r.status, 0) + 1
+    p1 = mixed_ph["primal_bytes"] / (mixed_ph["primal_ms"] * 1e-3) / 1e9 if mixed_ph["primal_ms"] else None
+    p2 = mixed_ph["dual_bytes"] / (mixed_ph["dual_ms"] * 1e-3) / 1e9 if mixed_ph["dual_ms"] else None
+    dom = ("batched_primal_phase", p1, mixed_ph["primal_ms"], mixed_ph["primal_bytes"]) \
+        if mixed_ph["primal_ms"] >= mixed_ph["dual_ms"] else \
+        ("batched_dual_phase", p2, mixed_ph["dual_ms"], mixed_ph["dual_bytes"])
+    cpu = None
+    if not args.no_cpu:
+        print("# cpu baseline sample ...", file=sys.stderr, flush=True)
+        cpu = cpu_batch_baseline(probs, args.cpu_budget)
+    n_s = max(mixed_ph["samples"], 1)
+    out = {
+        "metric": "time-to-1e-4 KKT & SpMV HBM GB/s; served LPs/s + p99 latency vs plain loop",
+        "value": count / step_s, "unit": "LPs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": step_s * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32-matrix/f64",
+        "data": "synthetic (seeded generator, feasible by construction)",
+        "config": {"workload": f"cfg3: {w['desc']}", "lps": count, "nnz_total": nnz,
+                   "m_total": sum(p.m for p, _ in probs), "n_total": sum(p.n for p, _ in probs),
+                   "parallelism": "one GPU, batched launches"},
+        "statuses": statuses,
+        "plain_loop": {"s": plain_s, "LPs_per_s": count / plain_s, "dtype": "f64",
+                       "iterations_mean": sum(r.iterations for r in plain) / count},
+        "speedup_vs_plain_loop": plain_s / step_s,
+        "batch_fp64": {"s": fp64_s, "LPs_per_s": count / fp64_s,
+                       "speedup_vs_plain_loop": plain_s / fp64_s, "counters": fp64_cnt},
+        "batch_counters": mixed_cnt,
+        "objective_rel_diff": {"mixed_vs_plain": obj_err(mixed, plain),
+                               "fp64_batch_vs_plain": obj_err(fp64, plain)},
+        "phase_ms_per_sample": {"primal": mixed_ph["primal_ms"] / n_s,
+                                "dual": mixed_ph["dual_ms"] / n_s, "samples": mixed_ph["samples"]},
+        "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[1], "peak": HBM_PEAK_GBPS,
+                     "unit": "GB/s", "frac": (dom[1] or 0) / HBM_PEAK_GBPS, "traffic": None,
+                     "algorithmic_bytes_per_launch": dom[3] / n_s},
+        "phase_GBps": {"primal": p1, "dual": p2},
+        "cpu_baseline": cpu, "generate_s": gen_s, "upload_s": up_s,
+    }
+    print(json.dumps(clean(out)))
+
+
+def bench_serve(args, dpdlp):
+    w = WORKLOADS["cfg4"]
+    params = dpdlp.default_params(presolve=False)
+    rates = [float(x) for x in args.rates.split(",")] if args.rates else list(w["rates"])
+    per = []
+    for rate in rates:
+        row = {"rate": rate}
+        for mode, name in ((0, "batched"), (1, "plain_loop")):
+            t = time.perf_counter()
+            rep, lat, idx, _ = dpdlp.serve(
+                params, rate=rate, duration_sec=args.serve_duration, drain_sec=args.serve_drain,
+                seed=4000, mode=mode, precision=dpdlp.PREC_MIXED if mode == 0 else dpdlp.PREC_FP64,
+                large_frac=w["large_frac"], large_m=w["large_m"], large_nnz=w["large_nnz"],
+                hbm_budget_gb=w["budget_gb"], max_lps=args.serve_max_lps)
+            rep["wall_s"] = time.perf_counter() - t
+            row[name] = rep
+            print(f"# rate {rate:.0f}/s {name}: arrived {rep['arrived']} done {rep['completed']} "
+                  f"{rep['lps_per_sec']:.1f} LPs/s p50 {rep['p50_sec']:.2f}s p99 {rep['p99_sec']:.2f}s "
+                  f"in-flight max {rep['max_in_flight']}", file=sys.stderr, flush=True)
+        per.append(row)
+    top = per[-1]
+    out = {
+        "metric": "time-to-1e-4 KKT & SpMV HBM GB/s; served LPs/s + p99 latency vs plain loop",
+        "value": top["batched"]["lps_per_sec"], "unit": "LPs/s", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": top["batched"]["makespan_sec"] * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-matrix/f64", "data": "synthetic (seeded generator, feasible by construction)",
+        "config": {"workload": f"cfg4: {w['desc']}", "rates": rates,
+                   "arrival_window_s": args.serve_duration, "drain_cap_s": args.serve_drain,
+                   "max_lps": args.serve_max_lps},
+        "per_rate": per,
+        "speedup_vs_plain_loop_at_top_rate": top["batched"]["lps_per_sec"] /
+        max(top["plain_loop"]["lps_per_sec"], 1e-9),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(clean(out)))
 
 
 def precision_compare(R, dpdlp, opt, eps):
@@ -125,6 +295,12 @@ def main():
                     help="also run a 256 MiB plain copy kernel (counter calibration)")
     ap.add_argument("--mixed", action="store_true", help="FP32 matrix / FP64 state")
     ap.add_argument("--verbose", action="store_true", help="print the solver's iteration log")
+    ap.add_argument("--batch-count", type=int, default=0, help="cfg3: LPs in the batch")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="cfg3: CPU sample seconds")
+    ap.add_argument("--rates", default="", help="cfg4: comma-separated arrival rates")
+    ap.add_argument("--serve-duration", type=float, default=30.0, help="cfg4: arrival window s")
+    ap.add_argument("--serve-drain", type=float, default=30.0, help="cfg4: drain cap s")
+    ap.add_argument("--serve-max-lps", type=int, default=512, help="cfg4: LPs in flight")
     ap.add_argument("--compare-precision", type=float, default=0.0, metavar="EPS",
                     help="solve to EPS in FP64 and in mixed precision and report both")
     args = ap.parse_args()
@@ -134,6 +310,10 @@ def main():
         sys.exit(2)
 
     import dpdlp
+    if args.workload == "cfg3":
+        return bench_batch(args, dpdlp)
+    if args.workload == "cfg4":
+        return bench_serve(args, dpdlp)
     w = WORKLOADS[args.workload]
     t = time.perf_counter()
     prob, opt = dpdlp.generate(w["m"], w["n"], w["nnz"], row_rule=w["row_rule"], seed=w["seed"],

@@ -36,8 +36,10 @@ template <> struct BEpi<K_PRIMAL> {
   __device__ static T make(const dpdlp_blp_t &L, const dpdlp_bctl_t &c) {
     T e;
     e.x_cur = L.x_cur; e.x_init = L.x_init; e.c = L.c; e.lv = L.lv; e.uv = L.uv;
-    e.x_ref = L.x_ref; e.x_pdhg = L.x_pdhg; e.z = L.dual_slack; e.dx = L.dx;
-    e.x_ref32 = L.x_ref32;
+    e.x_pdhg = L.x_pdhg; e.z = L.dual_slack; e.dx = L.dx;
+    /* FP32 rounds write only the FP32 copies the next SpMV gathers */
+    e.x_ref = c.use32 ? nullptr : L.x_ref;
+    e.x_ref32 = c.use32 ? L.x_ref32 : nullptr;
     e.tau = c.tau; e.w = c.w; e.rho = c.rho; e.major = c.major; e.fpe = c.fpe; e.partials = c.fpe;
     return e;
   }
@@ -46,8 +48,8 @@ template <> struct BEpi<K_DUAL> {
   using T = EpiDual;
   __device__ static T make(const dpdlp_blp_t &L, const dpdlp_bctl_t &c) {
     T e;
-    e.y_cur = L.y_cur; e.y_init = L.y_init; e.lc = L.lc; e.uc = L.uc; e.y_ref = L.y_ref;
-    e.y_pdhg = L.y_pdhg; e.dy = L.dy; e.y_cur32 = L.y_cur32;
+    e.y_cur = L.y_cur; e.y_init = L.y_init; e.lc = L.lc; e.uc = L.uc;
+    e.y_pdhg = L.y_pdhg; e.dy = L.dy; e.y_cur32 = c.use32 ? L.y_cur32 : nullptr;
     e.sigma = c.sigma; e.w = c.w; e.rho = c.rho; e.major = c.major; e.fpe = c.fpe;
     e.partials = c.fpe;
     return e;
@@ -356,6 +358,14 @@ int dpdlp_bdev_restart_copy(const dpdlp_batch_dev_t *b, void *stream) {
   if (b->neNM <= 0) return 0;
   k_brestart_copy<<<b->neNM, DPDLP_BS, 0, ST(stream)>>>(b->lps, b->ctl, b->eNM);
   return (int)hipGetLastError();
+}
+
+int dpdlp_bdev_primal(const dpdlp_batch_dev_t *b, void *stream) {
+  return (int)launch_b<K_PRIMAL>(b, DPDLP_SL_FPE, ST(stream));
+}
+
+int dpdlp_bdev_dual(const dpdlp_batch_dev_t *b, void *stream) {
+  return (int)launch_b<K_DUAL>(b, DPDLP_SL_FPE + 3, ST(stream));
 }
 
 int dpdlp_bdev_phases(const dpdlp_batch_dev_t *b, void *stream) {

@@ -21,16 +21,14 @@
 #define BLUE_SBIG 0x1p-538
 
 __device__ __forceinline__ void blue_acc(double x, double &s, double &m, double &b) {
+  /* branch-free (adds +0.0 to the two accumulators x does not belong to), so
+   * the accumulators stay in registers */
   double ax = fabs(x);
-  if (ax > BLUE_TBIG) {
-    double t = ax * BLUE_SBIG;
-    b += t * t;
-  } else if (ax < BLUE_TSML) {
-    double t = ax * BLUE_SSML;
-    s += t * t;
-  } else {
-    m += ax * ax;
-  }
+  bool big = ax > BLUE_TBIG, sml = ax < BLUE_TSML;
+  double tb = ax * BLUE_SBIG, ts = ax * BLUE_SSML;
+  b += big ? tb * tb : 0.0;
+  s += sml ? ts * ts : 0.0;
+  m += (big || sml) ? 0.0 : ax * ax;
 }
 
 /* Combines the three Blue accumulators into the norm (host and device). */

@@ -615,11 +615,12 @@ int dpdlp_dev_primal_phase(const dpdlp_state_t *st, double tau, double w, double
   e.c = st->c;
   e.lv = st->lv;
   e.uv = st->uv;
-  e.x_ref = st->x_ref;
+  /* FP32 iterations write only the FP32 copies the next SpMV gathers */
+  e.x_ref = use32 ? nullptr : st->x_ref;
   e.x_pdhg = st->x_pdhg;
   e.z = st->dual_slack;
   e.dx = st->dx;
-  e.x_ref32 = st->x_ref32;
+  e.x_ref32 = use32 ? st->x_ref32 : nullptr;
   e.tau = tau;
   e.w = w;
   e.rho = rho;
@@ -640,10 +641,9 @@ int dpdlp_dev_dual_phase(const dpdlp_state_t *st, double sigma, double w, double
   e.y_init = st->y_init;
   e.lc = st->lc;
   e.uc = st->uc;
-  e.y_ref = st->y_ref;
   e.y_pdhg = st->y_pdhg;
   e.dy = st->dy;
-  e.y_cur32 = st->y_cur32;
+  e.y_cur32 = use32 ? st->y_cur32 : nullptr;
   e.sigma = sigma;
   e.w = w;
   e.rho = rho;

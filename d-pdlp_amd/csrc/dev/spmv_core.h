@@ -100,6 +100,27 @@ __device__ __forceinline__ void rows_reduce(const double *__restrict__ prod,
   }
 }
 
+/* Epilogue operands that do not depend on the row's sum are loaded at the
+ * start of the row block, so their latency overlaps the matrix stream.
+ * Epilogues with a Pre type and pre()/apply() opt in; the rest load inside
+ * operator(). */
+template <class Epi, class = void> struct EpiPre {
+  struct T {};
+  __device__ static T load(const Epi &, int) { return T{}; }
+  __device__ static void apply(const Epi &e, int r, double ax, const T &, double *s,
+                               double *mx) {
+    e(r, ax, s, mx);
+  }
+};
+template <class Epi> struct EpiPre<Epi, decltype((void)sizeof(typename Epi::Pre))> {
+  using T = typename Epi::Pre;
+  __device__ static T load(const Epi &e, int r) { return e.pre(r); }
+  __device__ static void apply(const Epi &e, int r, double ax, const T &q, double *s,
+                               double *mx) {
+    e.apply(r, ax, q, s, mx);
+  }
+};
+
 /* One CSR-stream row block [r0, r1): stream, gather, stage in LDS, reduce each
  * row (rows_reduce) and call epi(row, sum, s, mx) once per row.  The caller
  * owns the LDS arrays (prod[CAP], srp[257], ssum[256]) and the per-thread
@@ -110,8 +131,11 @@ __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int
                                            int r0, int r1, const Epi &epi, double *s, double *mx,
                                            double *prod, int *srp, double *ssum) {
   constexpr int UNR = 8;
+  using PF = EpiPre<Epi>;
   const int tid = threadIdx.x;
   const int R = r1 - r0;
+  typename PF::T pre;
+  if (tid < R) pre = PF::load(epi, r0 + tid);
   const int p0 = rp[r0], p1 = rp[r1];
   const int nz = p1 - p0;
   if (nz > CAP) {
@@ -122,7 +146,7 @@ __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int
     __syncthreads();
     if (tid == 0) {
       double tot = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
-      epi(r0, tot, s, mx);
+      PF::apply(epi, r0, tot, pre, s, mx);
     }
     return;
   }
@@ -153,7 +177,7 @@ __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int
   __syncthreads();
   rows_reduce(prod, srp, R, [&](int r, double v) { ssum[r] = v; });
   __syncthreads();
-  if (tid < R) epi(r0 + tid, ssum[tid], s, mx);
+  if (tid < R) PF::apply(epi, r0 + tid, ssum[tid], pre, s, mx);
 }
 
 /* ---- epilogues ---- */
@@ -174,17 +198,23 @@ struct EpiPrimal {
   float *x_ref32;
   double tau, w, rho;
   int major, fpe, partials;
-  __device__ void operator()(int j, double aty, double *s, double *) const {
-    double xc = x_cur[j];
-    double t = xc - tau * (c[j] - aty);
-    double p = fmax(lv[j], fmin(t, uv[j]));
+  /* the column's operands, loadable before its A'y is known */
+  struct Pre {
+    double xc, cj, l, u, xi;
+  };
+  __device__ Pre pre(int j) const { return Pre{x_cur[j], c[j], lv[j], uv[j], x_init[j]}; }
+  __device__ void apply(int j, double aty, const Pre &q, double *s, double *) const {
+    double xc = q.xc;
+    double t = xc - tau * (q.cj - aty);
+    double p = fmax(q.l, fmin(t, q.u));
     double xr;
     if (major) {
       x_pdhg[j] = p;
       z[j] = (p - t) / tau;
     }
     xr = 2.0 * p - xc;
-    x_ref[j] = xr;
+    /* the dual phase gathers exactly one of the two copies */
+    if (x_ref) x_ref[j] = xr;
     if (x_ref32) x_ref32[j] = (float)xr;
     if (fpe) {
       double d = xr - xc;
@@ -192,23 +222,31 @@ struct EpiPrimal {
       blue_acc(d, s[0], s[1], s[2]);
     }
     double rr = rho * xr + (1.0 - rho) * xc;
-    x_cur[j] = w * rr + (1.0 - w) * x_init[j];
+    x_cur[j] = w * rr + (1.0 - w) * q.xi;
+  }
+  __device__ void operator()(int j, double aty, double *s, double *mx) const {
+    apply(j, aty, pre(j), s, mx);
   }
 };
 
-/* Phase 2: pdlp_core_op.cu:420-443 then the y half of :445-462. */
+/* Phase 2: pdlp_core_op.cu:420-443 then the y half of :445-462.  The
+ * reflected dual itself is not stored: nothing reads it after this step. */
 struct EpiDual {
   static constexpr int NS = 3, NM = 0;
   double *y_cur;
   const double *y_init, *lc, *uc;
-  double *y_ref, *y_pdhg, *dy;
+  double *y_pdhg, *dy;
   float *y_cur32;
   double sigma, w, rho;
   int major, fpe, partials;
-  __device__ void operator()(int i, double ax, double *s, double *) const {
-    double yc = y_cur[i];
+  struct Pre {
+    double yc, l, u, yi;
+  };
+  __device__ Pre pre(int i) const { return Pre{y_cur[i], lc[i], uc[i], y_init[i]}; }
+  __device__ void apply(int i, double ax, const Pre &q, double *s, double *) const {
+    double yc = q.yc;
     double t = yc / sigma - ax;
-    double p = fmax(-uc[i], fmin(t, -lc[i]));
+    double p = fmax(-q.u, fmin(t, -q.l));
     double yr;
     if (major) {
       double yp = (t - p) * sigma;
@@ -217,16 +255,18 @@ struct EpiDual {
     } else {
       yr = 2.0 * (t - p) * sigma - yc;
     }
-    y_ref[i] = yr;
     if (fpe) {
       double d = yr - yc;
       dy[i] = d;
       blue_acc(d, s[0], s[1], s[2]);
     }
     double rr = rho * yr + (1.0 - rho) * yc;
-    double yn = w * rr + (1.0 - w) * y_init[i];
+    double yn = w * rr + (1.0 - w) * q.yi;
     y_cur[i] = yn;
     if (y_cur32) y_cur32[i] = (float)yn;
+  }
+  __device__ void operator()(int i, double ax, double *s, double *mx) const {
+    apply(i, ax, pre(i), s, mx);
   }
 };
 

@@ -37,6 +37,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #define BCHK(call)                                                             \
   do {                                                                         \
@@ -73,6 +74,25 @@ typedef struct {
   double latency;
 } bdone_t;
 
+/* a retired LP whose result copies are in flight (complete at the next
+ * synchronisation); its arena and resident problem stay alive until then */
+typedef struct {
+  long long tag;
+  cupdlpx_result_t *res;
+  size_t off;
+  int n, m;
+  double t_add;
+  char *arena;
+  size_t arena_cap;
+  dpdlp_resident_t *R;
+  int own_R;
+} bpend_t;
+
+typedef struct {
+  char *p;
+  size_t cap;
+} barena_t;
+
 struct dpdlp_batch {
   pthread_mutex_t mu; /* slot reservation and READY hand-over */
   pdhg_parameters_t prm;
@@ -98,8 +118,22 @@ struct dpdlp_batch {
   dpdlp_batch_dev_t bd;
   bdone_t *done;
   int ndone, done_head, done_cap;
+  bpend_t *pend;
+  int npend;
+  double *h_stage; /* pinned staging of result vectors */
+  size_t stage_cap, stage_off;
+  barena_t *pool; /* recycled LP-state arenas (guarded by mu) */
+  int npool;
+  size_t pool_bytes;
+  dpdlp_resident_t **dead; /* retired owned problems awaiting free (guarded by mu) */
+  int ndead, dead_cap;
   long long windows, syncs, launches;
   size_t state_bytes;
+  /* live phase timing */
+  hipEvent_t pev[32][3];
+  double pev_bytes[32][2];
+  int npev_created, npev_used, psamples;
+  double p_ms[2], p_bytes[2];
 };
 
 static int stage_of(const bslot_t *q) { return __atomic_load_n(&q->stage, __ATOMIC_ACQUIRE); }
@@ -137,6 +171,17 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   }
   for (int i = b->done_head; i < b->ndone; ++i) cupdlpx_result_free(b->done[i].res);
   free(b->done);
+  for (int i = 0; i < b->npend; ++i) {
+    cupdlpx_result_free(b->pend[i].res);
+    if (b->pend[i].arena) hipFree(b->pend[i].arena);
+    if (b->pend[i].own_R) dpdlp_resident_free(b->pend[i].R);
+  }
+  free(b->pend);
+  for (int i = 0; i < b->npool; ++i) hipFree(b->pool[i].p);
+  free(b->pool);
+  for (int i = 0; i < b->ndead; ++i) dpdlp_resident_free(b->dead[i]);
+  free(b->dead);
+  if (b->h_stage) hipHostFree(b->h_stage);
   free(b->sl);
   void *dev[] = {b->d_lps, b->d_ctl_eval, b->d_ctl_win, b->d_scal,
                  b->d_list, b->d_rng,     b->d_part,    b->d_normals};
@@ -148,6 +193,8 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
     if (host[k]) hipHostFree(host[k]);
   for (int k = 0; k < 2; ++k)
     if (b->ev_win[k]) hipEventDestroy(b->ev_win[k]);
+  for (int k = 0; k < b->npev_created; ++k)
+    for (int q = 0; q < 3; ++q) hipEventDestroy(b->pev[k][q]);
   if (b->s) hipStreamDestroy(b->s);
   pthread_mutex_destroy(&b->mu);
   free(b);
@@ -193,7 +240,11 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
   b->sl = (bslot_t *)calloc((size_t)b->cap, sizeof(bslot_t));
   b->done_cap = 64;
   b->done = (bdone_t *)malloc((size_t)b->done_cap * sizeof(bdone_t));
-  if (!b->sl || !b->done) {
+  b->pend = (bpend_t *)calloc((size_t)b->cap, sizeof(bpend_t));
+  b->pool = (barena_t *)calloc((size_t)b->cap, sizeof(barena_t));
+  b->dead_cap = 64;
+  b->dead = (dpdlp_resident_t **)calloc((size_t)b->dead_cap, sizeof(*b->dead));
+  if (!b->sl || !b->done || !b->pend || !b->pool || !b->dead) {
     dpdlp_batch_free(b);
     return NULL;
   }
@@ -215,6 +266,12 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
   CCHK(hipMalloc((void **)&b->d_rng, C * 2 * NLIST * sizeof(int)));
   CCHK(hipHostMalloc((void **)&b->h_rng, C * 2 * NLIST * sizeof(int), 0));
   memset(b->h_lps, 0, C * sizeof(dpdlp_blp_t));
+  if (b->o.profile_phases) {
+    for (int k = 0; k < 32; ++k) {
+      for (int q = 0; q < 3; ++q) CCHK(hipEventCreate(&b->pev[k][q]));
+      b->npev_created = k + 1;
+    }
+  }
   b->state_bytes = C * (sizeof(dpdlp_blp_t) + sizeof(dpdlp_bctl_t) * (1 + b->W) +
                         DPDLP_SCAL_LEN * sizeof(double) + 2 * NLIST * sizeof(int));
   return b;
@@ -320,6 +377,76 @@ static void start_loop(dpdlp_batch_t *b, bslot_t *q) {
   q->fresh = 1;
 }
 
+/* smallest pooled arena of at least `need` bytes (and at most 4x), or none */
+static void take_arena(dpdlp_batch_t *b, size_t need, char **p, size_t *cap) {
+  *p = NULL;
+  *cap = 0;
+  pthread_mutex_lock(&b->mu);
+  int best = -1;
+  for (int i = 0; i < b->npool; ++i)
+    if (b->pool[i].cap >= need && b->pool[i].cap <= 4 * need &&
+        (best < 0 || b->pool[i].cap < b->pool[best].cap))
+      best = i;
+  if (best >= 0) {
+    *p = b->pool[best].p;
+    *cap = b->pool[best].cap;
+    b->pool_bytes -= *cap;
+    b->pool[best] = b->pool[--b->npool];
+  }
+  pthread_mutex_unlock(&b->mu);
+}
+
+/* pools an arena; returns one to free instead when the pool is full */
+static char *put_arena(dpdlp_batch_t *b, char *p, size_t cap) {
+  char *out = NULL;
+  pthread_mutex_lock(&b->mu);
+  if (b->npool < b->cap) {
+    b->pool[b->npool].p = p;
+    b->pool[b->npool].cap = cap;
+    b->npool++;
+    b->pool_bytes += cap;
+  } else {
+    out = p;
+  }
+  pthread_mutex_unlock(&b->mu);
+  return out;
+}
+
+/* Frees retired problems and trims the arena pool to `keep_bytes`; device
+ * frees may synchronise the device, so a set-up thread calls this, not the
+ * stepping thread. */
+void dpdlp_i_batch_reap(dpdlp_batch_t *b, size_t keep_bytes) {
+  for (;;) {
+    dpdlp_resident_t *R = NULL;
+    char *a = NULL;
+    pthread_mutex_lock(&b->mu);
+    if (b->ndead > 0) {
+      R = b->dead[--b->ndead];
+    } else if (b->pool_bytes > keep_bytes && b->npool > 0) {
+      int big = 0;
+      for (int i = 1; i < b->npool; ++i)
+        if (b->pool[i].cap > b->pool[big].cap) big = i;
+      a = b->pool[big].p;
+      b->pool_bytes -= b->pool[big].cap;
+      b->pool[big] = b->pool[--b->npool];
+    }
+    pthread_mutex_unlock(&b->mu);
+    if (R)
+      dpdlp_resident_free(R);
+    else if (a)
+      hipFree(a);
+    else
+      break;
+  }
+}
+
+size_t dpdlp_i_batch_pool_bytes(dpdlp_batch_t *b) {
+  pthread_mutex_lock(&b->mu);
+  size_t v = b->pool_bytes;
+  pthread_mutex_unlock(&b->mu);
+  return v;
+}
+
 int dpdlp_i_batch_reserve(dpdlp_batch_t *b) {
   pthread_mutex_lock(&b->mu);
   int i = 0;
@@ -352,6 +479,8 @@ int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take
   q->tag = tag;
   q->R = R;
   hipStream_t s = stream ? (hipStream_t)stream : b->s;
+  take_arena(b, dpdlp_i_ctx_footprint(R, b->o.precision == DPDLP_PREC_MIXED, 0), &q->c.arena,
+             &q->c.arena_cap);
   if (dpdlp_i_ctx_setup(&q->c, R, &b->prm, b->o.precision == DPDLP_PREC_MIXED, s,
                         b->d_scal + (size_t)i * DPDLP_SCAL_LEN,
                         b->h_scal + (size_t)i * DPDLP_SCAL_LEN, &q->rescale_time)) {
@@ -547,9 +676,49 @@ static void mark_eval(dpdlp_batch_t *b, bslot_t *q) {
   }
 }
 
+/* sampled phase events are complete after a synchronisation */
+static void harvest_phase_events(dpdlp_batch_t *b) {
+  for (int k = 0; k < b->npev_used; ++k) {
+    float a = 0.f, c = 0.f;
+    if (hipEventElapsedTime(&a, b->pev[k][0], b->pev[k][1]) == hipSuccess &&
+        hipEventElapsedTime(&c, b->pev[k][1], b->pev[k][2]) == hipSuccess) {
+      b->p_ms[0] += a;
+      b->p_ms[1] += c;
+      b->p_bytes[0] += b->pev_bytes[k][0];
+      b->p_bytes[1] += b->pev_bytes[k][1];
+      b->psamples++;
+    }
+  }
+  b->npev_used = 0;
+}
+
+void dpdlp_batch_phase_stats(const dpdlp_batch_t *b, int *samples, double *primal_ms,
+                             double *primal_bytes, double *dual_ms, double *dual_bytes) {
+  if (samples) *samples = b ? b->psamples : 0;
+  if (primal_ms) *primal_ms = b ? b->p_ms[0] : 0.0;
+  if (primal_bytes) *primal_bytes = b ? b->p_bytes[0] : 0.0;
+  if (dual_ms) *dual_ms = b ? b->p_ms[1] : 0.0;
+  if (dual_bytes) *dual_bytes = b ? b->p_bytes[1] : 0.0;
+}
+
+/* Algorithmic bytes of one LP's two phase kernels in a round: matrix stream
+ * (values + indices), row pointers, the gathered vector read once, and the
+ * epilogue's per-row vector reads and writes (as the epilogues do them). */
+static void phase_bytes(const dpdlp_resident_t *R, const dpdlp_bctl_t *c, double *p1,
+                        double *p2) {
+  double m = R->m, n = R->n, nnz = R->nnz;
+  double sv = c->use32 ? 4 : 8, sx = c->use32 ? 4 : 8;
+  double x_rw = 40 + 16 + (c->use32 ? 4 : 0) + (c->major ? 16 : 0) + (c->fpe ? 8 : 0);
+  double y_rw = 32 + 8 + (c->use32 ? 4 : 0) + (c->major ? 8 : 0) + (c->fpe ? 8 : 0);
+  *p1 = nnz * (sv + 4) + 4 * (n + 1) + sx * m + x_rw * n;
+  *p2 = nnz * (sv + 4) + 4 * (m + 1) + sx * n + y_rw * m;
+}
+
 /* launches the residual (and infeasibility) kernels of the marked slots and
  * reads every slot's scalar slots and power-iteration history back: one
  * synchronisation */
+static int flush_pending(dpdlp_batch_t *b);
+
 static int eval_and_read(dpdlp_batch_t *b, int any_eval, int any_inf) {
   if (b->dirty && rebuild_lists(b)) return -1;
   if (any_eval) {
@@ -578,6 +747,8 @@ static int eval_and_read(dpdlp_batch_t *b, int any_eval, int any_inf) {
                         (size_t)b->nhw, hipMemcpyDeviceToHost, b->s));
   BCHK(hipStreamSynchronize(b->s));
   b->syncs++;
+  if (b->npev_used) harvest_phase_events(b);
+  if (b->npend && flush_pending(b)) return -1;
   return 0;
 }
 
@@ -597,8 +768,54 @@ static int push_done(dpdlp_batch_t *b, long long tag, cupdlpx_result_t *res, dou
   return 0;
 }
 
-/* an LP terminated: polish (as the single-LP path does), copy its result out,
- * free its slot */
+/* completes the retired LPs whose result copies were issued before the last
+ * synchronisation: results become available, arenas go back to the pool */
+static int flush_pending(dpdlp_batch_t *b) {
+  double now = dpdlp_wall();
+  for (int k = 0; k < b->npend; ++k) {
+    bpend_t *e = &b->pend[k];
+    cupdlpx_result_t *r = e->res;
+    const double *st = b->h_stage + e->off;
+    if (e->n > 0) {
+      memcpy(r->primal_solution, st, (size_t)e->n * sizeof(double));
+      memcpy(r->reduced_cost, st + e->n, (size_t)e->n * sizeof(double));
+    }
+    if (e->m > 0) memcpy(r->dual_solution, st + 2 * (size_t)e->n, (size_t)e->m * sizeof(double));
+    if (push_done(b, e->tag, r, now - e->t_add)) return -1;
+    e->res = NULL;
+    char *drop = put_arena(b, e->arena, e->arena_cap);
+    if (drop) hipFree(drop);
+    dpdlp_resident_t *free_now = NULL;
+    pthread_mutex_lock(&b->mu);
+    if (e->own_R) {
+      /* the list grows as needed: retirements can outpace the set-up
+       * thread that frees them */
+      if (b->ndead == b->dead_cap) {
+        int nc = 2 * b->dead_cap;
+        dpdlp_resident_t **d =
+            (dpdlp_resident_t **)realloc(b->dead, (size_t)nc * sizeof(*b->dead));
+        if (d) {
+          b->dead = d;
+          b->dead_cap = nc;
+        }
+      }
+      if (b->ndead < b->dead_cap)
+        b->dead[b->ndead++] = e->R;
+      else
+        free_now = e->R;
+    }
+    b->in_flight--;
+    pthread_mutex_unlock(&b->mu);
+    if (free_now) dpdlp_resident_free(free_now);
+  }
+  b->npend = 0;
+  b->stage_off = 0;
+  return 0;
+}
+
+/* an LP terminated: polish (as the single-LP path does), issue its result
+ * copies into pinned staging, free its slot; the result completes at the
+ * next synchronisation (flush_pending) */
 static int finish(dpdlp_batch_t *b, int i) {
   bslot_t *q = &b->sl[i];
   hstate_t *h = &q->h;
@@ -611,22 +828,60 @@ static int finish(dpdlp_batch_t *b, int i) {
       return -1;
     }
   }
-  cupdlpx_result_t *res = dpdlp_i_ctx_result(&q->c, h, q->rescale_time, polish_time,
-                                             polish_iters);
+  int n = q->c.st.n, m = q->c.st.m;
+  size_t need = 2 * (size_t)n + (size_t)m;
+  if (b->stage_off + need > b->stage_cap) {
+    /* staging full: complete what is in flight first */
+    BCHK(hipStreamSynchronize(b->s));
+    b->syncs++;
+    if (flush_pending(b)) return -1;
+    if (need > b->stage_cap) {
+      size_t nc = need > 2 * b->stage_cap ? need : 2 * b->stage_cap;
+      if (nc < (1u << 20)) nc = 1u << 20;
+      if (b->h_stage) hipHostFree(b->h_stage);
+      b->h_stage = NULL;
+      b->stage_cap = 0;
+      BCHK(hipHostMalloc((void **)&b->h_stage, nc * sizeof(double), 0));
+      b->stage_cap = nc;
+    }
+  }
+  cupdlpx_result_t *res = (cupdlpx_result_t *)calloc(1, sizeof(*res));
   if (!res) return -1;
-  double lat = dpdlp_wall() - q->t_add;
-  if (push_done(b, q->tag, res, lat)) {
+  res->primal_solution = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+  res->dual_solution = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
+  res->reduced_cost = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+  if (!res->primal_solution || !res->dual_solution || !res->reduced_cost) {
     cupdlpx_result_free(res);
     return -1;
   }
+  dpdlp_i_result_scalars(res, &q->c, h, q->rescale_time, polish_time, polish_iters);
+  double *stg = b->h_stage + b->stage_off;
+  if (dpdlp_i_result_launch(&q->c, stg, stg + 2 * (size_t)n, stg + n)) {
+    cupdlpx_result_free(res);
+    return -1;
+  }
+  bpend_t *e = &b->pend[b->npend++];
+  e->tag = q->tag;
+  e->res = res;
+  e->off = b->stage_off;
+  e->n = n;
+  e->m = m;
+  e->t_add = q->t_add;
+  e->arena = q->c.arena;
+  e->arena_cap = q->c.arena_cap;
+  e->R = q->R;
+  e->own_R = q->own_R;
+  b->stage_off += need;
+  /* the arena moves to the pending entry; the slot's other allocations (if
+   * polishing made any) are freed now; no stream synchronisation here */
+  q->c.arena = NULL;
+  q->c.s = NULL;
   dpdlp_i_ctx_free(&q->c);
-  if (q->own_R) dpdlp_resident_free(q->R);
   memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
   memset(&b->h_lps[i], 0, sizeof(dpdlp_blp_t));
   b->dirty = 1;
   pthread_mutex_lock(&b->mu);
   set_stage(q, S_FREE);
-  b->in_flight--;
   pthread_mutex_unlock(&b->mu);
   while (b->nhw > 0 && !active(&b->sl[b->nhw - 1])) b->nhw--;
   return 0;
@@ -754,7 +1009,29 @@ static int launch_window(dpdlp_batch_t *b) {
       b->launches += 1;
     }
     if (any_phase[r]) {
-      if (dpdlp_bdev_phases(&b->bd, b->s)) return -1;
+      int timed = b->npev_used < b->npev_created && (b->windows & 3) == 0 && r == (W > 1 ? 1 : 0);
+      if (timed) {
+        hipEvent_t *ev = b->pev[b->npev_used];
+        double s1 = 0.0, s2 = 0.0;
+        for (int i = 0; i < nl; ++i) {
+          const dpdlp_bctl_t *c = &ctl[(size_t)r * nl + i];
+          if (!c->phase) continue;
+          double a1, a2;
+          phase_bytes(b->sl[i].R, c, &a1, &a2);
+          s1 += a1;
+          s2 += a2;
+        }
+        b->pev_bytes[b->npev_used][0] = s1;
+        b->pev_bytes[b->npev_used][1] = s2;
+        b->npev_used++;
+        BCHK(hipEventRecord(ev[0], b->s));
+        if (dpdlp_bdev_primal(&b->bd, b->s)) return -1;
+        BCHK(hipEventRecord(ev[1], b->s));
+        if (dpdlp_bdev_dual(&b->bd, b->s)) return -1;
+        BCHK(hipEventRecord(ev[2], b->s));
+      } else {
+        if (dpdlp_bdev_phases(&b->bd, b->s)) return -1;
+      }
       b->launches += 2;
     }
     if (any_fpe[r]) {
@@ -827,6 +1104,10 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
   if (b->nhw > 0) {
     if (b->dirty && rebuild_lists(b)) return -1;
     if (launch_window(b)) return -1;
+  } else if (b->npend) {
+    BCHK(hipStreamSynchronize(b->s));
+    b->syncs++;
+    if (flush_pending(b)) return -1;
   }
   return dpdlp_batch_in_flight(b);
 }
@@ -844,8 +1125,32 @@ int dpdlp_batch_take(dpdlp_batch_t *b, long long *tag, cupdlpx_result_t **res,
   return 1;
 }
 
+typedef struct {
+  dpdlp_batch_t *b;
+  dpdlp_resident_t *const *res;
+  int count;
+  hipStream_t s;
+  int done, err;
+} bloader_t;
+
+/* sets the LPs up on its own stream while the batch already steps */
+static void *solve_loader(void *arg) {
+  bloader_t *L = (bloader_t *)arg;
+  int err = 0;
+  for (int i = 0; i < L->count; ++i) {
+    int slot = dpdlp_i_batch_reserve(L->b);
+    if (slot < 0 || dpdlp_i_batch_prepare(L->b, slot, L->res[i], 0, i, L->s)) {
+      err = 1;
+      break;
+    }
+  }
+  __atomic_store_n(&L->err, err, __ATOMIC_RELEASE);
+  __atomic_store_n(&L->done, 1, __ATOMIC_RELEASE);
+  return NULL;
+}
+
 int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parameters_t *params,
-                      const dpdlp_batch_opts_t *o, cupdlpx_result_t **results) {
+                      dpdlp_batch_opts_t *o, cupdlpx_result_t **results) {
   if (count < 0 || (count > 0 && (!res || !results))) {
     dpdlp_set_error("invalid batch arguments");
     return -1;
@@ -857,30 +1162,67 @@ int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parame
     dpdlp_batch_default_opts(&local);
   if (local.max_lps < count) local.max_lps = count;
   for (int i = 0; i < count; ++i) results[i] = NULL;
+  for (int i = 0; i < count; ++i)
+    if (!res[i] || res[i]->cap != local.spmv_cap || res[i]->A_P != 1 || res[i]->At_P != 1) {
+      dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
+                      local.spmv_cap);
+      return -1;
+    }
   dpdlp_batch_t *b = dpdlp_batch_create(params, &local);
   if (!b) return -1;
-  int got = 0;
-  for (int i = 0; i < count; ++i)
-    if (dpdlp_batch_add(b, res[i], 0, i)) goto fail;
-  while (dpdlp_batch_in_flight(b) > 0)
-    if (dpdlp_batch_step(b) < 0) goto fail;
+  bloader_t L;
+  memset(&L, 0, sizeof(L));
+  L.b = b;
+  L.res = res;
+  L.count = count;
+  pthread_t th;
+  int started = 0, got = 0, failed = 0;
+  if (hipStreamCreateWithFlags(&L.s, hipStreamNonBlocking) != hipSuccess) {
+    dpdlp_set_error("stream creation failed");
+    failed = 1;
+  } else if (pthread_create(&th, NULL, solve_loader, &L) != 0) {
+    dpdlp_set_error("thread creation failed");
+    failed = 1;
+  } else {
+    started = 1;
+  }
+  while (!failed) {
+    int ld = __atomic_load_n(&L.done, __ATOMIC_ACQUIRE);
+    int k = dpdlp_batch_in_flight(b);
+    if (k > 0) {
+      if (dpdlp_batch_step(b) < 0) failed = 1;
+    } else if (ld) {
+      break;
+    } else {
+      struct timespec ts = {0, 50000};
+      nanosleep(&ts, NULL);
+    }
+  }
+  if (started) pthread_join(th, NULL);
+  if (L.err) failed = 1;
   long long tag;
   cupdlpx_result_t *r;
   while (dpdlp_batch_take(b, &tag, &r, NULL)) {
-    if (tag >= 0 && tag < count && !results[tag]) {
+    if (!failed && tag >= 0 && tag < count && !results[tag]) {
       results[tag] = r;
       got++;
     } else {
       cupdlpx_result_free(r);
     }
   }
-  dpdlp_batch_free(b);
-  return got;
-fail:
-  dpdlp_batch_free(b);
-  for (int i = 0; i < count; ++i) {
-    cupdlpx_result_free(results[i]);
-    results[i] = NULL;
+  if (o) {
+    dpdlp_batch_counters(b, &o->windows, &o->syncs, &o->launches);
+    dpdlp_batch_phase_stats(b, &o->phase_samples, &o->primal_ms, &o->primal_bytes, &o->dual_ms,
+                            &o->dual_bytes);
   }
-  return -1;
+  dpdlp_batch_free(b);
+  if (L.s) hipStreamDestroy(L.s);
+  if (failed) {
+    for (int i = 0; i < count; ++i) {
+      cupdlpx_result_free(results[i]);
+      results[i] = NULL;
+    }
+    return -1;
+  }
+  return got;
 }

@@ -108,6 +108,9 @@ static void *loader_main(void *arg) {
   const dpdlp_serve_spec_t *sp = S->spec;
   double budget = sp->hbm_budget_gb * 1e9;
   for (int k = 0; k < S->nreq; ++k) {
+    /* retired problems and surplus pooled arenas are freed here, off the
+     * stepping thread */
+    dpdlp_i_batch_reap(S->b, (size_t)(0.1 * budget));
     sleep_until(S->arrival[k]);
     pthread_mutex_lock(&S->mu);
     while (!S->stop && S->admitted_live > 0 &&

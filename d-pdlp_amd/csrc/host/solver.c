@@ -91,7 +91,7 @@ void dpdlp_i_ctx_free(ctx_t *c) {
   c->nalloc = 0;
   if (c->arena) hipFree(c->arena);
   c->arena = NULL;
-  c->arena_off = c->arena_len = 0;
+  c->arena_off = c->arena_len = c->arena_cap = 0;
   if (c->h_scal && c->owns_scal) hipHostFree(c->h_scal);
   c->h_scal = NULL;
   if (c->s && c->owns_stream) hipStreamDestroy(c->s);
@@ -675,16 +675,24 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
     HIPCHK(hipHostMalloc((void **)&c->h_scal, DPDLP_SCAL_LEN * sizeof(double), 0));
     c->owns_scal = 1;
   }
-  /* one device arena for the LP's state, cleared once */
-  c->arena_len = dpdlp_i_ctx_footprint(R, mixed, scal_dev == NULL);
-  if (hipMalloc((void **)&c->arena, c->arena_len) != hipSuccess) {
-    dpdlp_set_error("device allocation failed (%zu bytes of LP state)", c->arena_len);
-    c->arena = NULL;
-    c->arena_len = 0;
-    goto fail;
+  /* one device arena for the LP's state, cleared once; a caller may hand in
+   * a recycled arena that is large enough */
+  {
+    size_t need = dpdlp_i_ctx_footprint(R, mixed, scal_dev == NULL);
+    if (!c->arena || c->arena_cap < need) {
+      if (c->arena) hipFree(c->arena);
+      c->arena_cap = need;
+      if (hipMalloc((void **)&c->arena, c->arena_cap) != hipSuccess) {
+        dpdlp_set_error("device allocation failed (%zu bytes of LP state)", c->arena_cap);
+        c->arena = NULL;
+        c->arena_cap = 0;
+        goto fail;
+      }
+    }
+    c->arena_off = 0;
+    HIPCHK(hipMemsetAsync(c->arena, 0, need, c->s));
+    c->arena_len = need; /* bump allocation stays within what was cleared */
   }
-  c->arena_off = 0;
-  HIPCHK(hipMemsetAsync(c->arena, 0, c->arena_len, c->s));
 
   /* ---- state (pdlp_core_op.cu:42-391) ---- */
   st->m = m;
@@ -851,35 +859,11 @@ void dpdlp_i_init_hstate(hstate_t *h, const ctx_t *c, const pdhg_parameters_t *p
   h->m32_best = INFINITY;
 }
 
-cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale_time,
-                                     double polish_time, int polish_iters) {
-  dpdlp_state_t *st = &c->st;
-  int m = st->m, n = st->n, nnz = st->A.nnz;
-  /* ---- result (pdlp_core_op.cu:789-859) ---- */
-  cupdlpx_result_t *res = (cupdlpx_result_t *)calloc(1, sizeof(*res));
-  if (!res) goto fail;
-  res->primal_solution = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
-  res->dual_solution = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
-  res->reduced_cost = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
-  if (!res->primal_solution || !res->dual_solution || !res->reduced_cost) goto fail;
-  {
-    double *xo = c->pi_dual, *rco = st->dual_slack, *yo = c->pi_next; /* reuse work vectors */
-    if (n > 0 && st->At.nblk == 0) HIPCHK(hipMemsetAsync(rco, 0, (size_t)n * 8, c->s));
-    HIPCHK(dpdlp_dev_result(st, c->k.beta, c->k.gamma, xo, yo, rco, c->s));
-    if (n > 0) {
-      HIPCHK(hipMemcpyAsync(res->primal_solution, xo, (size_t)n * sizeof(double),
-                            hipMemcpyDeviceToHost, c->s));
-      HIPCHK(hipMemcpyAsync(res->reduced_cost, rco, (size_t)n * sizeof(double),
-                            hipMemcpyDeviceToHost, c->s));
-    }
-    if (m > 0)
-      HIPCHK(hipMemcpyAsync(res->dual_solution, yo, (size_t)m * sizeof(double),
-                            hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-  }
-  res->num_variables = n;
-  res->num_constraints = m;
-  res->num_nonzeros = nnz;
+void dpdlp_i_result_scalars(cupdlpx_result_t *res, const ctx_t *c, const hstate_t *h,
+                            double rescale_time, double polish_time, int polish_iters) {
+  res->num_variables = c->st.n;
+  res->num_constraints = c->st.m;
+  res->num_nonzeros = c->st.A.nnz;
   res->total_count = h->total;
   res->rescaling_time_sec = rescale_time;
   res->cumulative_time_sec = h->t_cum;
@@ -898,7 +882,41 @@ cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale
   res->termination_reason = h->reason;
   res->feasibility_polishing_time = polish_time;
   res->feasibility_iteration = polish_iters;
+}
 
+int dpdlp_i_result_launch(ctx_t *c, double *x_host, double *y_host, double *rc_host) {
+  dpdlp_state_t *st = &c->st;
+  int m = st->m, n = st->n;
+  /* unscaled vectors (pdlp_core_op.cu:789-859) into the LP's work vectors */
+  double *xo = c->pi_dual, *rco = st->dual_slack, *yo = c->pi_next;
+  if (n > 0 && st->At.nblk == 0) HIPCHK(hipMemsetAsync(rco, 0, (size_t)n * 8, c->s));
+  HIPCHK(dpdlp_dev_result(st, c->k.beta, c->k.gamma, xo, yo, rco, c->s));
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(x_host, xo, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(rc_host, rco, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                          c->s));
+  }
+  if (m > 0)
+    HIPCHK(hipMemcpyAsync(y_host, yo, (size_t)m * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  return 0;
+fail:
+  return -1;
+}
+
+cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale_time,
+                                     double polish_time, int polish_iters) {
+  dpdlp_state_t *st = &c->st;
+  int m = st->m, n = st->n;
+  cupdlpx_result_t *res = (cupdlpx_result_t *)calloc(1, sizeof(*res));
+  if (!res) goto fail;
+  res->primal_solution = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+  res->dual_solution = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
+  res->reduced_cost = (double *)malloc((size_t)(n ? n : 1) * sizeof(double));
+  if (!res->primal_solution || !res->dual_solution || !res->reduced_cost) goto fail;
+  if (dpdlp_i_result_launch(c, res->primal_solution, res->dual_solution, res->reduced_cost))
+    goto fail;
+  HIPCHK(hipStreamSynchronize(c->s));
+  dpdlp_i_result_scalars(res, c, h, rescale_time, polish_time, polish_iters);
   return res;
 fail:
   cupdlpx_result_free(res);

@@ -22,7 +22,7 @@ typedef struct {
   int mixed, use32;
   int owns_stream, owns_scal;
   char *arena; /* one device allocation for the LP state (zeroed) */
-  size_t arena_off, arena_len;
+  size_t arena_off, arena_len, arena_cap;
   double *pi_eig, *pi_next, *pi_dual, *rs, *cs;
   double *w_A_v, *w_At_v;
   float *w_A_v32, *w_At_v32;
@@ -43,6 +43,11 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
 /* Unscaled result of a finished LP (D2H, synchronous on c->s). */
 cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale_time,
                                      double polish_time, int polish_iters);
+/* The same in two halves: launch the unscale kernels and the device-to-host
+ * copies into caller buffers (asynchronous on c->s), and fill the scalars. */
+int dpdlp_i_result_launch(ctx_t *c, double *x_host, double *y_host, double *rc_host);
+void dpdlp_i_result_scalars(cupdlpx_result_t *res, const ctx_t *c, const hstate_t *h,
+                            double rescale_time, double polish_time, int polish_iters);
 void dpdlp_i_ctx_free(ctx_t *c);
 /* Initial control state after the step size is known. */
 void dpdlp_i_init_hstate(hstate_t *h, const ctx_t *c, const pdhg_parameters_t *params,
@@ -59,3 +64,7 @@ int dpdlp_i_batch_reserve(dpdlp_batch_t *b);
 void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int slot);
 int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int slot, dpdlp_resident_t *R, int take_ownership,
                           long long tag, void *stream);
+/* frees retired problems and trims pooled arenas to keep_bytes (call from a
+ * set-up thread: device frees may synchronise the device) */
+void dpdlp_i_batch_reap(dpdlp_batch_t *b, size_t keep_bytes);
+size_t dpdlp_i_batch_pool_bytes(dpdlp_batch_t *b);

@@ -76,6 +76,7 @@ def _load() -> C.CDLL:
         "dpdlp_batch_counters": (None, [C.c_void_p, C.POINTER(C.c_longlong),
                                         C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "dpdlp_batch_free": (None, [C.c_void_p]),
+        "dpdlp_batch_phase_stats": (None, [C.c_void_p, _ip, _dp, _dp, _dp, _dp]),
         "dpdlp_solve_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int,
                                         C.POINTER(PdhgParameters), C.POINTER(BatchOpts),
                                         C.POINTER(R)]),
@@ -354,12 +355,14 @@ class Resident:
             pass
 
 
-def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=1024, mixed_switch_tol=None):
+def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=1024, mixed_switch_tol=None,
+               profile_phases=0):
     o = BatchOpts()
     lib.dpdlp_batch_default_opts(C.byref(o))
     o.max_lps = max_lps
     o.precision = precision
     o.spmv_cap = spmv_cap
+    o.profile_phases = profile_phases
     if mixed_switch_tol is not None:
         o.mixed_switch_tol = mixed_switch_tol
     return o
@@ -407,6 +410,15 @@ class Batch:
         lib.dpdlp_batch_counters(self.h, C.byref(w), C.byref(s_), C.byref(l_))
         return {"windows": w.value, "syncs": s_.value, "launches": l_.value}
 
+    def phase_stats(self):
+        """Sampled phase-kernel timing: samples, ms and algorithmic bytes of each phase."""
+        n = C.c_int()
+        a, b, c, d = (C.c_double() for _ in range(4))
+        lib.dpdlp_batch_phase_stats(self.h, C.byref(n), C.byref(a), C.byref(b), C.byref(c),
+                                    C.byref(d))
+        return {"samples": n.value, "primal_ms": a.value, "primal_bytes": b.value,
+                "dual_ms": c.value, "dual_bytes": d.value}
+
     def run(self):
         """Steps until every LP finished; returns {tag: Result}."""
         res = {}
@@ -430,8 +442,9 @@ class Batch:
             pass
 
 
-def solve_batch(residents, params=None, **opt_kw):
-    """dpdlp_solve_batch over Resident objects; list of Results in input order."""
+def solve_batch(residents, params=None, stats=None, **opt_kw):
+    """dpdlp_solve_batch over Resident objects; list of Results in input order.
+    stats (a dict) receives the batch counters and phase timing."""
     o = batch_opts(**opt_kw)
     pp = params if params is not None else default_params()
     arr = (C.c_void_p * len(residents))(*[r.h for r in residents])
@@ -439,6 +452,10 @@ def solve_batch(residents, params=None, **opt_kw):
     k = lib.dpdlp_solve_batch(arr, len(residents), C.byref(pp), C.byref(o), out)
     if k != len(residents):
         raise RuntimeError("batch solve failed: " + last_error())
+    if stats is not None:
+        for name in ("windows", "syncs", "launches", "phase_samples", "primal_ms",
+                     "primal_bytes", "dual_ms", "dual_bytes"):
+            stats[name] = getattr(o, name)
     return [result_from_ptr(out[i]) for i in range(len(residents))]
 
 

@@ -122,6 +122,9 @@ typedef struct {
 /* Batched steps: every launch covers all LPs whose control flag is set. */
 int dpdlp_bdev_restart_copy(const dpdlp_batch_dev_t *b, void *stream);
 int dpdlp_bdev_phases(const dpdlp_batch_dev_t *b, void *stream);
+/* the two halves of dpdlp_bdev_phases, for timing them apart */
+int dpdlp_bdev_primal(const dpdlp_batch_dev_t *b, void *stream);
+int dpdlp_bdev_dual(const dpdlp_batch_dev_t *b, void *stream);
 int dpdlp_bdev_fpe(const dpdlp_batch_dev_t *b, void *stream);
 int dpdlp_bdev_residual(const dpdlp_batch_dev_t *b, void *stream);
 int dpdlp_bdev_infeasibility(const dpdlp_batch_dev_t *b, void *stream);

@@ -107,6 +107,12 @@ typedef struct {
   int precision;           /* DPDLP_PREC_FP64 (default) or DPDLP_PREC_MIXED */
   double mixed_switch_tol; /* mixed: FP64 from this max relative KKT on */
   int spmv_cap;            /* row-block capacity of every LP (default 1024) */
+  int profile_phases;      /* 1: HIP events around the two phase kernels of
+                              round 1 of every 4th window (dpdlp_batch_phase_stats) */
+  /* outputs of dpdlp_solve_batch */
+  long long windows, syncs, launches;
+  int phase_samples;
+  double primal_ms, primal_bytes, dual_ms, dual_bytes;
 } dpdlp_batch_opts_t;
 
 void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o);
@@ -134,12 +140,19 @@ size_t dpdlp_batch_bytes(const dpdlp_batch_t *b);
 /* Counters: windows run, host-device synchronisations, kernel launches. */
 void dpdlp_batch_counters(const dpdlp_batch_t *b, long long *windows, long long *syncs,
                           long long *launches);
+/* Live phase timing (profile_phases): sampled launches, their summed kernel
+ * time (ms) and summed algorithmic bytes, for the primal (A'y) and dual (Ax)
+ * phase kernels. */
+void dpdlp_batch_phase_stats(const dpdlp_batch_t *b, int *samples, double *primal_ms,
+                             double *primal_bytes, double *dual_ms, double *dual_bytes);
 void dpdlp_batch_free(dpdlp_batch_t *b);
 
 /* Solves `count` resident problems in one batch; results[i] belongs to
- * res[i].  Returns the number of results (count) or -1. */
+ * res[i].  The LPs are set up on a second host thread and stream while the
+ * batch already steps.  o (optional) also receives the counters and phase
+ * timing.  Returns the number of results (count) or -1. */
 int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parameters_t *params,
-                      const dpdlp_batch_opts_t *o, cupdlpx_result_t **results);
+                      dpdlp_batch_opts_t *o, cupdlpx_result_t **results);
 
 /* ---- serving: a Poisson request stream with continuous batching ---- */
 
