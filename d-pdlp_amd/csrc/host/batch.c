@@ -129,6 +129,16 @@ struct dpdlp_batch {
   int ndead, dead_cap;
   long long windows, syncs, launches;
   size_t state_bytes;
+  /* window graphs: captured launch sequences, keyed by the list epoch, the
+   * slot count and which kernel kinds each round launches */
+  struct {
+    long long epoch;
+    int nl;
+    unsigned long long sig;
+    hipGraphExec_t exec;
+    long long used;
+  } gcache[8];
+  long long epoch, graph_replays, graph_captures;
   /* live phase timing */
   hipEvent_t pev[32][3];
   double pev_bytes[32][2];
@@ -195,6 +205,8 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
     if (b->ev_win[k]) hipEventDestroy(b->ev_win[k]);
   for (int k = 0; k < b->npev_created; ++k)
     for (int q = 0; q < 3; ++q) hipEventDestroy(b->pev[k][q]);
+  for (int k = 0; k < 8; ++k)
+    if (b->gcache[k].exec) hipGraphExecDestroy(b->gcache[k].exec);
   if (b->s) hipStreamDestroy(b->s);
   pthread_mutex_destroy(&b->mu);
   free(b);
@@ -293,6 +305,8 @@ size_t dpdlp_batch_bytes(const dpdlp_batch_t *b) {
     if (active(&b->sl[i])) s += b->sl[i].c.arena_len;
   return s;
 }
+
+long long dpdlp_batch_graph_replays(const dpdlp_batch_t *b) { return b ? b->graph_replays : 0; }
 
 void dpdlp_batch_counters(const dpdlp_batch_t *b, long long *windows, long long *syncs,
                           long long *launches) {
@@ -653,6 +667,12 @@ static int rebuild_lists(dpdlp_batch_t *b) {
   d->stride = (int)need_stride;
   d->cap = b->o.spmv_cap;
   b->dirty = 0;
+  b->epoch++;
+  for (int k = 0; k < 8; ++k)
+    if (b->gcache[k].exec) {
+      hipGraphExecDestroy(b->gcache[k].exec);
+      b->gcache[k].exec = NULL;
+    }
   return 0;
 }
 
@@ -942,6 +962,56 @@ static void process_pi(dpdlp_batch_t *b, int i) {
 
 /* ---- one window ---- */
 
+/* the window's launches, round by round; ctl (host copy) non-NULL: time the
+ * two phase kernels of one round with events */
+static int issue_rounds(dpdlp_batch_t *b, int nl, int any_restart, const int *any_phase,
+                        const int *any_fpe, const int *any_pi, const dpdlp_bctl_t *ctl) {
+  const int W = b->W;
+  for (int r = 0; r < W; ++r) {
+    b->bd.ctl = b->d_ctl_win + (size_t)r * nl;
+    b->bd.nlp = nl;
+    if (r == 0 && any_restart) {
+      if (dpdlp_bdev_restart_copy(&b->bd, b->s)) return -1;
+      b->launches += 1;
+    }
+    if (any_phase[r]) {
+      int timed = ctl && b->npev_used < b->npev_created && r == (W > 1 ? 1 : 0);
+      if (timed) {
+        hipEvent_t *ev = b->pev[b->npev_used];
+        double s1 = 0.0, s2 = 0.0;
+        for (int i = 0; i < nl; ++i) {
+          const dpdlp_bctl_t *c = &ctl[(size_t)r * nl + i];
+          if (!c->phase) continue;
+          double a1, a2;
+          phase_bytes(b->sl[i].R, c, &a1, &a2);
+          s1 += a1;
+          s2 += a2;
+        }
+        b->pev_bytes[b->npev_used][0] = s1;
+        b->pev_bytes[b->npev_used][1] = s2;
+        b->npev_used++;
+        BCHK(hipEventRecord(ev[0], b->s));
+        if (dpdlp_bdev_primal(&b->bd, b->s)) return -1;
+        BCHK(hipEventRecord(ev[1], b->s));
+        if (dpdlp_bdev_dual(&b->bd, b->s)) return -1;
+        BCHK(hipEventRecord(ev[2], b->s));
+      } else {
+        if (dpdlp_bdev_phases(&b->bd, b->s)) return -1;
+      }
+      b->launches += 2;
+    }
+    if (any_fpe[r]) {
+      if (dpdlp_bdev_fpe(&b->bd, b->s)) return -1;
+      b->launches += 5;
+    }
+    if (any_pi[r]) {
+      if (dpdlp_bdev_power_step(&b->bd, b->s)) return -1;
+      b->launches += 5;
+    }
+  }
+  return 0;
+}
+
 static int launch_window(dpdlp_batch_t *b) {
   const int W = b->W, nl = b->nhw;
   const pdhg_parameters_t *p = &b->prm;
@@ -1001,47 +1071,54 @@ static int launch_window(dpdlp_batch_t *b) {
                       hipMemcpyHostToDevice, b->s));
   BCHK(hipEventRecord(b->ev_win[k], b->s));
   b->win_buf ^= 1;
-  for (int r = 0; r < W; ++r) {
-    b->bd.ctl = b->d_ctl_win + (size_t)r * nl;
-    b->bd.nlp = nl;
-    if (r == 0 && any_restart) {
-      if (dpdlp_bdev_restart_copy(&b->bd, b->s)) return -1;
-      b->launches += 1;
-    }
-    if (any_phase[r]) {
-      int timed = b->npev_used < b->npev_created && (b->windows & 3) == 0 && r == (W > 1 ? 1 : 0);
-      if (timed) {
-        hipEvent_t *ev = b->pev[b->npev_used];
-        double s1 = 0.0, s2 = 0.0;
-        for (int i = 0; i < nl; ++i) {
-          const dpdlp_bctl_t *c = &ctl[(size_t)r * nl + i];
-          if (!c->phase) continue;
-          double a1, a2;
-          phase_bytes(b->sl[i].R, c, &a1, &a2);
-          s1 += a1;
-          s2 += a2;
-        }
-        b->pev_bytes[b->npev_used][0] = s1;
-        b->pev_bytes[b->npev_used][1] = s2;
-        b->npev_used++;
-        BCHK(hipEventRecord(ev[0], b->s));
-        if (dpdlp_bdev_primal(&b->bd, b->s)) return -1;
-        BCHK(hipEventRecord(ev[1], b->s));
-        if (dpdlp_bdev_dual(&b->bd, b->s)) return -1;
-        BCHK(hipEventRecord(ev[2], b->s));
-      } else {
-        if (dpdlp_bdev_phases(&b->bd, b->s)) return -1;
+  int timed_window = b->npev_used < b->npev_created && (b->windows & 3) == 0;
+  unsigned long long sig = (unsigned long long)any_restart;
+  for (int r = 0; r < W; ++r)
+    sig |= (unsigned long long)(any_phase[r] | (any_fpe[r] << 1) | (any_pi[r] << 2)) << (1 + 3 * r);
+  if (b->o.use_graph && !timed_window) {
+    int slot = -1, lru = 0;
+    for (int g = 0; g < 8; ++g) {
+      if (b->gcache[g].exec && b->gcache[g].epoch == b->epoch && b->gcache[g].nl == nl &&
+          b->gcache[g].sig == sig) {
+        slot = g;
+        break;
       }
-      b->launches += 2;
+      if (!b->gcache[g].exec || b->gcache[g].used < b->gcache[lru].used) lru = g;
     }
-    if (any_fpe[r]) {
-      if (dpdlp_bdev_fpe(&b->bd, b->s)) return -1;
-      b->launches += 5;
+    if (slot < 0) {
+      /* capture this window's launches once; later windows with the same
+       * shape replay it (their per-LP parameters live in d_ctl_win) */
+      hipGraph_t graph = NULL;
+      BCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+      int e = issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi, NULL);
+      hipError_t ce = hipStreamEndCapture(b->s, &graph);
+      if (e || ce != hipSuccess) {
+        if (graph) hipGraphDestroy(graph);
+        dpdlp_set_error("window graph capture failed");
+        return -1;
+      }
+      if (b->gcache[lru].exec) hipGraphExecDestroy(b->gcache[lru].exec);
+      b->gcache[lru].exec = NULL;
+      hipError_t ie = hipGraphInstantiate(&b->gcache[lru].exec, graph, NULL, NULL, 0);
+      hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        b->gcache[lru].exec = NULL;
+        dpdlp_set_error("window graph instantiation failed: %s", hipGetErrorString(ie));
+        return -1;
+      }
+      b->gcache[lru].epoch = b->epoch;
+      b->gcache[lru].nl = nl;
+      b->gcache[lru].sig = sig;
+      slot = lru;
+      b->graph_captures++;
     }
-    if (any_pi[r]) {
-      if (dpdlp_bdev_power_step(&b->bd, b->s)) return -1;
-      b->launches += 5;
-    }
+    b->gcache[slot].used = b->windows;
+    BCHK(hipGraphLaunch(b->gcache[slot].exec, b->s));
+    b->graph_replays++;
+  } else {
+    if (issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi,
+                     timed_window ? ctl : NULL))
+      return -1;
   }
   b->windows++;
   return 0;
@@ -1212,6 +1289,7 @@ int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parame
   }
   if (o) {
     dpdlp_batch_counters(b, &o->windows, &o->syncs, &o->launches);
+    o->graph_replays = b->graph_replays;
     dpdlp_batch_phase_stats(b, &o->phase_samples, &o->primal_ms, &o->primal_bytes, &o->dual_ms,
                             &o->dual_bytes);
   }

@@ -948,6 +948,24 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     dpdlp_set_error("termination_evaluation_frequency must be positive");
     return NULL;
   }
+  if (opts->use_graph && R->A_P == 1 && R->At_P == 1 && opts->trace_points <= 0 &&
+      !opts->profile_phases) {
+    /* the windowed engine with a single LP: per-iteration parameters live in
+     * device memory, so each window of iterations replays one hipGraph */
+    dpdlp_batch_opts_t bo;
+    dpdlp_batch_default_opts(&bo);
+    bo.max_lps = 1;
+    bo.precision = opts->precision;
+    bo.mixed_switch_tol = opts->mixed_switch_tol;
+    bo.spmv_cap = R->cap;
+    bo.use_graph = 1;
+    cupdlpx_result_t *r1 = NULL;
+    if (dpdlp_solve_batch(&R, 1, params, &bo, &r1) != 1) return NULL;
+    opts->total_time_sec = dpdlp_wall() - t_begin;
+    opts->loop_time_sec = r1->cumulative_time_sec;
+    if (!opts->quiet) dpdlp_print_final(r1, params);
+    return r1;
+  }
   int m = R->m, n = R->n, nnz = R->nnz;
   dpdlp_state_t *st = &c->st;
   hstate_t H;

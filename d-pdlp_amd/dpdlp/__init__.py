@@ -77,6 +77,7 @@ def _load() -> C.CDLL:
                                         C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
         "dpdlp_batch_free": (None, [C.c_void_p]),
         "dpdlp_batch_phase_stats": (None, [C.c_void_p, _ip, _dp, _dp, _dp, _dp]),
+        "dpdlp_batch_graph_replays": (C.c_longlong, [C.c_void_p]),
         "dpdlp_solve_batch": (C.c_int, [C.POINTER(C.c_void_p), C.c_int,
                                         C.POINTER(PdhgParameters), C.POINTER(BatchOpts),
                                         C.POINTER(R)]),
@@ -264,13 +265,14 @@ def result_from_ptr(rp, free_fn=None) -> Result:
 
 
 def make_opts(precision=PREC_FP64, quiet=True, spmv_cap=0, profile_phases=0,
-              trace_iters=None, n=0, m=0, mixed_switch_tol=None):
+              trace_iters=None, n=0, m=0, mixed_switch_tol=None, use_graph=0):
     o = SolveOpts()
     lib.dpdlp_default_opts(C.byref(o))
     o.precision = precision
     o.quiet = int(quiet)
     o.spmv_cap = spmv_cap
     o.profile_phases = profile_phases
+    o.use_graph = use_graph
     if mixed_switch_tol is not None:
         o.mixed_switch_tol = mixed_switch_tol
     keep = []
@@ -356,13 +358,14 @@ class Resident:
 
 
 def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=1024, mixed_switch_tol=None,
-               profile_phases=0):
+               profile_phases=0, use_graph=0):
     o = BatchOpts()
     lib.dpdlp_batch_default_opts(C.byref(o))
     o.max_lps = max_lps
     o.precision = precision
     o.spmv_cap = spmv_cap
     o.profile_phases = profile_phases
+    o.use_graph = use_graph
     if mixed_switch_tol is not None:
         o.mixed_switch_tol = mixed_switch_tol
     return o
@@ -408,7 +411,8 @@ class Batch:
     def counters(self):
         w, s_, l_ = C.c_longlong(), C.c_longlong(), C.c_longlong()
         lib.dpdlp_batch_counters(self.h, C.byref(w), C.byref(s_), C.byref(l_))
-        return {"windows": w.value, "syncs": s_.value, "launches": l_.value}
+        return {"windows": w.value, "syncs": s_.value, "launches": l_.value,
+                "graph_replays": lib.dpdlp_batch_graph_replays(self.h)}
 
     def phase_stats(self):
         """Sampled phase-kernel timing: samples, ms and algorithmic bytes of each phase."""
@@ -453,7 +457,7 @@ def solve_batch(residents, params=None, stats=None, **opt_kw):
     if k != len(residents):
         raise RuntimeError("batch solve failed: " + last_error())
     if stats is not None:
-        for name in ("windows", "syncs", "launches", "phase_samples", "primal_ms",
+        for name in ("windows", "syncs", "launches", "graph_replays", "phase_samples", "primal_ms",
                      "primal_bytes", "dual_ms", "dual_bytes"):
             stats[name] = getattr(o, name)
     return [result_from_ptr(out[i]) for i in range(len(residents))]

@@ -109,8 +109,10 @@ typedef struct {
   int spmv_cap;            /* row-block capacity of every LP (default 1024) */
   int profile_phases;      /* 1: HIP events around the two phase kernels of
                               round 1 of every 4th window (dpdlp_batch_phase_stats) */
+  int use_graph;           /* 1: capture each window's launches in a hipGraph
+                              and replay it while the batch keeps its shape */
   /* outputs of dpdlp_solve_batch */
-  long long windows, syncs, launches;
+  long long windows, syncs, launches, graph_replays;
   int phase_samples;
   double primal_ms, primal_bytes, dual_ms, dual_bytes;
 } dpdlp_batch_opts_t;
@@ -137,9 +139,11 @@ int dpdlp_batch_take(dpdlp_batch_t *b, long long *tag, cupdlpx_result_t **res,
 int dpdlp_batch_in_flight(const dpdlp_batch_t *b);
 /* Device bytes the batch holds for LP state beyond the resident problems. */
 size_t dpdlp_batch_bytes(const dpdlp_batch_t *b);
-/* Counters: windows run, host-device synchronisations, kernel launches. */
+/* Counters: windows run, host-device synchronisations, kernel launches
+ * (captured ones included), windows replayed from a graph. */
 void dpdlp_batch_counters(const dpdlp_batch_t *b, long long *windows, long long *syncs,
                           long long *launches);
+long long dpdlp_batch_graph_replays(const dpdlp_batch_t *b);
 /* Live phase timing (profile_phases): sampled launches, their summed kernel
  * time (ms) and summed algorithmic bytes, for the primal (A'y) and dual (Ax)
  * phase kernels. */
