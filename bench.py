@@ -81,6 +81,16 @@ def cpu_batch_baseline(probs, budget_s):
                        f"to eps 1e-4 one after another: {done} LPs in {el:.1f} s")
 
 
+def batch_traffic(kernel):
+    """Mean HBM bytes per launch of a batched phase kernel, from the committed
+    counter profile (FETCH_SIZE/WRITE_SIZE passes over this same leg)."""
+    tfile = os.path.join(ROOT, "profiles", "cfg3_batch_hbm_traffic.json")
+    if not os.path.exists(tfile):
+        return None
+    with open(tfile) as f:
+        return json.load(f).get("bytes_per_launch", {}).get(kernel)
+
+
 def bench_batch(args, dpdlp):
     w = WORKLOADS["cfg3"]
     count = args.batch_count or w["count"]
@@ -90,18 +100,22 @@ def bench_batch(args, dpdlp):
     nnz = sum(p.nnz for p, _ in probs)
     print(f"# generated {count} LPs, nnz={nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
     t = time.perf_counter()
-    plain_res = [dpdlp.Resident(p) for p, _ in probs]
+    plain_res = [dpdlp.Resident(p) for p, _ in probs] if not args.no_plain else []
     batch_res = [dpdlp.Resident(p, spmv_cap=1024, panels=(1, 1)) for p, _ in probs]
     up_s = time.perf_counter() - t
     params = dpdlp.default_params(presolve=False)
     for k in range(args.warmup):
-        plain_res[0].solve(params)
+        if plain_res:
+            plain_res[0].solve(params)
         dpdlp.solve_batch(batch_res[:4], params, spmv_cap=1024, precision=dpdlp.PREC_MIXED)
     # plain loop: the single-LP FP64 path, one LP after another
-    t = time.perf_counter()
-    plain = [r.solve(params)[0] for r in plain_res]
-    plain_s = time.perf_counter() - t
-    print(f"# plain loop: {plain_s:.2f}s {count / plain_s:.1f} LPs/s", file=sys.stderr, flush=True)
+    plain, plain_s = None, None
+    if not args.no_plain:
+        t = time.perf_counter()
+        plain = [r.solve(params)[0] for r in plain_res]
+        plain_s = time.perf_counter() - t
+        print(f"# plain loop: {plain_s:.2f}s {count / plain_s:.1f} LPs/s", file=sys.stderr,
+              flush=True)
 
     def run_batch(prec, profile):
         st = {}
@@ -122,11 +136,16 @@ def bench_batch(args, dpdlp):
         print(f"# batch mixed step {k}: {el:.2f}s {count / el:.1f} LPs/s {cnt}", file=sys.stderr,
               flush=True)
     mixed, mixed_ph, mixed_cnt = res, ph, cnt
-    fp64, fp64_s, fp64_cnt, _ = run_batch(dpdlp.PREC_FP64, 0)
-    print(f"# batch fp64: {fp64_s:.2f}s {count / fp64_s:.1f} LPs/s", file=sys.stderr, flush=True)
+    fp64, fp64_s, fp64_cnt = None, None, None
+    if not args.no_fp64_batch:
+        fp64, fp64_s, fp64_cnt, _ = run_batch(dpdlp.PREC_FP64, 0)
+        print(f"# batch fp64: {fp64_s:.2f}s {count / fp64_s:.1f} LPs/s", file=sys.stderr,
+              flush=True)
     step_s = sum(runs) / len(runs)
 
     def obj_err(a, b):
+        if a is None or b is None:
+            return None
         return max(abs(x.primal_obj - y.primal_obj) / max(1.0, abs(y.primal_obj))
                    for x, y in zip(a, b))
     statuses = {}
@@ -152,18 +171,22 @@ def bench_batch(args, dpdlp):
                    "m_total": sum(p.m for p, _ in probs), "n_total": sum(p.n for p, _ in probs),
                    "parallelism": "one GPU, batched launches"},
         "statuses": statuses,
-        "plain_loop": {"s": plain_s, "LPs_per_s": count / plain_s, "dtype": "f64",
-                       "iterations_mean": sum(r.iterations for r in plain) / count},
-        "speedup_vs_plain_loop": plain_s / step_s,
-        "batch_fp64": {"s": fp64_s, "LPs_per_s": count / fp64_s,
-                       "speedup_vs_plain_loop": plain_s / fp64_s, "counters": fp64_cnt},
+        "plain_loop": None if plain is None else {
+            "s": plain_s, "LPs_per_s": count / plain_s, "dtype": "f64",
+            "iterations_mean": sum(r.iterations for r in plain) / count},
+        "speedup_vs_plain_loop": None if plain is None else plain_s / step_s,
+        "batch_fp64": None if fp64 is None else {
+            "s": fp64_s, "LPs_per_s": count / fp64_s,
+            "speedup_vs_plain_loop": None if plain is None else plain_s / fp64_s,
+            "counters": fp64_cnt},
         "batch_counters": mixed_cnt,
         "objective_rel_diff": {"mixed_vs_plain": obj_err(mixed, plain),
                                "fp64_batch_vs_plain": obj_err(fp64, plain)},
         "phase_ms_per_sample": {"primal": mixed_ph["primal_ms"] / n_s,
                                 "dual": mixed_ph["dual_ms"] / n_s, "samples": mixed_ph["samples"]},
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": dom[1], "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": (dom[1] or 0) / HBM_PEAK_GBPS, "traffic": None,
+                     "unit": "GB/s", "frac": (dom[1] or 0) / HBM_PEAK_GBPS,
+                     "traffic": batch_traffic(dom[0]),
                      "algorithmic_bytes_per_launch": dom[3] / n_s},
         "phase_GBps": {"primal": p1, "dual": p2},
         "cpu_baseline": cpu, "generate_s": gen_s, "upload_s": up_s,
@@ -297,6 +320,8 @@ def main():
     ap.add_argument("--verbose", action="store_true", help="print the solver's iteration log")
     ap.add_argument("--batch-count", type=int, default=0, help="cfg3: LPs in the batch")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="cfg3: CPU sample seconds")
+    ap.add_argument("--no-plain", action="store_true", help="cfg3: skip the plain-loop leg")
+    ap.add_argument("--no-fp64-batch", action="store_true", help="cfg3: skip the FP64 batch leg")
     ap.add_argument("--rates", default="", help="cfg4: comma-separated arrival rates")
     ap.add_argument("--serve-duration", type=float, default=30.0, help="cfg4: arrival window s")
     ap.add_argument("--serve-drain", type=float, default=30.0, help="cfg4: drain cap s")
