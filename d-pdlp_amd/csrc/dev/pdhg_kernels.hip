@@ -21,14 +21,6 @@
 #include "dpdlp_dev.h"
 
 
-/* grid for one wavefront per row (4 rows per workgroup), capped */
-static inline int wave_grid(long long rows) {
-  long long g = (rows + 3) / 4;
-  if (g < 1) g = 1;
-  if (g > 16384) g = 16384;
-  return (int)g;
-}
-
 static inline int ew_grid(long long n) { return dpdlp_ew_grid(n); }
 
 #include "spmv_core.h"
@@ -408,32 +400,35 @@ __global__ void k_unscale(const double *x, const double *y, const double *D_c, c
 
 /* ---- scaling (preconditioner.c) ---- */
 
-/* Per-row reduction of |a|, one wavefront per row with coalesced loads:
+/* Per-row reduction of |a| by a group of G lanes per row (G = 4..64, picked
+ * from the mean row length), loads coalesced within the group:
  * mode 0 = max (exact in any order); mode 1 = sum of |a|^p accumulated
- * strictly in storage order by lane 0 (so column sums over the transpose
- * match a row-major host accumulation bit for bit); pow is skipped for
- * p == 1, which is exact. */
+ * strictly in storage order by the group's first lane (so column sums over
+ * the transpose match a row-major host accumulation bit for bit); pow is
+ * skipped for p == 1, which is exact. */
+template <int G>
 __global__ __launch_bounds__(DPDLP_BS) void k_row_reduce(const int *rp, const double *v, int rows,
                                                         int mode, double p, double *out) {
-  const int lane = threadIdx.x & 63;
-  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long long nwave = ((long long)gridDim.x * blockDim.x) >> 6;
-  for (long long r = wave0; r < rows; r += nwave) {
+  const int gl = threadIdx.x & (G - 1);
+  const long long g0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const long long ng = ((long long)gridDim.x * blockDim.x) / G;
+  for (long long r = g0; r < rows; r += ng) {
     const int a = rp[r], e = rp[r + 1];
     double acc = 0.0;
     if (mode == 0) {
-      for (int q = a + lane; q < e; q += 64) acc = fmax(acc, fabs(v[q]));
-      acc = wave_max(acc);
+      for (int q = a + gl; q < e; q += G) acc = fmax(acc, fabs(v[q]));
+#pragma unroll
+      for (int off = G / 2; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off, G));
     } else {
-      for (int base = a; base < e; base += 64) {
-        const int q = base + lane;
+      for (int base = a; base < e; base += G) {
+        const int q = base + gl;
         double t = 0.0;
         if (q < e) t = (p == 1.0) ? fabs(v[q]) : pow(fabs(v[q]), p);
-        const int cnt = (e - base) < 64 ? (e - base) : 64;
-        for (int j = 0; j < cnt; ++j) acc += __shfl(t, j, 64);
+        const int cnt = (e - base) < G ? (e - base) : G;
+        for (int j = 0; j < cnt; ++j) acc += __shfl(t, j, G);
       }
     }
-    if (lane == 0) out[r] = acc;
+    if (gl == 0) out[r] = acc;
   }
 }
 
@@ -448,23 +443,77 @@ __global__ void k_scale_factors(double *f, double *D, int count) {
   }
 }
 
-/* a_k /= (rs[row] * cs[col]) for one CSR matrix, one wavefront per row;
+/* a_k /= (rs[row] * cs[col]) for one CSR matrix, G lanes per row;
  * `rowf` scales rows, `colf` the column index space (pass them swapped for
  * the transpose: the product is formed as rs[i] * cs[j] in both). */
+template <int G>
 __global__ __launch_bounds__(DPDLP_BS) void k_scale_values(const int *rp, const int *ci, double *v,
                                                           int rows, const double *rowf,
                                                           const double *colf, int row_first) {
-  const int lane = threadIdx.x & 63;
-  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const long long nwave = ((long long)gridDim.x * blockDim.x) >> 6;
-  for (long long r = wave0; r < rows; r += nwave) {
+  const int gl = threadIdx.x & (G - 1);
+  const long long g0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const long long ng = ((long long)gridDim.x * blockDim.x) / G;
+  for (long long r = g0; r < rows; r += ng) {
     const double fr = rowf[r];
     const int e = rp[r + 1];
-    for (int q = rp[r] + lane; q < e; q += 64) {
+    for (int q = rp[r] + gl; q < e; q += G) {
       const double fc = colf[ci[q]];
       const double d = row_first ? fr * fc : fc * fr;
       v[q] /= d;
     }
+  }
+}
+
+/* lanes per row for a mean row length: the next power of two, 4..64 */
+static inline int lanes_per_row(long long nnz, long long rows) {
+  long long mean = rows > 0 ? (nnz + rows - 1) / rows : 1;
+  int g = 4;
+  while (g < 64 && g < mean) g <<= 1;
+  return g;
+}
+
+static inline int group_grid(long long rows, int G) {
+  long long per = DPDLP_BS / G;
+  long long g = (rows + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > 16384) g = 16384;
+  return (int)g;
+}
+
+static void row_reduce(const dpdlp_csr_t *A, int rows, int mode, double p, double *out,
+                       hipStream_t s) {
+  const int G = lanes_per_row(A->nnz, rows);
+  const int grid = group_grid(rows, G);
+  switch (G) {
+  case 4: k_row_reduce<4><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
+  case 8: k_row_reduce<8><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
+  case 16: k_row_reduce<16><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
+  case 32: k_row_reduce<32><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
+  default: k_row_reduce<64><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
+  }
+}
+
+static void scale_values(const dpdlp_csr_t *A, int rows, const double *rowf, const double *colf,
+                         int row_first, hipStream_t s) {
+  const int G = lanes_per_row(A->nnz, rows);
+  const int grid = group_grid(rows, G);
+  double *v = (double *)A->v;
+  switch (G) {
+  case 4:
+    k_scale_values<4><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
+    break;
+  case 8:
+    k_scale_values<8><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
+    break;
+  case 16:
+    k_scale_values<16><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
+    break;
+  case 32:
+    k_scale_values<32><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
+    break;
+  default:
+    k_scale_values<64><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
+    break;
   }
 }
 
@@ -817,33 +866,28 @@ int dpdlp_dev_result(const dpdlp_state_t *st, double beta, double gamma, double 
 
 int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream) {
   hipStream_t s = ST(stream);
-  k_row_reduce<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 0, 1.0, rs);
-  k_row_reduce<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 0, 1.0, cs);
+  row_reduce(&st->A, st->m, 0, 1.0, rs, s);
+  row_reduce(&st->At, st->n, 0, 1.0, cs, s);
   k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
   k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
   k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
       st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
-  k_scale_values<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
-                                                     st->m, rs, cs, 1);
-  k_scale_values<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
-                                                     st->n, cs, rs, 0);
+  scale_values(&st->A, st->m, rs, cs, 1, s);
+  scale_values(&st->At, st->n, cs, rs, 0, s);
   return (int)hipGetLastError();
 }
 
 int dpdlp_dev_pock_chambolle(dpdlp_state_t *st, double alpha, double *rs, double *cs,
                              void *stream) {
   hipStream_t s = ST(stream);
-  k_row_reduce<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, 1, alpha, rs);
-  k_row_reduce<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.v, st->n, 1, 2.0 - alpha,
-                                                     cs);
+  row_reduce(&st->A, st->m, 1, alpha, rs, s);
+  row_reduce(&st->At, st->n, 1, 2.0 - alpha, cs, s);
   k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
   k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
   k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
       st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
-  k_scale_values<<<wave_grid(st->m), DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v,
-                                                     st->m, rs, cs, 1);
-  k_scale_values<<<wave_grid(st->n), DPDLP_BS, 0, s>>>(st->At.rp, st->At.ci, (double *)st->At.v,
-                                                     st->n, cs, rs, 0);
+  scale_values(&st->A, st->m, rs, cs, 1, s);
+  scale_values(&st->At, st->n, cs, rs, 0, s);
   return (int)hipGetLastError();
 }
 

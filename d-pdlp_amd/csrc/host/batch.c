@@ -115,7 +115,7 @@ struct dpdlp_batch {
   long long part_len;
   double *d_normals;
   int normals_len;
-  dpdlp_batch_dev_t bd;
+  dpdlp_batch_dev_t bd, bd_pi; /* lists of the iterating / power-iteration LPs */
   bdone_t *done;
   int ndone, done_head, done_cap;
   bpend_t *pend;
@@ -387,6 +387,7 @@ static void start_loop(dpdlp_batch_t *b, bslot_t *q) {
   q->power_time = dpdlp_wall() - q->t_pi0;
   q->t_loop0 = dpdlp_wall();
   set_stage(q, S_PDHG);
+  b->dirty = 1; /* the LP moves to the iterating lists */
   q->do_restart = 0;
   q->fresh = 1;
 }
@@ -557,21 +558,27 @@ int dpdlp_batch_add(dpdlp_batch_t *b, dpdlp_resident_t *R, int take_ownership, l
 /* ---- block lists ---- */
 
 static int rebuild_lists(dpdlp_batch_t *b) {
-  long long cnt[NLIST] = {0, 0, 0, 0, 0};
+  /* every list is split into the LPs iterating (group 0) and the LPs still in
+   * power iteration (group 1), so each launch only covers the LPs it serves */
+  long long cnt[2][NLIST];
+  memset(cnt, 0, sizeof(cnt));
   for (int i = 0; i < b->nhw; ++i) {
     bslot_t *q = &b->sl[i];
     if (!active(q)) continue;
+    const int g = stage_of(q) == S_PI;
     const dpdlp_resident_t *R = q->R;
-    cnt[0] += R->A_nblk;
-    cnt[1] += R->At_nblk;
-    cnt[2] += dpdlp_ew_grid(R->n);
-    cnt[3] += dpdlp_ew_grid(R->m);
-    cnt[4] += dpdlp_ew_grid((long long)R->n + R->m);
+    cnt[g][0] += R->A_nblk;
+    cnt[g][1] += R->At_nblk;
+    cnt[g][2] += dpdlp_ew_grid(R->n);
+    cnt[g][3] += dpdlp_ew_grid(R->m);
+    cnt[g][4] += dpdlp_ew_grid((long long)R->n + R->m);
   }
-  long long total = cnt[0] + cnt[1] + cnt[2] + cnt[3] + cnt[4];
-  long long need_stride = 1;
-  for (int k = 0; k < NLIST; ++k)
-    if (cnt[k] > need_stride) need_stride = cnt[k];
+  long long total = 0, need_stride = 1;
+  for (int g = 0; g < 2; ++g)
+    for (int k = 0; k < NLIST; ++k) {
+      total += cnt[g][k];
+      if (cnt[g][k] > need_stride) need_stride = cnt[g][k];
+    }
   if (need_stride > 0x7fffffffLL) {
     dpdlp_set_error("batch too large (%lld work items in one list)", need_stride);
     return -1;
@@ -599,28 +606,34 @@ static int rebuild_lists(dpdlp_batch_t *b) {
     BCHK(hipMalloc((void **)&b->d_part, (size_t)nl * sizeof(double)));
     b->part_len = nl;
   }
-  long long off[NLIST];
-  off[0] = 0;
-  for (int k = 1; k < NLIST; ++k) off[k] = off[k - 1] + cnt[k - 1];
-  long long pos[NLIST];
-  for (int k = 0; k < NLIST; ++k) pos[k] = 0;
+  /* kind k occupies [off[0][k], off[0][k] + cnt[0][k] + cnt[1][k]): group 0 then group 1 */
+  long long off[2][NLIST];
+  long long run = 0;
+  for (int k = 0; k < NLIST; ++k) {
+    off[0][k] = run;
+    off[1][k] = run + cnt[0][k];
+    run += cnt[0][k] + cnt[1][k];
+  }
+  long long pos[2][NLIST];
+  memset(pos, 0, sizeof(pos));
   int *rng = b->h_rng;
   memset(rng, 0, (size_t)b->cap * 2 * NLIST * sizeof(int));
+  dpdlp_bblk_t *L = b->h_list;
   for (int i = 0; i < b->nhw; ++i) {
     bslot_t *q = &b->sl[i];
     if (!active(q)) continue;
+    const int g = stage_of(q) == S_PI;
     const dpdlp_resident_t *R = q->R;
-    for (int k = 0; k < NLIST; ++k) rng[(size_t)k * 2 * b->cap + 2 * i] = (int)pos[k];
-    dpdlp_bblk_t *L = b->h_list;
+    for (int k = 0; k < NLIST; ++k) rng[(size_t)k * 2 * b->cap + 2 * i] = (int)pos[g][k];
     for (int j = 0; j < R->A_nblk; ++j) {
-      dpdlp_bblk_t *e = &L[off[0] + pos[0]++];
+      dpdlp_bblk_t *e = &L[off[g][0] + pos[g][0]++];
       e->lp = i;
       e->a = R->h_A_blk[j];
       e->b = R->h_A_blk[j + 1];
       e->pad = 0;
     }
     for (int j = 0; j < R->At_nblk; ++j) {
-      dpdlp_bblk_t *e = &L[off[1] + pos[1]++];
+      dpdlp_bblk_t *e = &L[off[g][1] + pos[g][1]++];
       e->lp = i;
       e->a = R->h_At_blk[j];
       e->b = R->h_At_blk[j + 1];
@@ -630,14 +643,14 @@ static int rebuild_lists(dpdlp_batch_t *b) {
     for (int k = 0; k < 3; ++k) {
       int G = dpdlp_ew_grid(cnts[k]);
       for (int j = 0; j < G; ++j) {
-        dpdlp_bblk_t *e = &L[off[2 + k] + pos[2 + k]++];
+        dpdlp_bblk_t *e = &L[off[g][2 + k] + pos[g][2 + k]++];
         e->lp = i;
         e->a = j;
         e->b = G;
         e->pad = 0;
       }
     }
-    for (int k = 0; k < NLIST; ++k) rng[(size_t)k * 2 * b->cap + 2 * i + 1] = (int)pos[k];
+    for (int k = 0; k < NLIST; ++k) rng[(size_t)k * 2 * b->cap + 2 * i + 1] = (int)pos[g][k];
   }
   if (total > 0)
     BCHK(hipMemcpyAsync(b->d_list, b->h_list, (size_t)total * sizeof(dpdlp_bblk_t),
@@ -646,26 +659,28 @@ static int rebuild_lists(dpdlp_batch_t *b) {
                       hipMemcpyHostToDevice, b->s));
   BCHK(hipMemcpyAsync(b->d_lps, b->h_lps, (size_t)b->nhw * sizeof(dpdlp_blp_t),
                       hipMemcpyHostToDevice, b->s));
-  dpdlp_batch_dev_t *d = &b->bd;
-  d->lps = b->d_lps;
-  d->bA = b->d_list + off[0];
-  d->bAt = b->d_list + off[1];
-  d->eN = b->d_list + off[2];
-  d->eM = b->d_list + off[3];
-  d->eNM = b->d_list + off[4];
-  d->nbA = (int)cnt[0];
-  d->nbAt = (int)cnt[1];
-  d->neN = (int)cnt[2];
-  d->neM = (int)cnt[3];
-  d->neNM = (int)cnt[4];
-  d->rA = b->d_rng;
-  d->rAt = b->d_rng + 2 * b->cap;
-  d->rN = b->d_rng + 4 * b->cap;
-  d->rM = b->d_rng + 6 * b->cap;
-  d->rNM = b->d_rng + 8 * b->cap;
-  d->part = b->d_part;
-  d->stride = (int)need_stride;
-  d->cap = b->o.spmv_cap;
+  for (int g = 0; g < 2; ++g) {
+    dpdlp_batch_dev_t *d = g ? &b->bd_pi : &b->bd;
+    d->lps = b->d_lps;
+    d->bA = b->d_list + off[g][0];
+    d->bAt = b->d_list + off[g][1];
+    d->eN = b->d_list + off[g][2];
+    d->eM = b->d_list + off[g][3];
+    d->eNM = b->d_list + off[g][4];
+    d->nbA = (int)cnt[g][0];
+    d->nbAt = (int)cnt[g][1];
+    d->neN = (int)cnt[g][2];
+    d->neM = (int)cnt[g][3];
+    d->neNM = (int)cnt[g][4];
+    d->rA = b->d_rng;
+    d->rAt = b->d_rng + 2 * b->cap;
+    d->rN = b->d_rng + 4 * b->cap;
+    d->rM = b->d_rng + 6 * b->cap;
+    d->rNM = b->d_rng + 8 * b->cap;
+    d->part = b->d_part;
+    d->stride = (int)need_stride;
+    d->cap = b->o.spmv_cap;
+  }
   b->dirty = 0;
   b->epoch++;
   for (int k = 0; k < 8; ++k)
@@ -1005,7 +1020,9 @@ static int issue_rounds(dpdlp_batch_t *b, int nl, int any_restart, const int *an
       b->launches += 5;
     }
     if (any_pi[r]) {
-      if (dpdlp_bdev_power_step(&b->bd, b->s)) return -1;
+      b->bd_pi.ctl = b->bd.ctl;
+      b->bd_pi.nlp = nl;
+      if (dpdlp_bdev_power_step(&b->bd_pi, b->s)) return -1;
       b->launches += 5;
     }
   }
