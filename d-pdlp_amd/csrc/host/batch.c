@@ -52,6 +52,8 @@
 /* slot stages: FREE -> RESERVED (being set up, possibly on another thread)
  * -> READY (set up, waiting for a window start) -> PI -> PDHG -> FREE */
 enum { S_FREE = 0, S_RESERVED, S_READY, S_PI, S_PDHG };
+
+
 enum { NLIST = 5 };
 
 typedef struct {
@@ -159,6 +161,7 @@ void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o) {
   o->precision = DPDLP_PREC_FP64;
   o->mixed_switch_tol = 1e-4 * 0.1;
   o->spmv_cap = 1024;
+  o->solo_lps = 4;
 }
 
 static int gcd(int a, int b) {
@@ -980,11 +983,35 @@ static void process_pi(dpdlp_batch_t *b, int i) {
 /* the window's launches, round by round; ctl (host copy) non-NULL: time the
  * two phase kernels of one round with events */
 static int issue_rounds(dpdlp_batch_t *b, int nl, int any_restart, const int *any_phase,
-                        const int *any_fpe, const int *any_pi, const dpdlp_bctl_t *ctl) {
+                        const int *any_fpe, const int *any_pi, const dpdlp_bctl_t *ctl,
+                        const dpdlp_bctl_t *solo) {
   const int W = b->W;
   for (int r = 0; r < W; ++r) {
     b->bd.ctl = b->d_ctl_win + (size_t)r * nl;
     b->bd.nlp = nl;
+    if (solo) {
+      /* few LPs iterate: their own single-LP launches (kernel arguments, no
+       * work lists), the same arithmetic as the batched kernels */
+      for (int i = 0; i < nl; ++i) {
+        const dpdlp_bctl_t *c = &solo[(size_t)r * nl + i];
+        if (!c->phase) continue;
+        const dpdlp_state_t *st = &b->sl[i].c.st;
+        if (c->restart && dpdlp_dev_restart_copy(st, c->restart, b->s)) return -1;
+        if (dpdlp_dev_primal_phase(st, c->tau, c->w, c->rho, c->major, c->fpe, c->use32, b->s) ||
+            dpdlp_dev_dual_phase(st, c->sigma, c->w, c->rho, c->major, c->fpe, c->use32, b->s))
+          return -1;
+        if (c->fpe && dpdlp_dev_fpe_finish(st, c->fpe_to & 1, (c->fpe_to >> 1) & 1, b->s))
+          return -1;
+        b->launches += 2 + (c->restart ? 1 : 0) + (c->fpe ? 4 : 0);
+      }
+      if (any_pi[r]) {
+        b->bd_pi.ctl = b->bd.ctl;
+        b->bd_pi.nlp = nl;
+        if (dpdlp_bdev_power_step(&b->bd_pi, b->s)) return -1;
+        b->launches += 5;
+      }
+      continue;
+    }
     if (r == 0 && any_restart) {
       if (dpdlp_bdev_restart_copy(&b->bd, b->s)) return -1;
       b->launches += 1;
@@ -1088,11 +1115,16 @@ static int launch_window(dpdlp_batch_t *b) {
                       hipMemcpyHostToDevice, b->s));
   BCHK(hipEventRecord(b->ev_win[k], b->s));
   b->win_buf ^= 1;
-  int timed_window = b->npev_used < b->npev_created && (b->windows & 3) == 0;
+  int n_iter = 0;
+  for (int i = 0; i < nl; ++i) n_iter += stage_of(&b->sl[i]) == S_PDHG;
+  /* few iterating LPs: their single-LP launches (shorter dependency chain per
+   * workgroup, no work lists) */
+  const dpdlp_bctl_t *solo = n_iter <= b->o.solo_lps ? ctl : NULL;
+  int timed_window = !solo && b->npev_used < b->npev_created && (b->windows & 3) == 0;
   unsigned long long sig = (unsigned long long)any_restart;
   for (int r = 0; r < W; ++r)
     sig |= (unsigned long long)(any_phase[r] | (any_fpe[r] << 1) | (any_pi[r] << 2)) << (1 + 3 * r);
-  if (b->o.use_graph && !timed_window) {
+  if (b->o.use_graph && !timed_window && !solo) {
     int slot = -1, lru = 0;
     for (int g = 0; g < 8; ++g) {
       if (b->gcache[g].exec && b->gcache[g].epoch == b->epoch && b->gcache[g].nl == nl &&
@@ -1107,7 +1139,7 @@ static int launch_window(dpdlp_batch_t *b) {
        * shape replay it (their per-LP parameters live in d_ctl_win) */
       hipGraph_t graph = NULL;
       BCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-      int e = issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi, NULL);
+      int e = issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi, NULL, NULL);
       hipError_t ce = hipStreamEndCapture(b->s, &graph);
       if (e || ce != hipSuccess) {
         if (graph) hipGraphDestroy(graph);
@@ -1134,7 +1166,7 @@ static int launch_window(dpdlp_batch_t *b) {
     b->graph_replays++;
   } else {
     if (issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi,
-                     timed_window ? ctl : NULL))
+                     timed_window ? ctl : NULL, solo))
       return -1;
   }
   b->windows++;

@@ -358,7 +358,7 @@ class Resident:
 
 
 def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=1024, mixed_switch_tol=None,
-               profile_phases=0, use_graph=0):
+               profile_phases=0, use_graph=0, solo_lps=None):
     o = BatchOpts()
     lib.dpdlp_batch_default_opts(C.byref(o))
     o.max_lps = max_lps
@@ -366,6 +366,8 @@ def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=1024, mixed_switch_to
     o.spmv_cap = spmv_cap
     o.profile_phases = profile_phases
     o.use_graph = use_graph
+    if solo_lps is not None:
+        o.solo_lps = solo_lps
     if mixed_switch_tol is not None:
         o.mixed_switch_tol = mixed_switch_tol
     return o

@@ -111,6 +111,8 @@ typedef struct {
                               round 1 of every 4th window (dpdlp_batch_phase_stats) */
   int use_graph;           /* 1: capture each window's launches in a hipGraph
                               and replay it while the batch keeps its shape */
+  int solo_lps;            /* windows with at most this many iterating LPs use
+                              their single-LP launches (default 4; 0: never) */
   /* outputs of dpdlp_solve_batch */
   long long windows, syncs, launches, graph_replays;
   int phase_samples;

@@ -169,7 +169,8 @@ def test_window_graphs_replay_identically(dp):
     for prec in (dp.PREC_FP64, dp.PREC_MIXED):
         params = dp.default_params(presolve=False)
         st = {}
-        g = dp.solve_batch(res, params, stats=st, spmv_cap=CAP, precision=prec, use_graph=1)
+        g = dp.solve_batch(res, params, stats=st, spmv_cap=CAP, precision=prec, use_graph=1,
+                           solo_lps=0)
         d = dp.solve_batch(res, params, spmv_cap=CAP, precision=prec, use_graph=0)
         assert st["graph_replays"] > 0
         for k in range(len(res)):
@@ -181,7 +182,7 @@ def test_window_graphs_replay_identically(dp):
         b, _ = r.solve(params)
         assert same(a, b)
     # a batch stepped by hand with eager host work between replays
-    b = dp.Batch(params, max_lps=4, spmv_cap=CAP, use_graph=1)
+    b = dp.Batch(params, max_lps=4, spmv_cap=CAP, use_graph=1, solo_lps=0)
     for k, r in enumerate(res):
         b.add(r, k)
     got = {}
@@ -195,3 +196,17 @@ def test_window_graphs_replay_identically(dp):
         assert same(got[k], ref[k]), k
     assert b.counters()["graph_replays"] > 0
     b.free()
+
+
+def test_tail_single_lp_launches_identical(dp):
+    """Windows that issue single-LP launches for the last few LPs give the
+    same bits as batched launches."""
+    lps = uneven_lps(dp, SIZES[:5], seed0=3800)
+    res = residents(dp, lps)
+    params = dp.default_params(presolve=False)
+    for prec in (dp.PREC_FP64, dp.PREC_MIXED):
+        a = dp.solve_batch(res, params, spmv_cap=CAP, precision=prec, solo_lps=0)
+        b = dp.solve_batch(res, params, spmv_cap=CAP, precision=prec, solo_lps=64)
+        c = dp.solve_batch(res, params, spmv_cap=CAP, precision=prec, solo_lps=2)
+        for k in range(len(res)):
+            assert same(a[k], b[k]) and same(a[k], c[k]), (prec, k)
