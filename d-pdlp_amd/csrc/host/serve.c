@@ -47,6 +47,7 @@ void dpdlp_serve_default_spec(dpdlp_serve_spec_t *s) {
   s->precision = DPDLP_PREC_MIXED;
   s->mode = 0;
   s->spmv_cap = 1024;
+  s->large_lane_nnz = 4000000;
 }
 
 /* SplitMix64 stream for arrival gaps and request draws */
@@ -101,7 +102,60 @@ typedef struct {
   double peak_bytes;
   int max_in_flight;
   double *t_admit;
+  /* large-LP lane: requests of at least large_lane_nnz nonzeros are solved
+   * one at a time by the single-LP path (XCD column panels, tuned block
+   * size) on its own thread and stream, beside the batch */
+  int *lq;
+  int lq_head, lq_tail, lane_on;
+  const pdhg_parameters_t *prm;
+  double *t_done;
+  cupdlpx_result_t **res;
+  int completed;
 } srv_t;
+
+static void *lane_main(void *arg) {
+  srv_t *S = (srv_t *)arg;
+  const dpdlp_serve_spec_t *sp = S->spec;
+  for (;;) {
+    pthread_mutex_lock(&S->mu);
+    while (!S->stop && S->lq_head == S->lq_tail && !S->loader_done)
+      pthread_cond_wait(&S->cv, &S->mu);
+    if (S->stop || (S->lq_head == S->lq_tail && S->loader_done)) {
+      pthread_mutex_unlock(&S->mu);
+      break;
+    }
+    int k = S->lq[S->lq_head++];
+    pthread_mutex_unlock(&S->mu);
+    pdhg_parameters_t prm = *S->prm;
+    double left = S->t_cap - dpdlp_wall();
+    if (left < prm.termination_criteria.time_sec_limit)
+      prm.termination_criteria.time_sec_limit = left > 0 ? left : 0.0;
+    dpdlp_solve_opts_t o;
+    dpdlp_default_opts(&o);
+    o.quiet = 1;
+    o.precision = sp->precision;
+    dpdlp_resident_t *R = dpdlp_upload(S->pool[S->req_pool[k]], 0);
+    cupdlpx_result_t *r = R ? dpdlp_solve_resident(R, &prm, &o) : NULL;
+    if (R) dpdlp_resident_free(R);
+    double now = dpdlp_wall();
+    pthread_mutex_lock(&S->mu);
+    if (!r) {
+      S->loader_error = 1;
+      S->stop = 1;
+    } else if (r->termination_reason == TERMINATION_REASON_TIME_LIMIT && now >= S->t_cap) {
+      cupdlpx_result_free(r); /* cut by the end of the run: unfinished */
+    } else {
+      S->t_done[k] = now;
+      S->res[k] = r;
+      S->completed++;
+    }
+    S->in_use -= S->bytes[k];
+    S->admitted_live--;
+    pthread_cond_broadcast(&S->cv);
+    pthread_mutex_unlock(&S->mu);
+  }
+  return NULL;
+}
 
 static void *loader_main(void *arg) {
   srv_t *S = (srv_t *)arg;
@@ -124,6 +178,14 @@ static void *loader_main(void *arg) {
     S->admitted_live++;
     if (S->in_use > S->peak_bytes) S->peak_bytes = S->in_use;
     if (S->admitted_live > S->max_in_flight) S->max_in_flight = S->admitted_live;
+    const lp_problem_t *P = S->pool[S->req_pool[k]];
+    if (S->lane_on && P->constraint_matrix_num_nonzeros >= sp->large_lane_nnz) {
+      S->lq[S->lq_tail++] = k;
+      S->t_admit[k] = dpdlp_wall();
+      pthread_cond_broadcast(&S->cv);
+      pthread_mutex_unlock(&S->mu);
+      continue;
+    }
     pthread_mutex_unlock(&S->mu);
     int slot = dpdlp_i_batch_reserve(S->b);
     dpdlp_resident_t *R = slot >= 0 ? dpdlp_upload_ex(S->pool[S->req_pool[k]], sp->spmv_cap, 1, 1)
@@ -141,6 +203,7 @@ static void *loader_main(void *arg) {
   }
   pthread_mutex_lock(&S->mu);
   S->loader_done = 1;
+  pthread_cond_broadcast(&S->cv);
   pthread_mutex_unlock(&S->mu);
   return NULL;
 }
@@ -189,8 +252,8 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
   dpdlp_batch_t *b = NULL;
   srv_t S;
   memset(&S, 0, sizeof(S));
-  pthread_t th;
-  int loader_started = 0;
+  pthread_t th, th_lane;
+  int loader_started = 0, lane_started = 0;
   if (!pool || !pool_bytes) goto out;
   for (int k = 0; k < sp.pool_small; ++k) {
     dpdlp_gen_spec_t g;
@@ -293,8 +356,21 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
     pthread_mutex_init(&S.mu, NULL);
     pthread_cond_init(&S.cv, NULL);
     if (hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking) != hipSuccess) goto out;
+    S.prm = &prm;
+    S.t_done = t_done;
+    S.res = res;
+    S.lane_on = sp.large_lane_nnz > 0;
+    S.lq = (int *)malloc((size_t)(nreq ? nreq : 1) * sizeof(int));
+    if (!S.lq) goto out;
     if (pthread_create(&th, NULL, loader_main, &S) != 0) goto out;
     loader_started = 1;
+    if (S.lane_on) {
+      if (pthread_create(&th_lane, NULL, lane_main, &S) != 0) {
+        S.lane_on = 0;
+        goto out;
+      }
+      lane_started = 1;
+    }
     for (;;) {
       int inflight = dpdlp_batch_in_flight(b);
       if (inflight > 0) {
@@ -303,8 +379,6 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
           S.stop = 1;
           pthread_cond_broadcast(&S.cv);
           pthread_mutex_unlock(&S.mu);
-          pthread_join(th, NULL);
-          loader_started = 0;
           goto out;
         }
       } else {
@@ -315,19 +389,21 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
       cupdlpx_result_t *r;
       int freed = 0;
       double freed_bytes = 0.0;
+      int got = 0;
       while (dpdlp_batch_take(b, &tag, &r, NULL)) {
         double now = dpdlp_wall();
         if (tag >= 0 && tag < nreq) {
           t_done[tag] = now;
           res[tag] = r;
           freed_bytes += bytes[tag];
-          completed++;
+          got++;
         } else {
           cupdlpx_result_free(r);
         }
         freed++;
       }
       pthread_mutex_lock(&S.mu);
+      S.completed += got;
       if (freed) {
         S.in_use -= freed_bytes;
         S.admitted_live -= freed;
@@ -338,11 +414,7 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
       if (dpdlp_wall() > t_cap) S.stop = 1;
       int stop = S.stop;
       pthread_mutex_unlock(&S.mu);
-      if (err) {
-        pthread_join(th, NULL);
-        loader_started = 0;
-        goto out;
-      }
+      if (err) goto out;
       if (done || (stop && S.loader_done)) break;
       if (stop) {
         pthread_mutex_lock(&S.mu);
@@ -352,6 +424,11 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
     }
     pthread_join(th, NULL);
     loader_started = 0;
+    if (lane_started) {
+      pthread_join(th_lane, NULL);
+      lane_started = 0;
+    }
+    completed = S.completed;
     rep->max_in_flight = S.max_in_flight;
     rep->peak_gb = S.peak_bytes / 1e9;
     dpdlp_batch_counters(b, &rep->windows, &rep->syncs, NULL);
@@ -403,13 +480,15 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
   }
   rc = 0;
 out:
-  if (loader_started) {
+  if (loader_started || lane_started) {
     pthread_mutex_lock(&S.mu);
     S.stop = 1;
     pthread_cond_broadcast(&S.cv);
     pthread_mutex_unlock(&S.mu);
-    pthread_join(th, NULL);
+    if (loader_started) pthread_join(th, NULL);
+    if (lane_started) pthread_join(th_lane, NULL);
   }
+  free(S.lq);
   if (b) dpdlp_batch_free(b);
   if (S.stream) hipStreamDestroy(S.stream);
   if (sp.mode != 1 && S.b) {

@@ -179,6 +179,9 @@ typedef struct {
   int precision;        /* batch precision (DPDLP_PREC_MIXED default) */
   int mode;             /* 0 continuous batching, 1 plain loop (one at a time, FP64) */
   int spmv_cap;         /* row-block capacity (1024) */
+  long long large_lane_nnz; /* requests with at least this many nonzeros are
+                               solved by the single-LP path on a second
+                               thread and stream, beside the batch (0: off) */
 } dpdlp_serve_spec_t;
 
 typedef struct {

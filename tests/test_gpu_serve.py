@@ -72,3 +72,17 @@ def test_drain_cap_reports_unfinished(dp):
     assert rep["arrived"] > 5
     assert rep["completed"] + rep["unfinished"] == rep["arrived"]
     assert np.isinf(lat).sum() == rep["unfinished"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_large_lane_results_equal_alone(dp, precision):
+    """Requests above the large-LP threshold go to the single-LP lane beside
+    the batch; every result still equals the LP solved alone."""
+    params = dp.default_params(presolve=False)
+    ref = alone_results(dp, params, precision)
+    rep, lat, idx, res = dp.serve(params, want_results=True, rate=300.0, duration_sec=0.2,
+                                  drain_sec=60.0, seed=12, max_lps=6, precision=precision,
+                                  large_lane_nnz=9000, **POOL)
+    assert rep["completed"] == rep["arrived"] > 20
+    for k in range(rep["arrived"]):
+        assert same(res[k], ref[idx[k]]), (k, idx[k])
