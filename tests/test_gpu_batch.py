@@ -210,3 +210,40 @@ def test_tail_single_lp_launches_identical(dp):
         c = dp.solve_batch(res, params, spmv_cap=CAP, precision=prec, solo_lps=2)
         for k in range(len(res)):
             assert same(a[k], b[k]) and same(a[k], c[k]), (prec, k)
+
+
+def degenerate_lps(dp):
+    """Edge shapes: no constraints, empty rows and columns, a free variable
+    with a zero objective, fixed variables."""
+    out = []
+    # m = 0: min c'x over the box
+    out.append(dp.Problem.from_csr(0, 3, [0], [], [], c=np.array([1.0, -1.0, 0.0]),
+                                   var_lb=np.zeros(3), var_ub=np.array([2.0, 3.0, 1.0])))
+    # empty row 1 and empty column 2 (x2 free with c = 0)
+    out.append(dp.Problem.from_csr(3, 3, [0, 2, 2, 3], [0, 1, 0], [1.0, 1.0, 1.0],
+                                   c=np.array([1.0, 2.0, 0.0]),
+                                   con_lb=np.array([1.0, -np.inf, 0.5]),
+                                   con_ub=np.array([np.inf, np.inf, 4.0]),
+                                   var_lb=np.array([0.0, 0.0, -np.inf]),
+                                   var_ub=np.array([np.inf, np.inf, np.inf])))
+    # fixed variables (l = u) and an equality row
+    out.append(dp.Problem.from_dense(np.array([[1.0, 1.0, 1.0]]), c=np.array([1.0, 2.0, 3.0]),
+                                     con_lb=np.array([3.0]), con_ub=np.array([3.0]),
+                                     var_lb=np.array([1.0, 0.0, 0.0]),
+                                     var_ub=np.array([1.0, 5.0, 5.0])))
+    return out
+
+
+def test_degenerate_shapes_batch_equals_single_and_oracle(dp, orc):
+    probs = degenerate_lps(dp)
+    params = dp.default_params(presolve=False, termination_criteria__iteration_limit=20000)
+    res = [dp.Resident(p, spmv_cap=CAP, panels=(1, 1)) for p in probs]
+    mix = dp.solve_batch(res, params, spmv_cap=CAP)
+    for k, (p, r) in enumerate(zip(probs, res)):
+        single, _ = r.solve(params)
+        assert same(mix[k], single), k
+        ref, _ = orc.solve(p.ptr, orc.default_params(presolve=False,
+                                                     termination_criteria__iteration_limit=20000))
+        assert mix[k].status == ref["status"], (k, mix[k].status, ref["status"])
+        assert abs(mix[k].primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
+            1.0, abs(ref["primal_objective_value"])), k
