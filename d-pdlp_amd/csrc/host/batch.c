@@ -120,10 +120,14 @@ struct dpdlp_batch {
   dpdlp_batch_dev_t bd, bd_pi; /* lists of the iterating / power-iteration LPs */
   bdone_t *done;
   int ndone, done_head, done_cap;
-  bpend_t *pend;
-  int npend;
-  double *h_stage; /* pinned staging of result vectors */
-  size_t stage_cap, stage_off;
+  /* two pinned staging sets: retirements of this window start go to set
+   * scur; the other set (copies complete at the last synchronisation) is
+   * completed on the host while the next window runs */
+  bpend_t *pend[2];
+  int npend[2];
+  double *h_stage[2];
+  size_t stage_cap[2], stage_off[2];
+  int scur;
   barena_t *pool; /* recycled LP-state arenas (guarded by mu) */
   int npool;
   size_t pool_bytes;
@@ -184,17 +188,19 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   }
   for (int i = b->done_head; i < b->ndone; ++i) cupdlpx_result_free(b->done[i].res);
   free(b->done);
-  for (int i = 0; i < b->npend; ++i) {
-    cupdlpx_result_free(b->pend[i].res);
-    if (b->pend[i].arena) hipFree(b->pend[i].arena);
-    if (b->pend[i].own_R) dpdlp_resident_free(b->pend[i].R);
+  for (int g = 0; g < 2; ++g) {
+    for (int i = 0; i < b->npend[g]; ++i) {
+      cupdlpx_result_free(b->pend[g][i].res);
+      if (b->pend[g][i].arena) hipFree(b->pend[g][i].arena);
+      if (b->pend[g][i].own_R) dpdlp_resident_free(b->pend[g][i].R);
+    }
+    free(b->pend[g]);
+    if (b->h_stage[g]) hipHostFree(b->h_stage[g]);
   }
-  free(b->pend);
   for (int i = 0; i < b->npool; ++i) hipFree(b->pool[i].p);
   free(b->pool);
   for (int i = 0; i < b->ndead; ++i) dpdlp_resident_free(b->dead[i]);
   free(b->dead);
-  if (b->h_stage) hipHostFree(b->h_stage);
   free(b->sl);
   void *dev[] = {b->d_lps, b->d_ctl_eval, b->d_ctl_win, b->d_scal,
                  b->d_list, b->d_rng,     b->d_part,    b->d_normals};
@@ -255,11 +261,12 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
   b->sl = (bslot_t *)calloc((size_t)b->cap, sizeof(bslot_t));
   b->done_cap = 64;
   b->done = (bdone_t *)malloc((size_t)b->done_cap * sizeof(bdone_t));
-  b->pend = (bpend_t *)calloc((size_t)b->cap, sizeof(bpend_t));
+  b->pend[0] = (bpend_t *)calloc((size_t)b->cap, sizeof(bpend_t));
+  b->pend[1] = (bpend_t *)calloc((size_t)b->cap, sizeof(bpend_t));
   b->pool = (barena_t *)calloc((size_t)b->cap, sizeof(barena_t));
   b->dead_cap = 64;
   b->dead = (dpdlp_resident_t **)calloc((size_t)b->dead_cap, sizeof(*b->dead));
-  if (!b->sl || !b->done || !b->pend || !b->pool || !b->dead) {
+  if (!b->sl || !b->done || !b->pend[0] || !b->pend[1] || !b->pool || !b->dead) {
     dpdlp_batch_free(b);
     return NULL;
   }
@@ -786,7 +793,6 @@ static int eval_and_read(dpdlp_batch_t *b, int any_eval, int any_inf) {
   BCHK(hipStreamSynchronize(b->s));
   b->syncs++;
   if (b->npev_used) harvest_phase_events(b);
-  if (b->npend && flush_pending(b)) return -1;
   return 0;
 }
 
@@ -808,12 +814,12 @@ static int push_done(dpdlp_batch_t *b, long long tag, cupdlpx_result_t *res, dou
 
 /* completes the retired LPs whose result copies were issued before the last
  * synchronisation: results become available, arenas go back to the pool */
-static int flush_pending(dpdlp_batch_t *b) {
+static int flush_set(dpdlp_batch_t *b, int g) {
   double now = dpdlp_wall();
-  for (int k = 0; k < b->npend; ++k) {
-    bpend_t *e = &b->pend[k];
+  for (int k = 0; k < b->npend[g]; ++k) {
+    bpend_t *e = &b->pend[g][k];
     cupdlpx_result_t *r = e->res;
-    const double *st = b->h_stage + e->off;
+    const double *st = b->h_stage[g] + e->off;
     if (e->n > 0) {
       memcpy(r->primal_solution, st, (size_t)e->n * sizeof(double));
       memcpy(r->reduced_cost, st + e->n, (size_t)e->n * sizeof(double));
@@ -846,8 +852,14 @@ static int flush_pending(dpdlp_batch_t *b) {
     pthread_mutex_unlock(&b->mu);
     if (free_now) dpdlp_resident_free(free_now);
   }
-  b->npend = 0;
-  b->stage_off = 0;
+  b->npend[g] = 0;
+  b->stage_off[g] = 0;
+  return 0;
+}
+
+/* completes every retired LP (only after a synchronisation) */
+static int flush_pending(dpdlp_batch_t *b) {
+  if (flush_set(b, 0) || flush_set(b, 1)) return -1;
   return 0;
 }
 
@@ -868,19 +880,20 @@ static int finish(dpdlp_batch_t *b, int i) {
   }
   int n = q->c.st.n, m = q->c.st.m;
   size_t need = 2 * (size_t)n + (size_t)m;
-  if (b->stage_off + need > b->stage_cap) {
+  const int g = b->scur;
+  if (b->stage_off[g] + need > b->stage_cap[g]) {
     /* staging full: complete what is in flight first */
     BCHK(hipStreamSynchronize(b->s));
     b->syncs++;
     if (flush_pending(b)) return -1;
-    if (need > b->stage_cap) {
-      size_t nc = need > 2 * b->stage_cap ? need : 2 * b->stage_cap;
+    if (need > b->stage_cap[g]) {
+      size_t nc = need > 2 * b->stage_cap[g] ? need : 2 * b->stage_cap[g];
       if (nc < (1u << 20)) nc = 1u << 20;
-      if (b->h_stage) hipHostFree(b->h_stage);
-      b->h_stage = NULL;
-      b->stage_cap = 0;
-      BCHK(hipHostMalloc((void **)&b->h_stage, nc * sizeof(double), 0));
-      b->stage_cap = nc;
+      if (b->h_stage[g]) hipHostFree(b->h_stage[g]);
+      b->h_stage[g] = NULL;
+      b->stage_cap[g] = 0;
+      BCHK(hipHostMalloc((void **)&b->h_stage[g], nc * sizeof(double), 0));
+      b->stage_cap[g] = nc;
     }
   }
   cupdlpx_result_t *res = (cupdlpx_result_t *)calloc(1, sizeof(*res));
@@ -893,15 +906,15 @@ static int finish(dpdlp_batch_t *b, int i) {
     return -1;
   }
   dpdlp_i_result_scalars(res, &q->c, h, q->rescale_time, polish_time, polish_iters);
-  double *stg = b->h_stage + b->stage_off;
+  double *stg = b->h_stage[g] + b->stage_off[g];
   if (dpdlp_i_result_launch(&q->c, stg, stg + 2 * (size_t)n, stg + n)) {
     cupdlpx_result_free(res);
     return -1;
   }
-  bpend_t *e = &b->pend[b->npend++];
+  bpend_t *e = &b->pend[g][b->npend[g]++];
   e->tag = q->tag;
   e->res = res;
-  e->off = b->stage_off;
+  e->off = b->stage_off[g];
   e->n = n;
   e->m = m;
   e->t_add = q->t_add;
@@ -909,7 +922,7 @@ static int finish(dpdlp_batch_t *b, int i) {
   e->arena_cap = q->c.arena_cap;
   e->R = q->R;
   e->own_R = q->own_R;
-  b->stage_off += need;
+  b->stage_off[g] += need;
   /* the arena moves to the pending entry; the slot's other allocations (if
    * polishing made any) are freed now; no stream synchronisation here */
   q->c.arena = NULL;
@@ -1191,6 +1204,11 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
   }
   if (any_eval || any_read) {
     if (eval_and_read(b, any_eval, any_inf)) return -1;
+    /* every result copy issued so far is complete: the set that collected the
+     * last window start's retirements is completed after this window is
+     * launched; this window start's retirements go to the other set */
+    b->scur ^= 1;
+    if (b->npend[b->scur] && flush_set(b, b->scur)) return -1;
     int any_fresh = 0;
     for (int i = 0; i < b->nhw; ++i) {
       bslot_t *q = &b->sl[i];
@@ -1230,7 +1248,9 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
   if (b->nhw > 0) {
     if (b->dirty && rebuild_lists(b)) return -1;
     if (launch_window(b)) return -1;
-  } else if (b->npend) {
+    /* host copies of completed results overlap the window on the device */
+    if (b->npend[b->scur ^ 1] && flush_set(b, b->scur ^ 1)) return -1;
+  } else if (b->npend[0] || b->npend[1]) {
     BCHK(hipStreamSynchronize(b->s));
     b->syncs++;
     if (flush_pending(b)) return -1;
