@@ -101,6 +101,12 @@ struct dpdlp_batch {
   dpdlp_batch_opts_t o;
   int W, freq, rbw;
   hipStream_t s;
+  /* power-iteration steps run on their own stream, beside the PDHG phases
+   * (disjoint LPs, disjoint partial-sum rows); ev_ctl orders them after the
+   * window's control upload, ev_pi joins them before the next window start */
+  hipStream_t s_pi;
+  hipEvent_t ev_ctl, ev_pi;
+  int pi_pending;
   int cap;
   bslot_t *sl;
   int nhw, in_flight, dirty;
@@ -180,6 +186,7 @@ static int gcd(int a, int b) {
 void dpdlp_batch_free(dpdlp_batch_t *b) {
   if (!b) return;
   if (b->s) hipStreamSynchronize(b->s);
+  if (b->s_pi) hipStreamSynchronize(b->s_pi);
   for (int i = 0; i < b->cap && b->sl; ++i) {
     bslot_t *q = &b->sl[i];
     if (stage_of(q) == S_FREE) continue;
@@ -217,6 +224,9 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   for (int k = 0; k < 8; ++k)
     if (b->gcache[k].exec) hipGraphExecDestroy(b->gcache[k].exec);
   if (b->s) hipStreamDestroy(b->s);
+  if (b->s_pi) hipStreamDestroy(b->s_pi);
+  if (b->ev_ctl) hipEventDestroy(b->ev_ctl);
+  if (b->ev_pi) hipEventDestroy(b->ev_pi);
   pthread_mutex_destroy(&b->mu);
   free(b);
 }
@@ -272,6 +282,9 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
   }
   size_t C = (size_t)b->cap;
   CCHK(hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking));
+  CCHK(hipStreamCreateWithFlags(&b->s_pi, hipStreamNonBlocking));
+  CCHK(hipEventCreateWithFlags(&b->ev_ctl, hipEventDisableTiming));
+  CCHK(hipEventCreateWithFlags(&b->ev_pi, hipEventDisableTiming));
   CCHK(hipMalloc((void **)&b->d_lps, C * sizeof(dpdlp_blp_t)));
   CCHK(hipHostMalloc((void **)&b->h_lps, C * sizeof(dpdlp_blp_t), 0));
   CCHK(hipMalloc((void **)&b->d_ctl_eval, C * sizeof(dpdlp_bctl_t)));
@@ -593,7 +606,8 @@ static int rebuild_lists(dpdlp_batch_t *b) {
     dpdlp_set_error("batch too large (%lld work items in one list)", need_stride);
     return -1;
   }
-  /* pinned buffers and the part array are rewritten: drain the stream */
+  /* pinned buffers and the part array are rewritten: drain the streams */
+  BCHK(hipStreamSynchronize(b->s_pi));
   BCHK(hipStreamSynchronize(b->s));
   b->syncs++;
   if (total > b->list_cap) {
@@ -764,7 +778,16 @@ static void phase_bytes(const dpdlp_resident_t *R, const dpdlp_bctl_t *c, double
  * synchronisation */
 static int flush_pending(dpdlp_batch_t *b);
 
+/* the batch stream waits for the last window's power-iteration steps */
+static int join_pi(dpdlp_batch_t *b) {
+  if (!b->pi_pending) return 0;
+  BCHK(hipStreamWaitEvent(b->s, b->ev_pi, 0));
+  b->pi_pending = 0;
+  return 0;
+}
+
 static int eval_and_read(dpdlp_batch_t *b, int any_eval, int any_inf) {
+  if (join_pi(b)) return -1;
   if (b->dirty && rebuild_lists(b)) return -1;
   if (any_eval) {
     memset(b->h_ctl_eval, 0, (size_t)b->nhw * sizeof(dpdlp_bctl_t));
@@ -997,8 +1020,9 @@ static void process_pi(dpdlp_batch_t *b, int i) {
  * two phase kernels of one round with events */
 static int issue_rounds(dpdlp_batch_t *b, int nl, int any_restart, const int *any_phase,
                         const int *any_fpe, const int *any_pi, const dpdlp_bctl_t *ctl,
-                        const dpdlp_bctl_t *solo) {
+                        const dpdlp_bctl_t *solo, hipStream_t spi) {
   const int W = b->W;
+  int pi_started = 0;
   for (int r = 0; r < W; ++r) {
     b->bd.ctl = b->d_ctl_win + (size_t)r * nl;
     b->bd.nlp = nl;
@@ -1018,9 +1042,11 @@ static int issue_rounds(dpdlp_batch_t *b, int nl, int any_restart, const int *an
         b->launches += 2 + (c->restart ? 1 : 0) + (c->fpe ? 4 : 0);
       }
       if (any_pi[r]) {
+        if (spi != b->s && !pi_started) BCHK(hipStreamWaitEvent(spi, b->ev_ctl, 0));
+        pi_started = 1;
         b->bd_pi.ctl = b->bd.ctl;
         b->bd_pi.nlp = nl;
-        if (dpdlp_bdev_power_step(&b->bd_pi, b->s)) return -1;
+        if (dpdlp_bdev_power_step(&b->bd_pi, spi)) return -1;
         b->launches += 5;
       }
       continue;
@@ -1060,11 +1086,17 @@ static int issue_rounds(dpdlp_batch_t *b, int nl, int any_restart, const int *an
       b->launches += 5;
     }
     if (any_pi[r]) {
+      if (spi != b->s && !pi_started) BCHK(hipStreamWaitEvent(spi, b->ev_ctl, 0));
+      pi_started = 1;
       b->bd_pi.ctl = b->bd.ctl;
       b->bd_pi.nlp = nl;
-      if (dpdlp_bdev_power_step(&b->bd_pi, b->s)) return -1;
+      if (dpdlp_bdev_power_step(&b->bd_pi, spi)) return -1;
       b->launches += 5;
     }
+  }
+  if (pi_started && spi != b->s) {
+    BCHK(hipEventRecord(b->ev_pi, spi));
+    b->pi_pending = 1;
   }
   return 0;
 }
@@ -1124,9 +1156,11 @@ static int launch_window(dpdlp_batch_t *b) {
       q->pi_issued = kk;
     }
   }
+  if (join_pi(b)) return -1;
   BCHK(hipMemcpyAsync(b->d_ctl_win, ctl, (size_t)W * nl * sizeof(dpdlp_bctl_t),
                       hipMemcpyHostToDevice, b->s));
   BCHK(hipEventRecord(b->ev_win[k], b->s));
+  BCHK(hipEventRecord(b->ev_ctl, b->s));
   b->win_buf ^= 1;
   int n_iter = 0;
   for (int i = 0; i < nl; ++i) n_iter += stage_of(&b->sl[i]) == S_PDHG;
@@ -1152,7 +1186,7 @@ static int launch_window(dpdlp_batch_t *b) {
        * shape replay it (their per-LP parameters live in d_ctl_win) */
       hipGraph_t graph = NULL;
       BCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-      int e = issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi, NULL, NULL);
+      int e = issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi, NULL, NULL, b->s);
       hipError_t ce = hipStreamEndCapture(b->s, &graph);
       if (e || ce != hipSuccess) {
         if (graph) hipGraphDestroy(graph);
@@ -1179,7 +1213,7 @@ static int launch_window(dpdlp_batch_t *b) {
     b->graph_replays++;
   } else {
     if (issue_rounds(b, nl, any_restart, any_phase, any_fpe, any_pi,
-                     timed_window ? ctl : NULL, solo))
+                     timed_window ? ctl : NULL, solo, b->s_pi))
       return -1;
   }
   b->windows++;
