@@ -51,7 +51,10 @@
 
 /* slot stages: FREE -> RESERVED (being set up, possibly on another thread)
  * -> READY (set up, waiting for a window start) -> PI -> PDHG -> FREE */
-enum { S_FREE = 0, S_RESERVED, S_READY, S_PI, S_PDHG };
+enum { S_FREE = 0, S_RESERVED, S_READY, S_PI, S_PDHG, S_ZOMBIE };
+/* S_ZOMBIE: retired, but the work lists still name the slot (its items
+ * return at once: the slot's control records are zero); it becomes FREE at
+ * the next list rebuild */
 
 
 enum { NLIST = 5 };
@@ -110,6 +113,9 @@ struct dpdlp_batch {
   int cap;
   bslot_t *sl;
   int nhw, in_flight, dirty;
+  long long list_items, zombie_items;
+  long long last_activation;
+  int reserve_failed;
   dpdlp_blp_t *d_lps, *h_lps;
   dpdlp_bctl_t *d_ctl_eval, *h_ctl_eval;
   dpdlp_bctl_t *d_ctl_win, *h_ctl_win[2];
@@ -489,7 +495,10 @@ int dpdlp_i_batch_reserve(dpdlp_batch_t *b) {
   pthread_mutex_lock(&b->mu);
   int i = 0;
   while (i < b->cap && stage_of(&b->sl[i]) != S_FREE) ++i;
-  if (i < b->cap) set_stage(&b->sl[i], S_RESERVED);
+  if (i < b->cap)
+    set_stage(&b->sl[i], S_RESERVED);
+  else
+    b->reserve_failed = 1; /* retired slots are reclaimed at the next rebuild */
   pthread_mutex_unlock(&b->mu);
   return i < b->cap ? i : -1;
 }
@@ -538,6 +547,17 @@ int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take
 /* READY slots join (main thread, at a window start): power iteration start
  * (utils.cu:80-135) or, without nonzeros, the loop directly */
 static int activate_ready(dpdlp_batch_t *b) {
+  /* joins are gathered (each one rebuilds the lists): at least 16 waiting,
+   * 4 windows since the last join, or nothing else to run */
+  int ready = 0, busy = 0;
+  for (int i = 0; i < b->cap; ++i) {
+    int st = stage_of(&b->sl[i]);
+    ready += st == S_READY;
+    busy += st == S_PI || st == S_PDHG;
+  }
+  if (ready == 0) return 0;
+  if (busy > 0 && ready < 16 && b->windows - b->last_activation < 4) return 0;
+  b->last_activation = b->windows;
   for (int i = 0; i < b->cap; ++i) {
     bslot_t *q = &b->sl[i];
     if (stage_of(q) != S_READY) continue;
@@ -565,12 +585,19 @@ static int activate_ready(dpdlp_batch_t *b) {
   return 0;
 }
 
+static int rebuild_lists(dpdlp_batch_t *b);
+
 int dpdlp_batch_add(dpdlp_batch_t *b, dpdlp_resident_t *R, int take_ownership, long long tag) {
   if (!b || !R) {
     dpdlp_set_error("NULL batch or problem");
     return -1;
   }
   int i = dpdlp_i_batch_reserve(b);
+  if (i < 0 && b->zombie_items > 0) {
+    /* reclaim retired slots (this is the stepping thread) */
+    if (rebuild_lists(b)) return -1;
+    i = dpdlp_i_batch_reserve(b);
+  }
   if (i < 0) {
     dpdlp_set_error("batch full (%d LPs)", b->cap);
     return -1;
@@ -581,6 +608,13 @@ int dpdlp_batch_add(dpdlp_batch_t *b, dpdlp_resident_t *R, int take_ownership, l
 /* ---- block lists ---- */
 
 static int rebuild_lists(dpdlp_batch_t *b) {
+  pthread_mutex_lock(&b->mu);
+  for (int i = 0; i < b->nhw; ++i)
+    if (stage_of(&b->sl[i]) == S_ZOMBIE) set_stage(&b->sl[i], S_FREE);
+  b->reserve_failed = 0;
+  pthread_mutex_unlock(&b->mu);
+  while (b->nhw > 0 && stage_of(&b->sl[b->nhw - 1]) == S_FREE) b->nhw--;
+  b->zombie_items = 0;
   /* every list is split into the LPs iterating (group 0) and the LPs still in
    * power iteration (group 1), so each launch only covers the LPs it serves */
   long long cnt[2][NLIST];
@@ -602,6 +636,7 @@ static int rebuild_lists(dpdlp_batch_t *b) {
       total += cnt[g][k];
       if (cnt[g][k] > need_stride) need_stride = cnt[g][k];
     }
+  b->list_items = total;
   if (need_stride > 0x7fffffffLL) {
     dpdlp_set_error("batch too large (%lld work items in one list)", need_stride);
     return -1;
@@ -948,16 +983,22 @@ static int finish(dpdlp_batch_t *b, int i) {
   b->stage_off[g] += need;
   /* the arena moves to the pending entry; the slot's other allocations (if
    * polishing made any) are freed now; no stream synchronisation here */
+  {
+    const dpdlp_resident_t *R = q->R;
+    b->zombie_items += R->A_nblk + R->At_nblk + dpdlp_ew_grid(R->n) + dpdlp_ew_grid(R->m) +
+                       dpdlp_ew_grid((long long)R->n + R->m);
+  }
   q->c.arena = NULL;
   q->c.s = NULL;
   dpdlp_i_ctx_free(&q->c);
   memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
   memset(&b->h_lps[i], 0, sizeof(dpdlp_blp_t));
-  b->dirty = 1;
   pthread_mutex_lock(&b->mu);
-  set_stage(q, S_FREE);
+  set_stage(q, S_ZOMBIE);
   pthread_mutex_unlock(&b->mu);
-  while (b->nhw > 0 && !active(&b->sl[b->nhw - 1])) b->nhw--;
+  /* the lists drop the slot at the next rebuild: when a quarter of their
+   * items are retired ones, when slots run out, or when LPs join */
+  if (b->zombie_items * 4 > b->list_items) b->dirty = 1;
   return 0;
 }
 
@@ -1220,10 +1261,27 @@ static int launch_window(dpdlp_batch_t *b) {
   return 0;
 }
 
+static int n_active(const dpdlp_batch_t *b) {
+  int k = 0;
+  for (int i = 0; i < b->nhw; ++i) k += active(&b->sl[i]);
+  return k;
+}
+
 int dpdlp_batch_step(dpdlp_batch_t *b) {
   if (!b) return -1;
   if (activate_ready(b)) return -1;
-  if (b->nhw == 0) return dpdlp_batch_in_flight(b);
+  pthread_mutex_lock(&b->mu);
+  if (b->reserve_failed && b->zombie_items > 0) b->dirty = 1;
+  pthread_mutex_unlock(&b->mu);
+  if (n_active(b) == 0) {
+    if (b->npend[0] || b->npend[1]) {
+      BCHK(hipStreamSynchronize(b->s));
+      b->syncs++;
+      if (flush_pending(b)) return -1;
+    }
+    if (b->dirty && b->zombie_items > 0 && rebuild_lists(b)) return -1;
+    return dpdlp_batch_in_flight(b);
+  }
   /* evaluation points and power-iteration reads at this window start */
   int any_eval = 0, any_inf = 0, any_read = 0;
   for (int i = 0; i < b->nhw; ++i) {
@@ -1279,7 +1337,7 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
       }
     }
   }
-  if (b->nhw > 0) {
+  if (n_active(b) > 0) {
     if (b->dirty && rebuild_lists(b)) return -1;
     if (launch_window(b)) return -1;
     /* host copies of completed results overlap the window on the device */

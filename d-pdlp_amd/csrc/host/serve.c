@@ -187,11 +187,20 @@ static void *loader_main(void *arg) {
       continue;
     }
     pthread_mutex_unlock(&S->mu);
-    int slot = dpdlp_i_batch_reserve(S->b);
-    dpdlp_resident_t *R = slot >= 0 ? dpdlp_upload_ex(S->pool[S->req_pool[k]], sp->spmv_cap, 1, 1)
-                                    : NULL;
+    /* a slot of a just-retired LP is reclaimed at the batch's next window */
+    int slot;
+    while ((slot = dpdlp_i_batch_reserve(S->b)) < 0) {
+      struct timespec ts = {0, 100000};
+      nanosleep(&ts, NULL);
+      pthread_mutex_lock(&S->mu);
+      int stop = S->stop;
+      pthread_mutex_unlock(&S->mu);
+      if (stop) break;
+    }
+    if (slot < 0) break;
+    dpdlp_resident_t *R = dpdlp_upload_ex(S->pool[S->req_pool[k]], sp->spmv_cap, 1, 1);
     if (!R || dpdlp_i_batch_prepare(S->b, slot, R, 1, k, S->stream)) {
-      if (slot >= 0 && !R) dpdlp_i_batch_unreserve(S->b, slot);
+      if (!R) dpdlp_i_batch_unreserve(S->b, slot);
       if (R) dpdlp_resident_free(R);
       pthread_mutex_lock(&S->mu);
       S->loader_error = 1;
@@ -382,6 +391,15 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
           goto out;
         }
       } else {
+        /* nothing iterating: still step, so retired slots are reclaimed for
+         * the loader */
+        if (dpdlp_batch_step(b) < 0) {
+          pthread_mutex_lock(&S.mu);
+          S.stop = 1;
+          pthread_cond_broadcast(&S.cv);
+          pthread_mutex_unlock(&S.mu);
+          goto out;
+        }
         struct timespec ts = {0, 200000};
         nanosleep(&ts, NULL);
       }
