@@ -36,107 +36,222 @@
 #include <string.h>
 #include <zlib.h>
 
-/* ---- string -> index map (open addressing, FNV-1a) ---- */
+/* ---- string -> index map (open addressing, FNV-1a).  A 32-byte slot holds
+ * the hash, the index and the key itself when it is shorter than 16 bytes
+ * (MPS names usually are), so a lookup normally touches one cache line. ---- */
 typedef struct {
-  char **keys;
-  int *vals;
+  uint64_t h;
+  int v;   /* -1: empty */
+  int len;
+  union {
+    char in[16];
+    char *out;
+  } k;
+} sslot_t;
+
+typedef struct {
+  sslot_t *slot;
   size_t cap, size;
 } smap_t;
 
-static uint64_t fnv1a(const char *s) {
+static uint64_t fnv1a_len(const char *s, int *len) {
   uint64_t h = 1469598103934665603ull;
-  for (; *s; ++s) {
-    h ^= (unsigned char)*s;
+  const char *p = s;
+  for (; *p; ++p) {
+    h ^= (unsigned char)*p;
     h *= 1099511628211ull;
   }
+  *len = (int)(p - s);
   return h;
+}
+
+static sslot_t *slots_new(size_t cap) {
+  sslot_t *s = (sslot_t *)malloc(cap * sizeof(sslot_t));
+  if (s)
+    for (size_t i = 0; i < cap; ++i) s[i].v = -1;
+  return s;
 }
 
 static int smap_init(smap_t *m) {
   m->cap = 1024;
   m->size = 0;
-  m->keys = (char **)calloc(m->cap, sizeof(char *));
-  m->vals = (int *)calloc(m->cap, sizeof(int));
-  return (m->keys && m->vals) ? 0 : -1;
+  m->slot = slots_new(m->cap);
+  return m->slot ? 0 : -1;
 }
 
 static void smap_free(smap_t *m) {
-  if (m->keys)
-    for (size_t i = 0; i < m->cap; ++i) free(m->keys[i]);
-  free(m->keys);
-  free(m->vals);
+  if (m->slot)
+    for (size_t i = 0; i < m->cap; ++i)
+      if (m->slot[i].v >= 0 && m->slot[i].len >= 16) free(m->slot[i].k.out);
+  free(m->slot);
   memset(m, 0, sizeof(*m));
 }
 
-static int smap_get(const smap_t *m, const char *k) {
-  size_t i = (size_t)(fnv1a(k) & (m->cap - 1));
-  while (m->keys[i]) {
-    if (strcmp(m->keys[i], k) == 0) return m->vals[i];
+static inline int slot_is(const sslot_t *e, const char *k, int len, uint64_t h) {
+  if (e->h != h || e->len != len) return 0;
+  return len < 16 ? memcmp(e->k.in, k, (size_t)len) == 0 : strcmp(e->k.out, k) == 0;
+}
+
+static int smap_get_h(const smap_t *m, const char *k, int len, uint64_t h) {
+  size_t i = (size_t)(h & (m->cap - 1));
+  while (m->slot[i].v >= 0) {
+    if (slot_is(&m->slot[i], k, len, h)) return m->slot[i].v;
     i = (i + 1) & (m->cap - 1);
   }
   return -1;
 }
 
+static int smap_get(const smap_t *m, const char *k) {
+  int len;
+  uint64_t h = fnv1a_len(k, &len);
+  return smap_get_h(m, k, len, h);
+}
+
 static int smap_grow(smap_t *m) {
-  smap_t n;
-  n.cap = m->cap * 2;
-  n.size = m->size;
-  n.keys = (char **)calloc(n.cap, sizeof(char *));
-  n.vals = (int *)calloc(n.cap, sizeof(int));
-  if (!n.keys || !n.vals) {
-    free(n.keys);
-    free(n.vals);
-    return -1;
-  }
+  size_t nc = m->cap * 2;
+  sslot_t *ns = slots_new(nc);
+  if (!ns) return -1;
   for (size_t i = 0; i < m->cap; ++i) {
-    if (!m->keys[i]) continue;
-    size_t j = (size_t)(fnv1a(m->keys[i]) & (n.cap - 1));
-    while (n.keys[j]) j = (j + 1) & (n.cap - 1);
-    n.keys[j] = m->keys[i];
-    n.vals[j] = m->vals[i];
+    if (m->slot[i].v < 0) continue;
+    size_t j = (size_t)(m->slot[i].h & (nc - 1));
+    while (ns[j].v >= 0) j = (j + 1) & (nc - 1);
+    ns[j] = m->slot[i];
   }
-  free(m->keys);
-  free(m->vals);
-  *m = n;
+  free(m->slot);
+  m->slot = ns;
+  m->cap = nc;
   return 0;
 }
 
 /* returns the existing index, or inserts with index = size */
 static int smap_put(smap_t *m, const char *k) {
-  int v = smap_get(m, k);
+  int len;
+  uint64_t h = fnv1a_len(k, &len);
+  int v = smap_get_h(m, k, len, h);
   if (v >= 0) return v;
-  if ((m->size + 1) * 4 > m->cap * 3 && smap_grow(m)) return -1;
-  size_t i = (size_t)(fnv1a(k) & (m->cap - 1));
-  while (m->keys[i]) i = (i + 1) & (m->cap - 1);
-  m->keys[i] = strdup(k);
-  if (!m->keys[i]) return -1;
-  m->vals[i] = (int)m->size++;
-  return m->vals[i];
+  if ((m->size + 1) * 2 > m->cap && smap_grow(m)) return -1;
+  size_t i = (size_t)(h & (m->cap - 1));
+  while (m->slot[i].v >= 0) i = (i + 1) & (m->cap - 1);
+  sslot_t *e = &m->slot[i];
+  if (len < 16) {
+    memcpy(e->k.in, k, (size_t)len);
+  } else {
+    e->k.out = strdup(k);
+    if (!e->k.out) return -1;
+  }
+  e->h = h;
+  e->len = len;
+  e->v = (int)m->size++;
+  return e->v;
 }
 
-/* ---- line source: plain stdio or zlib ---- */
+/* Decimal to double, as atof: numbers with at most 15 significant digits
+ * and a decimal exponent within 10^±22 are converted exactly with one
+ * correctly rounded multiply or divide (both operands are exact doubles,
+ * Clinger's fast path); everything else goes to strtod. */
+static double mps_atof(const char *s) {
+  static const double p10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,
+                                 1e8,  1e9,  1e10, 1e11, 1e12, 1e13, 1e14, 1e15,
+                                 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+  const char *p = s;
+  int neg = 0;
+  if (*p == '+' || *p == '-') neg = (*p++ == '-');
+  uint64_t mant = 0;
+  int digits = 0, scale = 0, any = 0;
+  while (*p >= '0' && *p <= '9') {
+    any = 1;
+    if (mant || *p != '0') {
+      if (++digits > 15) return atof(s);
+      mant = mant * 10 + (uint64_t)(*p - '0');
+    }
+    ++p;
+  }
+  if (*p == '.') {
+    ++p;
+    while (*p >= '0' && *p <= '9') {
+      any = 1;
+      if (mant || *p != '0') {
+        if (++digits > 15) return atof(s);
+        mant = mant * 10 + (uint64_t)(*p - '0');
+      }
+      --scale;
+      ++p;
+    }
+  }
+  if (!any) return atof(s);
+  if (*p == 'e' || *p == 'E') {
+    ++p;
+    int eneg = 0, e = 0, edig = 0;
+    if (*p == '+' || *p == '-') eneg = (*p++ == '-');
+    while (*p >= '0' && *p <= '9') {
+      if (e < 10000) e = e * 10 + (*p - '0');
+      ++p;
+      ++edig;
+    }
+    if (!edig) return atof(s);
+    scale += eneg ? -e : e;
+  }
+  if (*p != '\0') return atof(s);
+  double v = (double)mant; /* exact: at most 15 digits */
+  if (mant == 0) return neg ? -0.0 : 0.0;
+  if (scale > 22 || scale < -22) return atof(s);
+  v = scale >= 0 ? v * p10[scale] : v / p10[-scale];
+  return neg ? -v : v;
+}
+
+/* ---- line source: plain stdio or zlib, read in 4 MiB chunks ---- */
 typedef struct {
   gzFile gz;
   FILE *f;
   char *buf;
-  size_t cap;
+  size_t cap, beg, end; /* unread bytes are buf[beg, end) */
+  int eof;
 } src_t;
 
+static size_t src_fill(src_t *s, char *dst, size_t n) {
+  if (s->gz) {
+    int r = gzread(s->gz, dst, (unsigned)(n > (1u << 30) ? (1u << 30) : n));
+    return r > 0 ? (size_t)r : 0;
+  }
+  return fread(dst, 1, n, s->f);
+}
+
+/* next line, NUL-terminated in place without its newline; NULL at the end */
 static char *src_line(src_t *s) {
-  size_t len = 0;
   for (;;) {
-    if (len + 2 >= s->cap) {
-      size_t nc = s->cap ? s->cap * 2 : 8192;
+    char *nl = s->beg < s->end ? (char *)memchr(s->buf + s->beg, '\n', s->end - s->beg) : NULL;
+    if (nl) {
+      char *line = s->buf + s->beg;
+      *nl = '\0';
+      s->beg = (size_t)(nl - s->buf) + 1;
+      return line;
+    }
+    if (s->eof) {
+      if (s->beg < s->end) { /* last line without a newline */
+        char *line = s->buf + s->beg;
+        s->buf[s->end] = '\0';
+        s->beg = s->end;
+        return line;
+      }
+      return NULL;
+    }
+    /* keep the partial line, make room, read more */
+    size_t rest = s->end - s->beg;
+    if (s->beg > 0) {
+      memmove(s->buf, s->buf + s->beg, rest);
+      s->beg = 0;
+      s->end = rest;
+    }
+    if (s->cap - s->end < (1u << 20) + 1) {
+      size_t nc = s->cap ? s->cap * 2 : (4u << 20);
       char *nb = (char *)realloc(s->buf, nc);
       if (!nb) return NULL;
       s->buf = nb;
       s->cap = nc;
     }
-    char *r = s->gz ? gzgets(s->gz, s->buf + len, (int)(s->cap - len))
-                    : fgets(s->buf + len, (int)(s->cap - len), s->f);
-    if (!r) return len ? s->buf : NULL;
-    len += strlen(s->buf + len);
-    if (len > 0 && s->buf[len - 1] == '\n') return s->buf;
+    size_t got = src_fill(s, s->buf + s->end, s->cap - s->end - 1);
+    if (got == 0) s->eof = 1;
+    s->end += got;
   }
 }
 
@@ -155,6 +270,7 @@ typedef struct {
   size_t nz, nz_cap;
   char *obj_row;
   char *cur_col;
+  int cur_idx; /* index of cur_col (-1: not looked up yet) */
   double obj_const;
   bool maximize;
   int err;
@@ -269,9 +385,12 @@ static int columns_line(mps_t *P, char **tk, int nt) {
   if (strcmp(tk[1], "'MARKER'") == 0) return 0;
   int first;
   if (nt % 2 != 0) {
-    free(P->cur_col);
-    P->cur_col = strdup(tk[0]);
-    if (!P->cur_col) return -1;
+    if (!P->cur_col || strcmp(P->cur_col, tk[0]) != 0) {
+      free(P->cur_col);
+      P->cur_col = strdup(tk[0]);
+      if (!P->cur_col) return -1;
+      P->cur_idx = -1;
+    }
     first = 1;
   } else {
     if (!P->cur_col) {
@@ -281,10 +400,13 @@ static int columns_line(mps_t *P, char **tk, int nt) {
     first = 0;
   }
   if (ensure_cols(P)) return -1;
-  int col = smap_put(&P->cols, P->cur_col);
-  if (col < 0) return -1;
+  if (P->cur_idx < 0) {
+    P->cur_idx = smap_put(&P->cols, P->cur_col);
+    if (P->cur_idx < 0) return -1;
+  }
+  int col = P->cur_idx;
   for (int i = first; i + 1 < nt; i += 2) {
-    double v = atof(tk[i + 1]);
+    double v = mps_atof(tk[i + 1]);
     if (P->obj_row && strcmp(tk[i], P->obj_row) == 0) {
       P->obj[col] += v;
     } else {
@@ -298,7 +420,7 @@ static int columns_line(mps_t *P, char **tk, int nt) {
 /* mps_parser.c:652-678 */
 static void rhs_line(mps_t *P, char **tk, int nt) {
   for (int i = 1; i + 1 < nt; i += 2) {
-    double v = atof(tk[i + 1]);
+    double v = mps_atof(tk[i + 1]);
     if (P->obj_row && strcmp(tk[i], P->obj_row) == 0) {
       P->obj_const = -v;
       continue;
@@ -405,7 +527,7 @@ lp_problem_t *read_mps_file(const char *filename) {
   bool rows_done = false;
   char *line;
   while (!P.err && (line = src_line(&src)) != NULL) {
-    if (line[0] == '*' || line[0] == '\n' || line[0] == '\r') continue;
+    if (line[0] == '*' || line[0] == '\0' || line[0] == '\r') continue;
     char *tk[6];
     int nt = 0;
     char *save = NULL;

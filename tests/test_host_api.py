@@ -293,3 +293,27 @@ def test_extension_struct_layouts_match_c(dp, tmp_path):
         py = mirrors[cname]
         want = C.sizeof(py) if field == "size" else getattr(py, field).offset
         assert int(val) == want, (cname, field, int(val), want)
+
+
+def test_mps_number_formats_parse_exactly(dp, tmp_path):
+    """COLUMNS / RHS values in many textual forms read back as the correctly
+    rounded doubles (what strtod gives)."""
+    rng = np.random.default_rng(7)
+    forms = []
+    for k in range(400):
+        x = rng.standard_normal() * 10.0 ** rng.integers(-30, 30)
+        f = ["%.3g", "%.6g", "%.12g", "%.15g", "%.17g", "%e", "%.10E", "%.1f", "%+.8g"][k % 9]
+        forms.append(f % x)
+    forms += ["1.", ".5", "-0.0", "+.25e-2", "1e22", "1e23", "123456789012345",
+              "1234567890123456789", "0.000001", "5E-324", "1e-400", "1.5e+300"]
+    lines = ["NAME fmt", "ROWS", " N obj", " L r0", "COLUMNS"]
+    for j, t in enumerate(forms):
+        lines.append(f" x{j} r0 {t}")
+    lines += ["RHS", " rhs r0 1.0", "ENDATA"]
+    path = tmp_path / "fmt.mps"
+    path.write_text("\n".join(lines) + "\n")
+    p = dp.Problem.from_mps(str(path))
+    vals = p.arrays()["vals"]
+    want = np.array([float(t) for t in forms])
+    assert vals.shape == want.shape
+    assert np.array_equal(vals.view(np.int64), want.view(np.int64))
