@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime_api.h>
 #include <stdlib.h>
+#include <string.h>
 
 
 void dpdlp_resident_info(const dpdlp_resident_t *r, int *m, int *n, int *nnz, int *nblk_A,
@@ -28,7 +29,7 @@ double dpdlp_time_spmv(dpdlp_resident_t *r, int transpose, int reps, int use32) 
   dpdlp_resident_csr(r, transpose, &A, xpart);
   hipStream_t s = NULL;
   hipEvent_t e0 = NULL, e1 = NULL;
-  double *x = NULL, *y = NULL;
+  double *x = NULL, *y = NULL, *xp = NULL;
   float *x32 = NULL, *v32 = NULL, *pv32 = NULL;
   double ms = -1.0;
   if (hipStreamCreate(&s) != hipSuccess) return -1.0;
@@ -61,6 +62,7 @@ double dpdlp_time_spmv(dpdlp_resident_t *r, int transpose, int reps, int use32) 
     if (hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms = (double)t / reps;
   }
 out:
+  free(xp);
   if (s) hipStreamSynchronize(s);
   if (x) hipFree(x);
   if (y) hipFree(y);
@@ -82,14 +84,21 @@ int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, do
   long long xl = dpdlp_resident_xpart_len(r);
   if (xl > 0 && hipMalloc((void **)&xpart, (size_t)xl * sizeof(double)) != hipSuccess) return -1;
   dpdlp_resident_csr(r, transpose, &A, xpart);
-  double *x = NULL, *y = NULL;
+  double *x = NULL, *y = NULL, *xp = NULL;
   float *x32 = NULL, *v32 = NULL, *pv32 = NULL;
   int rc = -1;
   hipStream_t s = NULL;
   if (hipStreamCreate(&s) != hipSuccess) return -1;
   if (hipMalloc((void **)&x, (size_t)(A.cols ? A.cols : 1) * 8) != hipSuccess) goto out;
   if (hipMalloc((void **)&y, (size_t)(A.rows ? A.rows : 1) * 8) != hipSuccess) goto out;
-  if (A.cols && hipMemcpy(x, xh, (size_t)A.cols * 8, hipMemcpyHostToDevice) != hipSuccess)
+  if (transpose && r->h_row_perm && A.cols) {
+    /* A' x takes x in the caller's row order: renumber it */
+    xp = (double *)malloc((size_t)A.cols * sizeof(double));
+    if (!xp) goto out;
+    for (int i = 0; i < A.cols; ++i) xp[i] = xh[r->h_row_perm[i]];
+  }
+  if (A.cols &&
+      hipMemcpy(x, xp ? xp : xh, (size_t)A.cols * 8, hipMemcpyHostToDevice) != hipSuccess)
     goto out;
   if (use32) {
     if (hipMalloc((void **)&x32, (size_t)(A.cols ? A.cols : 1) * 4) != hipSuccess) goto out;
@@ -109,6 +118,14 @@ int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, do
   if (hipStreamSynchronize(s) != hipSuccess) goto out;
   if (A.rows && hipMemcpy(yh, y, (size_t)A.rows * 8, hipMemcpyDeviceToHost) != hipSuccess)
     goto out;
+  if (!transpose && r->h_row_perm && A.rows) {
+    /* A x comes out in the renumbered row order: back to the caller's */
+    double *t = (double *)malloc((size_t)A.rows * sizeof(double));
+    if (!t) goto out;
+    for (int i = 0; i < A.rows; ++i) t[r->h_row_perm[i]] = yh[i];
+    memcpy(yh, t, (size_t)A.rows * sizeof(double));
+    free(t);
+  }
   rc = 0;
 out:
   if (s) hipStreamSynchronize(s);

@@ -26,6 +26,9 @@ struct dpdlp_resident {
   double c0, obj_l2, obj_linf, bnd_l2, bnd_linf;
   double *h_pstart, *h_dstart;
   int *h_A_blk, *h_At_blk; /* host copies of the row-block plans */
+  int *h_row_perm;         /* rows renumbered: perm[new] = old (NULL: identity) */
+  int *h_perm_rp, *h_perm_ci; /* upload-time copies (freed after upload) */
+  double *h_perm_v, *h_perm_lc, *h_perm_uc;
   double upload_time;
 };
 

@@ -26,6 +26,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+static void free_perm_copies(dpdlp_resident_t *r);
+
 #define HIPCHK(call)                                                           \
   do {                                                                         \
     hipError_t _e = (hipError_t)(call);                                        \
@@ -165,6 +167,8 @@ void dpdlp_resident_free(dpdlp_resident_t *r) {
   free(r->h_dstart);
   free(r->h_A_blk);
   free(r->h_At_blk);
+  free(r->h_row_perm);
+  free_perm_copies(r);
   free(r);
 }
 
@@ -185,10 +189,11 @@ static hipError_t up(void **dst, const void *src, size_t bytes) {
   return hipSuccess;
 }
 
-static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt);
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt,
+                                     int order);
 
 dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
-  return upload_with(p, spmv_cap, 0, 0);
+  return upload_with(p, spmv_cap, 0, 0, -1);
 }
 
 dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panels_A,
@@ -198,7 +203,9 @@ dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panel
     dpdlp_set_error("panel counts must be 0 (automatic), 1, 2, 4 or 8");
     return NULL;
   }
-  return upload_with(p, spmv_cap, panels_A, panels_At);
+  /* explicit layouts (the batch's plain one) keep the row order */
+  return upload_with(p, spmv_cap, panels_A, panels_At,
+                     (panels_A > 0 || panels_At > 0) ? 0 : -1);
 }
 
 size_t dpdlp_resident_bytes(const dpdlp_resident_t *r) {
@@ -224,7 +231,7 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
   double bestA = 1e30, bestT = 1e30;
   int PA = 1, PT = 1;
   for (int i = 0; i < 4; ++i) {
-    dpdlp_resident_t *r = upload_with(p, spmv_cap, cand[i], cand[i]);
+    dpdlp_resident_t *r = upload_with(p, spmv_cap, cand[i], cand[i], -1);
     if (!r) continue;
     double ta, tt;
     time_both(r, &ta, &tt);
@@ -242,10 +249,56 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
     chosen[0] = PA;
     chosen[1] = PT;
   }
-  return upload_with(p, spmv_cap, PA, PT);
+  return upload_with(p, spmv_cap, PA, PT, -1);
 }
 
-static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt) {
+/* row lengths skewed enough for the row renumbering to pay: the longest row
+ * holds at least 16 times the mean (DPDLP_ROW_ORDER=0/1 overrides) */
+static int rows_skewed(int m, const int *rp) {
+  const char *env = getenv("DPDLP_ROW_ORDER");
+  if (env) return atoi(env) != 0;
+  if (m < 4096) return 0;
+  long long mx = 0;
+  for (int i = 0; i < m; ++i)
+    if (rp[i + 1] - rp[i] > mx) mx = rp[i + 1] - rp[i];
+  return mx * (long long)m >= 16LL * rp[m];
+}
+
+/* perm[new] = old: rows by decreasing length, ties in row order (a stable
+ * counting sort) */
+static int row_order_by_length(int m, const int *rp, int *perm) {
+  int mx = 0;
+  for (int i = 0; i < m; ++i)
+    if (rp[i + 1] - rp[i] > mx) mx = rp[i + 1] - rp[i];
+  int *cnt = (int *)calloc((size_t)mx + 2, sizeof(int));
+  if (!cnt) {
+    dpdlp_set_error("host allocation failed");
+    return -1;
+  }
+  for (int i = 0; i < m; ++i) cnt[mx - (rp[i + 1] - rp[i])]++;
+  int run = 0;
+  for (int L = 0; L <= mx; ++L) {
+    int c = cnt[L];
+    cnt[L] = run;
+    run += c;
+  }
+  for (int i = 0; i < m; ++i) perm[cnt[mx - (rp[i + 1] - rp[i])]++] = i;
+  free(cnt);
+  return 0;
+}
+
+static void free_perm_copies(dpdlp_resident_t *r) {
+  free(r->h_perm_rp);
+  free(r->h_perm_ci);
+  free(r->h_perm_v);
+  free(r->h_perm_lc);
+  free(r->h_perm_uc);
+  r->h_perm_rp = r->h_perm_ci = NULL;
+  r->h_perm_v = r->h_perm_lc = r->h_perm_uc = NULL;
+}
+
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt,
+                                     int order) {
   double t0 = dpdlp_wall();
   if (!p || p->num_variables < 0 || p->num_constraints < 0 ||
       p->constraint_matrix_num_nonzeros < 0) {
@@ -323,6 +376,54 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     memcpy(r->h_dstart, p->dual_start, (size_t)m * sizeof(double));
   }
 
+  /* Row order: with strongly skewed row lengths (automatic layouts only),
+   * rows are renumbered by decreasing length, so the dual entries that A'
+   * gathers most often share cache lines; vectors indexed by rows are
+   * permuted at upload and the dual result back at the end. */
+  const int *rp_u = rp, *ci_u = ci;
+  const double *v_u = p->constraint_matrix_values, *lc_u = p->constraint_lower_bound,
+               *uc_u = p->constraint_upper_bound;
+  if (order > 0 || (order < 0 && rows_skewed(m, rp))) {
+    int *perm = (int *)malloc((size_t)(m ? m : 1) * sizeof(int));
+    int *rp2 = (int *)malloc((size_t)(m + 1) * sizeof(int));
+    int *ci2 = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+    double *v2 = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+    double *lc2 = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
+    double *uc2 = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
+    r->h_row_perm = perm;
+    r->h_perm_rp = rp2;
+    r->h_perm_ci = ci2;
+    r->h_perm_v = v2;
+    r->h_perm_lc = lc2;
+    r->h_perm_uc = uc2;
+    if (!perm || !rp2 || !ci2 || !v2 || !lc2 || !uc2) {
+      dpdlp_set_error("host allocation failed");
+      goto ufail;
+    }
+    if (row_order_by_length(m, rp, perm)) goto ufail;
+    rp2[0] = 0;
+    for (int i = 0; i < m; ++i) {
+      int o = perm[i], a = rp[o], L = rp[o + 1] - a;
+      memcpy(ci2 + rp2[i], ci + a, (size_t)L * sizeof(int));
+      memcpy(v2 + rp2[i], p->constraint_matrix_values + a, (size_t)L * sizeof(double));
+      rp2[i + 1] = rp2[i] + L;
+      lc2[i] = p->constraint_lower_bound[o];
+      uc2[i] = p->constraint_upper_bound[o];
+    }
+    if (r->h_dstart) {
+      double *d2 = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
+      if (!d2) goto ufail;
+      for (int i = 0; i < m; ++i) d2[i] = r->h_dstart[perm[i]];
+      free(r->h_dstart);
+      r->h_dstart = d2;
+    }
+    rp_u = rp2;
+    ci_u = ci2;
+    v_u = v2;
+    lc_u = lc2;
+    uc_u = uc2;
+  }
+
   t_rp = (int *)malloc((size_t)(n + 1) * sizeof(int));
   t_ci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
   t_v = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
@@ -331,12 +432,12 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     dpdlp_set_error("host allocation failed");
     goto ufail;
   }
-  if (dpdlp_host_transpose(m, n, rp, ci, p->constraint_matrix_values, t_rp, t_ci, t_v))
+  if (dpdlp_host_transpose(m, n, rp_u, ci_u, v_u, t_rp, t_ci, t_v))
     goto ufail;
 
-  UPCHK(up((void **)&r->A_rp, rp, (size_t)(m + 1) * sizeof(int)));
-  UPCHK(up((void **)&r->A_ci, ci, (size_t)nnz * sizeof(int)));
-  UPCHK(up((void **)&r->A_v, p->constraint_matrix_values, (size_t)nnz * sizeof(double)));
+  UPCHK(up((void **)&r->A_rp, rp_u, (size_t)(m + 1) * sizeof(int)));
+  UPCHK(up((void **)&r->A_ci, ci_u, (size_t)nnz * sizeof(int)));
+  UPCHK(up((void **)&r->A_v, v_u, (size_t)nnz * sizeof(double)));
   UPCHK(up((void **)&r->At_rp, t_rp, (size_t)(n + 1) * sizeof(int)));
   UPCHK(up((void **)&r->At_ci, t_ci, (size_t)nnz * sizeof(int)));
   UPCHK(up((void **)&r->At_v, t_v, (size_t)nnz * sizeof(double)));
@@ -353,9 +454,9 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
       }
     }
   }
-  r->A_nblk = r->A_P > 1 ? dpdlp_plan_blocks_panels(m, n, rp, ci, r->cap, r->A_P,
+  r->A_nblk = r->A_P > 1 ? dpdlp_plan_blocks_panels(m, n, rp_u, ci_u, r->cap, r->A_P,
                                                     DPDLP_PANEL_RMAX, blk)
-                         : dpdlp_plan_blocks(m, rp, r->cap, blk);
+                         : dpdlp_plan_blocks(m, rp_u, r->cap, blk);
   UPCHK(up((void **)&r->A_blk, blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
   r->h_A_blk = (int *)malloc((size_t)(r->A_nblk + 1) * sizeof(int));
   if (!r->h_A_blk) goto ufail;
@@ -372,8 +473,8 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     int P = which ? r->At_P : r->A_P;
     if (P <= 1) continue;
     int rows = which ? n : m, cols = which ? m : n, nb = which ? r->At_nblk : r->A_nblk;
-    const int *hrp = which ? t_rp : rp, *hci = which ? t_ci : ci;
-    const double *hv = which ? t_v : p->constraint_matrix_values;
+    const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
+    const double *hv = which ? t_v : v_u;
     int *blkh = (int *)malloc((size_t)(nb + 1) * sizeof(int));
     int *toff = (int *)malloc(((size_t)nb * P + 1) * sizeof(int));
     unsigned short *rofs =
@@ -416,12 +517,13 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
   UPCHK(up((void **)&r->c, p->objective_vector, (size_t)n * sizeof(double)));
   UPCHK(up((void **)&r->lv, p->variable_lower_bound, (size_t)n * sizeof(double)));
   UPCHK(up((void **)&r->uv, p->variable_upper_bound, (size_t)n * sizeof(double)));
-  UPCHK(up((void **)&r->lc, p->constraint_lower_bound, (size_t)m * sizeof(double)));
-  UPCHK(up((void **)&r->uc, p->constraint_upper_bound, (size_t)m * sizeof(double)));
+  UPCHK(up((void **)&r->lc, lc_u, (size_t)m * sizeof(double)));
+  UPCHK(up((void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
   free(t_rp);
   free(t_ci);
   free(t_v);
   free(blk);
+  free_perm_copies(r);
   r->upload_time = dpdlp_wall() - t0;
   return r;
 ufail:
@@ -697,6 +799,7 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
   /* ---- state (pdlp_core_op.cu:42-391) ---- */
   st->m = m;
   st->n = n;
+  c->row_perm = R->h_row_perm;
   c->w_A_v = dcopy(c, R->A_v, nnz);
   c->w_At_v = dcopy(c, R->At_v, nnz);
   {
@@ -916,6 +1019,13 @@ cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale
   if (dpdlp_i_result_launch(c, res->primal_solution, res->dual_solution, res->reduced_cost))
     goto fail;
   HIPCHK(hipStreamSynchronize(c->s));
+  if (c->row_perm && m > 0) { /* dual back to the caller's row order */
+    double *t = (double *)malloc((size_t)m * sizeof(double));
+    if (!t) goto fail;
+    for (int i = 0; i < m; ++i) t[c->row_perm[i]] = res->dual_solution[i];
+    memcpy(res->dual_solution, t, (size_t)m * sizeof(double));
+    free(t);
+  }
   dpdlp_i_result_scalars(res, c, h, rescale_time, polish_time, polish_iters);
   return res;
 fail:
@@ -996,6 +1106,13 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     h_q = (double *)malloc((size_t)m * sizeof(double));
     if (!h_q) goto fail;
     dpdlp_normal_fill(1u, m, h_q);
+    if (R->h_row_perm) { /* draw i belongs to original row i */
+      double *t = (double *)malloc((size_t)m * sizeof(double));
+      if (!t) goto fail;
+      for (int i = 0; i < m; ++i) t[i] = h_q[R->h_row_perm[i]];
+      free(h_q);
+      h_q = t;
+    }
     HIPCHK(hipMemcpy(pi_eig, h_q, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
     double sigma_sq = 1.0;
     int done = 0, batch = 8;
@@ -1037,6 +1154,14 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
         HIPCHK(hipMemcpyAsync(opts->trace_y + (size_t)trace_k * m, st->y_cur,
                               (size_t)m * sizeof(double), hipMemcpyDeviceToHost, c->s));
       HIPCHK(hipStreamSynchronize(c->s));
+      if (R->h_row_perm && m > 0) {
+        double *ty = opts->trace_y + (size_t)trace_k * m;
+        double *t = (double *)malloc((size_t)m * sizeof(double));
+        if (!t) goto fail;
+        for (int i = 0; i < m; ++i) t[R->h_row_perm[i]] = ty[i];
+        memcpy(ty, t, (size_t)m * sizeof(double));
+        free(t);
+      }
       trace_k++;
       opts->trace_recorded = trace_k;
     }

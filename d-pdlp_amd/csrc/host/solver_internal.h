@@ -24,6 +24,7 @@ typedef struct {
   char *arena; /* one device allocation for the LP state (zeroed) */
   size_t arena_off, arena_len, arena_cap;
   double *pi_eig, *pi_next, *pi_dual, *rs, *cs;
+  const int *row_perm; /* the resident problem's row renumbering or NULL */
   double *w_A_v, *w_At_v;
   float *w_A_v32, *w_At_v32;
   /* live phase timing */

@@ -120,3 +120,40 @@ def test_mixed_precision_reaches_fp64_accuracy(dp, orc, panels, monkeypatch):
     assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-5 * max(
         1.0, abs(ref["primal_objective_value"]))
     assert rel(res.x, ref["x"]) < 1e-3
+
+
+
+def test_skewed_rows_renumbered_parity(dp, orc):
+    """Zipf row lengths trigger the row renumbering of the upload (rows by
+    decreasing length): iterates (in the caller's order), duals, status and
+    objective still match the oracle, and so does the standalone SpMV."""
+    from helpers import csr_matvec
+    prob, opt = dp.generate(6000, 12000, 120000, row_rule=1, pm1_frac=0.1, seed=41)
+    a = prob.arrays()
+    lens = np.diff(a["row_ptr"])
+    assert lens.max() * len(lens) >= 16 * a["row_ptr"][-1]  # skewed enough to renumber
+    iters = [0, 10, 100, 300, 600]
+    params = dp.default_params(presolve=False)
+    res, o, (tx, ty) = dp.solve(prob, params, trace_iters=iters)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(presolve=False),
+                                 trace_iters=iters, n=prob.n, m=prob.m)
+    assert res.status == ref["status"] == "OPTIMAL"
+    assert min(len(tx), len(ox)) == len(iters)
+    for i in range(len(iters)):
+        assert rel(tx[i], ox[i]) < 1e-8 and rel(ty[i], oy[i]) < 1e-8, iters[i]
+    assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
+        1.0, abs(ref["primal_objective_value"]))
+    if res.iterations == ref["total_count"]:
+        assert rel(res.y, ref["y"]) < 1e-6 and rel(res.x, ref["x"]) < 1e-6
+    # the standalone SpMV of a renumbered resident, in the caller's order
+    R = dp.Resident(prob)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(prob.n)
+    y = rng.standard_normal(prob.m)
+    ax = csr_matvec(a["row_ptr"], a["col_ind"], a["vals"], x)
+    assert np.allclose(R.spmv(x), ax, rtol=1e-12, atol=1e-12)
+    rows = np.repeat(np.arange(prob.m), lens)
+    aty = np.zeros(prob.n)
+    np.add.at(aty, a["col_ind"], a["vals"] * y[rows])
+    assert np.allclose(R.spmv(y, transpose=True), aty, rtol=1e-12, atol=1e-12)
+    R.free()
