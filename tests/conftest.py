@@ -23,3 +23,12 @@ def dp():
 def orc():
     import oracle_py
     return oracle_py
+
+
+@pytest.fixture(params=["auto", "renumbered"])
+def row_order(request, monkeypatch):
+    """Runs a single-LP test with the upload's automatic row order and with
+    rows renumbered by decreasing length (forced for any shape)."""
+    if request.param == "renumbered":
+        monkeypatch.setenv("DPDLP_ROW_ORDER", "1")
+    return request.param

@@ -49,7 +49,7 @@ def test_capi_example(dp):
 
 
 @pytest.mark.parametrize("seed", [0, 1])
-def test_parity_trace_vs_oracle(dp, orc, seed):
+def test_parity_trace_vs_oracle(dp, orc, seed, row_order):
     """Iterates within 1e-8 relative of the oracle over the first 1000
     iterations, same status, objective within 1e-6 relative."""
     prob, opt = dp.generate(2000, 5000, 50000, row_rule=0, seed=seed)
@@ -73,7 +73,7 @@ def test_parity_trace_vs_oracle(dp, orc, seed):
 CAP = dict(presolve=False, termination_criteria__iteration_limit=5000)
 
 
-def test_primal_infeasible(dp, orc):
+def test_primal_infeasible(dp, orc, row_order):
     # x + y >= 2 and x + y <= 1 over the unit box
     A = np.array([[1.0, 1.0], [1.0, 1.0]])
     prob = dp.Problem.from_dense(A, c=np.zeros(2), con_lb=np.array([2.0, -np.inf]),
@@ -85,7 +85,7 @@ def test_primal_infeasible(dp, orc):
     assert res.iterations == ref["total_count"] == 200
 
 
-def test_dual_infeasible(dp, orc):
+def test_dual_infeasible(dp, orc, row_order):
     # min -x s.t. x - y >= 0 (x, y >= 0): unbounded along x = y
     A = np.array([[1.0, -1.0]])
     prob = dp.Problem.from_dense(A, c=np.array([-1.0, 0.0]), con_lb=np.array([0.0]),

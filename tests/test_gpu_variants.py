@@ -41,7 +41,7 @@ def test_variant_parity(dp, orc, name):
     assert abs(res.iterations - ref["total_count"]) <= 10
 
 
-def test_warm_start_parity(dp, orc):
+def test_warm_start_parity(dp, orc, row_order):
     prob, opt = dp.generate(600, 1500, 12000, row_rule=0, seed=21)
     rng = np.random.default_rng(0)
     x0 = rng.uniform(0, 5, prob.n)
@@ -56,7 +56,7 @@ def test_warm_start_parity(dp, orc):
         assert rel(tx[i], ox[i]) < 1e-8 and rel(ty[i], oy[i]) < 1e-8
 
 
-def test_feasibility_polish_parity(dp, orc):
+def test_feasibility_polish_parity(dp, orc, row_order):
     prob, opt = dp.generate(700, 1800, 14000, row_rule=0, seed=31)
     kw = dict(presolve=False, feasibility_polishing=True)
     res = dp.solve_plain(prob, dp.default_params(**kw))
@@ -70,7 +70,7 @@ def test_feasibility_polish_parity(dp, orc):
     assert rel(res.x, ref["x"]) < 1e-5 and rel(res.y, ref["y"]) < 1e-5
 
 
-def test_result_vectors_match_oracle(dp, orc):
+def test_result_vectors_match_oracle(dp, orc, row_order):
     prob, opt = dp.generate(900, 2100, 17000, row_rule=2, seed=41, poisson_lambda=8.0)
     params = dict(presolve=False, termination_criteria__iteration_limit=1000)
     res = dp.solve_plain(prob, dp.default_params(**params))
