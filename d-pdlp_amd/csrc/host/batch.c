@@ -475,13 +475,34 @@ void dpdlp_i_batch_reap(dpdlp_batch_t *b, size_t keep_bytes) {
       b->pool[big] = b->pool[--b->npool];
     }
     pthread_mutex_unlock(&b->mu);
-    if (R)
+    if (R && R->pooled) {
+      /* the problem's device blob goes back to the pool */
+      size_t cap = 0;
+      char *blob = dpdlp_i_resident_release(R, &cap);
+      char *f = blob ? put_arena(b, blob, cap) : NULL;
+      if (f) hipFree(f);
+    } else if (R) {
       dpdlp_resident_free(R);
-    else if (a)
+    } else if (a) {
       hipFree(a);
-    else
+    } else {
       break;
+    }
   }
+}
+
+char *dpdlp_i_batch_get_mem(dpdlp_batch_t *b, size_t need, size_t *cap) {
+  char *p = NULL;
+  take_arena(b, need, &p, cap);
+  if (p) return p;
+  if (hipMalloc((void **)&p, need) != hipSuccess) return NULL;
+  *cap = need;
+  return p;
+}
+
+void dpdlp_i_batch_put_mem(dpdlp_batch_t *b, char *p, size_t cap) {
+  char *f = p ? put_arena(b, p, cap) : NULL;
+  if (f) hipFree(f);
 }
 
 size_t dpdlp_i_batch_pool_bytes(dpdlp_batch_t *b) {

@@ -29,8 +29,27 @@ struct dpdlp_resident {
   int *h_row_perm;         /* rows renumbered: perm[new] = old (NULL: identity) */
   int *h_perm_rp, *h_perm_ci; /* upload-time copies (freed after upload) */
   double *h_perm_v, *h_perm_lc, *h_perm_uc;
+  char *blob; /* the one device allocation every device array above lives in */
+  size_t blob_cap;
+  int pooled; /* blob came from an uploader's pool */
   double upload_time;
 };
+
+/* Source of a resident's device blob and pinned staging buffer (uploads
+ * that must not allocate or free device memory while the device is busy). */
+typedef struct {
+  char *(*get_blob)(void *ctx, size_t need, size_t *cap);
+  void (*put_blob)(void *ctx, char *p, size_t cap);
+  char *(*get_stage)(void *ctx, size_t need);
+  void *ctx;
+  void *stream; /* hipStream_t of the one asynchronous copy */
+} dpdlp_uploader_t;
+
+/* dpdlp_upload_ex through an uploader */
+struct dpdlp_resident *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, int panels_A,
+                                           int panels_At, const dpdlp_uploader_t *u);
+/* frees the host side of r and returns its device blob (and its size) */
+char *dpdlp_i_resident_release(struct dpdlp_resident *r, size_t *cap);
 
 /* Device descriptors of a resident problem's unscaled matrices (xpart must be
  * provided by the caller when P > 1). */

@@ -7,8 +7,11 @@
  * on its own HIP stream and hands the slot to the batch; the main thread
  * advances the batch one window at a time, retires finished LPs (result
  * copied out, slot and memory freed) and so makes room for the next
- * admissions.  Every request's result is the one a solve of that LP alone
- * would give (the batch guarantees it bit for bit).
+ * admissions.  Requests of at least large_lane_nnz nonzeros queue for the
+ * large-LP lane instead, which takes their memory only when it starts them,
+ * so a backlog of large requests never holds back the small ones.  Every
+ * request's result is the one a solve of that LP alone would give (the batch
+ * guarantees it bit for bit).
  *
  * The comparison mode runs the same stream through the plain loop: one
  * request at a time, in arrival order, each uploaded and solved by the
@@ -94,7 +97,7 @@ typedef struct {
   hipStream_t stream;
   pthread_mutex_t mu;
   pthread_cond_t cv;
-  double in_use;
+  double in_use, budget;
   int admitted_live;
   int stop;
   int loader_done;
@@ -106,16 +109,76 @@ typedef struct {
    * one at a time by the single-LP path (XCD column panels, tuned block
    * size) on its own thread and stream, beside the batch */
   int *lq;
-  int lq_head, lq_tail, lane_on;
+  int lq_head, lq_tail, lane_on, lane_busy;
   const pdhg_parameters_t *prm;
   double *t_done;
   cupdlpx_result_t **res;
   int completed;
 } srv_t;
 
+/* Uploads of one thread: device blobs from the batch's pool, one pinned
+ * staging buffer (grown, never shrunk), one stream for the copy. */
+typedef struct {
+  dpdlp_batch_t *b;
+  char *stage;
+  size_t stage_cap;
+  hipStream_t s;
+} uctx_t;
+
+static char *u_get_blob(void *ctx, size_t need, size_t *cap) {
+  return dpdlp_i_batch_get_mem(((uctx_t *)ctx)->b, need, cap);
+}
+
+static void u_put_blob(void *ctx, char *p, size_t cap) {
+  dpdlp_i_batch_put_mem(((uctx_t *)ctx)->b, p, cap);
+}
+
+static char *u_get_stage(void *ctx, size_t need) {
+  uctx_t *u = (uctx_t *)ctx;
+  if (need > u->stage_cap) {
+    size_t nc = need > 2 * u->stage_cap ? need : 2 * u->stage_cap;
+    if (nc < ((size_t)64 << 20)) nc = (size_t)64 << 20;
+    char *p = NULL;
+    if (hipHostMalloc((void **)&p, nc, 0) != hipSuccess) return NULL;
+    if (u->stage) hipHostFree(u->stage);
+    u->stage = p;
+    u->stage_cap = nc;
+  }
+  return u->stage;
+}
+
+static int uctx_init(uctx_t *u, dpdlp_uploader_t *up, dpdlp_batch_t *b) {
+  memset(u, 0, sizeof(*u));
+  u->b = b;
+  if (hipStreamCreateWithFlags(&u->s, hipStreamNonBlocking) != hipSuccess) return -1;
+  up->get_blob = u_get_blob;
+  up->put_blob = u_put_blob;
+  up->get_stage = u_get_stage;
+  up->ctx = u;
+  up->stream = u->s;
+  return 0;
+}
+
+static void uctx_free(uctx_t *u) {
+  if (u->s) hipStreamSynchronize(u->s);
+  if (u->stage) hipHostFree(u->stage);
+  if (u->s) hipStreamDestroy(u->s);
+  memset(u, 0, sizeof(*u));
+}
+
 static void *lane_main(void *arg) {
   srv_t *S = (srv_t *)arg;
   const dpdlp_serve_spec_t *sp = S->spec;
+  uctx_t uc;
+  dpdlp_uploader_t up;
+  if (uctx_init(&uc, &up, S->b)) {
+    pthread_mutex_lock(&S->mu);
+    S->loader_error = 1;
+    S->stop = 1;
+    pthread_cond_broadcast(&S->cv);
+    pthread_mutex_unlock(&S->mu);
+    return NULL;
+  }
   for (;;) {
     pthread_mutex_lock(&S->mu);
     while (!S->stop && S->lq_head == S->lq_tail && !S->loader_done)
@@ -125,6 +188,21 @@ static void *lane_main(void *arg) {
       break;
     }
     int k = S->lq[S->lq_head++];
+    S->lane_busy = 1;
+    /* a lane request takes its memory when the lane starts it, so queued
+     * large requests never hold back the small ones */
+    while (!S->stop && S->admitted_live > 0 && S->in_use + S->bytes[k] > S->budget)
+      pthread_cond_wait(&S->cv, &S->mu);
+    if (S->stop) {
+      S->lane_busy = 0;
+      pthread_mutex_unlock(&S->mu);
+      break;
+    }
+    S->in_use += S->bytes[k];
+    S->admitted_live++;
+    if (S->in_use > S->peak_bytes) S->peak_bytes = S->in_use;
+    if (S->admitted_live > S->max_in_flight) S->max_in_flight = S->admitted_live;
+    S->t_admit[k] = dpdlp_wall();
     pthread_mutex_unlock(&S->mu);
     pdhg_parameters_t prm = *S->prm;
     double left = S->t_cap - dpdlp_wall();
@@ -134,9 +212,13 @@ static void *lane_main(void *arg) {
     dpdlp_default_opts(&o);
     o.quiet = 1;
     o.precision = sp->precision;
-    dpdlp_resident_t *R = dpdlp_upload(S->pool[S->req_pool[k]], 0);
+    dpdlp_resident_t *R = dpdlp_i_upload_into(S->pool[S->req_pool[k]], 0, 0, 0, &up);
     cupdlpx_result_t *r = R ? dpdlp_solve_resident(R, &prm, &o) : NULL;
-    if (R) dpdlp_resident_free(R);
+    if (R) {
+      size_t cap = 0;
+      char *blob = dpdlp_i_resident_release(R, &cap);
+      dpdlp_i_batch_put_mem(S->b, blob, cap);
+    }
     double now = dpdlp_wall();
     pthread_mutex_lock(&S->mu);
     if (!r) {
@@ -151,9 +233,11 @@ static void *lane_main(void *arg) {
     }
     S->in_use -= S->bytes[k];
     S->admitted_live--;
+    S->lane_busy = 0;
     pthread_cond_broadcast(&S->cv);
     pthread_mutex_unlock(&S->mu);
   }
+  uctx_free(&uc);
   return NULL;
 }
 
@@ -161,12 +245,30 @@ static void *loader_main(void *arg) {
   srv_t *S = (srv_t *)arg;
   const dpdlp_serve_spec_t *sp = S->spec;
   double budget = sp->hbm_budget_gb * 1e9;
+  uctx_t uc;
+  dpdlp_uploader_t up;
+  if (uctx_init(&uc, &up, S->b)) {
+    pthread_mutex_lock(&S->mu);
+    S->loader_error = 1;
+    S->stop = 1;
+    S->loader_done = 1;
+    pthread_cond_broadcast(&S->cv);
+    pthread_mutex_unlock(&S->mu);
+    return NULL;
+  }
   for (int k = 0; k < S->nreq; ++k) {
     /* retired problems and surplus pooled arenas are freed here, off the
      * stepping thread */
     dpdlp_i_batch_reap(S->b, (size_t)(0.1 * budget));
     sleep_until(S->arrival[k]);
     pthread_mutex_lock(&S->mu);
+    const lp_problem_t *P = S->pool[S->req_pool[k]];
+    if (S->lane_on && P->constraint_matrix_num_nonzeros >= sp->large_lane_nnz) {
+      S->lq[S->lq_tail++] = k; /* admitted by the lane when it starts it */
+      pthread_cond_broadcast(&S->cv);
+      pthread_mutex_unlock(&S->mu);
+      continue;
+    }
     while (!S->stop && S->admitted_live > 0 &&
            (S->in_use + S->bytes[k] > budget || S->admitted_live >= sp->max_lps))
       pthread_cond_wait(&S->cv, &S->mu);
@@ -178,14 +280,6 @@ static void *loader_main(void *arg) {
     S->admitted_live++;
     if (S->in_use > S->peak_bytes) S->peak_bytes = S->in_use;
     if (S->admitted_live > S->max_in_flight) S->max_in_flight = S->admitted_live;
-    const lp_problem_t *P = S->pool[S->req_pool[k]];
-    if (S->lane_on && P->constraint_matrix_num_nonzeros >= sp->large_lane_nnz) {
-      S->lq[S->lq_tail++] = k;
-      S->t_admit[k] = dpdlp_wall();
-      pthread_cond_broadcast(&S->cv);
-      pthread_mutex_unlock(&S->mu);
-      continue;
-    }
     pthread_mutex_unlock(&S->mu);
     /* a slot of a just-retired LP is reclaimed at the batch's next window */
     int slot;
@@ -198,10 +292,14 @@ static void *loader_main(void *arg) {
       if (stop) break;
     }
     if (slot < 0) break;
-    dpdlp_resident_t *R = dpdlp_upload_ex(S->pool[S->req_pool[k]], sp->spmv_cap, 1, 1);
+    dpdlp_resident_t *R = dpdlp_i_upload_into(S->pool[S->req_pool[k]], sp->spmv_cap, 1, 1, &up);
     if (!R || dpdlp_i_batch_prepare(S->b, slot, R, 1, k, S->stream)) {
       if (!R) dpdlp_i_batch_unreserve(S->b, slot);
-      if (R) dpdlp_resident_free(R);
+      if (R) {
+        size_t cap = 0;
+        char *blob = dpdlp_i_resident_release(R, &cap);
+        dpdlp_i_batch_put_mem(S->b, blob, cap);
+      }
       pthread_mutex_lock(&S->mu);
       S->loader_error = 1;
       S->stop = 1;
@@ -210,6 +308,7 @@ static void *loader_main(void *arg) {
     }
     S->t_admit[k] = dpdlp_wall();
   }
+  uctx_free(&uc);
   pthread_mutex_lock(&S->mu);
   S->loader_done = 1;
   pthread_cond_broadcast(&S->cv);
@@ -361,6 +460,7 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
     S.bytes = bytes;
     S.nreq = nreq;
     S.t_cap = t_cap;
+    S.budget = sp.hbm_budget_gb * 1e9;
     S.t_admit = t_admit;
     pthread_mutex_init(&S.mu, NULL);
     pthread_cond_init(&S.cv, NULL);
@@ -427,7 +527,8 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
         S.admitted_live -= freed;
         pthread_cond_broadcast(&S.cv);
       }
-      int done = S.loader_done && dpdlp_batch_in_flight(b) == 0 && S.admitted_live == 0;
+      int done = S.loader_done && dpdlp_batch_in_flight(b) == 0 && S.admitted_live == 0 &&
+                 S.lq_head == S.lq_tail && !S.lane_busy;
       int err = S.loader_error;
       if (dpdlp_wall() > t_cap) S.stop = 1;
       int stop = S.stop;

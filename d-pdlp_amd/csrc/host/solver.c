@@ -155,14 +155,33 @@ void dpdlp_default_opts(dpdlp_solve_opts_t *o) {
   o->time_to_1e6_sec = -1.0;
 }
 
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt,
+                                     int order, const dpdlp_uploader_t *u);
+
+/* frees the host side of r and hands back its device blob */
+char *dpdlp_i_resident_release(dpdlp_resident_t *r, size_t *cap) {
+  if (!r) return NULL;
+  char *b = r->blob;
+  if (cap) *cap = r->blob_cap;
+  free(r->h_pstart);
+  free(r->h_dstart);
+  free(r->h_A_blk);
+  free(r->h_At_blk);
+  free(r->h_row_perm);
+  free_perm_copies(r);
+  free(r);
+  return b;
+}
+
+dpdlp_resident_t *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, int panels_A,
+                                      int panels_At, const dpdlp_uploader_t *u) {
+  return upload_with(p, spmv_cap, panels_A, panels_At,
+                     (panels_A > 0 || panels_At > 0) ? 0 : -1, u);
+}
+
 void dpdlp_resident_free(dpdlp_resident_t *r) {
   if (!r) return;
-  void *ptrs[] = {r->A_rp,    r->A_ci,    r->At_rp,   r->At_ci,   r->A_blk,   r->At_blk,
-                  r->A_v,     r->At_v,    r->c,       r->lv,      r->uv,      r->lc,
-                  r->uc,      r->A_toff,  r->At_toff, r->A_perm,  r->At_perm, r->A_pci,
-                  r->At_pci,  r->A_rofs,  r->At_rofs, r->A_pv,    r->At_pv};
-  for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
-    if (ptrs[i]) hipFree(ptrs[i]);
+  if (r->blob) hipFree(r->blob);
   free(r->h_pstart);
   free(r->h_dstart);
   free(r->h_A_blk);
@@ -182,18 +201,70 @@ void dpdlp_resident_free(dpdlp_resident_t *r) {
     }                                                                          \
   } while (0)
 
-static hipError_t up(void **dst, const void *src, size_t bytes) {
-  hipError_t e = hipMalloc(dst, bytes ? bytes : 8);
-  if (e != hipSuccess) return e;
-  if (bytes && src) return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+/* bytes an array takes in a resident's blob */
+static size_t ublk(size_t bytes) { return ((bytes ? bytes : 8) + 255) & ~(size_t)255; }
+
+/* Carves a resident's device arrays out of one blob.  Without an uploader
+ * the blob is allocated here and every array is copied directly; with one,
+ * the blob comes from the uploader (a recycled allocation) and the arrays
+ * are gathered in its pinned staging buffer and sent in one asynchronous
+ * copy on its stream, so an upload under load neither frees device memory
+ * nor waits behind the device's kernels for each array. */
+typedef struct {
+  char *blob, *stage;
+  size_t cap, off;
+  hipStream_t s;
+} usink_t;
+
+static int sink_open(usink_t *k, const dpdlp_uploader_t *u, size_t need) {
+  memset(k, 0, sizeof(*k));
+  if (u) {
+    k->blob = u->get_blob(u->ctx, need, &k->cap);
+    k->stage = u->get_stage(u->ctx, need);
+    k->s = (hipStream_t)u->stream;
+    if (!k->blob || !k->stage) {
+      dpdlp_set_error("no device memory or staging buffer for %zu bytes", need);
+      if (k->blob) u->put_blob(u->ctx, k->blob, k->cap);
+      k->blob = NULL;
+      return -1;
+    }
+    return 0;
+  }
+  if (hipMalloc((void **)&k->blob, need) != hipSuccess) {
+    dpdlp_set_error("device allocation failed (%zu bytes of problem data)", need);
+    k->blob = NULL;
+    return -1;
+  }
+  k->cap = need;
+  return 0;
+}
+
+static hipError_t up(usink_t *k, void **dst, const void *src, size_t bytes) {
+  size_t step = ublk(bytes);
+  if (k->off + step > k->cap) return hipErrorOutOfMemory;
+  *dst = k->blob + k->off;
+  if (bytes && src) {
+    if (k->stage)
+      memcpy(k->stage + k->off, src, bytes);
+    else {
+      hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+      if (e != hipSuccess) return e;
+    }
+  }
+  k->off += step;
   return hipSuccess;
 }
 
-static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt,
-                                     int order);
+static hipError_t sink_close(usink_t *k) {
+  if (!k->stage || k->off == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(k->blob, k->stage, k->off, hipMemcpyHostToDevice, k->s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(k->s);
+}
+
 
 dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
-  return upload_with(p, spmv_cap, 0, 0, -1);
+  return upload_with(p, spmv_cap, 0, 0, -1, NULL);
 }
 
 dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panels_A,
@@ -205,7 +276,7 @@ dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panel
   }
   /* explicit layouts (the batch's plain one) keep the row order */
   return upload_with(p, spmv_cap, panels_A, panels_At,
-                     (panels_A > 0 || panels_At > 0) ? 0 : -1);
+                     (panels_A > 0 || panels_At > 0) ? 0 : -1, NULL);
 }
 
 size_t dpdlp_resident_bytes(const dpdlp_resident_t *r) {
@@ -231,7 +302,7 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
   double bestA = 1e30, bestT = 1e30;
   int PA = 1, PT = 1;
   for (int i = 0; i < 4; ++i) {
-    dpdlp_resident_t *r = upload_with(p, spmv_cap, cand[i], cand[i], -1);
+    dpdlp_resident_t *r = upload_with(p, spmv_cap, cand[i], cand[i], -1, NULL);
     if (!r) continue;
     double ta, tt;
     time_both(r, &ta, &tt);
@@ -249,7 +320,7 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
     chosen[0] = PA;
     chosen[1] = PT;
   }
-  return upload_with(p, spmv_cap, PA, PT, -1);
+  return upload_with(p, spmv_cap, PA, PT, -1, NULL);
 }
 
 /* row lengths skewed enough for the row renumbering to pay: the longest row
@@ -298,8 +369,10 @@ static void free_perm_copies(dpdlp_resident_t *r) {
 }
 
 static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt,
-                                     int order) {
+                                     int order, const dpdlp_uploader_t *u) {
   double t0 = dpdlp_wall();
+  usink_t K;
+  memset(&K, 0, sizeof(K));
   if (!p || p->num_variables < 0 || p->num_constraints < 0 ||
       p->constraint_matrix_num_nonzeros < 0) {
     dpdlp_set_error("invalid problem");
@@ -435,12 +508,6 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
   if (dpdlp_host_transpose(m, n, rp_u, ci_u, v_u, t_rp, t_ci, t_v))
     goto ufail;
 
-  UPCHK(up((void **)&r->A_rp, rp_u, (size_t)(m + 1) * sizeof(int)));
-  UPCHK(up((void **)&r->A_ci, ci_u, (size_t)nnz * sizeof(int)));
-  UPCHK(up((void **)&r->A_v, v_u, (size_t)nnz * sizeof(double)));
-  UPCHK(up((void **)&r->At_rp, t_rp, (size_t)(n + 1) * sizeof(int)));
-  UPCHK(up((void **)&r->At_ci, t_ci, (size_t)nnz * sizeof(int)));
-  UPCHK(up((void **)&r->At_v, t_v, (size_t)nnz * sizeof(double)));
   /* XCD column-panel tiles */
   r->A_P = PA > 0 ? PA : pick_panels(n);
   r->At_P = PAt > 0 ? PAt : pick_panels(m);
@@ -454,20 +521,55 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
       }
     }
   }
-  r->A_nblk = r->A_P > 1 ? dpdlp_plan_blocks_panels(m, n, rp_u, ci_u, r->cap, r->A_P,
-                                                    DPDLP_PANEL_RMAX, blk)
-                         : dpdlp_plan_blocks(m, rp_u, r->cap, blk);
-  UPCHK(up((void **)&r->A_blk, blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
-  r->h_A_blk = (int *)malloc((size_t)(r->A_nblk + 1) * sizeof(int));
-  if (!r->h_A_blk) goto ufail;
-  memcpy(r->h_A_blk, blk, (size_t)(r->A_nblk + 1) * sizeof(int));
-  r->At_nblk = r->At_P > 1 ? dpdlp_plan_blocks_panels(n, m, t_rp, t_ci, r->cap, r->At_P,
-                                                      DPDLP_PANEL_RMAX, blk)
-                           : dpdlp_plan_blocks(n, t_rp, r->cap, blk);
-  UPCHK(up((void **)&r->At_blk, blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
-  r->h_At_blk = (int *)malloc((size_t)(r->At_nblk + 1) * sizeof(int));
-  if (!r->h_At_blk) goto ufail;
-  memcpy(r->h_At_blk, blk, (size_t)(r->At_nblk + 1) * sizeof(int));
+  /* row-block plans (host copies kept for the batch's work lists) */
+  for (int which = 0; which < 2; ++which) {
+    int P = which ? r->At_P : r->A_P, rows = which ? n : m, cols = which ? m : n;
+    const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
+    int nb = P > 1 ? dpdlp_plan_blocks_panels(rows, cols, hrp, hci, r->cap, P, DPDLP_PANEL_RMAX,
+                                              blk)
+                   : dpdlp_plan_blocks(rows, hrp, r->cap, blk);
+    int *h = (int *)malloc((size_t)(nb + 1) * sizeof(int));
+    if (!h) {
+      dpdlp_set_error("host allocation failed");
+      goto ufail;
+    }
+    memcpy(h, blk, (size_t)(nb + 1) * sizeof(int));
+    if (which) {
+      r->At_nblk = nb;
+      r->h_At_blk = h;
+    } else {
+      r->A_nblk = nb;
+      r->h_A_blk = h;
+    }
+  }
+
+  /* every device array lives in one blob, carved in 256-byte steps in the
+   * order below */
+  {
+    size_t need = ublk(4 * ((size_t)m + 1)) + ublk(4 * (size_t)nnz) + ublk(8 * (size_t)nnz) +
+                  ublk(4 * ((size_t)n + 1)) + ublk(4 * (size_t)nnz) + ublk(8 * (size_t)nnz) +
+                  ublk(4 * ((size_t)r->A_nblk + 1)) + ublk(4 * ((size_t)r->At_nblk + 1)) +
+                  3 * ublk(8 * (size_t)n) + 2 * ublk(8 * (size_t)m);
+    for (int which = 0; which < 2; ++which) {
+      int P = which ? r->At_P : r->A_P, rows = which ? n : m;
+      size_t nb = (size_t)(which ? r->At_nblk : r->A_nblk);
+      if (P > 1)
+        need += ublk(4 * (nb * P + 1)) + ublk(2 * (size_t)P * ((size_t)rows + nb)) +
+                2 * ublk(4 * (size_t)nnz) + ublk(8 * (size_t)nnz);
+    }
+    if (sink_open(&K, u, need)) goto ufail;
+    r->blob = K.blob;
+    r->blob_cap = K.cap;
+    r->pooled = u != NULL;
+  }
+  UPCHK(up(&K, (void **)&r->A_rp, rp_u, (size_t)(m + 1) * sizeof(int)));
+  UPCHK(up(&K, (void **)&r->A_ci, ci_u, (size_t)nnz * sizeof(int)));
+  UPCHK(up(&K, (void **)&r->A_v, v_u, (size_t)nnz * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->At_rp, t_rp, (size_t)(n + 1) * sizeof(int)));
+  UPCHK(up(&K, (void **)&r->At_ci, t_ci, (size_t)nnz * sizeof(int)));
+  UPCHK(up(&K, (void **)&r->At_v, t_v, (size_t)nnz * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->A_blk, r->h_A_blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
+  UPCHK(up(&K, (void **)&r->At_blk, r->h_At_blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
 
   for (int which = 0; which < 2; ++which) {
     int P = which ? r->At_P : r->A_P;
@@ -475,18 +577,15 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     int rows = which ? n : m, cols = which ? m : n, nb = which ? r->At_nblk : r->A_nblk;
     const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
     const double *hv = which ? t_v : v_u;
-    int *blkh = (int *)malloc((size_t)(nb + 1) * sizeof(int));
+    const int *blkh = which ? r->h_At_blk : r->h_A_blk;
     int *toff = (int *)malloc(((size_t)nb * P + 1) * sizeof(int));
     unsigned short *rofs =
         (unsigned short *)malloc((size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
     int *perm = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
     int *pci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
     double *pv = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
-    int ok = blkh && toff && rofs && perm && pci && pv;
-    if (ok) {
-      dpdlp_plan_blocks_panels(rows, cols, hrp, hci, r->cap, P, DPDLP_PANEL_RMAX, blkh);
-      ok = dpdlp_build_panels(rows, cols, hrp, hci, blkh, nb, P, toff, rofs, perm, pci) == 0;
-    }
+    int ok = toff && rofs && perm && pci && pv;
+    if (ok) ok = dpdlp_build_panels(rows, cols, hrp, hci, blkh, nb, P, toff, rofs, perm, pci) == 0;
     if (ok) {
       for (int k = 0; k < nnz; ++k) pv[k] = hv[perm[k]];
       hipError_t e = hipSuccess;
@@ -494,16 +593,16 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
       int **dc = which ? &r->At_pci : &r->A_pci;
       unsigned short **dr = which ? &r->At_rofs : &r->A_rofs;
       double **dv = which ? &r->At_pv : &r->A_pv;
-      if (e == hipSuccess) e = up((void **)dt, toff, ((size_t)nb * P + 1) * sizeof(int));
+      if (e == hipSuccess) e = up(&K, (void **)dt, toff, ((size_t)nb * P + 1) * sizeof(int));
       if (e == hipSuccess)
-        e = up((void **)dr, rofs, (size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
-      if (e == hipSuccess) e = up((void **)dp, perm, (size_t)nnz * sizeof(int));
-      if (e == hipSuccess) e = up((void **)dc, pci, (size_t)nnz * sizeof(int));
-      if (e == hipSuccess) e = up((void **)dv, pv, (size_t)nnz * sizeof(double));
+        e = up(&K, (void **)dr, rofs,
+               (size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
+      if (e == hipSuccess) e = up(&K, (void **)dp, perm, (size_t)nnz * sizeof(int));
+      if (e == hipSuccess) e = up(&K, (void **)dc, pci, (size_t)nnz * sizeof(int));
+      if (e == hipSuccess) e = up(&K, (void **)dv, pv, (size_t)nnz * sizeof(double));
       ok = e == hipSuccess;
       if (!ok) dpdlp_set_error("HIP error uploading panel tiles: %s", hipGetErrorString(e));
     }
-    free(blkh);
     free(toff);
     free(rofs);
     free(perm);
@@ -514,11 +613,12 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
       goto ufail;
     }
   }
-  UPCHK(up((void **)&r->c, p->objective_vector, (size_t)n * sizeof(double)));
-  UPCHK(up((void **)&r->lv, p->variable_lower_bound, (size_t)n * sizeof(double)));
-  UPCHK(up((void **)&r->uv, p->variable_upper_bound, (size_t)n * sizeof(double)));
-  UPCHK(up((void **)&r->lc, lc_u, (size_t)m * sizeof(double)));
-  UPCHK(up((void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->c, p->objective_vector, (size_t)n * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->lv, p->variable_lower_bound, (size_t)n * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->uv, p->variable_upper_bound, (size_t)n * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->lc, lc_u, (size_t)m * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
+  UPCHK(sink_close(&K));
   free(t_rp);
   free(t_ci);
   free(t_v);

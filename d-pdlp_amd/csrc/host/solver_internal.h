@@ -68,4 +68,7 @@ int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int slot, dpdlp_resident_t *R, int t
 /* frees retired problems and trims pooled arenas to keep_bytes (call from a
  * set-up thread: device frees may synchronise the device) */
 void dpdlp_i_batch_reap(dpdlp_batch_t *b, size_t keep_bytes);
+/* device memory from / back to the batch's pool of recycled allocations */
+char *dpdlp_i_batch_get_mem(dpdlp_batch_t *b, size_t need, size_t *cap);
+void dpdlp_i_batch_put_mem(dpdlp_batch_t *b, char *p, size_t cap);
 size_t dpdlp_i_batch_pool_bytes(dpdlp_batch_t *b);
