@@ -46,6 +46,9 @@ WORKLOADS = {
 }
 
 
+BATCH_CAP = 4096  # row-block capacity of the batched LPs (nonzeros per workgroup)
+
+
 def finite(v):
     """JSON-safe number (None for inf / nan)."""
     import math
@@ -101,13 +104,13 @@ def bench_batch(args, dpdlp):
     print(f"# generated {count} LPs, nnz={nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
     t = time.perf_counter()
     plain_res = [dpdlp.Resident(p) for p, _ in probs] if not args.no_plain else []
-    batch_res = [dpdlp.Resident(p, spmv_cap=1024, panels=(1, 1)) for p, _ in probs]
+    batch_res = [dpdlp.Resident(p, spmv_cap=BATCH_CAP, panels=(1, 1)) for p, _ in probs]
     up_s = time.perf_counter() - t
     params = dpdlp.default_params(presolve=False)
     for k in range(args.warmup):
         if plain_res:
             plain_res[0].solve(params)
-        dpdlp.solve_batch(batch_res[:4], params, spmv_cap=1024, precision=dpdlp.PREC_MIXED)
+        dpdlp.solve_batch(batch_res[:4], params, spmv_cap=BATCH_CAP, precision=dpdlp.PREC_MIXED)
     # plain loop: the single-LP FP64 path, one LP after another
     plain, plain_s = None, None
     if not args.no_plain:
@@ -120,7 +123,7 @@ def bench_batch(args, dpdlp):
     def run_batch(prec, profile):
         st = {}
         t0 = time.perf_counter()
-        got = dpdlp.solve_batch(batch_res, params, stats=st, max_lps=count, spmv_cap=1024,
+        got = dpdlp.solve_batch(batch_res, params, stats=st, max_lps=count, spmv_cap=BATCH_CAP,
                                 precision=prec, profile_phases=profile)
         el = time.perf_counter() - t0
         cnt = {k: st[k] for k in ("windows", "syncs", "launches")}

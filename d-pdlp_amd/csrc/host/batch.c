@@ -176,7 +176,7 @@ void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o) {
   o->max_lps = 1024;
   o->precision = DPDLP_PREC_FP64;
   o->mixed_switch_tol = 1e-4 * 0.1;
-  o->spmv_cap = 1024;
+  o->spmv_cap = 4096;
   o->solo_lps = 4;
 }
 
@@ -263,7 +263,7 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
     dpdlp_batch_default_opts(&b->o);
   if (b->o.max_lps <= 0) b->o.max_lps = 1024;
   if (b->o.spmv_cap != 1024 && b->o.spmv_cap != 2048 && b->o.spmv_cap != 4096)
-    b->o.spmv_cap = 1024;
+    b->o.spmv_cap = 4096;
   if (b->prm.termination_evaluation_frequency <= 0) {
     dpdlp_set_error("termination_evaluation_frequency must be positive");
     pthread_mutex_destroy(&b->mu);

@@ -49,7 +49,7 @@ void dpdlp_serve_default_spec(dpdlp_serve_spec_t *s) {
   s->max_lps = 512;
   s->precision = DPDLP_PREC_MIXED;
   s->mode = 0;
-  s->spmv_cap = 1024;
+  s->spmv_cap = 4096;
   s->large_lane_nnz = 4000000;
 }
 

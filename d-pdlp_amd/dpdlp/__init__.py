@@ -357,7 +357,7 @@ class Resident:
             pass
 
 
-def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=1024, mixed_switch_tol=None,
+def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=4096, mixed_switch_tol=None,
                profile_phases=0, use_graph=0, solo_lps=None):
     o = BatchOpts()
     lib.dpdlp_batch_default_opts(C.byref(o))

@@ -106,7 +106,7 @@ typedef struct {
   int max_lps;             /* slot capacity (default 1024) */
   int precision;           /* DPDLP_PREC_FP64 (default) or DPDLP_PREC_MIXED */
   double mixed_switch_tol; /* mixed: FP64 from this max relative KKT on */
-  int spmv_cap;            /* row-block capacity of every LP (default 1024) */
+  int spmv_cap;            /* row-block capacity of every LP (default 4096) */
   int profile_phases;      /* 1: HIP events around the two phase kernels of
                               round 1 of every 4th window (dpdlp_batch_phase_stats) */
   int use_graph;           /* 1: capture each window's launches in a hipGraph
@@ -178,7 +178,7 @@ typedef struct {
   int max_lps;          /* admission: LPs in flight */
   int precision;        /* batch precision (DPDLP_PREC_MIXED default) */
   int mode;             /* 0 continuous batching, 1 plain loop (one at a time, FP64) */
-  int spmv_cap;         /* row-block capacity (1024) */
+  int spmv_cap;         /* row-block capacity (default 4096) */
   long long large_lane_nnz; /* requests with at least this many nonzeros are
                                solved by the single-LP path on a second
                                thread and stream, beside the batch (0: off) */

@@ -6,7 +6,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-POOL = dict(pool_small=6, pool_large=0, large_frac=0.0, m_lo=300, m_hi=3000,
+POOL = dict(pool_small=6, pool_large=0, large_frac=0.0, m_lo=300, m_hi=3000, spmv_cap=1024,
             member_seed=3600)
 
 
