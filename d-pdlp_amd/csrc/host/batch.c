@@ -1229,12 +1229,10 @@ static int launch_window(dpdlp_batch_t *b) {
   /* few iterating LPs: their single-LP launches (shorter dependency chain per
    * workgroup, no work lists) */
   const dpdlp_bctl_t *solo = n_iter <= b->o.solo_lps ? ctl : NULL;
-  /* phase timing samples windows without power-iteration steps, whose
-   * kernels would share the device with the timed ones */
-  int pi_window = 0;
-  for (int r = 0; r < W; ++r) pi_window |= any_pi[r];
-  int timed_window =
-      !solo && !pi_window && b->npev_used < b->npev_created && (b->windows & 3) == 0;
+  /* phase timing samples every 4th batched window, power-iteration steps
+   * running beside it included (the launch population a counter profile of
+   * the same kernels averages over) */
+  int timed_window = !solo && b->npev_used < b->npev_created && (b->windows & 3) == 0;
   unsigned long long sig = (unsigned long long)any_restart;
   for (int r = 0; r < W; ++r)
     sig |= (unsigned long long)(any_phase[r] | (any_fpe[r] << 1) | (any_pi[r] << 2)) << (1 + 3 * r);
