@@ -405,61 +405,97 @@ __global__ void k_unscale(const double *x, const double *y, const double *D_c, c
  * mode 0 = max (exact in any order); mode 1 = sum of |a|^p accumulated
  * strictly in storage order by the group's first lane (so column sums over
  * the transpose match a row-major host accumulation bit for bit); pow is
- * skipped for p == 1, which is exact. */
+ * skipped for p == 1, which is exact.  The result does not depend on G. */
 template <int G>
-__global__ __launch_bounds__(DPDLP_BS) void k_row_reduce(const int *rp, const double *v, int rows,
-                                                        int mode, double p, double *out) {
-  const int gl = threadIdx.x & (G - 1);
-  const long long g0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  const long long ng = ((long long)gridDim.x * blockDim.x) / G;
-  for (long long r = g0; r < rows; r += ng) {
-    const int a = rp[r], e = rp[r + 1];
-    double acc = 0.0;
-    if (mode == 0) {
-      for (int q = a + gl; q < e; q += G) acc = fmax(acc, fabs(v[q]));
+__device__ __forceinline__ double row_abs_reduce(const int *rp, const double *v, long long r,
+                                                 int gl, int mode, double p) {
+  const int a = rp[r], e = rp[r + 1];
+  double acc = 0.0;
+  if (mode == 0) {
+    for (int q = a + gl; q < e; q += G) acc = fmax(acc, fabs(v[q]));
 #pragma unroll
-      for (int off = G / 2; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off, G));
-    } else {
-      for (int base = a; base < e; base += G) {
-        const int q = base + gl;
-        double t = 0.0;
-        if (q < e) t = (p == 1.0) ? fabs(v[q]) : pow(fabs(v[q]), p);
-        const int cnt = (e - base) < G ? (e - base) : G;
-        for (int j = 0; j < cnt; ++j) acc += __shfl(t, j, G);
-      }
+    for (int off = G / 2; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off, G));
+  } else {
+    for (int base = a; base < e; base += G) {
+      const int q = base + gl;
+      double t = 0.0;
+      if (q < e) t = (p == 1.0) ? fabs(v[q]) : pow(fabs(v[q]), p);
+      const int cnt = (e - base) < G ? (e - base) : G;
+      for (int j = 0; j < cnt; ++j) acc += __shfl(t, j, G);
     }
-    if (gl == 0) out[r] = acc;
   }
+  return acc;
 }
 
-/* rs/cs <- sqrt(.) or 1 below 1e-12; cumulative D *= factor */
-__global__ void k_scale_factors(double *f, double *D, int count) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
-       i += (long long)gridDim.x * blockDim.x) {
-    double v = f[i];
-    v = (v < 1e-12) ? 1.0 : sqrt(v);
-    f[i] = v;
-    D[i] *= v;
-  }
-}
-
-/* a_k /= (rs[row] * cs[col]) for one CSR matrix, G lanes per row;
- * `rowf` scales rows, `colf` the column index space (pass them swapped for
- * the transpose: the product is formed as rs[i] * cs[j] in both). */
+/* Row reductions of A (rows [0, mA), exponent pA, into outA) and of A'
+ * (the next mB groups, exponent pB, into outB) in one launch. */
 template <int G>
-__global__ __launch_bounds__(DPDLP_BS) void k_scale_values(const int *rp, const int *ci, double *v,
-                                                          int rows, const double *rowf,
-                                                          const double *colf, int row_first) {
+__global__ __launch_bounds__(DPDLP_BS) void k_row_reduce2(const int *rpA, const double *vA, int mA,
+                                                         double pA, double *outA,
+                                                         const int *rpB, const double *vB,
+                                                         int mB, double pB, double *outB,
+                                                         int mode) {
   const int gl = threadIdx.x & (G - 1);
   const long long g0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
   const long long ng = ((long long)gridDim.x * blockDim.x) / G;
-  for (long long r = g0; r < rows; r += ng) {
-    const double fr = rowf[r];
-    const int e = rp[r + 1];
-    for (int q = rp[r] + gl; q < e; q += G) {
-      const double fc = colf[ci[q]];
-      const double d = row_first ? fr * fc : fc * fr;
-      v[q] /= d;
+  for (long long r = g0; r < (long long)mA + mB; r += ng) {
+    const bool a = r < mA;
+    const long long rr = a ? r : r - mA;
+    const double acc = a ? row_abs_reduce<G>(rpA, vA, rr, gl, mode, pA)
+                         : row_abs_reduce<G>(rpB, vB, rr, gl, mode, pB);
+    if (gl == 0) (a ? outA : outB)[rr] = acc;
+  }
+}
+
+/* The factors of one Ruiz / Pock-Chambolle step and the vectors they scale,
+ * one index per thread: f <- sqrt(f) (1 below 1e-12), D *= f; then
+ * c /= cs, lv,uv *= cs and lc,uc /= rs (preconditioner.c:92-100,126-156). */
+__global__ void k_scale_factors_vectors(double *cs, double *D_c, double *c, double *lv,
+                                        double *uv, int n, double *rs, double *D_r, double *lc,
+                                        double *uc, int m) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n + m;
+       i += (long long)gridDim.x * blockDim.x) {
+    if (i < n) {
+      double f = cs[i];
+      f = (f < 1e-12) ? 1.0 : sqrt(f);
+      cs[i] = f;
+      D_c[i] *= f;
+      c[i] /= f;
+      uv[i] *= f;
+      lv[i] *= f;
+    } else {
+      int k = (int)(i - n);
+      double f = rs[k];
+      f = (f < 1e-12) ? 1.0 : sqrt(f);
+      rs[k] = f;
+      D_r[k] *= f;
+      lc[k] /= f;
+      uc[k] /= f;
+    }
+  }
+}
+
+/* a_k /= (rs[row] * cs[col]) for A and A' in one launch, G lanes per row;
+ * the product is formed as rs[i] * cs[j] in both matrices (for A' the row
+ * index is a column of A). */
+template <int G>
+__global__ __launch_bounds__(DPDLP_BS) void k_scale_values2(const int *rpA, const int *ciA,
+                                                           double *vA, int mA, const int *rpB,
+                                                           const int *ciB, double *vB, int mB,
+                                                           const double *rs, const double *cs) {
+  const int gl = threadIdx.x & (G - 1);
+  const long long g0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  const long long ng = ((long long)gridDim.x * blockDim.x) / G;
+  for (long long r = g0; r < (long long)mA + mB; r += ng) {
+    if (r < mA) {
+      const double fr = rs[r];
+      const int e = rpA[r + 1];
+      for (int q = rpA[r] + gl; q < e; q += G) vA[q] /= fr * cs[ciA[q]];
+    } else {
+      const long long rr = r - mA;
+      const double fc = cs[rr];
+      const int e = rpB[rr + 1];
+      for (int q = rpB[rr] + gl; q < e; q += G) vB[q] /= rs[ciB[q]] * fc;
     }
   }
 }
@@ -480,60 +516,40 @@ static inline int group_grid(long long rows, int G) {
   return (int)g;
 }
 
-static void row_reduce(const dpdlp_csr_t *A, int rows, int mode, double p, double *out,
-                       hipStream_t s) {
-  const int G = lanes_per_row(A->nnz, rows);
-  const int grid = group_grid(rows, G);
+/* rs <- row reduction of A (exponent pA), cs <- of A' (exponent pB) */
+static void row_reduce_both(const dpdlp_state_t *st, int mode, double pA, double pB, double *rs,
+                            double *cs, hipStream_t s) {
+  const int G = lanes_per_row(2LL * st->A.nnz, (long long)st->m + st->n);
+  const int grid = group_grid((long long)st->m + st->n, G);
+#define RR(GG)                                                                              \
+  k_row_reduce2<GG><<<grid, DPDLP_BS, 0, s>>>(st->A.rp, st->A.v, st->m, pA, rs, st->At.rp, \
+                                              st->At.v, st->n, pB, cs, mode)
   switch (G) {
-  case 4: k_row_reduce<4><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
-  case 8: k_row_reduce<8><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
-  case 16: k_row_reduce<16><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
-  case 32: k_row_reduce<32><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
-  default: k_row_reduce<64><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->v, rows, mode, p, out); break;
+  case 4: RR(4); break;
+  case 8: RR(8); break;
+  case 16: RR(16); break;
+  case 32: RR(32); break;
+  default: RR(64); break;
   }
+#undef RR
 }
 
-static void scale_values(const dpdlp_csr_t *A, int rows, const double *rowf, const double *colf,
-                         int row_first, hipStream_t s) {
-  const int G = lanes_per_row(A->nnz, rows);
-  const int grid = group_grid(rows, G);
-  double *v = (double *)A->v;
+static void scale_values_both(dpdlp_state_t *st, const double *rs, const double *cs,
+                              hipStream_t s) {
+  const int G = lanes_per_row(2LL * st->A.nnz, (long long)st->m + st->n);
+  const int grid = group_grid((long long)st->m + st->n, G);
+#define SV(GG)                                                                         \
+  k_scale_values2<GG><<<grid, DPDLP_BS, 0, s>>>(st->A.rp, st->A.ci, (double *)st->A.v, \
+                                                st->m, st->At.rp, st->At.ci,           \
+                                                (double *)st->At.v, st->n, rs, cs)
   switch (G) {
-  case 4:
-    k_scale_values<4><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
-    break;
-  case 8:
-    k_scale_values<8><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
-    break;
-  case 16:
-    k_scale_values<16><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
-    break;
-  case 32:
-    k_scale_values<32><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
-    break;
-  default:
-    k_scale_values<64><<<grid, DPDLP_BS, 0, s>>>(A->rp, A->ci, v, rows, rowf, colf, row_first);
-    break;
+  case 4: SV(4); break;
+  case 8: SV(8); break;
+  case 16: SV(16); break;
+  case 32: SV(32); break;
+  default: SV(64); break;
   }
-}
-
-/* c /= cs, lv,uv *= cs, lc,uc /= rs (preconditioner.c:92-100) */
-__global__ void k_scale_vectors(double *c, double *lv, double *uv, const double *cs, int n,
-                                double *lc, double *uc, const double *rs, int m) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n + m;
-       i += (long long)gridDim.x * blockDim.x) {
-    if (i < n) {
-      double f = cs[i];
-      c[i] /= f;
-      uv[i] *= f;
-      lv[i] *= f;
-    } else {
-      int k = (int)(i - n);
-      double f = rs[k];
-      lc[k] /= f;
-      uc[k] /= f;
-    }
-  }
+#undef SV
 }
 
 /* bound norm^2 and objective norm^2 partials (preconditioner.c:227-247) */
@@ -557,8 +573,16 @@ __global__ __launch_bounds__(DPDLP_BS) void k_bound_obj_sums(const double *lc, c
   block_partials<2, 0>(s, mx, red, part, stride, slot0);
 }
 
-__global__ void k_bound_obj_apply(double *lc, double *uc, int m, double *lv, double *uv,
-                                  double *c, int n, double beta, double gamma) {
+/* l_c, u_c, l_v, u_v *= beta and c *= gamma (preconditioner.c:252-265), with
+ * beta = 1/(sqrt(bound sum of squares) + 1) and gamma =
+ * 1/(sqrt(objective sum of squares) + 1) formed from the reduced sums in
+ * S[DPDLP_SL_SCAL], S[DPDLP_SL_SCAL + 1] (preconditioner.c:249-250); beta
+ * and gamma are also stored to S[DPDLP_SL_SCAL + 2], S[DPDLP_SL_SCAL + 3]
+ * for the host. */
+__global__ void k_bound_obj_apply_scal(double *lc, double *uc, int m, double *lv, double *uv,
+                                       double *c, int n, double *S) {
+  const double beta = 1.0 / (sqrt(S[DPDLP_SL_SCAL]) + 1.0);
+  const double gamma = 1.0 / (sqrt(S[DPDLP_SL_SCAL + 1]) + 1.0);
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)m + n;
        i += (long long)gridDim.x * blockDim.x) {
     if (i < m) {
@@ -570,6 +594,10 @@ __global__ void k_bound_obj_apply(double *lc, double *uc, int m, double *lv, dou
       uv[k] *= beta;
       c[k] *= gamma;
     }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S[DPDLP_SL_SCAL + 2] = beta;
+    S[DPDLP_SL_SCAL + 3] = gamma;
   }
 }
 
@@ -864,30 +892,24 @@ int dpdlp_dev_result(const dpdlp_state_t *st, double beta, double gamma, double 
   return (int)hipGetLastError();
 }
 
+/* one Ruiz step: three launches (row maxima of A and A', factors and
+ * vectors, matrix values) */
 int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream) {
   hipStream_t s = ST(stream);
-  row_reduce(&st->A, st->m, 0, 1.0, rs, s);
-  row_reduce(&st->At, st->n, 0, 1.0, cs, s);
-  k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
-  k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
-  k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
-      st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
-  scale_values(&st->A, st->m, rs, cs, 1, s);
-  scale_values(&st->At, st->n, cs, rs, 0, s);
+  row_reduce_both(st, 0, 1.0, 1.0, rs, cs, s);
+  k_scale_factors_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
+      cs, st->D_c, st->c, st->lv, st->uv, st->n, rs, st->D_r, st->lc, st->uc, st->m);
+  scale_values_both(st, rs, cs, s);
   return (int)hipGetLastError();
 }
 
 int dpdlp_dev_pock_chambolle(dpdlp_state_t *st, double alpha, double *rs, double *cs,
                              void *stream) {
   hipStream_t s = ST(stream);
-  row_reduce(&st->A, st->m, 1, alpha, rs, s);
-  row_reduce(&st->At, st->n, 1, 2.0 - alpha, cs, s);
-  k_scale_factors<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(rs, st->D_r, st->m);
-  k_scale_factors<<<ew_grid(st->n), DPDLP_BS, 0, s>>>(cs, st->D_c, st->n);
-  k_scale_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
-      st->c, st->lv, st->uv, cs, st->n, st->lc, st->uc, rs, st->m);
-  scale_values(&st->A, st->m, rs, cs, 1, s);
-  scale_values(&st->At, st->n, cs, rs, 0, s);
+  row_reduce_both(st, 1, alpha, 2.0 - alpha, rs, cs, s);
+  k_scale_factors_vectors<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, s>>>(
+      cs, st->D_c, st->c, st->lv, st->uv, st->n, rs, st->D_r, st->lc, st->uc, st->m);
+  scale_values_both(st, rs, cs, s);
   return (int)hipGetLastError();
 }
 
@@ -900,9 +922,9 @@ int dpdlp_dev_bound_obj_sums(const dpdlp_state_t *st, void *stream) {
   return (int)finalize(st, g, DPDLP_SL_SCAL, 2, 0u, DPDLP_SL_SCAL, -1, -1, s);
 }
 
-int dpdlp_dev_bound_obj_apply(dpdlp_state_t *st, double beta, double gamma, void *stream) {
-  k_bound_obj_apply<<<ew_grid((long long)st->m + st->n), DPDLP_BS, 0, ST(stream)>>>(
-      st->lc, st->uc, st->m, st->lv, st->uv, st->c, st->n, beta, gamma);
+int dpdlp_dev_bound_obj_apply_scal(dpdlp_state_t *st, void *stream) {
+  k_bound_obj_apply_scal<<<ew_grid((long long)st->m + st->n), DPDLP_BS, 0, ST(stream)>>>(
+      st->lc, st->uc, st->m, st->lv, st->uv, st->c, st->n, st->scal);
   return (int)hipGetLastError();
 }
 

@@ -61,6 +61,7 @@ enum { NLIST = 5 };
 
 typedef struct {
   int stage; /* read and written with atomics: set-up threads move it */
+  hipEvent_t ev_ready; /* end of the set-up work on the set-up stream */
   long long tag;
   dpdlp_resident_t *R;
   int own_R;
@@ -214,6 +215,8 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   free(b->pool);
   for (int i = 0; i < b->ndead; ++i) dpdlp_resident_free(b->dead[i]);
   free(b->dead);
+  for (int i = 0; b->sl && i < b->cap; ++i)
+    if (b->sl[i].ev_ready) hipEventDestroy(b->sl[i].ev_ready);
   free(b->sl);
   void *dev[] = {b->d_lps, b->d_ctl_eval, b->d_ctl_win, b->d_scal,
                  b->d_list, b->d_rng,     b->d_part,    b->d_normals};
@@ -558,6 +561,16 @@ int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take
     return -1;
   }
   q->own_R = take_ownership;
+  /* the batch's stream waits for the set-up work (warm start, FP32 copies)
+   * before the LP's first kernels there */
+  if ((!q->ev_ready && hipEventCreateWithFlags(&q->ev_ready, hipEventDisableTiming) != hipSuccess) ||
+      hipEventRecord(q->ev_ready, s) != hipSuccess) {
+    dpdlp_set_error("set-up event failed");
+    dpdlp_i_ctx_free(&q->c);
+    memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
+    dpdlp_i_batch_unreserve(b, i);
+    return -1;
+  }
   pthread_mutex_lock(&b->mu);
   set_stage(q, S_READY);
   b->in_flight++;
@@ -584,6 +597,10 @@ static int activate_ready(dpdlp_batch_t *b) {
     if (stage_of(q) != S_READY) continue;
     dpdlp_resident_t *R = q->R;
     q->c.s = b->s; /* it may have been set up on another stream */
+    if (hipStreamWaitEvent(b->s, q->ev_ready, 0) != hipSuccess) {
+      dpdlp_set_error("waiting for an LP's set-up failed");
+      return -1;
+    }
     fill_blp(&b->h_lps[i], &q->c);
     q->t_pi0 = dpdlp_wall();
     q->sigma_sq = 1.0;

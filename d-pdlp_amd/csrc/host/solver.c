@@ -38,10 +38,12 @@ static void free_perm_copies(dpdlp_resident_t *r);
     }                                                                          \
   } while (0)
 
+/* bytes an array takes in an arena or a resident's blob (256-byte steps) */
+static size_t pad256(size_t bytes) { return ((bytes ? bytes : 8) + 255) & ~(size_t)255; }
+
 static void *dmalloc(ctx_t *c, size_t bytes) {
   void *p = NULL;
-  size_t need = (bytes ? bytes : 8);
-  need = (need + 255) & ~(size_t)255;
+  size_t need = pad256(bytes);
   if (c->arena && c->arena_off + need <= c->arena_len) {
     p = c->arena + c->arena_off;
     c->arena_off += need;
@@ -56,7 +58,7 @@ static void *dmalloc(ctx_t *c, size_t bytes) {
 /* zero-filled: arena memory is cleared once when the arena is made */
 static double *dzero(ctx_t *c, long long count) {
   size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(double);
-  int from_arena = c->arena && c->arena_off + ((bytes + 255) & ~(size_t)255) <= c->arena_len;
+  int from_arena = c->arena && c->arena_off + pad256(bytes) <= c->arena_len;
   double *p = (double *)dmalloc(c, bytes);
   if (p && !from_arena && hipMemsetAsync(p, 0, bytes, c->s) != hipSuccess) return NULL;
   return p;
@@ -201,8 +203,6 @@ void dpdlp_resident_free(dpdlp_resident_t *r) {
     }                                                                          \
   } while (0)
 
-/* bytes an array takes in a resident's blob */
-static size_t ublk(size_t bytes) { return ((bytes ? bytes : 8) + 255) & ~(size_t)255; }
 
 /* Carves a resident's device arrays out of one blob.  Without an uploader
  * the blob is allocated here and every array is copied directly; with one,
@@ -240,7 +240,7 @@ static int sink_open(usink_t *k, const dpdlp_uploader_t *u, size_t need) {
 }
 
 static hipError_t up(usink_t *k, void **dst, const void *src, size_t bytes) {
-  size_t step = ublk(bytes);
+  size_t step = pad256(bytes);
   if (k->off + step > k->cap) return hipErrorOutOfMemory;
   *dst = k->blob + k->off;
   if (bytes && src) {
@@ -546,16 +546,16 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
   /* every device array lives in one blob, carved in 256-byte steps in the
    * order below */
   {
-    size_t need = ublk(4 * ((size_t)m + 1)) + ublk(4 * (size_t)nnz) + ublk(8 * (size_t)nnz) +
-                  ublk(4 * ((size_t)n + 1)) + ublk(4 * (size_t)nnz) + ublk(8 * (size_t)nnz) +
-                  ublk(4 * ((size_t)r->A_nblk + 1)) + ublk(4 * ((size_t)r->At_nblk + 1)) +
-                  3 * ublk(8 * (size_t)n) + 2 * ublk(8 * (size_t)m);
+    size_t need = pad256(4 * ((size_t)m + 1)) + pad256(4 * (size_t)nnz) + pad256(8 * (size_t)nnz) +
+                  pad256(4 * ((size_t)n + 1)) + pad256(4 * (size_t)nnz) + pad256(8 * (size_t)nnz) +
+                  pad256(4 * ((size_t)r->A_nblk + 1)) + pad256(4 * ((size_t)r->At_nblk + 1)) +
+                  3 * pad256(8 * (size_t)n) + 2 * pad256(8 * (size_t)m);
     for (int which = 0; which < 2; ++which) {
       int P = which ? r->At_P : r->A_P, rows = which ? n : m;
       size_t nb = (size_t)(which ? r->At_nblk : r->A_nblk);
       if (P > 1)
-        need += ublk(4 * (nb * P + 1)) + ublk(2 * (size_t)P * ((size_t)rows + nb)) +
-                2 * ublk(4 * (size_t)nnz) + ublk(8 * (size_t)nnz);
+        need += pad256(4 * (nb * P + 1)) + pad256(2 * (size_t)P * ((size_t)rows + nb)) +
+                2 * pad256(4 * (size_t)nnz) + pad256(8 * (size_t)nnz);
     }
     if (sink_open(&K, u, need)) goto ufail;
     r->blob = K.blob;
@@ -861,7 +861,6 @@ size_t dpdlp_i_ctx_footprint(const dpdlp_resident_t *R, int mixed, int own_scal)
 int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params, int mixed,
                       hipStream_t stream, double *scal_dev, double *scal_host,
                       double *rescale_time) {
-  double *d_start = NULL;
   int m = R->m, n = R->n, nnz = R->nnz;
   dpdlp_state_t *st = &c->st;
   c->mixed = mixed;
@@ -920,11 +919,29 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
       goto fail;
     }
   }
-  st->c = dcopy(c, R->c, n);
-  st->lv = dcopy(c, R->lv, n);
-  st->uv = dcopy(c, R->uv, n);
-  st->lc = dcopy(c, R->lc, m);
-  st->uc = dcopy(c, R->uc, m);
+  /* c, l_v, u_v, l_c, u_c are laid out as in the resident's blob (same
+   * order and padding), so one copy brings all five */
+  st->c = (double *)dmalloc(c, (size_t)(n ? n : 1) * sizeof(double));
+  st->lv = (double *)dmalloc(c, (size_t)(n ? n : 1) * sizeof(double));
+  st->uv = (double *)dmalloc(c, (size_t)(n ? n : 1) * sizeof(double));
+  st->lc = (double *)dmalloc(c, (size_t)(m ? m : 1) * sizeof(double));
+  st->uc = (double *)dmalloc(c, (size_t)(m ? m : 1) * sizeof(double));
+  if (st->c && st->lv && st->uv && st->lc && st->uc) {
+    const char *rb = (const char *)R->c, *cb = (const char *)st->c;
+    if ((const char *)R->uc - rb == (const char *)st->uc - cb &&
+        (const char *)R->lv - rb == (const char *)st->lv - cb &&
+        (const char *)R->uv - rb == (const char *)st->uv - cb &&
+        (const char *)R->lc - rb == (const char *)st->lc - cb) {
+      HIPCHK(hipMemcpyAsync(st->c, R->c, (size_t)((const char *)R->uc - rb) + (size_t)m * 8,
+                            hipMemcpyDeviceToDevice, c->s));
+    } else {
+      HIPCHK(hipMemcpyAsync(st->c, R->c, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(st->lv, R->lv, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(st->uv, R->uv, (size_t)n * 8, hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(st->lc, R->lc, (size_t)m * 8, hipMemcpyDeviceToDevice, c->s));
+      HIPCHK(hipMemcpyAsync(st->uc, R->uc, (size_t)m * 8, hipMemcpyDeviceToDevice, c->s));
+    }
+  }
   st->D_r = (double *)dmalloc(c, (size_t)(m ? m : 1) * sizeof(double));
   st->D_c = (double *)dmalloc(c, (size_t)(n ? n : 1) * sizeof(double));
   st->x_init = dzero(c, n);
@@ -967,51 +984,54 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
 
   /* ---- scaling on device (preconditioner.c:197-273) ---- */
   double t_sc0 = dpdlp_wall();
-  HIPCHK(dpdlp_dev_fill(st->D_r, 1.0, m, c->s));
-  HIPCHK(dpdlp_dev_fill(st->D_c, 1.0, n, c->s));
+  if ((char *)st->D_c - (char *)st->D_r == (ptrdiff_t)pad256((size_t)m * 8)) {
+    /* adjacent: one fill covers both (and the padding between them) */
+    HIPCHK(dpdlp_dev_fill(st->D_r, 1.0, (int)(st->D_c - st->D_r) + n, c->s));
+  } else {
+    HIPCHK(dpdlp_dev_fill(st->D_r, 1.0, m, c->s));
+    HIPCHK(dpdlp_dev_fill(st->D_c, 1.0, n, c->s));
+  }
   for (int it = 0; it < params->l_inf_ruiz_iterations; ++it)
     HIPCHK(dpdlp_dev_ruiz_step(st, rs, cs, c->s));
   if (params->has_pock_chambolle_alpha)
     HIPCHK(dpdlp_dev_pock_chambolle(st, params->pock_chambolle_alpha, rs, cs, c->s));
+  /* beta and gamma are formed on the device; the host reads them at the one
+   * synchronisation that ends the scaling */
   if (params->bound_objective_rescaling) {
     HIPCHK(dpdlp_dev_bound_obj_sums(st, c->s));
-    if (sync_scal(c, 64)) {
-      dpdlp_set_error("device sync failed");
-      goto fail;
-    }
-    c->k.beta = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL]) + 1.0);
-    c->k.gamma = 1.0 / (sqrt(c->h_scal[DPDLP_SL_SCAL + 1]) + 1.0);
-    HIPCHK(dpdlp_dev_bound_obj_apply(st, c->k.beta, c->k.gamma, c->s));
-  } else {
-    c->k.beta = 1.0;
-    c->k.gamma = 1.0;
+    HIPCHK(dpdlp_dev_bound_obj_apply_scal(st, c->s));
   }
   /* scaled values in tile order */
   if (st->A.P > 1) HIPCHK(dpdlp_dev_permute(c->w_A_v, R->A_perm, (double *)st->A.pv, nnz, c->s));
   if (st->At.P > 1)
     HIPCHK(dpdlp_dev_permute(c->w_At_v, R->At_perm, (double *)st->At.pv, nnz, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
+  if (sync_scal(c, 64)) {
+    dpdlp_set_error("device sync failed");
+    goto fail;
+  }
+  if (params->bound_objective_rescaling) {
+    c->k.beta = c->h_scal[DPDLP_SL_SCAL + 2];
+    c->k.gamma = c->h_scal[DPDLP_SL_SCAL + 3];
+  } else {
+    c->k.beta = 1.0;
+    c->k.gamma = 1.0;
+  }
   *rescale_time = dpdlp_wall() - t_sc0;
 
-  /* warm start, pdlp_core_op.cu:174-201 */
+  /* warm start, pdlp_core_op.cu:174-201: the start vectors are staged in the
+   * power-iteration work vectors, which that iteration overwrites before
+   * reading them */
   if (R->h_pstart && n > 0) {
-    d_start = NULL;
-    HIPCHK(hipMalloc((void **)&d_start, (size_t)n * sizeof(double)));
-    HIPCHK(hipMemcpy(d_start, R->h_pstart, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(dpdlp_dev_warm_start(st->x_init, st->x_cur, st->x_pdhg, d_start, st->D_c, c->k.beta, n,
-                                c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    hipFree(d_start);
-    d_start = NULL;
+    HIPCHK(hipMemcpyAsync(c->pi_dual, R->h_pstart, (size_t)n * sizeof(double),
+                          hipMemcpyHostToDevice, c->s));
+    HIPCHK(dpdlp_dev_warm_start(st->x_init, st->x_cur, st->x_pdhg, c->pi_dual, st->D_c,
+                                c->k.beta, n, c->s));
   }
   if (R->h_dstart && m > 0) {
-    HIPCHK(hipMalloc((void **)&d_start, (size_t)m * sizeof(double)));
-    HIPCHK(hipMemcpy(d_start, R->h_dstart, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(dpdlp_dev_warm_start(st->y_init, st->y_cur, st->y_pdhg, d_start, st->D_r, c->k.gamma, m,
-                                c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    hipFree(d_start);
-    d_start = NULL;
+    HIPCHK(hipMemcpyAsync(c->pi_next, R->h_dstart, (size_t)m * sizeof(double),
+                          hipMemcpyHostToDevice, c->s));
+    HIPCHK(dpdlp_dev_warm_start(st->y_init, st->y_cur, st->y_pdhg, c->pi_next, st->D_r,
+                                c->k.gamma, m, c->s));
   }
 
   /* mixed precision: FP32 values and gather shadows */
@@ -1046,7 +1066,6 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
 
   return 0;
 fail:
-  if (d_start) hipFree(d_start);
   return -1;
 }
 

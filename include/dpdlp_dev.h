@@ -222,8 +222,10 @@ int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream)
 int dpdlp_dev_pock_chambolle(dpdlp_state_t *st, double alpha, double *rs,
                              double *cs, void *stream);
 int dpdlp_dev_bound_obj_sums(const dpdlp_state_t *st, void *stream);
-int dpdlp_dev_bound_obj_apply(dpdlp_state_t *st, double beta, double gamma,
-                              void *stream);
+/* Bound/objective rescaling with beta and gamma formed on the device from
+ * the sums dpdlp_dev_bound_obj_sums left in scal (and stored to
+ * scal[DPDLP_SL_SCAL + 2], scal[DPDLP_SL_SCAL + 3]): no host round trip. */
+int dpdlp_dev_bound_obj_apply_scal(dpdlp_state_t *st, void *stream);
 /* FP32 copies of the scaled values (mixed precision). */
 int dpdlp_dev_make_f32(const double *src, float *dst, long long count, void *stream);
 /* Polishing helpers: v <- 0 where finite (pdlp_core_op.cu:1136-1143). */
