@@ -536,8 +536,8 @@ void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int i) {
 /* Sets up a reserved slot (any thread, on `stream`; NULL = the batch's
  * stream): the same scaled problem a single-LP solve would build.  The slot
  * joins at the next window start. */
-int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take_ownership,
-                          long long tag, void *stream) {
+int dpdlp_i_batch_prepare_issue(dpdlp_batch_t *b, int i, dpdlp_resident_t *R,
+                                int take_ownership, long long tag, void *stream) {
   if (R->cap != b->o.spmv_cap || R->A_P != 1 || R->At_P != 1 || R->h_row_perm) {
     dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
                     b->o.spmv_cap);
@@ -549,23 +549,31 @@ int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take
   q->t_add = dpdlp_wall();
   q->tag = tag;
   q->R = R;
+  q->own_R = take_ownership;
   hipStream_t s = stream ? (hipStream_t)stream : b->s;
   take_arena(b, dpdlp_i_ctx_footprint(R, b->o.precision == DPDLP_PREC_MIXED, 0), &q->c.arena,
              &q->c.arena_cap);
-  if (dpdlp_i_ctx_setup(&q->c, R, &b->prm, b->o.precision == DPDLP_PREC_MIXED, s,
-                        b->d_scal + (size_t)i * DPDLP_SCAL_LEN,
-                        b->h_scal + (size_t)i * DPDLP_SCAL_LEN, &q->rescale_time)) {
+  if (dpdlp_i_ctx_setup_issue(&q->c, R, &b->prm, b->o.precision == DPDLP_PREC_MIXED, s,
+                              b->d_scal + (size_t)i * DPDLP_SCAL_LEN,
+                              b->h_scal + (size_t)i * DPDLP_SCAL_LEN)) {
     dpdlp_i_ctx_free(&q->c);
     memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
     dpdlp_i_batch_unreserve(b, i);
     return -1;
   }
-  q->own_R = take_ownership;
-  /* the batch's stream waits for the set-up work (warm start, FP32 copies)
-   * before the LP's first kernels there */
-  if ((!q->ev_ready && hipEventCreateWithFlags(&q->ev_ready, hipEventDisableTiming) != hipSuccess) ||
+  return 0;
+}
+
+int dpdlp_i_batch_prepare_complete(dpdlp_batch_t *b, int i) {
+  bslot_t *q = &b->sl[i];
+  hipStream_t s = q->c.s;
+  /* the batch's stream waits for the rest of the set-up (warm start, FP32
+   * copies) before the LP's first kernels there */
+  if (dpdlp_i_ctx_setup_complete(&q->c, q->R, &b->prm, &q->rescale_time) ||
+      (!q->ev_ready &&
+       hipEventCreateWithFlags(&q->ev_ready, hipEventDisableTiming) != hipSuccess) ||
       hipEventRecord(q->ev_ready, s) != hipSuccess) {
-    dpdlp_set_error("set-up event failed");
+    if (!dpdlp_last_error()[0]) dpdlp_set_error("set-up event failed");
     dpdlp_i_ctx_free(&q->c);
     memset(&q->tag, 0, sizeof(*q) - offsetof(bslot_t, tag));
     dpdlp_i_batch_unreserve(b, i);
@@ -576,6 +584,26 @@ int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take
   b->in_flight++;
   pthread_mutex_unlock(&b->mu);
   return 0;
+}
+
+/* drops an issued set-up whose stream has been synchronised */
+void dpdlp_i_batch_abandon(dpdlp_batch_t *b, int i) {
+  dpdlp_i_ctx_free(&b->sl[i].c);
+  memset(&b->sl[i].tag, 0, sizeof(bslot_t) - offsetof(bslot_t, tag));
+  dpdlp_i_batch_unreserve(b, i);
+}
+
+int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int i, dpdlp_resident_t *R, int take_ownership,
+                          long long tag, void *stream) {
+  if (dpdlp_i_batch_prepare_issue(b, i, R, take_ownership, tag, stream)) return -1;
+  if (hipStreamSynchronize(b->sl[i].c.s) != hipSuccess) {
+    dpdlp_set_error("device sync failed during set-up");
+    dpdlp_i_ctx_free(&b->sl[i].c);
+    memset(&b->sl[i].tag, 0, sizeof(bslot_t) - offsetof(bslot_t, tag));
+    dpdlp_i_batch_unreserve(b, i);
+    return -1;
+  }
+  return dpdlp_i_batch_prepare_complete(b, i);
 }
 
 /* READY slots join (main thread, at a window start): power iteration start
@@ -1412,15 +1440,29 @@ typedef struct {
   int done, err;
 } bloader_t;
 
-/* sets the LPs up on its own stream while the batch already steps */
+/* sets the LPs up on its own stream while the batch already steps: set-ups
+ * are issued in groups and completed after one synchronisation per group */
+enum { LOAD_GROUP = 32 };
+
 static void *solve_loader(void *arg) {
   bloader_t *L = (bloader_t *)arg;
   int err = 0;
-  for (int i = 0; i < L->count; ++i) {
-    int slot = dpdlp_i_batch_reserve(L->b);
-    if (slot < 0 || dpdlp_i_batch_prepare(L->b, slot, L->res[i], 0, i, L->s)) {
-      err = 1;
-      break;
+  for (int i0 = 0; i0 < L->count && !err; i0 += LOAD_GROUP) {
+    int slots[LOAD_GROUP], ns = 0;
+    for (int i = i0; i < L->count && i < i0 + LOAD_GROUP; ++i) {
+      int slot = dpdlp_i_batch_reserve(L->b);
+      if (slot < 0 || dpdlp_i_batch_prepare_issue(L->b, slot, L->res[i], 0, i, L->s)) {
+        err = 1;
+        break;
+      }
+      slots[ns++] = slot;
+    }
+    if (hipStreamSynchronize((hipStream_t)L->s) != hipSuccess) err = 1;
+    for (int k = 0; k < ns; ++k) {
+      if (err)
+        dpdlp_i_batch_abandon(L->b, slots[k]);
+      else if (dpdlp_i_batch_prepare_complete(L->b, slots[k]))
+        err = 1;
     }
   }
   __atomic_store_n(&L->err, err, __ATOMIC_RELEASE);

@@ -861,6 +861,16 @@ size_t dpdlp_i_ctx_footprint(const dpdlp_resident_t *R, int mixed, int own_scal)
 int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params, int mixed,
                       hipStream_t stream, double *scal_dev, double *scal_host,
                       double *rescale_time) {
+  if (dpdlp_i_ctx_setup_issue(c, R, params, mixed, stream, scal_dev, scal_host)) return -1;
+  if (hipStreamSynchronize(c->s) != hipSuccess) {
+    dpdlp_set_error("device sync failed");
+    return -1;
+  }
+  return dpdlp_i_ctx_setup_complete(c, R, params, rescale_time);
+}
+
+int dpdlp_i_ctx_setup_issue(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params,
+                            int mixed, hipStream_t stream, double *scal_dev, double *scal_host) {
   int m = R->m, n = R->n, nnz = R->nnz;
   dpdlp_state_t *st = &c->st;
   c->mixed = mixed;
@@ -1005,10 +1015,18 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
   if (st->A.P > 1) HIPCHK(dpdlp_dev_permute(c->w_A_v, R->A_perm, (double *)st->A.pv, nnz, c->s));
   if (st->At.P > 1)
     HIPCHK(dpdlp_dev_permute(c->w_At_v, R->At_perm, (double *)st->At.pv, nnz, c->s));
-  if (sync_scal(c, 64)) {
-    dpdlp_set_error("device sync failed");
-    goto fail;
-  }
+  /* the scalars come back with the stream's next synchronisation */
+  HIPCHK(hipMemcpyAsync(c->h_scal, st->scal, 64 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  c->t_issue = t_sc0;
+  return 0;
+fail:
+  return -1;
+}
+
+int dpdlp_i_ctx_setup_complete(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params,
+                               double *rescale_time) {
+  int m = R->m, n = R->n, nnz = R->nnz;
+  dpdlp_state_t *st = &c->st;
   if (params->bound_objective_rescaling) {
     c->k.beta = c->h_scal[DPDLP_SL_SCAL + 2];
     c->k.gamma = c->h_scal[DPDLP_SL_SCAL + 3];
@@ -1016,7 +1034,7 @@ int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *pa
     c->k.beta = 1.0;
     c->k.gamma = 1.0;
   }
-  *rescale_time = dpdlp_wall() - t_sc0;
+  *rescale_time = dpdlp_wall() - c->t_issue;
 
   /* warm start, pdlp_core_op.cu:174-201: the start vectors are staged in the
    * power-iteration work vectors, which that iteration overwrites before

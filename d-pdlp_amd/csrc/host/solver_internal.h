@@ -31,6 +31,7 @@ typedef struct {
   hipEvent_t ev[64][3];
   int nev_created, nev_used;
   double p1_ms, p2_ms;
+  double t_issue; /* wall time the set-up was issued */
   int nsamp;
 } ctx_t;
 
@@ -41,6 +42,14 @@ typedef struct {
 int dpdlp_i_ctx_setup(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params, int mixed,
                       hipStream_t stream, double *scal_dev, double *scal_host,
                       double *rescale_time);
+/* The same in two halves: issue queues every set-up operation on the stream
+ * (the scalars' copy back included) without waiting; complete, called once
+ * the stream has been synchronised, reads beta/gamma and queues the rest
+ * (warm start, FP32 copies). */
+int dpdlp_i_ctx_setup_issue(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params,
+                            int mixed, hipStream_t stream, double *scal_dev, double *scal_host);
+int dpdlp_i_ctx_setup_complete(ctx_t *c, dpdlp_resident_t *R, const pdhg_parameters_t *params,
+                               double *rescale_time);
 /* Unscaled result of a finished LP (D2H, synchronous on c->s). */
 cupdlpx_result_t *dpdlp_i_ctx_result(ctx_t *c, const hstate_t *h, double rescale_time,
                                      double polish_time, int polish_iters);
@@ -65,6 +74,13 @@ int dpdlp_i_batch_reserve(dpdlp_batch_t *b);
 void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int slot);
 int dpdlp_i_batch_prepare(dpdlp_batch_t *b, int slot, dpdlp_resident_t *R, int take_ownership,
                           long long tag, void *stream);
+/* dpdlp_i_batch_prepare in two halves around a synchronisation of the
+ * set-up stream (many issues may share one synchronisation); abandon drops
+ * an issued set-up after that synchronisation instead of completing it */
+int dpdlp_i_batch_prepare_issue(dpdlp_batch_t *b, int slot, dpdlp_resident_t *R,
+                                int take_ownership, long long tag, void *stream);
+int dpdlp_i_batch_prepare_complete(dpdlp_batch_t *b, int slot);
+void dpdlp_i_batch_abandon(dpdlp_batch_t *b, int slot);
 /* frees retired problems and trims pooled arenas to keep_bytes (call from a
  * set-up thread: device frees may synchronise the device) */
 void dpdlp_i_batch_reap(dpdlp_batch_t *b, size_t keep_bytes);
