@@ -20,6 +20,8 @@
 #include "dev_common.h"
 #include "dpdlp_dev.h"
 
+/* a row block's rows map one-to-one onto a workgroup's threads */
+static_assert(DPDLP_BLOCK_ROWS == DPDLP_BS, "row-block plan and SpMV workgroup disagree");
 
 static inline int ew_grid(long long n) { return dpdlp_ew_grid(n); }
 
@@ -651,28 +653,6 @@ __global__ void k_warm_start(double *a, double *b, double *c, const double *star
   } while (0)
 
 extern "C" {
-
-int dpdlp_plan_blocks(int rows, const int *rp, int cap, int *blk) {
-  int nb = 0, r = 0;
-  while (r < rows) {
-    int start = r;
-    long long len = (long long)rp[r + 1] - rp[r];
-    if (len > cap) {
-      r++;
-    } else {
-      long long nz = 0;
-      while (r < rows && r - start < DPDLP_BS) {
-        long long l = (long long)rp[r + 1] - rp[r];
-        if (l > cap || nz + l > cap) break;
-        nz += l;
-        r++;
-      }
-    }
-    blk[nb++] = start;
-  }
-  blk[nb] = rows;
-  return nb;
-}
 
 int dpdlp_dev_spmv(const dpdlp_csr_t *A, const double *x, double *y, void *stream) {
   EpiStore e{y, 0};

@@ -1,10 +1,34 @@
 /*
- * Host construction of the XCD column-panel tiles (layout documented at
- * dpdlp_csr_t in include/dpdlp_dev.h).
+ * Host planning of the SpMV layouts: row blocks and the XCD column-panel
+ * tiles (layout documented at dpdlp_csr_t in include/dpdlp_dev.h).
  */
 #include "dpdlp_dev.h"
 
 #include <stdlib.h>
+
+/* Consecutive rows per block: at most DPDLP_BLOCK_ROWS rows and cap nonzeros; a row
+ * longer than cap gets a block of its own. */
+int dpdlp_plan_blocks(int rows, const int *rp, int cap, int *blk) {
+  int nb = 0, r = 0;
+  while (r < rows) {
+    int start = r;
+    long long len = (long long)rp[r + 1] - rp[r];
+    if (len > cap) {
+      r++;
+    } else {
+      long long nz = 0;
+      while (r < rows && r - start < DPDLP_BLOCK_ROWS) {
+        long long l = (long long)rp[r + 1] - rp[r];
+        if (l > cap || nz + l > cap) break;
+        nz += l;
+        r++;
+      }
+    }
+    blk[nb++] = start;
+  }
+  blk[nb] = rows;
+  return nb;
+}
 
 int dpdlp_build_panels(int rows, int cols, const int *rp, const int *ci, const int *blk,
                        int nblk, int P, int *toff, unsigned short *rofs, int *perm, int *pci) {

@@ -150,6 +150,9 @@ enum {
  * of their own, other blocks hold <= 256 rows and <= cap nonzeros.  Writes
  * blk[0..nblk] and returns nblk. */
 int dpdlp_plan_blocks(int rows, const int *rp_host, int cap, int *blk_out);
+/* rows per row block of dpdlp_plan_blocks: one per thread of the SpMV's
+ * 256-thread workgroup */
+#define DPDLP_BLOCK_ROWS 256
 
 /* Host construction of the column-panel tiles of a CSR matrix with a given
  * block plan: toff[nblk*P+1], rofs[P*(rows+nblk)], perm[nnz] (tiled position
