@@ -76,6 +76,20 @@ out:
   return ms;
 }
 
+int dpdlp_resident_transpose(const dpdlp_resident_t *r, int *t_rp, int *t_ci, double *t_v) {
+  if (!r || !t_rp || !t_ci || !t_v || r->At_P > 1) return -1;
+  if (hipMemcpy(t_rp, r->At_rp, (size_t)(r->n + 1) * sizeof(int), hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return -1;
+  if (r->nnz > 0 &&
+      (hipMemcpy(t_ci, r->At_ci, (size_t)r->nnz * sizeof(int), hipMemcpyDeviceToHost) !=
+           hipSuccess ||
+       hipMemcpy(t_v, r->At_v, (size_t)r->nnz * sizeof(double), hipMemcpyDeviceToHost) !=
+           hipSuccess))
+    return -1;
+  return 0;
+}
+
 int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, double *yh,
                         int use32) {
   if (!r || !xh || !yh) return -1;

@@ -81,6 +81,8 @@ double dpdlp_wall(void);
 /* transpose.c: stable CSR transpose on the host */
 int dpdlp_host_transpose(int m, int n, const int *rp, const int *ci, const double *v,
                          int *t_rp, int *t_ci, double *t_v);
+/* A''s row pointers only (the device transposes the entries) */
+void dpdlp_host_col_ptr(int m, int n, const int *rp, const int *ci, int *t_rp);
 
 /* rng.cc: std::mt19937(seed) + std::normal_distribution<double>(0, 1) */
 void dpdlp_normal_fill(uint32_t seed, int count, double *out);

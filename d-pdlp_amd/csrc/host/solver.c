@@ -497,17 +497,6 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     uc_u = uc2;
   }
 
-  t_rp = (int *)malloc((size_t)(n + 1) * sizeof(int));
-  t_ci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
-  t_v = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
-  blk = (int *)malloc((size_t)((m > n ? m : n) + 2) * sizeof(int));
-  if (!t_rp || !t_ci || !t_v || !blk) {
-    dpdlp_set_error("host allocation failed");
-    goto ufail;
-  }
-  if (dpdlp_host_transpose(m, n, rp_u, ci_u, v_u, t_rp, t_ci, t_v))
-    goto ufail;
-
   /* XCD column-panel tiles */
   r->A_P = PA > 0 ? PA : pick_panels(n);
   r->At_P = PAt > 0 ? PAt : pick_panels(m);
@@ -521,6 +510,25 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
       }
     }
   }
+
+  /* A': an untiled A' is transposed on the device after the upload (the
+   * host only counts columns for its row pointers); A' tiles are cut on the
+   * host from a host transpose */
+  const int dev_tr = r->At_P == 1;
+  t_rp = (int *)malloc((size_t)(n + 1) * sizeof(int));
+  blk = (int *)malloc((size_t)((m > n ? m : n) + 2) * sizeof(int));
+  if (!dev_tr) {
+    t_ci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+    t_v = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+  }
+  if (!t_rp || !blk || (!dev_tr && (!t_ci || !t_v))) {
+    dpdlp_set_error("host allocation failed");
+    goto ufail;
+  }
+  if (dev_tr)
+    dpdlp_host_col_ptr(m, n, rp_u, ci_u, t_rp);
+  else if (dpdlp_host_transpose(m, n, rp_u, ci_u, v_u, t_rp, t_ci, t_v))
+    goto ufail;
   /* row-block plans (host copies kept for the batch's work lists) */
   for (int which = 0; which < 2; ++which) {
     int P = which ? r->At_P : r->A_P, rows = which ? n : m, cols = which ? m : n;
@@ -619,6 +627,26 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
   UPCHK(up(&K, (void **)&r->lc, lc_u, (size_t)m * sizeof(double)));
   UPCHK(up(&K, (void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
   UPCHK(sink_close(&K));
+  if (dev_tr && nnz > 0) {
+    size_t sb = dpdlp_dev_transpose_scratch_bytes(nnz), scap = sb;
+    char *sc = NULL;
+    if (u)
+      sc = u->get_blob(u->ctx, sb, &scap);
+    else if (hipMalloc((void **)&sc, sb) != hipSuccess)
+      sc = NULL;
+    if (!sc) {
+      dpdlp_set_error("device allocation failed (transpose scratch)");
+      goto ufail;
+    }
+    hipError_t e = (hipError_t)dpdlp_dev_transpose(m, n, nnz, r->A_rp, r->A_ci, r->A_v,
+                                                   r->At_ci, r->At_v, sc, K.s);
+    if (e == hipSuccess) e = hipStreamSynchronize(K.s);
+    if (u)
+      u->put_blob(u->ctx, sc, scap);
+    else
+      hipFree(sc);
+    UPCHK(e);
+  }
   free(t_rp);
   free(t_ci);
   free(t_v);

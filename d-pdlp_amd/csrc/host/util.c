@@ -28,6 +28,13 @@ double dpdlp_wall(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+/* Row pointers of A' (column counts of A, exclusive prefix). */
+void dpdlp_host_col_ptr(int m, int n, const int *rp, const int *ci, int *t_rp) {
+  memset(t_rp, 0, (size_t)(n + 1) * sizeof(int));
+  for (int k = 0; k < rp[m]; ++k) t_rp[ci[k] + 1]++;
+  for (int j = 0; j < n; ++j) t_rp[j + 1] += t_rp[j];
+}
+
 /* Counting-sort transpose; entries of each output row keep increasing source
  * row order (the layout csr2csc produces at pdlp_core_op.cu:111-128). */
 int dpdlp_host_transpose(int m, int n, const int *rp, const int *ci, const double *v,

@@ -57,6 +57,7 @@ def _load() -> C.CDLL:
         "dpdlp_copy_calibration": (C.c_double, [C.c_longlong, C.c_int]),
         "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
+        "dpdlp_resident_transpose": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
         "dpdlp_upload_ex": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int]),
         "dpdlp_resident_bytes": (C.c_size_t, [C.c_void_p]),
         "dpdlp_batch_member_spec": (None, [C.c_int, C.c_uint64, C.c_int, C.c_int,
@@ -341,6 +342,16 @@ class Resident:
         if rc != 0:
             raise RuntimeError("spmv failed: " + last_error())
         return y
+
+    def transpose(self):
+        """The stored unscaled A' as (row_ptr, col_ind, vals) host arrays."""
+        rp = np.zeros(self.n + 1, dtype=np.int32)
+        ci = np.zeros(max(self.nnz, 1), dtype=np.int32)
+        v = np.zeros(max(self.nnz, 1))
+        rc = lib.dpdlp_resident_transpose(self.h, _ptr(rp, C.c_int), _ptr(ci, C.c_int), _ptr(v))
+        if rc != 0:
+            raise RuntimeError("A' is stored as tiles only or the copy failed")
+        return rp, ci[: self.nnz], v[: self.nnz]
 
     def time_spmv(self, transpose=False, reps=20, use32=False):
         return lib.dpdlp_time_spmv(self.h, int(transpose), reps, int(use32))

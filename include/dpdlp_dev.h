@@ -10,6 +10,8 @@
  */
 #pragma once
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -219,6 +221,14 @@ int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next,
 /* Result of pdlp_core_op.cu:789-859: reduced cost into rc, unscaled x, y. */
 int dpdlp_dev_result(const dpdlp_state_t *st, double beta, double gamma,
                      double *x_out, double *y_out, double *rc_out, void *stream);
+
+/* Stable transpose on the device (csr2csc, pdlp_core_op.cu:93-130): fills
+ * t_ci/t_v of A' (row pointers from the host's column counts) from A's CSR.
+ * t_v's storage doubles as the sort's position buffers; scratch needs
+ * dpdlp_dev_transpose_scratch_bytes(nnz) bytes. */
+size_t dpdlp_dev_transpose_scratch_bytes(long long nnz);
+int dpdlp_dev_transpose(int m, int n, long long nnz, const int *rp, const int *ci,
+                        const double *v, int *t_ci, double *t_v, void *scratch, void *stream);
 
 /* Scaling on device (preconditioner.c:89-273).  rs/cs are work vectors. */
 int dpdlp_dev_ruiz_step(dpdlp_state_t *st, double *rs, double *cs, void *stream);

@@ -91,6 +91,11 @@ int dpdlp_resident_spmv(dpdlp_resident_t *res, int transpose, const double *x, d
  * mean kernel time in ms (reference kernel for counter calibration). */
 double dpdlp_copy_calibration(long long bytes, int reps);
 
+/* Copies a resident problem's unscaled A' (CSR: n+1 row pointers, nnz row
+ * indices of A, nnz values) to the host, in its stored row order; -1 when
+ * A' is stored as column-panel tiles only (for transpose tests). */
+int dpdlp_resident_transpose(const dpdlp_resident_t *res, int *t_rp, int *t_ci, double *t_v);
+
 /* Problem sizes of a resident problem (m, n, nnz, row blocks of A and A'). */
 void dpdlp_resident_info(const dpdlp_resident_t *res, int *m, int *n, int *nnz,
                          int *nblk_A, int *nblk_At);

@@ -158,3 +158,41 @@ def test_parity_trace_with_panels(dp, orc, panels):
         assert rel(tx[i], ox[i]) < 1e-8 and rel(ty[i], oy[i]) < 1e-8, iters[i]
     assert abs(res.primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
         1.0, abs(ref["primal_objective_value"]))
+
+
+def _stable_transpose(rp, ci, v, n):
+    """A' by a stable sort of the row-major nonzeros on their column."""
+    rows = np.repeat(np.arange(len(rp) - 1, dtype=np.int32), np.diff(rp))
+    order = np.argsort(ci, kind="stable")
+    t_rp = np.zeros(n + 1, dtype=np.int32)
+    np.cumsum(np.bincount(ci, minlength=n), out=t_rp[1:])
+    return t_rp, rows[order], v[order]
+
+
+@pytest.mark.parametrize("case", ["uniform", "zipf", "poisson", "wide", "one_col", "duplicates"])
+def test_device_transpose_matches_stable_sort(dp, case):
+    """The untiled A' is transposed on the device (radix sort of the nonzero
+    positions by column): bit for bit the stable counting-sort transpose,
+    including the 1-, 2- and 3-pass key widths, empty rows and columns and a
+    repeated (row, column) entry."""
+    if case == "uniform":
+        prob, _ = dp.generate(700, 300, 6000, row_rule=0, seed=5)        # 2 passes
+    elif case == "zipf":
+        prob, _ = dp.generate(5000, 70000, 90000, row_rule=1, seed=6)    # 3 passes, long rows
+    elif case == "poisson":
+        prob, _ = dp.generate(3000, 200, 0, row_rule=2, seed=7)          # 1 pass
+    elif case == "wide":
+        prob, _ = dp.generate(300, 20000, 0, row_rule=2, poisson_lambda=40.0, seed=8)
+    elif case == "one_col":
+        prob = dp.Problem.from_csr(4, 1, [0, 1, 1, 2, 3], [0, 0, 0], [1.0, 2.0, 3.0],
+                                   c=np.ones(1), var_lb=np.zeros(1), var_ub=np.ones(1))
+    else:
+        prob = dp.Problem.from_csr(3, 4, [0, 3, 3, 5], [2, 0, 2, 3, 2], [1.0, 2.0, 3.0, 4.0, 5.0],
+                                   c=np.ones(4), var_lb=np.zeros(4), var_ub=np.ones(4))
+    a = prob.arrays()
+    R = dp.Resident(prob, spmv_cap=1024, panels=(1, 1))
+    got = R.transpose()
+    want = _stable_transpose(a["row_ptr"], a["col_ind"], a["vals"], prob.n)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+    R.free()
