@@ -538,7 +538,7 @@ void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int i) {
  * joins at the next window start. */
 int dpdlp_i_batch_prepare_issue(dpdlp_batch_t *b, int i, dpdlp_resident_t *R,
                                 int take_ownership, long long tag, void *stream) {
-  if (R->cap != b->o.spmv_cap || R->A_P != 1 || R->At_P != 1 || R->h_row_perm) {
+  if (R->cap != b->o.spmv_cap || R->At_cap != b->o.spmv_cap || R->A_P != 1 || R->At_P != 1 || R->h_row_perm) {
     dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
                     b->o.spmv_cap);
     dpdlp_i_batch_unreserve(b, i);
@@ -1484,7 +1484,7 @@ int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parame
   if (local.max_lps < count) local.max_lps = count;
   for (int i = 0; i < count; ++i) results[i] = NULL;
   for (int i = 0; i < count; ++i)
-    if (!res[i] || res[i]->cap != local.spmv_cap || res[i]->A_P != 1 || res[i]->At_P != 1 ||
+    if (!res[i] || res[i]->cap != local.spmv_cap || res[i]->At_cap != local.spmv_cap || res[i]->A_P != 1 || res[i]->At_P != 1 ||
         res[i]->h_row_perm) {
       dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
                       local.spmv_cap);

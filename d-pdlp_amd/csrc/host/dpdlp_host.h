@@ -13,7 +13,8 @@ extern "C" {
 
 /* A problem resident in HBM (unscaled), see dpdlp_upload(). */
 struct dpdlp_resident {
-  int m, n, nnz, cap;
+  int m, n, nnz, cap; /* cap: row-block capacity of A */
+  int At_cap;         /* of A' */
   int *A_rp, *A_ci, *At_rp, *At_ci, *A_blk, *At_blk;
   int A_nblk, At_nblk;
   double *A_v, *At_v;

@@ -132,7 +132,7 @@ void dpdlp_resident_csr(const dpdlp_resident_t *r, int transpose, void *out, dou
                        r->cap, r->A_P, r->A_toff, r->A_rofs, r->A_pci, r->A_pv, NULL, xpart};
   } else {
     *A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
-                       r->At_nblk, r->cap, r->At_P, r->At_toff, r->At_rofs, r->At_pci, r->At_pv,
+                       r->At_nblk, r->At_cap, r->At_P, r->At_toff, r->At_rofs, r->At_pci, r->At_pv,
                        NULL, xpart};
   }
 }
@@ -511,6 +511,11 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     }
   }
 
+  /* an untiled A' of a large problem with an automatic block size takes
+   * 4096-nonzero blocks (its rows, the columns of A, are short and many) */
+  r->At_cap = r->cap;
+  if (spmv_cap <= 0 && r->At_P == 1 && nnz >= 4000000) r->At_cap = 4096;
+
   /* A': an untiled A' is transposed on the device after the upload (the
    * host only counts columns for its row pointers); A' tiles are cut on the
    * host from a host transpose */
@@ -533,9 +538,9 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
   for (int which = 0; which < 2; ++which) {
     int P = which ? r->At_P : r->A_P, rows = which ? n : m, cols = which ? m : n;
     const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
-    int nb = P > 1 ? dpdlp_plan_blocks_panels(rows, cols, hrp, hci, r->cap, P, DPDLP_PANEL_RMAX,
-                                              blk)
-                   : dpdlp_plan_blocks(rows, hrp, r->cap, blk);
+    int cap = which ? r->At_cap : r->cap;
+    int nb = P > 1 ? dpdlp_plan_blocks_panels(rows, cols, hrp, hci, cap, P, DPDLP_PANEL_RMAX, blk)
+                   : dpdlp_plan_blocks(rows, hrp, cap, blk);
     int *h = (int *)malloc((size_t)(nb + 1) * sizeof(int));
     if (!h) {
       dpdlp_set_error("host allocation failed");
@@ -1223,7 +1228,8 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     dpdlp_set_error("termination_evaluation_frequency must be positive");
     return NULL;
   }
-  if (opts->use_graph && R->A_P == 1 && R->At_P == 1 && opts->trace_points <= 0 &&
+  if (opts->use_graph && R->A_P == 1 && R->At_P == 1 && R->At_cap == R->cap &&
+      opts->trace_points <= 0 &&
       !opts->profile_phases) {
     /* the windowed engine with a single LP: per-iteration parameters live in
      * device memory, so each window of iterations replays one hipGraph */
