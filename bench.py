@@ -411,6 +411,15 @@ def main():
         nbytes = 256 << 20
         cms = dpdlp.lib.dpdlp_copy_calibration(nbytes, 5)
         copy_GBps = 2 * nbytes / (cms * 1e-3) / 1e9
+    # what bounds the phases: every nonzero gathers one vector entry per
+    # launch; the ceiling is the measured rate of L2-resident random gathers
+    gceil = dpdlp.lib.dpdlp_gather_calibration(2 << 20, 20_000_000, 10)
+    gather = dict(gathers_per_launch=nnz,
+                  phase1_G_per_s=nnz / (p1 * 1e-3) / 1e9, phase2_G_per_s=nnz / (p2 * 1e-3) / 1e9,
+                  ceiling_G_per_s=gceil if gceil > 0 else None,
+                  ceiling_kernel="random 8-byte gathers, every workgroup within a 2 MiB "
+                                 "slice of its own XCD (L2-resident)",
+                  dominant_frac=(nnz / (dom[2] * 1e-3) / 1e9) / gceil if gceil > 0 else None)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "cfg1_fp64_hbm_traffic.json")
     if args.workload == "cfg1" and not args.mixed and os.path.exists(tfile):
@@ -452,6 +461,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom[0], "achieved": achieved,
                      "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": traffic, "algorithmic_bytes_per_launch": dom[1]},
+        "gather_bound": gather,
         "copy_kernel_GBps": copy_GBps,
         "cpu_baseline": cpu,
         "xcd_panels": R.panels,

@@ -629,6 +629,28 @@ __global__ __launch_bounds__(DPDLP_BS) void k_copy_calib(const double4 *__restri
     dst[i] = src[i];
 }
 
+/* random 8-byte gathers, L per thread, every workgroup from the slice of x
+ * that belongs to its XCD (workgroup w runs on XCD w mod 8): the rate of
+ * L2-resident random gathers, the ceiling of the SpMVs' gather streams */
+__global__ __launch_bounds__(DPDLP_BS) void k_gather_calib(const int *__restrict__ idx,
+                                                          const double *__restrict__ x,
+                                                          double *__restrict__ y, long long rows,
+                                                          int L, long long slice) {
+  const long long i = (long long)blockIdx.x * DPDLP_BS + threadIdx.x;
+  if (i >= rows) return;
+  const int *p = idx + i * L;
+  const double *xs = x + (long long)(blockIdx.x & 7) * slice;
+  double acc = 0.0;
+  for (int k = 0; k + 9 < L; k += 10) {
+    double v[10];
+#pragma unroll
+    for (int u = 0; u < 10; ++u) v[u] = xs[__builtin_nontemporal_load(p + k + u)];
+#pragma unroll
+    for (int u = 0; u < 10; ++u) acc += v[u];
+  }
+  y[i] = acc;
+}
+
 /* warm start: v = start * D * s into init, cur and pdhg (pdlp_core_op.cu:174-201) */
 __global__ void k_warm_start(double *a, double *b, double *c, const double *start,
                              const double *D, double s, int count) {
@@ -937,6 +959,14 @@ int dpdlp_dev_permute_f32(const float *src, const int *perm, float *dst, long lo
                           void *stream) {
   if (count <= 0) return 0;
   k_permute<float><<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(src, perm, dst, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long rows, int L,
+                           long long slice, void *stream) {
+  if (rows <= 0 || L % 10 != 0) return (int)hipErrorInvalidValue;
+  long long g = (rows + DPDLP_BS - 1) / DPDLP_BS;
+  k_gather_calib<<<(unsigned)g, DPDLP_BS, 0, ST(stream)>>>(idx, x, y, rows, L, slice);
   return (int)hipGetLastError();
 }
 

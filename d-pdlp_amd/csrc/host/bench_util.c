@@ -7,6 +7,7 @@
 #include "dpdlp_host.h"
 
 #include <hip/hip_runtime_api.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -151,6 +152,51 @@ out:
   if (xpart) hipFree(xpart);
   if (s) hipStreamDestroy(s);
   return rc;
+}
+
+double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps) {
+  const int L = 20;
+  long long slice = slice_bytes / 8, rows = gathers / L;
+  if (slice < 1 || rows < 1 || reps < 1) return -1.0;
+  int *h = (int *)malloc((size_t)(rows * L) * sizeof(int));
+  int *idx = NULL;
+  double *x = NULL, *y = NULL;
+  hipEvent_t e0 = NULL, e1 = NULL;
+  double rate = -1.0;
+  if (!h) return -1.0;
+  uint64_t st = 0x9e3779b97f4a7c15ull; /* xorshift64: fixed random columns */
+  for (long long k = 0; k < rows * L; ++k) {
+    st ^= st << 13;
+    st ^= st >> 7;
+    st ^= st << 17;
+    h[k] = (int)(st % (uint64_t)slice);
+  }
+  if (hipMalloc((void **)&idx, (size_t)(rows * L) * sizeof(int)) != hipSuccess) goto out;
+  if (hipMalloc((void **)&x, (size_t)(8 * slice) * sizeof(double)) != hipSuccess) goto out;
+  if (hipMalloc((void **)&y, (size_t)rows * sizeof(double)) != hipSuccess) goto out;
+  if (hipMemcpy(idx, h, (size_t)(rows * L) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(x, 0, (size_t)(8 * slice) * sizeof(double)) != hipSuccess)
+    goto out;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto out;
+  if (dpdlp_dev_gather_calib(idx, x, y, rows, L, slice, NULL)) goto out;
+  if (hipEventRecord(e0, NULL) != hipSuccess) goto out;
+  for (int k = 0; k < reps; ++k)
+    if (dpdlp_dev_gather_calib(idx, x, y, rows, L, slice, NULL)) goto out;
+  if (hipEventRecord(e1, NULL) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) goto out;
+  {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, e0, e1) == hipSuccess && t > 0.f)
+      rate = (double)(rows * L) * reps / ((double)t * 1e-3) / 1e9;
+  }
+out:
+  hipDeviceSynchronize();
+  free(h);
+  if (idx) hipFree(idx);
+  if (x) hipFree(x);
+  if (y) hipFree(y);
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  return rate;
 }
 
 double dpdlp_copy_calibration(long long bytes, int reps) {

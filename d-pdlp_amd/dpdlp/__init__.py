@@ -55,6 +55,7 @@ def _load() -> C.CDLL:
         "dpdlp_time_spmv": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "dpdlp_generate": (P, [C.POINTER(GenSpec), _dp]),
         "dpdlp_copy_calibration": (C.c_double, [C.c_longlong, C.c_int]),
+        "dpdlp_gather_calibration": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int]),
         "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
         "dpdlp_resident_transpose": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),

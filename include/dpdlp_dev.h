@@ -244,6 +244,11 @@ int dpdlp_dev_make_f32(const double *src, float *dst, long long count, void *str
 /* Polishing helpers: v <- 0 where finite (pdlp_core_op.cu:1136-1143). */
 int dpdlp_dev_zero_finite(double *v, int count, void *stream);
 int dpdlp_dev_fill(double *v, double value, long long count, void *stream);
+/* y[i] = sum of x[slice_base + idx[i*L + k]], k < L (L a multiple of 10),
+ * where workgroup w reads the slice (w mod 8) * slice of x: random
+ * L2-resident gathers (gather-rate calibration). */
+int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long rows, int L,
+                           long long slice, void *stream);
 /* Plain device copy of `bytes` (multiple of 32): the reference point that
  * scales the FETCH_SIZE / WRITE_SIZE counters to bytes. */
 int dpdlp_dev_copy_calib(const void *src, void *dst, long long bytes, void *stream);

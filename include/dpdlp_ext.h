@@ -87,6 +87,11 @@ double dpdlp_time_spmv(dpdlp_resident_t *res, int transpose, int reps, int use32
 int dpdlp_resident_spmv(dpdlp_resident_t *res, int transpose, const double *x, double *y,
                         int use32);
 
+/* Rate (G gathers/s) of `gathers` random 8-byte reads in which every
+ * workgroup reads only the slice_bytes slice of its own XCD (L2-resident
+ * for slice_bytes up to a few MB): the ceiling of an SpMV's gathers. */
+double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps);
+
 /* Copies `bytes` device-to-device with a plain kernel `reps` times; returns the
  * mean kernel time in ms (reference kernel for counter calibration). */
 double dpdlp_copy_calibration(long long bytes, int reps);
