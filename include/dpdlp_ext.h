@@ -118,8 +118,8 @@ typedef struct {
   double mixed_switch_tol; /* mixed: FP64 from this max relative KKT on */
   int spmv_cap;            /* row-block capacity of every LP (default 4096) */
   int profile_phases;      /* 1: HIP events around the two phase kernels of
-                              round 1 of every 4th batched window, power
-                              iteration beside it included
+                              the first iteration of every 4th batched
+                              window, power iteration beside it included
                               (dpdlp_batch_phase_stats) */
   int use_graph;           /* 1: capture each window's launches in a hipGraph
                               and replay it while the batch keeps its shape */
