@@ -40,7 +40,7 @@ WORKLOADS = {
                  desc="batch of 512 LPs: m log-uniform [500, 2e5], n = m*U[1.5,3], "
                       "Poisson(8)+1 nnz/row, seeds 3000..3511, eps 1e-4"),
     "cfg4": dict(rates=(50.0, 200.0, 800.0), large_frac=0.05, large_m=1_000_000,
-                 large_nnz=10_000_000, budget_gb=200.0,
+                 large_nnz=10_000_000, budget_gb=200.0, pool_small=64,
                  desc="serving: Poisson arrivals, LPs from the batch generator + 5% "
                       "m=1e6 nnz=1e7, 200 GB admission, eps 1e-4"),
 }
@@ -198,6 +198,7 @@ def bench_batch(args, dpdlp):
 
 
 def bench_serve(args, dpdlp):
+    import numpy as np
     w = WORKLOADS["cfg4"]
     params = dpdlp.default_params(presolve=False)
     rates = [float(x) for x in args.rates.split(",")] if args.rates else list(w["rates"])
@@ -210,8 +211,19 @@ def bench_serve(args, dpdlp):
                 params, rate=rate, duration_sec=args.serve_duration, drain_sec=args.serve_drain,
                 seed=4000, mode=mode, precision=dpdlp.PREC_MIXED if mode == 0 else dpdlp.PREC_FP64,
                 large_frac=w["large_frac"], large_m=w["large_m"], large_nnz=w["large_nnz"],
+                pool_small=w["pool_small"],
                 hbm_budget_gb=w["budget_gb"], max_lps=args.serve_max_lps)
             rep["wall_s"] = time.perf_counter() - t
+            # per request class (pool entries past the small pool are the
+            # m = 1e6 requests): arrival -> result latency, inf = unfinished
+            small = np.asarray(idx) < w["pool_small"]
+            lat = np.asarray(lat)
+            for cls, sel in (("small", small), ("large", ~small)):
+                lc = lat[sel]
+                done = np.isfinite(lc)
+                rep[cls] = {"arrived": int(sel.sum()), "completed": int(done.sum()),
+                            "p50_sec": float(np.percentile(lc, 50)) if lc.size else None,
+                            "p99_sec": float(np.percentile(lc, 99)) if lc.size else None}
             row[name] = rep
             print(f"# rate {rate:.0f}/s {name}: arrived {rep['arrived']} done {rep['completed']} "
                   f"{rep['lps_per_sec']:.1f} LPs/s p50 {rep['p50_sec']:.2f}s p99 {rep['p99_sec']:.2f}s "
