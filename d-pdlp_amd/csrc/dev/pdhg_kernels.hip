@@ -629,6 +629,62 @@ __global__ __launch_bounds__(DPDLP_BS) void k_copy_calib(const double4 *__restri
     dst[i] = src[i];
 }
 
+/* Standalone reductions (get_vector_inf_norm, get_vector_sum and the cuBLAS
+ * nrm2 / dot calls, utils.cu:616-637): partials of a fixed grid, combined by
+ * k_reduce_final in a fixed order; nrm2 accumulates Blue's three scaled sums,
+ * so it neither overflows nor underflows. */
+__global__ __launch_bounds__(DPDLP_BS) void k_reduce_partials(int op, const double *__restrict__ x,
+                                                             const double *__restrict__ y,
+                                                             long long n,
+                                                             double *__restrict__ part, int nb) {
+  __shared__ double red[4 * 4];
+  double s[3] = {0.0, 0.0, 0.0};
+  double mx[1] = {0.0};
+  for (long long i = (long long)blockIdx.x * DPDLP_BS + threadIdx.x; i < n;
+       i += (long long)gridDim.x * DPDLP_BS) {
+    const double v = x[i];
+    if (op == DPDLP_RED_NRM2)
+      blue_acc(v, s[0], s[1], s[2]);
+    else if (op == DPDLP_RED_DOT)
+      s[0] += v * y[i];
+    else if (op == DPDLP_RED_SUM)
+      s[0] += v;
+    else
+      mx[0] = fmax(mx[0], fabs(v));
+  }
+  block_partials<3, 1>(s, mx, red, part, nb, 0);
+}
+
+__global__ __launch_bounds__(DPDLP_BS) void k_reduce_final(int op, const double *__restrict__ part,
+                                                          int nb, double *__restrict__ out) {
+  __shared__ double red[4];
+  double v[4];
+  for (int k = 0; k < 4; ++k) {
+    const double *p = part + (size_t)k * nb;
+    double acc = 0.0;
+    if (k == 3) {
+      for (int i = threadIdx.x; i < nb; i += DPDLP_BS) acc = fmax(acc, p[i]);
+      acc = wave_max(acc);
+    } else {
+      for (int i = threadIdx.x; i < nb; i += DPDLP_BS) acc += p[i];
+      acc = wave_sum(acc);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    v[k] = k == 3 ? fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))
+                  : (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (op == DPDLP_RED_NRM2)
+      out[0] = blue_combine(v[0], v[1], v[2]);
+    else if (op == DPDLP_RED_AMAX)
+      out[0] = v[3];
+    else
+      out[0] = v[0];
+  }
+}
+
 /* random 8-byte gathers, L per thread, every workgroup from the slice of x
  * that belongs to its XCD (workgroup w runs on XCD w mod 8): the rate of
  * L2-resident random gathers, the ceiling of the SpMVs' gather streams */
@@ -959,6 +1015,24 @@ int dpdlp_dev_permute_f32(const float *src, const int *perm, float *dst, long lo
                           void *stream) {
   if (count <= 0) return 0;
   k_permute<float><<<ew_grid(count), DPDLP_BS, 0, ST(stream)>>>(src, perm, dst, count);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_reduce_blocks(long long n) {
+  long long g = (n + DPDLP_BS - 1) / DPDLP_BS;
+  if (g < 1) g = 1;
+  if (g > 1024) g = 1024;
+  return (int)g;
+}
+
+int dpdlp_dev_reduce(int op, const double *x, const double *y, long long n, double *part,
+                     double *out, void *stream) {
+  if (op < DPDLP_RED_NRM2 || op > DPDLP_RED_SUM || n < 0 || !part || !out ||
+      (n > 0 && (!x || (op == DPDLP_RED_DOT && !y))))
+    return (int)hipErrorInvalidValue;
+  const int nb = dpdlp_dev_reduce_blocks(n);
+  k_reduce_partials<<<nb, DPDLP_BS, 0, ST(stream)>>>(op, x, y, n, part, nb);
+  k_reduce_final<<<1, DPDLP_BS, 0, ST(stream)>>>(op, part, nb, out);
   return (int)hipGetLastError();
 }
 

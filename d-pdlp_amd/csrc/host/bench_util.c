@@ -154,6 +154,31 @@ out:
   return rc;
 }
 
+int dpdlp_reduce_host(int op, const double *xh, const double *yh, long long n, double *out) {
+  double *x = NULL, *y = NULL, *part = NULL, *o = NULL;
+  int rc = -1;
+  size_t bytes = (size_t)(n > 0 ? n : 1) * sizeof(double);
+  if (!out || n < 0) return -1;
+  if (hipMalloc((void **)&x, bytes) != hipSuccess || hipMalloc((void **)&y, bytes) != hipSuccess ||
+      hipMalloc((void **)&part, 4 * (size_t)dpdlp_dev_reduce_blocks(n) * sizeof(double)) !=
+          hipSuccess ||
+      hipMalloc((void **)&o, sizeof(double)) != hipSuccess)
+    goto out;
+  if (n > 0 && (hipMemcpy(x, xh, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+                (yh && hipMemcpy(y, yh, (size_t)n * sizeof(double), hipMemcpyHostToDevice) !=
+                           hipSuccess)))
+    goto out;
+  if (dpdlp_dev_reduce(op, x, y, n, part, o, NULL) != 0) goto out;
+  if (hipMemcpy(out, o, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) goto out;
+  rc = 0;
+out:
+  if (x) hipFree(x);
+  if (y) hipFree(y);
+  if (part) hipFree(part);
+  if (o) hipFree(o);
+  return rc;
+}
+
 double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps) {
   const int L = 20;
   long long slice = slice_bytes / 8, rows = gathers / L;

@@ -56,6 +56,7 @@ def _load() -> C.CDLL:
         "dpdlp_generate": (P, [C.POINTER(GenSpec), _dp]),
         "dpdlp_copy_calibration": (C.c_double, [C.c_longlong, C.c_int]),
         "dpdlp_gather_calibration": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int]),
+        "dpdlp_reduce_host": (C.c_int, [C.c_int, _dp, _dp, C.c_longlong, _dp]),
         "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
         "dpdlp_resident_transpose": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
@@ -520,6 +521,19 @@ def batch_member(k, base_seed=3000, m_lo=500, m_hi=200_000):
     opt = C.c_double(0.0)
     ptr = lib.dpdlp_generate(C.byref(spec), C.byref(opt))
     return Problem(ptr), opt.value
+
+
+RED_NRM2, RED_DOT, RED_AMAX, RED_SUM = 0, 1, 2, 3
+
+
+def reduce(op, x, y=None):
+    """One device reduction (dpdlp_dev_reduce) of host vectors."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
+    out = C.c_double(0.0)
+    if lib.dpdlp_reduce_host(op, _ptr(x), _ptr(y), len(x), C.byref(out)) != 0:
+        raise RuntimeError("reduction failed: " + last_error())
+    return out.value
 
 
 def last_error() -> str:

@@ -244,6 +244,16 @@ int dpdlp_dev_make_f32(const double *src, float *dst, long long count, void *str
 /* Polishing helpers: v <- 0 where finite (pdlp_core_op.cu:1136-1143). */
 int dpdlp_dev_zero_finite(double *v, int count, void *stream);
 int dpdlp_dev_fill(double *v, double value, long long count, void *stream);
+/* Standalone reductions into out[0] (device memory): ||x||_2 (Blue's scaled
+ * sums: no overflow or underflow), x.y, max |x_i|, sum x_i; 0 for n = 0.
+ * Partials of a fixed grid combined in a fixed order (deterministic).  part
+ * needs 4 * dpdlp_dev_reduce_blocks(n) doubles.  Replace cublasDnrm2 /
+ * cublasDdot / get_vector_inf_norm / get_vector_sum (utils.cu:616-637). */
+enum { DPDLP_RED_NRM2 = 0, DPDLP_RED_DOT = 1, DPDLP_RED_AMAX = 2, DPDLP_RED_SUM = 3 };
+int dpdlp_dev_reduce_blocks(long long n);
+int dpdlp_dev_reduce(int op, const double *x, const double *y, long long n, double *part,
+                     double *out, void *stream);
+
 /* y[i] = sum of x[slice_base + idx[i*L + k]], k < L (L a multiple of 10),
  * where workgroup w reads the slice (w mod 8) * slice of x: random
  * L2-resident gathers (gather-rate calibration). */

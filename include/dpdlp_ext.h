@@ -87,6 +87,10 @@ double dpdlp_time_spmv(dpdlp_resident_t *res, int transpose, int reps, int use32
 int dpdlp_resident_spmv(dpdlp_resident_t *res, int transpose, const double *x, double *y,
                         int use32);
 
+/* One dpdlp_dev_reduce (op DPDLP_RED_*) of host vectors of length n into
+ * *out (y is read for the dot product only); for tests. */
+int dpdlp_reduce_host(int op, const double *x, const double *y, long long n, double *out);
+
 /* Rate (G gathers/s) of `gathers` random 8-byte reads in which every
  * workgroup reads only the slice_bytes slice of its own XCD (L2-resident
  * for slice_bytes up to a few MB): the ceiling of an SpMV's gathers. */

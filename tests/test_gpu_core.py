@@ -196,3 +196,33 @@ def test_device_transpose_matches_stable_sort(dp, case):
     for g, w in zip(got, want):
         assert np.array_equal(g, w)
     R.free()
+
+
+def test_device_reductions(dp):
+    """Standalone device reductions against exactly rounded host references:
+    nrm2 (Blue's scaled sums) without overflow at 1e200 or underflow at
+    1e-200, dot, sum, max |x|; n = 0 gives 0; repeated calls give the same
+    bits (fixed combination order)."""
+    import math
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(1_000_003)
+    y = rng.standard_normal(1_000_003)
+    nrm = dp.reduce(dp.RED_NRM2, x)
+    assert abs(nrm - math.sqrt(math.fsum(x * x))) <= 1e-14 * nrm
+    dot = dp.reduce(dp.RED_DOT, x, y)
+    ref = math.fsum(x * y)
+    assert abs(dot - ref) <= 1e-12 * math.fsum(np.abs(x * y))
+    sm = dp.reduce(dp.RED_SUM, x)
+    assert abs(sm - math.fsum(x)) <= 1e-12 * math.fsum(np.abs(x))
+    assert dp.reduce(dp.RED_AMAX, -np.abs(x)) == np.max(np.abs(x))
+    assert dp.reduce(dp.RED_NRM2, x) == nrm and dp.reduce(dp.RED_DOT, x, y) == dot
+    for scale in (1e200, 1e-200):
+        v = np.full(1000, scale)
+        r = dp.reduce(dp.RED_NRM2, v)
+        assert np.isfinite(r) and r > 0
+        assert abs(r - scale * math.sqrt(1000.0)) <= 1e-14 * scale * math.sqrt(1000.0)
+    mixed = np.array([1e200, 3.0, -4.0, 1e-200])
+    assert abs(dp.reduce(dp.RED_NRM2, mixed) - 1e200) <= 1e-15 * 1e200
+    for op in (dp.RED_NRM2, dp.RED_SUM, dp.RED_AMAX):
+        assert dp.reduce(op, np.zeros(0)) == 0.0
+    assert dp.reduce(dp.RED_DOT, np.zeros(0), np.zeros(0)) == 0.0
