@@ -5,6 +5,7 @@
 #include "dpdlp_dev.h"
 #include "dpdlp_ext.h"
 #include "dpdlp_host.h"
+#include "solver_internal.h"
 
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
@@ -151,6 +152,35 @@ out:
   if (pv32) hipFree(pv32);
   if (xpart) hipFree(xpart);
   if (s) hipStreamDestroy(s);
+  return rc;
+}
+
+int dpdlp_scaling_host(const lp_problem_t *p, const pdhg_parameters_t *params, double *val,
+                       double *c, double *lv, double *uv, double *lc, double *uc, double *D_r,
+                       double *D_c, double *beta_gamma) {
+  dpdlp_resident_t *R = dpdlp_upload_ex(p, 0, 1, 1);
+  if (!R) return -1;
+  ctx_t cx;
+  memset(&cx, 0, sizeof(cx));
+  double rt = 0.0;
+  int rc = -1;
+  const size_t m = (size_t)R->m, n = (size_t)R->n, nnz = (size_t)R->nnz;
+  if (dpdlp_i_ctx_setup(&cx, R, params, 0, NULL, NULL, NULL, &rt)) goto out;
+  if ((nnz && hipMemcpy(val, cx.w_A_v, nnz * 8, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (n && (hipMemcpy(c, cx.st.c, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(lv, cx.st.lv, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(uv, cx.st.uv, n * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(D_c, cx.st.D_c, n * 8, hipMemcpyDeviceToHost) != hipSuccess)) ||
+      (m && (hipMemcpy(lc, cx.st.lc, m * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(uc, cx.st.uc, m * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(D_r, cx.st.D_r, m * 8, hipMemcpyDeviceToHost) != hipSuccess)))
+    goto out;
+  beta_gamma[0] = cx.k.beta;
+  beta_gamma[1] = cx.k.gamma;
+  rc = 0;
+out:
+  dpdlp_i_ctx_free(&cx);
+  dpdlp_resident_free(R);
   return rc;
 }
 

@@ -57,6 +57,7 @@ def _load() -> C.CDLL:
         "dpdlp_copy_calibration": (C.c_double, [C.c_longlong, C.c_int]),
         "dpdlp_gather_calibration": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int]),
         "dpdlp_reduce_host": (C.c_int, [C.c_int, _dp, _dp, C.c_longlong, _dp]),
+        "dpdlp_scaling_host": (C.c_int, [P, C.POINTER(PdhgParameters)] + [_dp] * 9),
         "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
         "dpdlp_resident_transpose": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
@@ -521,6 +522,26 @@ def batch_member(k, base_seed=3000, m_lo=500, m_hi=200_000):
     opt = C.c_double(0.0)
     ptr = lib.dpdlp_generate(C.byref(spec), C.byref(opt))
     return Problem(ptr), opt.value
+
+
+def device_scaling(problem: Problem, params: PdhgParameters):
+    """The device scaling of a solve, copied back (dpdlp_scaling_host): dict of
+    scaled values (CSR order), c, lv, uv, lc, uc, Dr, Dc, beta, gamma."""
+    m, n, nnz = problem.m, problem.n, problem.nnz
+    out = dict(val=np.zeros(max(nnz, 1)), c=np.zeros(max(n, 1)), lv=np.zeros(max(n, 1)),
+               uv=np.zeros(max(n, 1)), lc=np.zeros(max(m, 1)), uc=np.zeros(max(m, 1)),
+               Dr=np.zeros(max(m, 1)), Dc=np.zeros(max(n, 1)))
+    bg = np.zeros(2)
+    rc = lib.dpdlp_scaling_host(problem.ptr, C.byref(params), _ptr(out["val"]), _ptr(out["c"]),
+                                _ptr(out["lv"]), _ptr(out["uv"]), _ptr(out["lc"]),
+                                _ptr(out["uc"]), _ptr(out["Dr"]), _ptr(out["Dc"]), _ptr(bg))
+    if rc != 0:
+        raise RuntimeError("device scaling failed: " + last_error())
+    for k, length in (("val", nnz), ("c", n), ("lv", n), ("uv", n), ("lc", m), ("uc", m),
+                      ("Dr", m), ("Dc", n)):
+        out[k] = out[k][:length]
+    out["beta"], out["gamma"] = float(bg[0]), float(bg[1])
+    return out
 
 
 RED_NRM2, RED_DOT, RED_AMAX, RED_SUM = 0, 1, 2, 3

@@ -87,6 +87,13 @@ double dpdlp_time_spmv(dpdlp_resident_t *res, int transpose, int reps, int use32
 int dpdlp_resident_spmv(dpdlp_resident_t *res, int transpose, const double *x, double *y,
                         int use32);
 
+/* The device scaling of a solve (Ruiz, Pock-Chambolle, bound/objective as
+ * params ask) of p in its given row order, copied back: scaled A values (CSR
+ * order), c, l_v, u_v, l_c, u_c, D_r, D_c and {beta, gamma}; for tests. */
+int dpdlp_scaling_host(const lp_problem_t *p, const pdhg_parameters_t *params, double *val,
+                       double *c, double *lv, double *uv, double *lc, double *uc, double *D_r,
+                       double *D_c, double *beta_gamma);
+
 /* One dpdlp_dev_reduce (op DPDLP_RED_*) of host vectors of length n into
  * *out (y is read for the dot product only); for tests. */
 int dpdlp_reduce_host(int op, const double *x, const double *y, long long n, double *out);

@@ -226,3 +226,60 @@ def test_device_reductions(dp):
     for op in (dp.RED_NRM2, dp.RED_SUM, dp.RED_AMAX):
         assert dp.reduce(op, np.zeros(0)) == 0.0
     assert dp.reduce(dp.RED_DOT, np.zeros(0), np.zeros(0)) == 0.0
+
+
+def _oracle_rescale(orc, prob, **kw):
+    import ctypes as C
+    prm = orc.default_params(**kw)
+    m, n, nnz = prob.m, prob.n, prob.nnz
+    out = dict(val=np.zeros(nnz), c=np.zeros(n), lv=np.zeros(n), uv=np.zeros(n), lc=np.zeros(m),
+               uc=np.zeros(m), Dr=np.zeros(m), Dc=np.zeros(n))
+    beta, gamma = C.c_double(), C.c_double()
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    orc.lib().orc_rescale(C.byref(prm), prob.ptr, P(out["val"]), P(out["c"]), P(out["lv"]),
+                          P(out["uv"]), P(out["lc"]), P(out["uc"]), P(out["Dr"]), P(out["Dc"]),
+                          C.byref(beta), C.byref(gamma))
+    out["beta"], out["gamma"] = beta.value, gamma.value
+    return out
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(l_inf_ruiz_iterations=0, has_pock_chambolle_alpha=False),
+                                dict(bound_objective_rescaling=False)],
+                         ids=["default", "bound_obj_only", "no_bound_obj"])
+@pytest.mark.parametrize("shape", ["uniform", "zipf", "empty_rows"])
+def test_device_scaling_bitwise_equals_oracle(dp, orc, kw, shape):
+    """The device's Ruiz and Pock-Chambolle scaling gives the oracle's
+    (preconditioner.c) scaled matrix, D_r and D_c bit for bit: maxima are
+    exact, sums of |a| run in storage order over A and A' as the host
+    accumulates them, and the factors are correctly rounded square roots and
+    quotients.  The bound/objective step's two sums of squares are reduced in
+    the device's fixed tree order instead of the host's sequential one, so
+    beta, gamma and the vectors they scale agree to 1e-14 relative (the
+    reordered sums of ~1e4 squares; bit for bit when that step is off)."""
+    if shape == "uniform":
+        prob, _ = dp.generate(600, 1400, 9000, row_rule=0, seed=21)
+    elif shape == "zipf":
+        prob, _ = dp.generate(2000, 5000, 40000, row_rule=1, pm1_frac=0.1, seed=22)
+    else:
+        prob = dp.Problem.from_csr(4, 3, [0, 2, 2, 3, 3], [0, 2, 1], [2.0, -3.0, 0.5],
+                                   c=np.array([1.0, 0.0, -2.0]),
+                                   con_lb=np.array([1.0, -np.inf, 0.0, 2.0]),
+                                   con_ub=np.array([4.0, np.inf, 0.0, np.inf]),
+                                   var_lb=np.array([0.0, -np.inf, -1.0]),
+                                   var_ub=np.array([np.inf, 3.0, 1.0]))
+    params = dp.default_params(presolve=False, **kw)
+    dev = dp.device_scaling(prob, params)
+    ref = _oracle_rescale(orc, prob, presolve=False, **kw)
+    for k in ("val", "Dr", "Dc"):
+        assert np.array_equal(dev[k], ref[k]), k
+    if params.bound_objective_rescaling:
+        for k in ("beta", "gamma"):
+            assert abs(dev[k] - ref[k]) <= 1e-14 * abs(ref[k]), k
+        for k in ("c", "lv", "uv", "lc", "uc"):
+            fin = np.isfinite(ref[k])
+            assert np.array_equal(np.isfinite(dev[k]), fin), k
+            assert np.all(np.abs(dev[k][fin] - ref[k][fin]) <= 1e-14 * np.abs(ref[k][fin])), k
+    else:
+        for k in ("c", "lv", "uv", "lc", "uc"):
+            assert np.array_equal(dev[k], ref[k]), k
+        assert dev["beta"] == ref["beta"] == 1.0 and dev["gamma"] == ref["gamma"] == 1.0
