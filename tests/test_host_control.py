@@ -1,0 +1,147 @@
+"""The library's host decisions, called directly (CPU, no device): the
+termination truth table (utils.cu:214-263), the adaptive restart test
+(utils.cu:265-293) and the PID primal-weight update of a restart
+(pdlp_core_op.cu:593-619), against expectations worked out by hand from
+those rules (strict '<' for optimality, the OPTIMAL > PRIMAL_INF > DUAL_INF >
+ITERATION > TIME order, the 0.2 / 0.5 / 0.36 restart thresholds, the PID
+guard branches)."""
+import ctypes as C
+import math
+
+import pytest
+
+
+class HState(C.Structure):
+    """Mirror of hstate_t (d-pdlp_amd/csrc/host/control.h)."""
+    _fields_ = ([(n, C.c_double) for n in ("step", "omega", "best_omega", "pw_err_sum",
+                                            "pw_last_err", "best_gap")]
+                + [(n, C.c_int) for n in ("total", "inner", "is_major")]
+                + [(n, C.c_double) for n in ("fpe", "fpe0", "fpe_last", "abs_p", "rel_p",
+                                              "abs_d", "rel_d", "pobj", "dobj", "gap", "rel_gap",
+                                              "max_p_ray_inf", "max_d_ray_inf", "p_ray_lin_obj",
+                                              "d_ray_obj")]
+                + [("reason", C.c_int), ("t_cum", C.c_double), ("pend_rst", C.c_int),
+                   ("pend_maj", C.c_int), ("m32_best", C.c_double), ("m32_best_T", C.c_int)])
+
+
+OPTIMAL, PRIMAL_INF, DUAL_INF, TIME, ITER = 1, 2, 3, 5, 6
+
+
+@pytest.fixture(scope="module")
+def ctl(dp):
+    L = dp.lib
+    L.ctl_check_termination.argtypes = [C.POINTER(HState), C.POINTER(dp.TerminationCriteria)]
+    L.ctl_should_restart.argtypes = [C.POINTER(HState), C.POINTER(dp.RestartParams), C.c_int]
+    L.ctl_should_restart.restype = C.c_int
+    L.ctl_pid_update.argtypes = [C.POINTER(HState), C.POINTER(dp.RestartParams), C.c_double,
+                                 C.c_double]
+    return L
+
+
+def _criteria(dp, **kw):
+    tc = dp.default_params().termination_criteria
+    for k, v in kw.items():
+        setattr(tc, k, v)
+    return tc
+
+
+def _term(ctl, dp, tc, **st):
+    h = HState()
+    h.rel_p = h.rel_d = h.rel_gap = 1.0
+    h.d_ray_obj = 0.0
+    h.p_ray_lin_obj = 0.0
+    h.total = 0
+    h.t_cum = 0.0
+    for k, v in st.items():
+        setattr(h, k, v)
+    ctl.ctl_check_termination(C.byref(h), C.byref(tc))
+    return h.reason
+
+
+def test_termination_truth_table(dp, ctl):
+    tc = _criteria(dp, eps_optimal_relative=1e-4, eps_feasible_relative=1e-4,
+                   eps_infeasible=1e-10, iteration_limit=1000, time_sec_limit=60.0)
+    # optimality: all three strictly below their tolerances
+    assert _term(ctl, dp, tc, rel_p=9e-5, rel_d=9e-5, rel_gap=9e-5) == OPTIMAL
+    assert _term(ctl, dp, tc, rel_p=1e-4, rel_d=9e-5, rel_gap=9e-5) == 0     # '<', not '<='
+    assert _term(ctl, dp, tc, rel_p=9e-5, rel_d=9e-5, rel_gap=2e-4) == 0
+    # primal infeasibility: dual ray objective > 0 and max infeasibility / objective <= eps
+    assert _term(ctl, dp, tc, d_ray_obj=2.0, max_d_ray_inf=2e-10) == PRIMAL_INF
+    assert _term(ctl, dp, tc, d_ray_obj=2.0, max_d_ray_inf=3e-10) == 0
+    assert _term(ctl, dp, tc, d_ray_obj=0.0, max_d_ray_inf=0.0) == 0       # needs > 0
+    # dual infeasibility: primal ray objective < 0, ratio against its magnitude
+    assert _term(ctl, dp, tc, p_ray_lin_obj=-4.0, max_p_ray_inf=4e-10) == DUAL_INF
+    assert _term(ctl, dp, tc, p_ray_lin_obj=-4.0, max_p_ray_inf=5e-10) == 0
+    assert _term(ctl, dp, tc, p_ray_lin_obj=0.0, max_p_ray_inf=0.0) == 0
+    # limits: '>='
+    assert _term(ctl, dp, tc, total=1000) == ITER
+    assert _term(ctl, dp, tc, total=999) == 0
+    assert _term(ctl, dp, tc, t_cum=60.0) == TIME
+    # order: optimal over infeasible over limits; iteration limit over time
+    assert _term(ctl, dp, tc, rel_p=0, rel_d=0, rel_gap=0, d_ray_obj=1.0, total=5000,
+                 t_cum=99.0) == OPTIMAL
+    assert _term(ctl, dp, tc, d_ray_obj=1.0, p_ray_lin_obj=-1.0) == PRIMAL_INF
+    assert _term(ctl, dp, tc, p_ray_lin_obj=-1.0, total=5000) == DUAL_INF
+    assert _term(ctl, dp, tc, total=5000, t_cum=99.0) == ITER
+
+
+def test_restart_sequence(dp, ctl):
+    rp = dp.default_params().restart_params      # 0.36 artificial, 0.2 sufficient, 0.5 necessary
+    freq = 200
+    h = HState()
+    h.fpe0 = 1.0
+    h.fpe_last = math.inf
+    steps = [  # (total, inner, fpe, expected)
+        (100, 100, 0.9, 0),     # before the first evaluation frequency: never
+        (200, 200, 0.9, 1),     # at it: forced
+        (400, 50, 0.6, 0),      # no criterion (0.6 > 0.5, inner < 0.36 T)
+        (600, 60, 0.45, 0),     # necessary reduction but still improving (0.45 < 0.6)
+        (800, 80, 0.48, 1),     # necessary reduction and no progress (0.48 > 0.45)
+        (1000, 100, 0.19, 1),   # sufficient reduction
+        (1200, 432, 0.9, 1),    # artificial: inner >= 0.36 * 1200 = 432
+        (1400, 503, 0.9, 0),    # 503 < 0.36 * 1400 = 504
+    ]
+    for total, inner, fpe, want in steps:
+        h.total, h.inner, h.fpe = total, inner, fpe
+        assert ctl.ctl_should_restart(C.byref(h), C.byref(rp), freq) == want, total
+        assert h.fpe_last == fpe      # the trial value is remembered every time
+
+
+def _pid(rp, st, pd, dd):
+    """The PID step written out from pdlp_core_op.cu:593-619."""
+    ratio = st["rel_d"] / st["rel_p"]
+    if 1e-16 < pd < 1e12 and 1e-16 < dd < 1e12 and 1e-8 < ratio < 1e8:
+        err = math.log(dd) - math.log(pd) - math.log(st["omega"])
+        st["pw_err_sum"] = st["pw_err_sum"] * rp.i_smooth + err
+        derr = err - st["pw_last_err"]
+        st["omega"] *= math.exp(rp.k_p * err + rp.k_i * st["pw_err_sum"] + rp.k_d * derr)
+        st["pw_last_err"] = err
+    else:
+        st["omega"] = st["best_omega"]
+        st["pw_err_sum"] = 0.0
+        st["pw_last_err"] = 0.0
+    g = abs(math.log10(st["rel_d"] / st["rel_p"]))
+    if g < st["best_gap"]:
+        st["best_gap"] = g
+        st["best_omega"] = st["omega"]
+
+
+def test_pid_primal_weight_sequence(dp, ctl):
+    rp = dp.default_params().restart_params      # k_p 0.99, k_i 0.01, k_d 0, i_smooth 0.3
+    st = dict(omega=1.0, best_omega=1.0, pw_err_sum=0.0, pw_last_err=0.0, best_gap=math.inf)
+    h = HState(**st)
+    seq = [  # (rel_p, rel_d, primal distance, dual distance)
+        (1e-2, 1e-3, 2.0, 8.0),
+        (5e-3, 4e-3, 1.5, 0.5),
+        (1e-3, 1e-3, 3.0, 3.0),
+        (1e-3, 1e-12, 1.0, 1.0),    # ratio 1e-9: guard -> best weight, sums reset
+        (1e-4, 2e-4, 1e-17, 1.0),   # primal distance below 1e-16: guard
+        (2e-4, 1e-4, 0.7, 2.2),
+    ]
+    for rel_p, rel_d, pd, dd in seq:
+        h.rel_p, h.rel_d = rel_p, rel_d
+        st["rel_p"], st["rel_d"] = rel_p, rel_d
+        ctl.ctl_pid_update(C.byref(h), C.byref(rp), pd, dd)
+        _pid(rp, st, pd, dd)
+        for k in ("omega", "best_omega", "pw_err_sum", "pw_last_err", "best_gap"):
+            assert math.isclose(getattr(h, k), st[k], rel_tol=1e-15, abs_tol=1e-300), (k, rel_p)
