@@ -26,6 +26,10 @@ def test_cfg1_solution_properties(dp):
     viol = ax - np.clip(ax, lc, uc)
     bnorm = np.linalg.norm(np.concatenate([lc[np.isfinite(lc)], uc[np.isfinite(uc)]]))
     assert np.linalg.norm(viol) <= 1e-3 * (1 + bnorm)
+    # dual signs of the projection (pdlp_core_op.cu:420-443): >= rows y >= 0, <= rows y <= 0
+    ge = np.isfinite(lc) & ~np.isfinite(uc)
+    le = ~np.isfinite(lc) & np.isfinite(uc)
+    assert ge.any() and np.all(y[ge] >= 0) and np.all(y[le] <= 0)
     # reduced costs are c - A'y of the returned duals
     r = c - A.T @ y
     assert np.linalg.norm(rc - r) <= 1e-9 * (1 + np.linalg.norm(c))

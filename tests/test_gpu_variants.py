@@ -157,3 +157,40 @@ def test_skewed_rows_renumbered_parity(dp, orc):
     np.add.at(aty, a["col_ind"], a["vals"] * y[rows])
     assert np.allclose(R.spmv(y, transpose=True), aty, rtol=1e-12, atol=1e-12)
     R.free()
+
+
+def test_projection_invariants_every_row_and_bound_type(dp, orc):
+    """The returned x (the projected PDHG iterate) lies inside its box
+    (finite, one-sided and free bounds, fixed variables) and y has the sign its
+    row type allows (>= rows y >= 0, <= rows y <= 0, equality and ranged rows
+    free); the iterates match the oracle's at the trace points."""
+    rng = np.random.default_rng(9)
+    m, n = 40, 60
+    A = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.3)
+    x0 = rng.uniform(0.0, 2.0, n)
+    ax = A @ x0
+    kinds = np.arange(m) % 4          # 0: >=, 1: <=, 2: =, 3: ranged
+    con_lb = np.where(kinds == 1, -np.inf, ax - np.where(kinds == 2, 0.0, 0.5))
+    con_ub = np.where(kinds == 0, np.inf, ax + np.where(kinds == 2, 0.0, 0.5))
+    con_lb[kinds == 2] = con_ub[kinds == 2] = ax[kinds == 2]
+    vk = np.arange(n) % 4             # 0: [0, inf), 1: (-inf, 3], 2: free, 3: fixed
+    var_lb = np.select([vk == 0, vk == 1, vk == 2, vk == 3], [0.0, -np.inf, -np.inf, x0])
+    var_ub = np.select([vk == 0, vk == 1, vk == 2, vk == 3], [np.inf, 3.0, np.inf, x0])
+    c = rng.standard_normal(n) * (vk != 2)
+    prob = dp.Problem.from_dense(A, c=c, con_lb=con_lb, con_ub=con_ub, var_lb=var_lb,
+                                 var_ub=var_ub)
+    iters = [1, 10, 100, 400]
+    params = dict(presolve=False, termination_criteria__iteration_limit=2000)
+    res, o, (tx, ty) = dp.solve(prob, dp.default_params(**params), trace_iters=iters)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(**params), trace_iters=iters,
+                                 n=prob.n, m=prob.m)
+    assert res.status == ref["status"]
+    for i in range(min(len(tx), len(ox))):
+        assert rel(tx[i], ox[i]) < 1e-8 and rel(ty[i], oy[i]) < 1e-8, iters[i]
+    x, y = res.x, res.y
+    tol = 1e-12 * (1 + np.abs(x0))
+    assert np.all(x >= np.where(np.isfinite(var_lb), var_lb, -np.inf) - tol)
+    assert np.all(x <= np.where(np.isfinite(var_ub), var_ub, np.inf) + tol)
+    assert np.array_equal(x[vk == 3], x0[vk == 3]) or np.allclose(x[vk == 3], x0[vk == 3],
+                                                                   rtol=1e-15, atol=0)
+    assert np.all(y[kinds == 0] >= 0) and np.all(y[kinds == 1] <= 0)
