@@ -283,3 +283,45 @@ def test_device_scaling_bitwise_equals_oracle(dp, orc, kw, shape):
         for k in ("c", "lv", "uv", "lc", "uc"):
             assert np.array_equal(dev[k], ref[k]), k
         assert dev["beta"] == ref["beta"] == 1.0 and dev["gamma"] == ref["gamma"] == 1.0
+
+
+@pytest.mark.parametrize("case", ["diagonal", "rank_one", "random"])
+def test_power_iteration_step_size(dp, orc, case):
+    """The device power iteration (scaling off, so it sees A itself) gives the
+    step 0.998 / sigma_max with sigma_max of the closed form (diagonal: max
+    |a_ii|; rank one u v': |u| |v|) or of numpy's SVD, and the same step and
+    iteration count as the oracle's power iteration from the same start."""
+    import ctypes as C
+    rng = np.random.default_rng(12)
+    if case == "diagonal":
+        d = np.array([3.0, -7.0, 2.0, 0.5, 6.0])
+        A = np.diag(d)
+        sigma = 7.0
+    elif case == "rank_one":
+        u, w = np.array([1.0, 2.0, 2.0]), np.array([3.0, 4.0])
+        A = np.outer(u, w)
+        sigma = np.linalg.norm(u) * np.linalg.norm(w)
+    else:
+        A = rng.standard_normal((30, 20)) * (rng.random((30, 20)) < 0.3)
+        sigma = np.linalg.svd(A, compute_uv=False)[0]
+    m, n = A.shape
+    prob = dp.Problem.from_dense(A, c=np.ones(n), con_lb=np.zeros(m), con_ub=np.ones(m),
+                                 var_lb=np.zeros(n), var_ub=np.ones(n))
+    params = dp.default_params(presolve=False, l_inf_ruiz_iterations=0,
+                               has_pock_chambolle_alpha=False, bound_objective_rescaling=False,
+                               termination_criteria__iteration_limit=1)
+    res, o, _ = dp.solve(prob, params)
+    assert abs(o.step_size - 0.998 / sigma) <= 1e-6 * (0.998 / sigma)
+    a = prob.arrays()
+    nnz = len(a["vals"])
+    trp, tci, tv = np.zeros(n + 1, np.int32), np.zeros(nnz, np.int32), np.zeros(nnz)
+    P = lambda x, t=C.c_double: x.ctypes.data_as(C.POINTER(t))
+    L = orc.lib()
+    L.orc_transpose(m, n, P(a["row_ptr"], C.c_int), P(a["col_ind"], C.c_int), P(a["vals"]),
+                    P(trp, C.c_int), P(tci, C.c_int), P(tv))
+    it = C.c_int()
+    s_orc = L.orc_power_iteration(m, n, P(a["row_ptr"], C.c_int), P(a["col_ind"], C.c_int),
+                                  P(a["vals"]), P(trp, C.c_int), P(tci, C.c_int), P(tv),
+                                  params.sv_max_iter, params.sv_tol, C.byref(it))
+    assert o.power_iterations == it.value
+    assert abs(o.step_size - 0.998 / s_orc) <= 1e-12 * o.step_size
