@@ -325,3 +325,21 @@ def test_power_iteration_step_size(dp, orc, case):
                                   params.sv_max_iter, params.sv_tol, C.byref(it))
     assert o.power_iterations == it.value
     assert abs(o.step_size - 0.998 / s_orc) <= 1e-12 * o.step_size
+
+
+def test_residuals_vanish_at_known_optimum(dp, orc):
+    """Residual closed form (utils.cu:639-719): started at the C API example's
+    optimum x* = (1, 2), y* = (1, -1, 0), the evaluation at iteration 0 finds
+    zero primal and dual residuals and gap, primal = dual objective = 3, and
+    the solve ends there, as the oracle's does."""
+    A, c, l, u = capi_example()
+    prob = dp.Problem.from_dense(A, c=c, con_lb=l, con_ub=u)
+    prob.set_start(np.array([1.0, 2.0]), np.array([1.0, -1.0, 0.0]))
+    res = dp.solve_plain(prob, dp.default_params(presolve=False))
+    ref, _ = orc.solve(prob.ptr, orc.default_params(presolve=False))
+    assert res.status == ref["status"] == "OPTIMAL"
+    assert res.iterations == ref["total_count"] == 0
+    assert res.rel_primal < 1e-12 and res.rel_dual < 1e-12 and res.rel_gap < 1e-12
+    assert abs(res.primal_obj - 3.0) < 1e-12 and abs(res.dual_obj - 3.0) < 1e-12
+    assert np.allclose(res.x, [1.0, 2.0], rtol=0, atol=1e-12)
+    assert np.allclose(res.y, [1.0, -1.0, 0.0], rtol=0, atol=1e-12)
