@@ -63,7 +63,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_xp_partial(
   constexpr int UNR = 8;
   __shared__ double prod[CAP];
   __shared__ int srp[DPDLP_PANEL_RMAX + 1];
-  __shared__ double ssum[DPDLP_BS];
+  __shared__ double ssum[4];
   const int tid = threadIdx.x;
   const int w = blockIdx.x, b = w / P, k = w - b * P;
   const int r0 = blk[b], r1 = blk[b + 1];
