@@ -75,11 +75,19 @@ void dpdlp_print_polish_final(termination_reason_t pr, int piters, double ptime,
                               termination_reason_t dr, int diters, double dtime,
                               double prel, double drel, double gap);
 
-/* error.c */
+/* util.c: error text, wall clock */
 void dpdlp_set_error(const char *fmt, ...);
 double dpdlp_wall(void);
 
-/* transpose.c: stable CSR transpose on the host */
+/* util.c: host worker threads for the data-parallel loops of ingestion and
+ * upload.  dpdlp_host_threads() is DPDLP_HOST_THREADS when set, else the
+ * online CPUs, clamped to [1, 32].  dpdlp_par_for runs fn(ctx, t) for every
+ * t in [0, ntask) on up to that many threads (the caller's included; tasks
+ * are taken in increasing order) and returns when all have finished. */
+int dpdlp_host_threads(void);
+void dpdlp_par_for(int ntask, void (*fn)(void *ctx, int task), void *ctx);
+
+/* util.c: stable CSR transpose on the host */
 int dpdlp_host_transpose(int m, int n, const int *rp, const int *ci, const double *v,
                          int *t_rp, int *t_ci, double *t_v);
 /* A''s row pointers only (the device transposes the entries) */

@@ -22,10 +22,21 @@
  * Differences: a file that cannot be opened returns NULL (the reference
  * dereferences NULL); lines longer than 4095 bytes are read whole instead of
  * ending the file.
+ *
+ * The input is read whole into memory.  The body of a COLUMNS section (the
+ * bulk of a file) is parsed by host worker threads: it is cut at line starts
+ * into chunks; each chunk's lines are split into tokens, their numbers
+ * converted and their row names looked up (the row map is complete and read
+ * only by then) into runs of entries, one run per column name change; one
+ * thread then walks the chunks in file order, numbering the columns and
+ * appending the entries, so the result is the one a line-by-line read gives.
+ * DPDLP_HOST_THREADS sets the thread count, DPDLP_MPS_CHUNK the smallest
+ * chunk in bytes (default 1 MiB).
  */
 #define _POSIX_C_SOURCE 200809L
 #include "mps_parser.h"
 #include "cupdlpx.h"
+#include "dpdlp_host.h"
 
 #include <ctype.h>
 #include <math.h>
@@ -35,6 +46,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <zlib.h>
+
+enum { S_NONE, S_ROWS, S_COLUMNS, S_RHS, S_RANGES, S_BOUNDS, S_OBJSENSE, S_END };
 
 /* ---- string -> index map (open addressing, FNV-1a).  A 32-byte slot holds
  * the hash, the index and the key itself when it is shorter than 16 bytes
@@ -107,8 +120,8 @@ static int smap_get(const smap_t *m, const char *k) {
   return smap_get_h(m, k, len, h);
 }
 
-static int smap_grow(smap_t *m) {
-  size_t nc = m->cap * 2;
+/* capacity for `count` keys at load factor 1/2 (rehashes at most once) */
+static int smap_grow_to(smap_t *m, size_t nc) {
   sslot_t *ns = slots_new(nc);
   if (!ns) return -1;
   for (size_t i = 0; i < m->cap; ++i) {
@@ -123,10 +136,16 @@ static int smap_grow(smap_t *m) {
   return 0;
 }
 
+static int smap_grow(smap_t *m) { return smap_grow_to(m, m->cap * 2); }
+
+static int smap_reserve(smap_t *m, size_t count) {
+  size_t nc = m->cap;
+  while ((count + 1) * 2 > nc) nc *= 2;
+  return nc == m->cap ? 0 : smap_grow_to(m, nc);
+}
+
 /* returns the existing index, or inserts with index = size */
-static int smap_put(smap_t *m, const char *k) {
-  int len;
-  uint64_t h = fnv1a_len(k, &len);
+static int smap_put_h(smap_t *m, const char *k, int len, uint64_t h) {
   int v = smap_get_h(m, k, len, h);
   if (v >= 0) return v;
   if ((m->size + 1) * 2 > m->cap && smap_grow(m)) return -1;
@@ -143,6 +162,12 @@ static int smap_put(smap_t *m, const char *k) {
   e->len = len;
   e->v = (int)m->size++;
   return e->v;
+}
+
+static int smap_put(smap_t *m, const char *k) {
+  int len;
+  uint64_t h = fnv1a_len(k, &len);
+  return smap_put_h(m, k, len, h);
 }
 
 /* Decimal to double, as atof: numbers with at most 15 significant digits
@@ -199,61 +224,114 @@ static double mps_atof(const char *s) {
   return neg ? -v : v;
 }
 
-/* ---- line source: plain stdio or zlib, read in 4 MiB chunks ---- */
-typedef struct {
-  gzFile gz;
-  FILE *f;
-  char *buf;
-  size_t cap, beg, end; /* unread bytes are buf[beg, end) */
-  int eof;
-} src_t;
-
-static size_t src_fill(src_t *s, char *dst, size_t n) {
-  if (s->gz) {
-    int r = gzread(s->gz, dst, (unsigned)(n > (1u << 30) ? (1u << 30) : n));
-    return r > 0 ? (size_t)r : 0;
+/* ---- the whole input, NUL-terminated: plain stdio or zlib ---- */
+static char *load_text(const char *filename, size_t *len_out) {
+  size_t fl = strlen(filename), len = 0, cap = 4u << 20;
+  gzFile gz = NULL;
+  FILE *f = NULL;
+  if (fl > 3 && strcmp(filename + fl - 3, ".gz") == 0) {
+    gz = gzopen(filename, "rb");
+  } else {
+    f = fopen(filename, "rb");
+    if (f && fseek(f, 0, SEEK_END) == 0) {
+      long sz = ftell(f);
+      if (sz >= 0) cap = (size_t)sz + 2;
+      if (fseek(f, 0, SEEK_SET) != 0) {
+        fclose(f);
+        f = NULL;
+      }
+    }
   }
-  return fread(dst, 1, n, s->f);
+  *len_out = 0;
+  if (!gz && !f) return NULL;
+  char *buf = (char *)malloc(cap);
+  for (;;) {
+    if (!buf) break;
+    if (cap - len < 2) {
+      char *nb = (char *)realloc(buf, cap * 2);
+      if (!nb) {
+        free(buf);
+        buf = NULL;
+        break;
+      }
+      buf = nb;
+      cap *= 2;
+    }
+    size_t want = cap - len - 1, got;
+    if (gz) {
+      int r = gzread(gz, buf + len, (unsigned)(want > (1u << 30) ? (1u << 30) : want));
+      got = r > 0 ? (size_t)r : 0;
+    } else {
+      got = fread(buf + len, 1, want, f);
+    }
+    len += got;
+    if (got == 0) break;
+  }
+  if (gz) gzclose(gz);
+  if (f) fclose(f);
+  if (!buf) {
+    fprintf(stderr, "ERROR: Out of memory reading %s\n", filename);
+    *len_out = 1; /* opened, not read */
+    return NULL;
+  }
+  buf[len] = '\0';
+  *len_out = len;
+  return buf;
 }
+
+typedef struct {
+  char *buf;
+  size_t len, pos; /* unread text is buf[pos, len) */
+} text_t;
 
 /* next line, NUL-terminated in place without its newline; NULL at the end */
-static char *src_line(src_t *s) {
-  for (;;) {
-    char *nl = s->beg < s->end ? (char *)memchr(s->buf + s->beg, '\n', s->end - s->beg) : NULL;
-    if (nl) {
-      char *line = s->buf + s->beg;
-      *nl = '\0';
-      s->beg = (size_t)(nl - s->buf) + 1;
-      return line;
-    }
-    if (s->eof) {
-      if (s->beg < s->end) { /* last line without a newline */
-        char *line = s->buf + s->beg;
-        s->buf[s->end] = '\0';
-        s->beg = s->end;
-        return line;
-      }
-      return NULL;
-    }
-    /* keep the partial line, make room, read more */
-    size_t rest = s->end - s->beg;
-    if (s->beg > 0) {
-      memmove(s->buf, s->buf + s->beg, rest);
-      s->beg = 0;
-      s->end = rest;
-    }
-    if (s->cap - s->end < (1u << 20) + 1) {
-      size_t nc = s->cap ? s->cap * 2 : (4u << 20);
-      char *nb = (char *)realloc(s->buf, nc);
-      if (!nb) return NULL;
-      s->buf = nb;
-      s->cap = nc;
-    }
-    size_t got = src_fill(s, s->buf + s->end, s->cap - s->end - 1);
-    if (got == 0) s->eof = 1;
-    s->end += got;
+static char *next_line(text_t *t) {
+  if (t->pos >= t->len) return NULL;
+  char *line = t->buf + t->pos;
+  char *nl = (char *)memchr(line, '\n', t->len - t->pos);
+  if (nl) {
+    *nl = '\0';
+    t->pos = (size_t)(nl - t->buf) + 1;
+  } else {
+    t->pos = t->len; /* the last line has no newline; buf[len] is NUL */
   }
+  return line;
 }
+
+static inline int is_delim(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; }
+
+/* the tokens of a line as strtok_r(" \t\n\r") splits them, at most 6,
+ * each NUL-terminated in place */
+static int tokenize(char *p, char **tk) {
+  int nt = 0;
+  for (;;) {
+    while (*p && is_delim(*p)) ++p;
+    if (!*p || nt == 6) break;
+    tk[nt++] = p;
+    while (*p && !is_delim(*p)) ++p;
+    if (!*p) break;
+    *p++ = '\0';
+  }
+  return nt;
+}
+
+/* the line [p, e) (e: its newline or the end of the text) is a section
+ * header: not a comment or blank line, exactly one token, a letter first */
+static int is_header(const char *p, const char *e) {
+  if (p == e || *p == '*' || *p == '\r' || *p == '\0') return 0;
+  while (p < e && *p && is_delim(*p)) ++p;
+  if (p == e || !*p || !isalpha((unsigned char)*p)) return 0;
+  while (p < e && *p && !is_delim(*p)) ++p;
+  while (p < e && *p && is_delim(*p)) ++p;
+  return p == e || !*p;
+}
+
+typedef struct {
+  int *er;    /* constraint (row) index */
+  int *ec;    /* column index */
+  double *ev; /* value */
+  size_t ne;
+} eblk_t;
 
 typedef struct {
   smap_t rows, cols;
@@ -265,9 +343,9 @@ typedef struct {
   double *obj, *lb, *ub;
   size_t col_cap;
   double *clb, *cub;
-  int *ci_r, *ci_c;
-  double *cv;
-  size_t nz, nz_cap;
+  eblk_t *blk; /* entries, one block per parsed COLUMNS chunk, in file order */
+  size_t nblk, blk_cap;
+  size_t nz;
   char *obj_row;
   char *cur_col;
   int cur_idx; /* index of cur_col (-1: not looked up yet) */
@@ -297,27 +375,6 @@ static int ensure_cols(mps_t *P) {
   return 0;
 }
 
-static int add_entry(mps_t *P, int r, int c, double v) {
-  if (P->nz >= P->nz_cap) {
-    size_t nc = P->nz_cap ? P->nz_cap * 2 : 1024;
-    int *a = (int *)realloc(P->ci_r, nc * sizeof(int));
-    if (!a) return -1;
-    P->ci_r = a;
-    int *b = (int *)realloc(P->ci_c, nc * sizeof(int));
-    if (!b) return -1;
-    P->ci_c = b;
-    double *d = (double *)realloc(P->cv, nc * sizeof(double));
-    if (!d) return -1;
-    P->cv = d;
-    P->nz_cap = nc;
-  }
-  P->ci_r[P->nz] = r;
-  P->ci_c[P->nz] = c;
-  P->cv[P->nz] = v;
-  P->nz++;
-  return 0;
-}
-
 /* mps_parser.c:511-594 */
 static int finalize_rows(mps_t *P) {
   long obj = -1;
@@ -327,6 +384,7 @@ static int finalize_rows(mps_t *P) {
       break;
     }
   if (obj < 0 && P->npend > 0) obj = 0;
+  if (smap_reserve(&P->rows, P->rows.size + P->npend)) return -1;
   if (obj >= 0) {
     P->obj_row = strdup(P->pend_names[obj]);
     if (!P->obj_row) return -1;
@@ -379,112 +437,425 @@ static int rows_line(mps_t *P, char **tk, int nt) {
   return 0;
 }
 
-/* mps_parser.c:596-650 */
-static int columns_line(mps_t *P, char **tk, int nt) {
-  if (nt < 2) return 0;
-  if (strcmp(tk[1], "'MARKER'") == 0) return 0;
-  int first;
-  if (nt % 2 != 0) {
-    if (!P->cur_col || strcmp(P->cur_col, tk[0]) != 0) {
-      free(P->cur_col);
-      P->cur_col = strdup(tk[0]);
-      if (!P->cur_col) return -1;
-      P->cur_idx = -1;
-    }
-    first = 1;
-  } else {
-    if (!P->cur_col) {
-      fprintf(stderr, "ERROR: Column data found before any column name was defined.\n");
-      return -1;
-    }
-    first = 0;
+/* ---- COLUMNS (mps_parser.c:596-650) ---- */
+
+typedef struct {
+  const char *name; /* column name (NUL-terminated in the text); NULL: the
+                       chunk's leading lines that continue the previous column */
+  uint64_t h;
+  int len;
+  size_t beg; /* first entry of the run */
+  int col;    /* column index, set when the chunks are merged */
+} crun_t;
+
+typedef struct {
+  char *beg, *end; /* whole lines */
+  char *hdr;       /* header scan: first header line in [beg, end), or NULL */
+  crun_t *run;
+  size_t nrun, run_cap;
+  int *er; /* constraint entries: row index, value, column (set by the merge) */
+  double *ev;
+  int *ec;
+  size_t ne, e_cap;
+  size_t *orun; /* objective entries: run, value */
+  double *ov;
+  size_t no, o_cap;
+  struct srec *rec; /* RHS, RANGES, BOUNDS: one record per name/value */
+  size_t nrec, rec_cap;
+  int err;
+} cchunk_t;
+
+typedef struct {
+  const mps_t *P; /* the maps and the objective row name: read only here */
+  cchunk_t *ch;
+  int sec;
+} cjob_t;
+
+static int chunk_run(cchunk_t *c, const char *name) {
+  if (c->nrun == c->run_cap) {
+    size_t nc = c->run_cap ? c->run_cap * 2 : 256;
+    crun_t *nr = (crun_t *)realloc(c->run, nc * sizeof(crun_t));
+    if (!nr) return -1;
+    c->run = nr;
+    c->run_cap = nc;
   }
-  if (ensure_cols(P)) return -1;
-  if (P->cur_idx < 0) {
-    P->cur_idx = smap_put(&P->cols, P->cur_col);
-    if (P->cur_idx < 0) return -1;
+  crun_t *r = &c->run[c->nrun++];
+  r->name = name;
+  r->len = 0;
+  r->h = name ? fnv1a_len(name, &r->len) : 0;
+  r->beg = c->ne;
+  return 0;
+}
+
+static int chunk_obj(cchunk_t *c, double v) {
+  if (c->no == c->o_cap) {
+    size_t nc = c->o_cap ? c->o_cap * 2 : 1024;
+    size_t *a = (size_t *)realloc(c->orun, nc * sizeof(size_t));
+    if (!a) return -1;
+    c->orun = a;
+    double *b = (double *)realloc(c->ov, nc * sizeof(double));
+    if (!b) return -1;
+    c->ov = b;
+    c->o_cap = nc;
   }
-  int col = P->cur_idx;
-  for (int i = first; i + 1 < nt; i += 2) {
-    double v = mps_atof(tk[i + 1]);
-    if (P->obj_row && strcmp(tk[i], P->obj_row) == 0) {
-      P->obj[col] += v;
+  c->orun[c->no] = c->nrun - 1;
+  c->ov[c->no] = v;
+  c->no++;
+  return 0;
+}
+
+static int chunk_entry(cchunk_t *c, int row, double v) {
+  if (c->ne == c->e_cap) {
+    size_t nc = c->e_cap ? c->e_cap * 2 : 4096;
+    int *a = (int *)realloc(c->er, nc * sizeof(int));
+    if (!a) return -1;
+    c->er = a;
+    double *b = (double *)realloc(c->ev, nc * sizeof(double));
+    if (!b) return -1;
+    c->ev = b;
+    c->e_cap = nc;
+  }
+  c->er[c->ne] = row;
+  c->ev[c->ne] = v;
+  c->ne++;
+  return 0;
+}
+
+/* worker: the first header line of a chunk */
+static void chunk_find_header(void *arg, int t) {
+  cchunk_t *c = &((cjob_t *)arg)->ch[t];
+  c->hdr = NULL;
+  for (char *p = c->beg; p < c->end;) {
+    char *nl = (char *)memchr(p, '\n', (size_t)(c->end - p));
+    char *e = nl ? nl : c->end;
+    if (is_header(p, e)) {
+      c->hdr = p;
+      return;
+    }
+    p = nl ? nl + 1 : c->end;
+  }
+}
+
+/* worker: the lines of one chunk of a COLUMNS body (no header among them).
+ * An odd token count names the column (a change of name starts a run), an
+ * even count continues the current one; 'MARKER' lines and lines of fewer
+ * than two tokens are skipped; entries of unknown rows are dropped. */
+static void chunk_parse(void *arg, int t) {
+  const cjob_t *J = (const cjob_t *)arg;
+  const mps_t *P = J->P;
+  cchunk_t *c = &J->ch[t];
+  const char *cur = NULL;
+  for (char *p = c->beg; p < c->end && !c->err;) {
+    char *line = p;
+    char *nl = (char *)memchr(p, '\n', (size_t)(c->end - p));
+    if (nl) {
+      *nl = '\0';
+      p = nl + 1;
     } else {
-      int r = smap_get(&P->rows, tk[i]);
-      if (r >= 0 && add_entry(P, r, col, v)) return -1;
+      p = c->end; /* the text's last line: NUL-terminated at the end */
     }
+    if (line[0] == '*' || line[0] == '\0' || line[0] == '\r') continue;
+    char *tk[6];
+    int nt = tokenize(line, tk);
+    if (nt < 2 || strcmp(tk[1], "'MARKER'") == 0) continue;
+    int first = nt % 2;
+    if (first) {
+      if (!cur || strcmp(cur, tk[0]) != 0) {
+        if (chunk_run(c, tk[0])) c->err = 1;
+        cur = tk[0];
+      }
+    } else if (c->nrun == 0) {
+      if (chunk_run(c, NULL)) c->err = 1;
+    }
+    for (int i = first; i + 1 < nt && !c->err; i += 2) {
+      double v = mps_atof(tk[i + 1]);
+      if (P->obj_row && strcmp(tk[i], P->obj_row) == 0) {
+        if (chunk_obj(c, v)) c->err = 1;
+      } else {
+        int r = smap_get(&P->rows, tk[i]);
+        if (r >= 0 && chunk_entry(c, r, v)) c->err = 1;
+      }
+    }
+  }
+}
+
+/* worker: the column of every constraint entry of a chunk */
+static void chunk_cols(void *arg, int t) {
+  cchunk_t *c = &((cjob_t *)arg)->ch[t];
+  if (c->ne == 0) return;
+  c->ec = (int *)malloc(c->ne * sizeof(int));
+  if (!c->ec) {
+    c->err = 1;
+    return;
+  }
+  for (size_t k = 0; k < c->nrun; ++k) {
+    size_t end = k + 1 < c->nrun ? c->run[k + 1].beg : c->ne;
+    for (size_t e = c->run[k].beg; e < end; ++e) c->ec[e] = c->run[k].col;
+  }
+}
+
+/* in file order: number the columns (first appearance) and add the
+ * objective entries to c; then hand the chunks' entries to the problem */
+static int chunks_merge(mps_t *P, cjob_t *job, int nch) {
+  cchunk_t *ch = job->ch;
+  for (int t = 0; t < nch; ++t)
+    if (ch[t].err) return -1;
+  size_t runs = 0;
+  for (int t = 0; t < nch; ++t) runs += ch[t].nrun;
+  if (smap_reserve(&P->cols, P->cols.size + runs)) return -1;
+  const char *cur = P->cur_col;
+  int col = P->cur_idx;
+  for (int t = 0; t < nch; ++t) {
+    cchunk_t *c = &ch[t];
+    for (size_t k = 0; k < c->nrun; ++k) {
+      crun_t *r = &c->run[k];
+      if (!r->name) {
+        if (!cur) {
+          fprintf(stderr, "ERROR: Column data found before any column name was defined.\n");
+          return -1;
+        }
+      } else if (!cur || strcmp(cur, r->name) != 0) {
+        cur = r->name;
+        col = -1;
+      }
+      if (col < 0) {
+        if (ensure_cols(P)) return -1;
+        int clen = r->len;
+        uint64_t h = r->h; /* hashed by the worker */
+        if (!r->name || cur != r->name) h = fnv1a_len(cur, &clen);
+        col = smap_put_h(&P->cols, cur, clen, h);
+        if (col < 0) return -1;
+      }
+      r->col = col;
+    }
+    for (size_t o = 0; o < c->no; ++o) P->obj[c->run[c->orun[o]].col] += c->ov[o];
+  }
+  if (cur != P->cur_col) { /* the text is freed after parsing: keep a copy */
+    char *own = strdup(cur);
+    if (!own) return -1;
+    free(P->cur_col);
+    P->cur_col = own;
+  }
+  P->cur_idx = col;
+  dpdlp_par_for(nch, chunk_cols, job);
+  if (P->nblk + (size_t)nch > P->blk_cap) {
+    size_t nc = P->nblk + (size_t)nch + 16;
+    eblk_t *nb = (eblk_t *)realloc(P->blk, nc * sizeof(eblk_t));
+    if (!nb) return -1;
+    P->blk = nb;
+    P->blk_cap = nc;
+  }
+  for (int t = 0; t < nch; ++t) {
+    cchunk_t *c = &ch[t];
+    if (c->err) return -1;
+    if (c->ne == 0) continue;
+    P->blk[P->nblk++] = (eblk_t){c->er, c->ec, c->ev, c->ne};
+    P->nz += c->ne;
+    c->er = c->ec = NULL;
+    c->ev = NULL;
   }
   return 0;
 }
 
-/* mps_parser.c:652-678 */
-static void rhs_line(mps_t *P, char **tk, int nt) {
-  for (int i = 1; i + 1 < nt; i += 2) {
-    double v = mps_atof(tk[i + 1]);
-    if (P->obj_row && strcmp(tk[i], P->obj_row) == 0) {
-      P->obj_const = -v;
+/* cut [b, e) at line starts into nch chunks */
+static void cut_chunks(char *b, char *e, int nch, cchunk_t *ch) {
+  size_t len = (size_t)(e - b);
+  char *prev = b;
+  for (int k = 0; k < nch; ++k) {
+    char *s = k == 0 ? b : b + (size_t)((double)len * k / nch);
+    if (s < prev) s = prev;
+    if (k > 0 && s > b && s < e && s[-1] != '\n') {
+      char *nl = (char *)memchr(s, '\n', (size_t)(e - s));
+      s = nl ? nl + 1 : e;
+    }
+    ch[k].beg = s;
+    if (k > 0) ch[k - 1].end = s;
+    prev = s;
+  }
+  ch[nch - 1].end = e;
+}
+
+/* ---- RHS, RANGES, BOUNDS (mps_parser.c:652-742): worker threads turn the
+ * lines of a section body into records (numbers converted, names looked up;
+ * unknown names and bound types dropped) that one thread applies in file
+ * order ---- */
+typedef struct srec {
+  int kind, idx;
+  double v;
+} srec_t;
+
+enum { R_OBJ = 1, R_ROW, B_LO, B_UP, B_FX, B_FR, B_MI, B_PL, B_BV };
+
+static int chunk_rec(cchunk_t *c, int kind, int idx, double v) {
+  if (c->nrec == c->rec_cap) {
+    size_t nc = c->rec_cap ? c->rec_cap * 2 : 1024;
+    srec_t *nr = (srec_t *)realloc(c->rec, nc * sizeof(srec_t));
+    if (!nr) return -1;
+    c->rec = nr;
+    c->rec_cap = nc;
+  }
+  c->rec[c->nrec++] = (srec_t){kind, idx, v};
+  return 0;
+}
+
+static int bound_kind(const char *t) {
+  static const char *names[] = {"LO", "UP", "FX", "FR", "MI", "PL", "BV"};
+  for (int k = 0; k < 7; ++k)
+    if (!strcmp(t, names[k])) return B_LO + k;
+  return 0;
+}
+
+static void chunk_records(void *arg, int t) {
+  const cjob_t *J = (const cjob_t *)arg;
+  const mps_t *P = J->P;
+  cchunk_t *c = &J->ch[t];
+  for (char *p = c->beg; p < c->end && !c->err;) {
+    char *line = p;
+    char *nl = (char *)memchr(p, '\n', (size_t)(c->end - p));
+    if (nl) {
+      *nl = '\0';
+      p = nl + 1;
+    } else {
+      p = c->end;
+    }
+    if (line[0] == '*' || line[0] == '\0' || line[0] == '\r') continue;
+    char *tk[6];
+    int nt = tokenize(line, tk);
+    if (J->sec == S_BOUNDS) {
+      if (nt < 3) continue;
+      int col = smap_get(&P->cols, tk[2]);
+      int kind = bound_kind(tk[0]);
+      if (col < 0 || !kind) continue;
+      if (chunk_rec(c, kind, col, nt > 3 ? atof(tk[3]) : 0.0)) c->err = 1;
       continue;
     }
-    int r = smap_get(&P->rows, tk[i]);
-    if (r < 0) continue;
-    char t = P->types[r];
-    if (t == 'L') {
-      P->cub[r] = v;
-    } else if (t == 'G') {
-      P->clb[r] = v;
-    } else {
-      P->clb[r] = v;
-      P->cub[r] = v;
+    for (int i = 1; i + 1 < nt && !c->err; i += 2) {
+      if (J->sec == S_RHS) {
+        double v = mps_atof(tk[i + 1]);
+        if (P->obj_row && strcmp(tk[i], P->obj_row) == 0) {
+          if (chunk_rec(c, R_OBJ, 0, v)) c->err = 1;
+          continue;
+        }
+        int r = smap_get(&P->rows, tk[i]);
+        if (r >= 0 && chunk_rec(c, R_ROW, r, v)) c->err = 1;
+      } else { /* RANGES */
+        double R = atof(tk[i + 1]);
+        int r = smap_get(&P->rows, tk[i]);
+        if (r >= 0 && chunk_rec(c, R_ROW, r, R)) c->err = 1;
+      }
     }
+  }
+}
+
+/* mps_parser.c:652-678 */
+static void apply_rhs(mps_t *P, const srec_t *q) {
+  if (q->kind == R_OBJ) {
+    P->obj_const = -q->v;
+    return;
+  }
+  int r = q->idx;
+  char t = P->types[r];
+  if (t == 'L') {
+    P->cub[r] = q->v;
+  } else if (t == 'G') {
+    P->clb[r] = q->v;
+  } else {
+    P->clb[r] = q->v;
+    P->cub[r] = q->v;
   }
 }
 
 /* mps_parser.c:680-707 */
-static void ranges_line(mps_t *P, char **tk, int nt) {
-  for (int i = 1; i + 1 < nt; i += 2) {
-    double R = atof(tk[i + 1]);
-    int r = smap_get(&P->rows, tk[i]);
-    if (r < 0) continue;
-    char t = P->types[r];
-    double rhs = (t == 'L') ? P->cub[r] : P->clb[r];
-    if (t == 'G') {
-      P->cub[r] = rhs + fabs(R);
-    } else if (t == 'L') {
-      P->clb[r] = rhs - fabs(R);
-    } else if (t == 'E') {
-      if (R >= 0)
-        P->cub[r] = rhs + R;
-      else
-        P->clb[r] = rhs + R;
-    }
+static void apply_range(mps_t *P, const srec_t *q) {
+  int r = q->idx;
+  double R = q->v;
+  char t = P->types[r];
+  double rhs = (t == 'L') ? P->cub[r] : P->clb[r];
+  if (t == 'G') {
+    P->cub[r] = rhs + fabs(R);
+  } else if (t == 'L') {
+    P->clb[r] = rhs - fabs(R);
+  } else if (t == 'E') {
+    if (R >= 0)
+      P->cub[r] = rhs + R;
+    else
+      P->clb[r] = rhs + R;
   }
 }
 
 /* mps_parser.c:709-742 */
-static void bounds_line(mps_t *P, char **tk, int nt) {
-  if (nt < 3) return;
-  int col = smap_get(&P->cols, tk[2]);
-  if (col < 0) return;
-  double v = nt > 3 ? atof(tk[3]) : 0.0;
-  const char *t = tk[0];
-  if (!strcmp(t, "LO")) {
-    P->lb[col] = v;
-  } else if (!strcmp(t, "UP")) {
-    P->ub[col] = v;
-  } else if (!strcmp(t, "FX")) {
-    P->lb[col] = v;
-    P->ub[col] = v;
-  } else if (!strcmp(t, "FR")) {
-    P->lb[col] = -INFINITY;
-    P->ub[col] = INFINITY;
-  } else if (!strcmp(t, "MI")) {
-    P->lb[col] = -INFINITY;
-  } else if (!strcmp(t, "PL")) {
-    P->ub[col] = INFINITY;
-  } else if (!strcmp(t, "BV")) {
-    P->lb[col] = 0.0;
-    P->ub[col] = 1.0;
+static void apply_bound(mps_t *P, const srec_t *q) {
+  int col = q->idx;
+  double v = q->v;
+  switch (q->kind) {
+  case B_LO: P->lb[col] = v; break;
+  case B_UP: P->ub[col] = v; break;
+  case B_FX: P->lb[col] = v; P->ub[col] = v; break;
+  case B_FR: P->lb[col] = -INFINITY; P->ub[col] = INFINITY; break;
+  case B_MI: P->lb[col] = -INFINITY; break;
+  case B_PL: P->ub[col] = INFINITY; break;
+  case B_BV: P->lb[col] = 0.0; P->ub[col] = 1.0; break;
+  default: break;
   }
+}
+
+/* The body of the section whose header was just read (it runs from the
+ * text's read position to the next header line, or the end): cut into
+ * chunks, parsed by the worker threads, merged or applied in file order; the
+ * read position is left at the header that ends it. */
+static int section_body(mps_t *P, text_t *T, int sec) {
+  char *b = T->buf + T->pos, *end = T->buf + T->len;
+  const char *env = getenv("DPDLP_MPS_CHUNK");
+  long long min_chunk = env ? atoll(env) : (1LL << 20);
+  if (min_chunk < 1) min_chunk = 1;
+  int nth = dpdlp_host_threads();
+  long long want = (long long)(end - b) / min_chunk;
+  int nch = want < 1 ? 1 : (want > 8LL * nth ? 8 * nth : (int)want);
+  cchunk_t *ch = (cchunk_t *)calloc((size_t)nch, sizeof(cchunk_t));
+  if (!ch) return -1;
+  cjob_t job = {P, ch, sec};
+  cut_chunks(b, end, nch, ch);
+  dpdlp_par_for(nch, chunk_find_header, &job);
+  char *e = end;
+  for (int k = 0; k < nch; ++k)
+    if (ch[k].hdr) {
+      e = ch[k].hdr;
+      break;
+    }
+  want = (long long)(e - b) / min_chunk;
+  if (want < nch) nch = want < 1 ? 1 : (int)want;
+  cut_chunks(b, e, nch, ch);
+  int rc = 0;
+  if (sec == S_COLUMNS) {
+    dpdlp_par_for(nch, chunk_parse, &job);
+    rc = chunks_merge(P, &job, nch);
+  } else {
+    dpdlp_par_for(nch, chunk_records, &job);
+    for (int k = 0; k < nch && !rc; ++k) {
+      if (ch[k].err) rc = -1;
+      for (size_t i = 0; i < ch[k].nrec && !rc; ++i) {
+        const srec_t *q = &ch[k].rec[i];
+        if (sec == S_RHS)
+          apply_rhs(P, q);
+        else if (sec == S_RANGES)
+          apply_range(P, q);
+        else
+          apply_bound(P, q);
+      }
+    }
+  }
+  for (int k = 0; k < nch; ++k) {
+    free(ch[k].run);
+    free(ch[k].er);
+    free(ch[k].ec);
+    free(ch[k].ev);
+    free(ch[k].orun);
+    free(ch[k].ov);
+    free(ch[k].rec);
+  }
+  free(ch);
+  T->pos = (size_t)(e - T->buf);
+  return rc;
 }
 
 static void mps_free(mps_t *P) {
@@ -499,41 +870,139 @@ static void mps_free(mps_t *P) {
   free(P->ub);
   free(P->clb);
   free(P->cub);
-  free(P->ci_r);
-  free(P->ci_c);
-  free(P->cv);
+  for (size_t b = 0; b < P->nblk; ++b) {
+    free(P->blk[b].er);
+    free(P->blk[b].ec);
+    free(P->blk[b].ev);
+  }
+  free(P->blk);
   free(P->obj_row);
   free(P->cur_col);
 }
 
-enum { S_NONE, S_ROWS, S_COLUMNS, S_RHS, S_RANGES, S_BOUNDS, S_OBJSENSE, S_END };
+/* ---- entries -> CSR: a stable counting sort by row over G groups of
+ * consecutive entry blocks; each group counts its rows, the counts become
+ * per-group write positions (row by row, groups in file order), and each
+ * group scatters its entries, so every row keeps file order. ---- */
+typedef struct {
+  const mps_t *P;
+  int m, G, R;
+  size_t *gb;  /* group g holds blocks [gb[g], gb[g + 1]) */
+  int *cnt;    /* cnt[g * (m + 1) + r] */
+  int *rp, *ci;
+  double *v;
+} csr_job_t;
+
+static void csr_count(void *arg, int g) {
+  csr_job_t *J = (csr_job_t *)arg;
+  int *c = J->cnt + (size_t)g * (J->m + 1);
+  for (size_t b = J->gb[g]; b < J->gb[g + 1]; ++b) {
+    const eblk_t *B = &J->P->blk[b];
+    for (size_t e = 0; e < B->ne; ++e) c[B->er[e]]++;
+  }
+}
+
+static void csr_rows(int m, int R, int t, int *r0, int *r1) {
+  *r0 = (int)((long long)m * t / R);
+  *r1 = (int)((long long)m * (t + 1) / R);
+}
+
+static void csr_totals(void *arg, int t) {
+  csr_job_t *J = (csr_job_t *)arg;
+  int r0, r1;
+  csr_rows(J->m, J->R, t, &r0, &r1);
+  for (int r = r0; r < r1; ++r) {
+    int s = 0;
+    for (int g = 0; g < J->G; ++g) s += J->cnt[(size_t)g * (J->m + 1) + r];
+    J->rp[r + 1] = s;
+  }
+}
+
+static void csr_offsets(void *arg, int t) {
+  csr_job_t *J = (csr_job_t *)arg;
+  int r0, r1;
+  csr_rows(J->m, J->R, t, &r0, &r1);
+  for (int r = r0; r < r1; ++r) {
+    int run = J->rp[r];
+    for (int g = 0; g < J->G; ++g) {
+      int *c = &J->cnt[(size_t)g * (J->m + 1) + r];
+      int k = *c;
+      *c = run;
+      run += k;
+    }
+  }
+}
+
+static void csr_scatter(void *arg, int g) {
+  csr_job_t *J = (csr_job_t *)arg;
+  int *pos = J->cnt + (size_t)g * (J->m + 1);
+  for (size_t b = J->gb[g]; b < J->gb[g + 1]; ++b) {
+    const eblk_t *B = &J->P->blk[b];
+    for (size_t e = 0; e < B->ne; ++e) {
+      int d = pos[B->er[e]]++;
+      J->ci[d] = B->ec[e];
+      J->v[d] = B->ev[e];
+    }
+  }
+}
+
+static int build_csr(const mps_t *P, int m, int *rp, int *ci, double *v) {
+  rp[0] = 0;
+  if (m == 0 || P->nz == 0) {
+    for (int i = 0; i < m; ++i) rp[i + 1] = 0;
+    return 0;
+  }
+  int nth = dpdlp_host_threads();
+  long long gmax = (32LL << 20) / ((long long)m + 1); /* counts stay within 128 MB */
+  int G = (int)P->nblk < nth ? (int)P->nblk : nth;
+  if (G > gmax) G = gmax < 1 ? 1 : (int)gmax;
+  size_t *gb = (size_t *)malloc(((size_t)G + 1) * sizeof(size_t));
+  int *cnt = (int *)calloc((size_t)G * ((size_t)m + 1), sizeof(int));
+  if (!gb || !cnt) {
+    free(gb);
+    free(cnt);
+    return -1;
+  }
+  /* groups of consecutive blocks with about nz / G entries each */
+  gb[0] = 0;
+  size_t acc = 0, b = 0;
+  for (int g = 1; g < G; ++g) {
+    size_t target = P->nz / (size_t)G * (size_t)g;
+    while (b < P->nblk && acc < target) acc += P->blk[b++].ne;
+    gb[g] = b;
+  }
+  gb[G] = P->nblk;
+  int R = m < 4096 ? 1 : 4 * nth;
+  csr_job_t J = {P, m, G, R, gb, cnt, rp, ci, v};
+  dpdlp_par_for(G, csr_count, &J);
+  dpdlp_par_for(R, csr_totals, &J);
+  for (int i = 0; i < m; ++i) rp[i + 1] += rp[i];
+  dpdlp_par_for(R, csr_offsets, &J);
+  dpdlp_par_for(G, csr_scatter, &J);
+  free(gb);
+  free(cnt);
+  return 0;
+}
 
 lp_problem_t *read_mps_file(const char *filename) {
   if (!filename) return NULL;
-  src_t src = {0};
-  size_t fl = strlen(filename);
-  if (fl > 3 && strcmp(filename + fl - 3, ".gz") == 0)
-    src.gz = gzopen(filename, "rb");
-  else
-    src.f = fopen(filename, "r");
-  if (!src.gz && !src.f) {
-    fprintf(stderr, "ERROR: Could not open file %s\n", filename);
+  text_t T = {NULL, 0, 0};
+  T.buf = load_text(filename, &T.len);
+  if (!T.buf) {
+    if (T.len == 0) fprintf(stderr, "ERROR: Could not open file %s\n", filename);
     return NULL;
   }
   mps_t P;
   memset(&P, 0, sizeof(P));
+  P.cur_idx = -1;
   if (smap_init(&P.rows) || smap_init(&P.cols)) P.err = 1;
   int sec = S_NONE;
   bool rows_done = false;
   char *line;
-  while (!P.err && (line = src_line(&src)) != NULL) {
+  while (!P.err && (line = next_line(&T)) != NULL) {
     if (line[0] == '*' || line[0] == '\0' || line[0] == '\r') continue;
     char *tk[6];
-    int nt = 0;
-    char *save = NULL;
-    for (char *t = strtok_r(line, " \t\n\r", &save); t && nt < 6;
-         t = strtok_r(NULL, " \t\n\r", &save))
-      tk[nt++] = t;
+    int nt = tokenize(line, tk);
     if (nt == 0) continue;
     if (nt == 1 && isalpha((unsigned char)tk[0][0])) {
       int next = S_NONE;
@@ -550,6 +1019,9 @@ lp_problem_t *read_mps_file(const char *filename) {
       }
       sec = next;
       if (sec == S_END) break;
+      if ((sec == S_COLUMNS || sec == S_RHS || sec == S_RANGES || sec == S_BOUNDS) && !P.err &&
+          section_body(&P, &T, sec))
+        P.err = 1;
       continue;
     }
     switch (sec) {
@@ -559,27 +1031,18 @@ lp_problem_t *read_mps_file(const char *filename) {
     case S_ROWS:
       if (rows_line(&P, tk, nt)) P.err = 1;
       break;
-    case S_COLUMNS:
-      if (columns_line(&P, tk, nt)) P.err = 1;
-      break;
-    case S_RHS:
-      rhs_line(&P, tk, nt);
-      break;
-    case S_RANGES:
-      ranges_line(&P, tk, nt);
-      break;
-    case S_BOUNDS:
-      bounds_line(&P, tk, nt);
-      break;
     default:
       break;
     }
   }
-  if (src.gz) gzclose(src.gz);
-  if (src.f) fclose(src.f);
-  free(src.buf);
+  free(T.buf);
   if (P.err) {
     fprintf(stderr, "ERROR: Failed to parse MPS file.\n");
+    mps_free(&P);
+    return NULL;
+  }
+  if (P.nz > 0x7fffffff) {
+    fprintf(stderr, "ERROR: More than 2^31-1 nonzeros.\n");
     mps_free(&P);
     return NULL;
   }
@@ -601,28 +1064,18 @@ lp_problem_t *read_mps_file(const char *filename) {
   P.obj = P.lb = P.ub = P.clb = P.cub = NULL;
   if (P.maximize)
     for (int j = 0; j < n; ++j) p->objective_vector[j] *= -1.0;
-  /* COO -> CSR keeping file order within rows (mps_parser.c:744-777) */
+  /* entries -> CSR keeping file order within rows (mps_parser.c:744-777) */
   p->constraint_matrix_row_pointers = (int *)calloc((size_t)m + 1, sizeof(int));
   p->constraint_matrix_col_indices = (int *)malloc((P.nz ? P.nz : 1) * sizeof(int));
   p->constraint_matrix_values = (double *)malloc((P.nz ? P.nz : 1) * sizeof(double));
-  int *pos = (int *)malloc(((size_t)m + 1) * sizeof(int));
   if (!p->constraint_matrix_row_pointers || !p->constraint_matrix_col_indices ||
-      !p->constraint_matrix_values || !pos) {
-    free(pos);
+      !p->constraint_matrix_values ||
+      build_csr(&P, m, p->constraint_matrix_row_pointers, p->constraint_matrix_col_indices,
+                p->constraint_matrix_values)) {
     lp_problem_free(p);
     mps_free(&P);
     return NULL;
   }
-  int *rp = p->constraint_matrix_row_pointers;
-  for (size_t k = 0; k < P.nz; ++k) rp[P.ci_r[k] + 1]++;
-  for (int i = 0; i < m; ++i) rp[i + 1] += rp[i];
-  memcpy(pos, rp, ((size_t)m + 1) * sizeof(int));
-  for (size_t k = 0; k < P.nz; ++k) {
-    int d = pos[P.ci_r[k]]++;
-    p->constraint_matrix_col_indices[d] = P.ci_c[k];
-    p->constraint_matrix_values[d] = P.cv[k];
-  }
-  free(pos);
   mps_free(&P);
   return p;
 }

@@ -1,14 +1,17 @@
 /*
- * Host utilities: thread-local error text, wall clock, stable transpose.
+ * Host utilities: thread-local error text, wall clock, worker threads,
+ * stable transpose.
  */
 #define _POSIX_C_SOURCE 200809L
 #include "dpdlp_ext.h"
 #include "dpdlp_host.h"
 
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <unistd.h>
 
 static _Thread_local char g_err[512];
 
@@ -26,6 +29,46 @@ double dpdlp_wall(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+int dpdlp_host_threads(void) {
+  const char *env = getenv("DPDLP_HOST_THREADS");
+  long t = env ? atol(env) : sysconf(_SC_NPROCESSORS_ONLN);
+  if (t < 1) t = 1;
+  if (t > 32) t = 32;
+  return (int)t;
+}
+
+typedef struct {
+  void (*fn)(void *, int);
+  void *ctx;
+  int ntask;
+  int next; /* next task to take (atomic) */
+} par_job_t;
+
+static void *par_worker(void *arg) {
+  par_job_t *j = (par_job_t *)arg;
+  for (;;) {
+    int t = __atomic_fetch_add(&j->next, 1, __ATOMIC_RELAXED);
+    if (t >= j->ntask) break;
+    j->fn(j->ctx, t);
+  }
+  return NULL;
+}
+
+void dpdlp_par_for(int ntask, void (*fn)(void *ctx, int task), void *ctx) {
+  if (ntask <= 0) return;
+  par_job_t job = {fn, ctx, ntask, 0};
+  int nth = dpdlp_host_threads();
+  if (nth > ntask) nth = ntask;
+  pthread_t th[32];
+  int started = 0;
+  for (int i = 1; i < nth; ++i) {
+    if (pthread_create(&th[started], NULL, par_worker, &job) != 0) break;
+    ++started;
+  }
+  par_worker(&job); /* the caller works too; a thread that failed to start is not needed */
+  for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
 }
 
 /* Row pointers of A' (column counts of A, exclusive prefix). */

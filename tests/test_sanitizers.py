@@ -5,7 +5,8 @@ that need no device (MPS reader, create_lp_problem's converters, generator,
 transpose, row-block and panel planners, logging) and the FP64 oracle with
 gcc -fsanitize=address,undefined, then runs it over the MPS fixtures, a
 gzip-compressed copy, files the reference's reader accepts leniently, a
-missing file and a malformed file."""
+missing file and a malformed file.  The MPS reader's worker threads also run
+under ThreadSanitizer on a file cut into many small chunks."""
 import gzip
 import os
 import shutil
@@ -41,7 +42,7 @@ def test_host_code_under_sanitizers(tmp_path):
            os.path.join(ROOT, "tests", "sanitize_harness.c"),
            os.path.join(ROOT, "oracle", "pdlp_oracle.c")]
     cmd += [os.path.join(HOST, s) for s in SOURCES]
-    cmd += ["-o", exe, "-lz", "-lm"]
+    cmd += ["-o", exe, "-lz", "-lm", "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-4000:]
 
@@ -68,3 +69,44 @@ def test_host_code_under_sanitizers(tmp_path):
     assert r.returncode == 0, report[-6000:]
     assert "harness ok" in r.stdout
     assert "runtime error" not in report and "AddressSanitizer" not in report
+
+
+def _tsan_works(tmp):
+    src = os.path.join(tmp, "tprobe.c")
+    with open(src, "w") as f:
+        f.write("#include <pthread.h>\nstatic void *f(void *a) { return a; }\n"
+                "int main(void) { pthread_t t; pthread_create(&t, 0, f, 0);"
+                " return pthread_join(t, 0); }\n")
+    exe = os.path.join(tmp, "tprobe")
+    r = subprocess.run(["gcc", "-fsanitize=thread", src, "-o", exe, "-lpthread"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        return False
+    return subprocess.run([exe], capture_output=True).returncode == 0
+
+
+def test_mps_reader_threads_under_thread_sanitizer(tmp_path):
+    if shutil.which("gcc") is None or not _tsan_works(str(tmp_path)):
+        pytest.skip("gcc with ThreadSanitizer is not usable here")
+    import test_mps_parallel as tmp_mod
+    exe = str(tmp_path / "harness_tsan")
+    cmd = ["gcc", "-std=gnu11", "-O1", "-g", "-fsanitize=thread",
+           "-I", os.path.join(ROOT, "include"), "-I", HOST, "-I", os.path.join(ROOT, "oracle"),
+           os.path.join(ROOT, "tests", "sanitize_harness.c"),
+           os.path.join(ROOT, "oracle", "pdlp_oracle.c")]
+    cmd += [os.path.join(HOST, s) for s in SOURCES]
+    cmd += ["-o", exe, "-lz", "-lm", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-4000:]
+    files = []
+    for seed in range(4):
+        f = tmp_path / f"rnd{seed}.mps"
+        f.write_text(tmp_mod.random_mps(seed, ncol=2000, nrow=300))
+        files.append(str(f))
+    env = dict(os.environ, DPDLP_HOST_THREADS="6", DPDLP_MPS_CHUNK="256",
+               TSAN_OPTIONS="halt_on_error=1")
+    r = subprocess.run([exe, "--mps-only"] + files, capture_output=True, text=True, env=env,
+                       timeout=300)
+    report = r.stdout + r.stderr
+    assert r.returncode == 0, report[-6000:]
+    assert "harness ok" in r.stdout and "ThreadSanitizer" not in report
