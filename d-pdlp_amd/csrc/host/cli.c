@@ -49,14 +49,7 @@ static void save_vector(const double *data, int size, const char *dir, const cha
     free(path);
     return;
   }
-  FILE *f = fopen(path, "w");
-  if (!f) {
-    perror("Error opening solution file");
-    free(path);
-    return;
-  }
-  for (int i = 0; i < size; ++i) fprintf(f, "%.10g\n", data[i]);
-  fclose(f);
+  if (dpdlp_write_vector(path, data, size)) perror("Error opening solution file");
   free(path);
 }
 

@@ -71,6 +71,58 @@ void dpdlp_par_for(int ntask, void (*fn)(void *ctx, int task), void *ctx) {
   for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
 }
 
+/* ---- solution files: "%.10g\n" per value, formatted by worker threads
+ * into one buffer per slice, written in order ---- */
+typedef struct {
+  const double *data;
+  int size, nsl;
+  char **buf;
+  size_t *len;
+} fmt_job_t;
+
+static void fmt_slice(void *arg, int t) {
+  fmt_job_t *J = (fmt_job_t *)arg;
+  int i0 = (int)((long long)J->size * t / J->nsl), i1 = (int)((long long)J->size * (t + 1) / J->nsl);
+  size_t cap = (size_t)(i1 - i0) * 24 + 1; /* "%.10g" is at most 17 characters */
+  char *b = (char *)malloc(cap);
+  if (!b) return;
+  size_t o = 0;
+  for (int i = i0; i < i1; ++i) o += (size_t)snprintf(b + o, cap - o, "%.10g\n", J->data[i]);
+  J->buf[t] = b;
+  J->len[t] = o;
+}
+
+int dpdlp_write_vector(const char *path, const double *data, int size) {
+  if (!path || (size > 0 && !data) || size < 0) {
+    dpdlp_set_error("dpdlp_write_vector: bad arguments");
+    return -1;
+  }
+  FILE *f = fopen(path, "w");
+  if (!f) {
+    dpdlp_set_error("cannot open %s for writing", path);
+    return -1;
+  }
+  int nsl = size / 65536 + 1, nth = dpdlp_host_threads();
+  if (nsl > 4 * nth) nsl = 4 * nth;
+  char **buf = (char **)calloc((size_t)nsl, sizeof(char *));
+  size_t *len = (size_t *)calloc((size_t)nsl, sizeof(size_t));
+  int rc = 0;
+  if (!buf || !len) {
+    rc = -1;
+  } else {
+    fmt_job_t J = {data, size, nsl, buf, len};
+    dpdlp_par_for(nsl, fmt_slice, &J);
+    for (int t = 0; t < nsl && rc == 0; ++t)
+      if (!buf[t] || fwrite(buf[t], 1, len[t], f) != len[t]) rc = -1;
+  }
+  for (int t = 0; buf && t < nsl; ++t) free(buf[t]);
+  free(buf);
+  free(len);
+  if (fclose(f) != 0) rc = -1;
+  if (rc) dpdlp_set_error("writing %s failed", path);
+  return rc;
+}
+
 /* Row pointers of A' (column counts of A, exclusive prefix). */
 void dpdlp_host_col_ptr(int m, int n, const int *rp, const int *ci, int *t_rp) {
   memset(t_rp, 0, (size_t)(n + 1) * sizeof(int));

@@ -52,6 +52,7 @@ def _load() -> C.CDLL:
         "dpdlp_resident_free": (None, [C.c_void_p]),
         "dpdlp_resident_info": (None, [C.c_void_p, _ip, _ip, _ip, _ip, _ip]),
         "dpdlp_last_error": (C.c_char_p, []),
+        "dpdlp_write_vector": (C.c_int, [C.c_char_p, _dp, C.c_int]),
         "dpdlp_time_spmv": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
         "dpdlp_generate": (P, [C.POINTER(GenSpec), _dp]),
         "dpdlp_copy_calibration": (C.c_double, [C.c_longlong, C.c_int]),

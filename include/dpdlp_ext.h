@@ -79,6 +79,11 @@ size_t dpdlp_resident_bytes(const dpdlp_resident_t *res);
 /* Last error message of this thread's most recent failing call ("" if none). */
 const char *dpdlp_last_error(void);
 
+/* Writes `size` values to `path`, one "%.10g" per line: the solution files of
+ * the command line (replaces save_vector, src/cli.c:59-79).  The text is
+ * formatted by host worker threads and written in order.  0 or -1. */
+int dpdlp_write_vector(const char *path, const double *data, int size);
+
 /* Time one standalone FP64 SpMV (A x or A' y) on a resident problem with HIP
  * events: returns the mean kernel time in ms over `reps` launches. */
 double dpdlp_time_spmv(dpdlp_resident_t *res, int transpose, int reps, int use32);

@@ -6,7 +6,9 @@ unknown rows, columns and bound types, repeated and non-contiguous column
 names, objective entries and the objective RHS, ranges on every row type, a
 second COLUMNS section, CR-LF endings) must give exactly what a line-by-line
 reading gives: the restatement below, written from those rules, is the
-checker, and every thread count and chunk size must match it bit for bit."""
+checker, and every thread count and chunk size must match it bit for bit.
+The command line's solution writer, which formats on the same threads, must
+give printf's text."""
 import math
 import os
 import random
@@ -309,3 +311,22 @@ def test_restatement_on_the_c_api_example():
     assert ref["con_lb"].tolist() == [5, -math.inf, -math.inf]
     assert ref["con_ub"].tolist() == [5, 2, 8]
     assert ref["var_lb"].tolist() == [-math.inf] * 2
+
+
+def test_solution_writer_matches_printf(dp, tmp_path, monkeypatch):
+    """dpdlp_write_vector (the command line's solution files, cli.c:59-79):
+    one "%.10g" per line, in order, whatever the thread count."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.normal(0, 1, 150_000) * 10.0 ** rng.integers(-30, 30, 150_000),
+                        [0.0, -0.0, 1.0, -1.0, 1e308, -1e-308, 5e-324, 123456789012.0,
+                         math.inf, -math.inf]])
+    want = "".join("%.10g\n" % v for v in x)
+    for threads in ("1", "5"):
+        monkeypatch.setenv("DPDLP_HOST_THREADS", threads)
+        path = tmp_path / f"v{threads}.txt"
+        assert dp.lib.dpdlp_write_vector(str(path).encode(), dp._ptr(x), len(x)) == 0
+        assert path.read_text() == want
+    empty = tmp_path / "empty.txt"
+    assert dp.lib.dpdlp_write_vector(str(empty).encode(), None, 0) == 0
+    assert empty.read_text() == ""
+    assert dp.lib.dpdlp_write_vector(str(tmp_path / "no" / "dir.txt").encode(), dp._ptr(x), 3) == -1
