@@ -3,6 +3,7 @@
  * tiles (layout documented at dpdlp_csr_t in include/dpdlp_dev.h).
  */
 #include "dpdlp_dev.h"
+#include "dpdlp_host.h"
 
 #include <stdlib.h>
 
@@ -30,63 +31,107 @@ int dpdlp_plan_blocks(int rows, const int *rp, int cap, int *blk) {
   return nb;
 }
 
-int dpdlp_build_panels(int rows, int cols, const int *rp, const int *ci, const int *blk,
-                       int nblk, int P, int *toff, unsigned short *rofs, int *perm, int *pci) {
-  long long W = ((long long)cols + P - 1) / P;
-  if (W < 1) W = 1;
-  int *cnt = (int *)calloc((size_t)P + 1, sizeof(int));
-  if (!cnt) return -1;
-  int pos = 0;
-  for (int b = 0; b < nblk; ++b) {
-    int r0 = blk[b], r1 = blk[b + 1], R = r1 - r0;
-    /* tile sizes of this block */
-    for (int k = 0; k <= P; ++k) cnt[k] = 0;
-    for (int e = rp[r0]; e < rp[r1]; ++e) cnt[(int)(ci[e] / W) + 1]++;
-    int base[9];
-    base[0] = pos;
+typedef struct {
+  const int *rp, *ci, *blk;
+  int nblk, P, ntask;
+  long long W;
+  int *toff, *perm, *pci;
+  unsigned short *rofs;
+  int bad; /* a multi-row tile above 65535 entries */
+} bp_job_t;
+
+/* the tiles of blocks [nblk * t / ntask, nblk * (t + 1) / ntask): block b's
+ * tiles fill its own nonzero range [rp[blk[b]], rp[blk[b + 1]]) */
+static void bp_task(void *arg, int t) {
+  bp_job_t *J = (bp_job_t *)arg;
+  const int P = J->P;
+  const long long W = J->W;
+  int b0 = (int)((long long)J->nblk * t / J->ntask), b1 = (int)((long long)J->nblk * (t + 1) / J->ntask);
+  for (int b = b0; b < b1; ++b) {
+    int r0 = J->blk[b], r1 = J->blk[b + 1], R = r1 - r0;
+    int cnt[8] = {0}, base[9], fill[8];
+    for (int e = J->rp[r0]; e < J->rp[r1]; ++e) cnt[(int)(J->ci[e] / W)]++;
+    base[0] = J->rp[r0];
     for (int k = 0; k < P; ++k) {
-      base[k + 1] = base[k] + cnt[k + 1];
-      toff[(size_t)b * P + k] = base[k];
-      if (R > 1 && cnt[k + 1] > 65535) {
-        free(cnt);
-        return -1;
-      }
+      base[k + 1] = base[k] + cnt[k];
+      J->toff[(size_t)b * P + k] = base[k];
+      if (R > 1 && cnt[k] > 65535) __atomic_store_n(&J->bad, 1, __ATOMIC_RELAXED);
+      fill[k] = base[k];
     }
-    int fill[8];
-    for (int k = 0; k < P; ++k) fill[k] = base[k];
-    unsigned short *ro = rofs + (size_t)P * (r0 + b);
+    unsigned short *ro = J->rofs + (size_t)P * (r0 + b);
     for (int i = r0; i < r1; ++i) {
       for (int k = 0; k < P; ++k) ro[(size_t)k * (R + 1) + (i - r0)] = (unsigned short)(fill[k] - base[k]);
-      for (int e = rp[i]; e < rp[i + 1]; ++e) {
-        int k = (int)(ci[e] / W);
-        perm[fill[k]] = e;
-        pci[fill[k]] = ci[e];
+      for (int e = J->rp[i]; e < J->rp[i + 1]; ++e) {
+        int k = (int)(J->ci[e] / W);
+        J->perm[fill[k]] = e;
+        J->pci[fill[k]] = J->ci[e];
         fill[k]++;
       }
     }
     for (int k = 0; k < P; ++k) ro[(size_t)k * (R + 1) + R] = (unsigned short)(fill[k] - base[k]);
-    pos = base[P];
   }
-  toff[(size_t)nblk * P] = pos;
-  free(cnt);
-  return 0;
+}
+
+int dpdlp_build_panels(int rows, int cols, const int *rp, const int *ci, const int *blk,
+                       int nblk, int P, int *toff, unsigned short *rofs, int *perm, int *pci) {
+  (void)rows;
+  if (P < 1 || P > 8) return -1;
+  long long W = ((long long)cols + P - 1) / P;
+  if (W < 1) W = 1;
+  int ntask = nblk < 1 ? 1 : (nblk < 4 * dpdlp_host_threads() ? nblk : 4 * dpdlp_host_threads());
+  bp_job_t J = {rp, ci, blk, nblk, P, ntask, W, toff, perm, pci, rofs, 0};
+  if (nblk > 0) dpdlp_par_for(ntask, bp_task, &J);
+  toff[(size_t)nblk * P] = rp[blk[nblk]];
+  return J.bad ? -1 : 0;
 }
 
 /* Row blocks for the panel-tiled SpMV: consecutive rows while the block has at
  * most rmax rows and every one of its P tiles at most cap entries; a row that
  * cannot share a tile (a panel count above cap) gets a block of its own. */
+/* per-row panel counts, saturated at 65535 (above any cap) */
+typedef struct {
+  const int *rp, *ci;
+  long long W;
+  int P, rows, ntask;
+  unsigned short *rc;
+} pc_job_t;
+
+static void row_panel_counts(const int *rp, const int *ci, long long W, int P, int r, int *c) {
+  for (int k = 0; k < P; ++k) c[k] = 0;
+  for (int e = rp[r]; e < rp[r + 1]; ++e) c[(int)(ci[e] / W)]++;
+}
+
+static void pc_task(void *arg, int t) {
+  pc_job_t *J = (pc_job_t *)arg;
+  int r0 = (int)((long long)J->rows * t / J->ntask), r1 = (int)((long long)J->rows * (t + 1) / J->ntask);
+  int c[8];
+  for (int r = r0; r < r1; ++r) {
+    row_panel_counts(J->rp, J->ci, J->W, J->P, r, c);
+    for (int k = 0; k < J->P; ++k) J->rc[(size_t)r * J->P + k] = (unsigned short)(c[k] > 65535 ? 65535 : c[k]);
+  }
+}
+
 int dpdlp_plan_blocks_panels(int rows, int cols, const int *rp, const int *ci, int cap, int P,
                              int rmax, int *blk) {
   long long W = ((long long)cols + P - 1) / P;
   if (W < 1) W = 1;
+  /* the rows' panel counts, counted by the worker threads (counted while
+   * planning if there is no memory for them) */
+  unsigned short *rc = rows > 0 ? (unsigned short *)malloc((size_t)rows * P * sizeof(unsigned short)) : NULL;
+  if (rc) {
+    pc_job_t J = {rp, ci, W, P, rows, 4 * dpdlp_host_threads(), rc};
+    dpdlp_par_for(J.ntask, pc_task, &J);
+  }
   int tile[8], rowc[8];
   int nb = 0, r = 0;
   while (r < rows) {
     int start = r;
     for (int k = 0; k < P; ++k) tile[k] = 0;
     while (r < rows && r - start < rmax) {
-      for (int k = 0; k < P; ++k) rowc[k] = 0;
-      for (int e = rp[r]; e < rp[r + 1]; ++e) rowc[(int)(ci[e] / W)]++;
+      if (rc)
+        for (int k = 0; k < P; ++k) rowc[k] = rc[(size_t)r * P + k];
+      else
+        row_panel_counts(rp, ci, W, P, r, rowc);
       int fits = 1, alone = 0;
       for (int k = 0; k < P; ++k) {
         if (rowc[k] > cap) alone = 1;
@@ -103,5 +148,6 @@ int dpdlp_plan_blocks_panels(int rows, int cols, const int *rp, const int *ci, i
     blk[nb++] = start;
   }
   blk[nb] = rows;
+  free(rc);
   return nb;
 }

@@ -335,6 +335,26 @@ static int rows_skewed(int m, const int *rp) {
   return mx * (long long)m >= 16LL * rp[m];
 }
 
+/* the entries of renumbered rows [m t / ntask, m (t + 1) / ntask) */
+typedef struct {
+  int m, ntask;
+  const int *perm, *rp, *ci;
+  const double *v;
+  const int *rp2;
+  int *ci2;
+  double *v2;
+} rowcopy_job_t;
+
+static void rowcopy_task(void *arg, int t) {
+  rowcopy_job_t *J = (rowcopy_job_t *)arg;
+  int i0 = (int)((long long)J->m * t / J->ntask), i1 = (int)((long long)J->m * (t + 1) / J->ntask);
+  for (int i = i0; i < i1; ++i) {
+    int o = J->perm[i], a = J->rp[o], L = J->rp[o + 1] - a;
+    memcpy(J->ci2 + J->rp2[i], J->ci + a, (size_t)L * sizeof(int));
+    memcpy(J->v2 + J->rp2[i], J->v + a, (size_t)L * sizeof(double));
+  }
+}
+
 /* perm[new] = old: rows by decreasing length, ties in row order (a stable
  * counting sort) */
 static int row_order_by_length(int m, const int *rp, int *perm) {
@@ -476,13 +496,14 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     if (row_order_by_length(m, rp, perm)) goto ufail;
     rp2[0] = 0;
     for (int i = 0; i < m; ++i) {
-      int o = perm[i], a = rp[o], L = rp[o + 1] - a;
-      memcpy(ci2 + rp2[i], ci + a, (size_t)L * sizeof(int));
-      memcpy(v2 + rp2[i], p->constraint_matrix_values + a, (size_t)L * sizeof(double));
-      rp2[i + 1] = rp2[i] + L;
+      int o = perm[i];
+      rp2[i + 1] = rp2[i] + (rp[o + 1] - rp[o]);
       lc2[i] = p->constraint_lower_bound[o];
       uc2[i] = p->constraint_upper_bound[o];
     }
+    rowcopy_job_t J = {m, 4 * dpdlp_host_threads(), perm, rp, ci, p->constraint_matrix_values,
+                       rp2, ci2, v2};
+    dpdlp_par_for(J.ntask, rowcopy_task, &J);
     if (r->h_dstart) {
       double *d2 = (double *)malloc((size_t)(m ? m : 1) * sizeof(double));
       if (!d2) goto ufail;
@@ -593,18 +614,15 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     if (P <= 1) continue;
     int rows = which ? n : m, cols = which ? m : n, nb = which ? r->At_nblk : r->A_nblk;
     const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
-    const double *hv = which ? t_v : v_u;
     const int *blkh = which ? r->h_At_blk : r->h_A_blk;
     int *toff = (int *)malloc(((size_t)nb * P + 1) * sizeof(int));
     unsigned short *rofs =
         (unsigned short *)malloc((size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
     int *perm = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
     int *pci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
-    double *pv = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
-    int ok = toff && rofs && perm && pci && pv;
+    int ok = toff && rofs && perm && pci;
     if (ok) ok = dpdlp_build_panels(rows, cols, hrp, hci, blkh, nb, P, toff, rofs, perm, pci) == 0;
     if (ok) {
-      for (int k = 0; k < nnz; ++k) pv[k] = hv[perm[k]];
       hipError_t e = hipSuccess;
       int **dt = which ? &r->At_toff : &r->A_toff, **dp = which ? &r->At_perm : &r->A_perm;
       int **dc = which ? &r->At_pci : &r->A_pci;
@@ -616,7 +634,9 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
                (size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
       if (e == hipSuccess) e = up(&K, (void **)dp, perm, (size_t)nnz * sizeof(int));
       if (e == hipSuccess) e = up(&K, (void **)dc, pci, (size_t)nnz * sizeof(int));
-      if (e == hipSuccess) e = up(&K, (void **)dv, pv, (size_t)nnz * sizeof(double));
+      /* the tiled values, pv[t] = v[perm[t]], are gathered on the device
+       * once the CSR values have arrived */
+      if (e == hipSuccess) e = up(&K, (void **)dv, NULL, (size_t)nnz * sizeof(double));
       ok = e == hipSuccess;
       if (!ok) dpdlp_set_error("HIP error uploading panel tiles: %s", hipGetErrorString(e));
     }
@@ -624,7 +644,6 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
     free(rofs);
     free(perm);
     free(pci);
-    free(pv);
     if (!ok) {
       dpdlp_set_error("building column-panel tiles failed");
       goto ufail;
@@ -636,6 +655,10 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
   UPCHK(up(&K, (void **)&r->lc, lc_u, (size_t)m * sizeof(double)));
   UPCHK(up(&K, (void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
   UPCHK(sink_close(&K));
+  if (r->A_P > 1 && nnz > 0) UPCHK((hipError_t)dpdlp_dev_permute(r->A_v, r->A_perm, r->A_pv, nnz, K.s));
+  if (r->At_P > 1 && nnz > 0)
+    UPCHK((hipError_t)dpdlp_dev_permute(r->At_v, r->At_perm, r->At_pv, nnz, K.s));
+  if (!dev_tr || nnz == 0) UPCHK(hipStreamSynchronize(K.s));
   if (dev_tr && nnz > 0) {
     size_t sb = dpdlp_dev_transpose_scratch_bytes(nnz), scap = sb;
     char *sc = NULL;

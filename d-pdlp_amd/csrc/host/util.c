@@ -123,10 +123,33 @@ int dpdlp_write_vector(const char *path, const double *data, int size) {
   return rc;
 }
 
-/* Row pointers of A' (column counts of A, exclusive prefix). */
+/* Row pointers of A' (column counts of A, exclusive prefix).  Each task
+ * owns a range of columns and counts the entries that fall in it. */
+typedef struct {
+  int n, ntask;
+  long long nnz;
+  const int *ci;
+  int *t_rp;
+} colcnt_job_t;
+
+static void colcnt_task(void *arg, int t) {
+  colcnt_job_t *J = (colcnt_job_t *)arg;
+  int c0 = (int)((long long)J->n * t / J->ntask), c1 = (int)((long long)J->n * (t + 1) / J->ntask);
+  unsigned w = (unsigned)(c1 - c0);
+  int *cnt = J->t_rp + 1 + c0;
+  for (long long k = 0; k < J->nnz; ++k) {
+    unsigned d = (unsigned)(J->ci[k] - c0);
+    if (d < w) cnt[d]++;
+  }
+}
+
 void dpdlp_host_col_ptr(int m, int n, const int *rp, const int *ci, int *t_rp) {
   memset(t_rp, 0, (size_t)(n + 1) * sizeof(int));
-  for (int k = 0; k < rp[m]; ++k) t_rp[ci[k] + 1]++;
+  long long nnz = rp[m];
+  int ntask = nnz < (1 << 20) ? 1 : dpdlp_host_threads();
+  if (ntask > n) ntask = n > 0 ? n : 1;
+  colcnt_job_t J = {n, ntask, nnz, ci, t_rp};
+  dpdlp_par_for(ntask, colcnt_task, &J);
   for (int j = 0; j < n; ++j) t_rp[j + 1] += t_rp[j];
 }
 
