@@ -247,6 +247,32 @@ def test_panel_tiles_reconstruct_spmv(dp, P):
     assert np.allclose(y, ref, rtol=1e-13, atol=1e-12)
 
 
+@pytest.mark.parametrize("threads", ["1", "6"])
+def test_column_counts_and_panel_plan_any_thread_count(dp, monkeypatch, threads):
+    """A' row pointers (column counts, split over worker threads by column
+    range) equal numpy's; the panel block plan (per-row panel counts on
+    worker threads) is the same at every thread count."""
+    monkeypatch.setenv("DPDLP_HOST_THREADS", threads)
+    L = dp.lib
+    rng = np.random.default_rng(7)
+    m, n = 3000, 5000
+    lens = np.minimum((rng.pareto(1.1, m) * 6).astype(int), 4000)
+    rp, ci, _ = random_csr(m, n, lens, 3)
+    Pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    # column counts over 2^21 entries (enough for the threaded count)
+    big_ci = rng.integers(0, n, 1 << 21).astype(np.int32)
+    big_rp = np.linspace(0, len(big_ci), m + 1).astype(np.int32)
+    trp = np.zeros(n + 1, np.int32)
+    L.dpdlp_host_col_ptr(m, n, Pi(big_rp), Pi(big_ci), Pi(trp))
+    assert np.array_equal(trp, np.concatenate([[0], np.cumsum(np.bincount(big_ci, minlength=n))]))
+    blk = np.zeros(m + 2, np.int32)
+    nb = L.dpdlp_plan_blocks_panels(m, n, Pi(rp), Pi(ci), 1024, 8, 1024, Pi(blk))
+    monkeypatch.setenv("DPDLP_HOST_THREADS", "1")
+    blk1 = np.zeros(m + 2, np.int32)
+    nb1 = L.dpdlp_plan_blocks_panels(m, n, Pi(rp), Pi(ci), 1024, 8, 1024, Pi(blk1))
+    assert nb == nb1 and np.array_equal(blk[:nb + 1], blk1[:nb1 + 1])
+
+
 def test_generator_rules(dp):
     p, opt = dp.generate(3000, 7000, 45000, row_rule=1, seed=5)
     assert p.nnz == 45000
