@@ -80,8 +80,9 @@ void dpdlp_set_error(const char *fmt, ...);
 double dpdlp_wall(void);
 
 /* util.c: host worker threads for the data-parallel loops of ingestion and
- * upload.  dpdlp_host_threads() is DPDLP_HOST_THREADS when set, else the
- * online CPUs, clamped to [1, 32].  dpdlp_par_for runs fn(ctx, t) for every
+ * upload.  dpdlp_host_threads() is DPDLP_HOST_THREADS when set, else
+ * OMP_NUM_THREADS when set, else the CPUs this process may run on, clamped
+ * to [1, 32].  dpdlp_par_for runs fn(ctx, t) for every
  * t in [0, ntask) on up to that many threads (the caller's included; tasks
  * are taken in increasing order) and returns when all have finished. */
 int dpdlp_host_threads(void);

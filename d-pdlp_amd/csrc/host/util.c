@@ -2,11 +2,12 @@
  * Host utilities: thread-local error text, wall clock, worker threads,
  * stable transpose.
  */
-#define _POSIX_C_SOURCE 200809L
+#define _GNU_SOURCE
 #include "dpdlp_ext.h"
 #include "dpdlp_host.h"
 
 #include <pthread.h>
+#include <sched.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -33,7 +34,13 @@ double dpdlp_wall(void) {
 
 int dpdlp_host_threads(void) {
   const char *env = getenv("DPDLP_HOST_THREADS");
-  long t = env ? atol(env) : sysconf(_SC_NPROCESSORS_ONLN);
+  if (!env || !*env) env = getenv("OMP_NUM_THREADS");
+  long t = env && *env ? atol(env) : 0;
+  if (t < 1) {
+    cpu_set_t set;
+    t = sched_getaffinity(0, sizeof(set), &set) == 0 ? CPU_COUNT(&set)
+                                                      : sysconf(_SC_NPROCESSORS_ONLN);
+  }
   if (t < 1) t = 1;
   if (t > 32) t = 32;
   return (int)t;
