@@ -162,22 +162,43 @@ void dpdlp_host_col_ptr(int m, int n, const int *rp, const int *ci, int *t_rp) {
 
 /* Counting-sort transpose; entries of each output row keep increasing source
  * row order (the layout csr2csc produces at pdlp_core_op.cu:111-128). */
+/* the entries of columns [n t / ntask, n (t + 1) / ntask): a scan of A in
+ * row order that keeps its own columns, so each A' row lists its entries in
+ * increasing row order */
+typedef struct {
+  int m, n, ntask;
+  const int *rp, *ci;
+  const double *v;
+  int *cursor, *t_ci;
+  double *t_v;
+} tr_job_t;
+
+static void tr_task(void *arg, int t) {
+  tr_job_t *J = (tr_job_t *)arg;
+  int c0 = (int)((long long)J->n * t / J->ntask), c1 = (int)((long long)J->n * (t + 1) / J->ntask);
+  unsigned w = (unsigned)(c1 - c0);
+  int *cur = J->cursor + c0;
+  for (int i = 0; i < J->m; ++i)
+    for (int k = J->rp[i]; k < J->rp[i + 1]; ++k) {
+      unsigned d = (unsigned)(J->ci[k] - c0);
+      if (d < w) {
+        int dst = cur[d]++;
+        J->t_ci[dst] = i;
+        J->t_v[dst] = J->v[k];
+      }
+    }
+}
+
 int dpdlp_host_transpose(int m, int n, const int *rp, const int *ci, const double *v,
                          int *t_rp, int *t_ci, double *t_v) {
   int *cursor = (int *)malloc((size_t)(n + 1) * sizeof(int));
   if (!cursor) return -1;
-  memset(t_rp, 0, (size_t)(n + 1) * sizeof(int));
-  for (int i = 0; i < m; ++i)
-    for (int k = rp[i]; k < rp[i + 1]; ++k) t_rp[ci[k] + 1]++;
-  for (int j = 0; j < n; ++j) t_rp[j + 1] += t_rp[j];
+  dpdlp_host_col_ptr(m, n, rp, ci, t_rp);
   memcpy(cursor, t_rp, (size_t)(n + 1) * sizeof(int));
-  for (int i = 0; i < m; ++i) {
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      int dst = cursor[ci[k]]++;
-      t_ci[dst] = i;
-      t_v[dst] = v[k];
-    }
-  }
+  int ntask = rp[m] < (1 << 20) ? 1 : dpdlp_host_threads();
+  if (ntask > n) ntask = n > 0 ? n : 1;
+  tr_job_t J = {m, n, ntask, rp, ci, v, cursor, t_ci, t_v};
+  dpdlp_par_for(ntask, tr_task, &J);
   free(cursor);
   return 0;
 }

@@ -273,6 +273,30 @@ def test_column_counts_and_panel_plan_any_thread_count(dp, monkeypatch, threads)
     assert nb == nb1 and np.array_equal(blk[:nb + 1], blk1[:nb1 + 1])
 
 
+@pytest.mark.parametrize("threads", ["1", "5"])
+def test_host_transpose_is_a_stable_sort(dp, monkeypatch, threads):
+    """The host transpose (worker threads own column ranges) equals a stable
+    sort of the nonzeros by column: each A' row in increasing row order, the
+    layout csr2csc produces (pdlp_core_op.cu:111-128)."""
+    monkeypatch.setenv("DPDLP_HOST_THREADS", threads)
+    rng = np.random.default_rng(11)
+    m, n, nnz = 50_000, 70_000, 1 << 21
+    rp = np.sort(rng.integers(0, nnz + 1, m - 1)).astype(np.int32)
+    rp = np.concatenate([[0], rp, [nnz]]).astype(np.int32)
+    ci = rng.integers(0, n, nnz).astype(np.int32)
+    v = rng.standard_normal(nnz)
+    t_rp = np.zeros(n + 1, np.int32)
+    t_ci = np.zeros(nnz, np.int32)
+    t_v = np.zeros(nnz)
+    Pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    assert dp.lib.dpdlp_host_transpose(m, n, Pi(rp), Pi(ci), dp._ptr(v), Pi(t_rp), Pi(t_ci),
+                                       dp._ptr(t_v)) == 0
+    order = np.argsort(ci, kind="stable")
+    rows = np.repeat(np.arange(m, dtype=np.int32), np.diff(rp))
+    assert np.array_equal(t_rp, np.concatenate([[0], np.cumsum(np.bincount(ci, minlength=n))]))
+    assert np.array_equal(t_ci, rows[order]) and np.array_equal(t_v, v[order])
+
+
 def test_generator_rules(dp):
     p, opt = dp.generate(3000, 7000, 45000, row_rule=1, seed=5)
     assert p.nnz == 45000
