@@ -30,16 +30,16 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 HBM_PEAK_GBPS = 8000.0
 
 WORKLOADS = {
-    "cfg1": dict(m=1_000_000, n=2_000_000, nnz=20_000_000, row_rule=1, pm1_frac=0.1, seed=1000,
+    "zipf-1m": dict(m=1_000_000, n=2_000_000, nnz=20_000_000, row_rule=1, pm1_frac=0.1, seed=1000,
                  desc="m=1e6 n=2e6 nnz=2e7 Zipf(1.5) rows, FP64, eps 1e-4"),
-    "cfg0": dict(m=2_000, n=5_000, nnz=50_000, row_rule=0, pm1_frac=0.0, seed=0,
+    "small-2k": dict(m=2_000, n=5_000, nnz=50_000, row_rule=0, pm1_frac=0.0, seed=0,
                  desc="m=2e3 n=5e3 nnz=5e4 uniform positions, FP64, eps 1e-4"),
-    "cfg2": dict(m=4_000_000, n=8_000_000, nnz=120_000_000, row_rule=1, pm1_frac=0.0,
+    "zipf-4m": dict(m=4_000_000, n=8_000_000, nnz=120_000_000, row_rule=1, pm1_frac=0.0,
                  seed=2000, desc="m=4e6 n=8e6 nnz=1.2e8 Zipf(1.5) rows"),
-    "cfg3": dict(count=512, base_seed=3000, m_lo=500, m_hi=200_000,
+    "batch-512": dict(count=512, base_seed=3000, m_lo=500, m_hi=200_000,
                  desc="batch of 512 LPs: m log-uniform [500, 2e5], n = m*U[1.5,3], "
                       "Poisson(8)+1 nnz/row, seeds 3000..3511, eps 1e-4"),
-    "cfg4": dict(rates=(50.0, 200.0, 800.0), large_frac=0.05, large_m=1_000_000,
+    "serve-poisson": dict(rates=(50.0, 200.0, 800.0), large_frac=0.05, large_m=1_000_000,
                  large_nnz=10_000_000, budget_gb=200.0, pool_small=64,
                  desc="serving: Poisson arrivals, LPs from the batch generator + 5% "
                       "m=1e6 nnz=1e7, 200 GB admission, eps 1e-4"),
@@ -87,7 +87,7 @@ def cpu_batch_baseline(probs, budget_s):
 def batch_traffic(kernel):
     """Mean HBM bytes per launch of a batched phase kernel, from the committed
     counter profile (FETCH_SIZE/WRITE_SIZE passes over this same leg)."""
-    tfile = os.path.join(ROOT, "profiles", "cfg3_batch_hbm_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "batch512_hbm_traffic.json")
     if not os.path.exists(tfile):
         return None
     with open(tfile) as f:
@@ -95,7 +95,7 @@ def batch_traffic(kernel):
 
 
 def bench_batch(args, dpdlp):
-    w = WORKLOADS["cfg3"]
+    w = WORKLOADS["batch-512"]
     count = args.batch_count or w["count"]
     t = time.perf_counter()
     probs = [dpdlp.batch_member(k, w["base_seed"], w["m_lo"], w["m_hi"]) for k in range(count)]
@@ -104,13 +104,16 @@ def bench_batch(args, dpdlp):
     print(f"# generated {count} LPs, nnz={nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
     t = time.perf_counter()
     plain_res = [dpdlp.Resident(p) for p, _ in probs] if not args.no_plain else []
-    batch_res = [dpdlp.Resident(p, spmv_cap=BATCH_CAP, panels=(1, 1)) for p, _ in probs]
+    cap_at = args.batch_cap_at or BATCH_CAP
+    batch_res = [dpdlp.Resident(p, spmv_cap=BATCH_CAP, panels=(1, 1), spmv_cap_At=cap_at)
+                 for p, _ in probs]
     up_s = time.perf_counter() - t
     params = dpdlp.default_params(presolve=False)
     for k in range(args.warmup):
         if plain_res:
             plain_res[0].solve(params)
-        dpdlp.solve_batch(batch_res[:4], params, spmv_cap=BATCH_CAP, precision=dpdlp.PREC_MIXED)
+        dpdlp.solve_batch(batch_res[:4], params, spmv_cap=BATCH_CAP, spmv_cap_At=cap_at,
+                          precision=dpdlp.PREC_MIXED)
     # plain loop: the single-LP FP64 path, one LP after another
     plain, plain_s = None, None
     if not args.no_plain:
@@ -124,7 +127,7 @@ def bench_batch(args, dpdlp):
         st = {}
         t0 = time.perf_counter()
         got = dpdlp.solve_batch(batch_res, params, stats=st, max_lps=count, spmv_cap=BATCH_CAP,
-                                precision=prec, profile_phases=profile)
+                                spmv_cap_At=cap_at, precision=prec, profile_phases=profile)
         el = time.perf_counter() - t0
         cnt = {k: st[k] for k in ("windows", "syncs", "launches")}
         ph = {"samples": st["phase_samples"], "primal_ms": st["primal_ms"],
@@ -170,7 +173,7 @@ def bench_batch(args, dpdlp):
         "warmup": args.warmup, "ms_per_step": step_s * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32-matrix/f64",
         "data": "synthetic (seeded generator, feasible by construction)",
-        "config": {"workload": f"cfg3: {w['desc']}", "lps": count, "nnz_total": nnz,
+        "config": {"workload": f"batch-512: {w['desc']}", "lps": count, "nnz_total": nnz,
                    "m_total": sum(p.m for p, _ in probs), "n_total": sum(p.n for p, _ in probs),
                    "parallelism": "one GPU, batched launches"},
         "statuses": statuses,
@@ -199,7 +202,7 @@ def bench_batch(args, dpdlp):
 
 def bench_serve(args, dpdlp):
     import numpy as np
-    w = WORKLOADS["cfg4"]
+    w = WORKLOADS["serve-poisson"]
     params = dpdlp.default_params(presolve=False)
     rates = [float(x) for x in args.rates.split(",")] if args.rates else list(w["rates"])
     per = []
@@ -236,7 +239,7 @@ def bench_serve(args, dpdlp):
         "warmup": 0, "ms_per_step": top["batched"]["makespan_sec"] * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32-matrix/f64", "data": "synthetic (seeded generator, feasible by construction)",
-        "config": {"workload": f"cfg4: {w['desc']}", "rates": rates,
+        "config": {"workload": f"serve-poisson: {w['desc']}", "rates": rates,
                    "arrival_window_s": args.serve_duration, "drain_cap_s": args.serve_drain,
                    "max_lps": args.serve_max_lps},
         "per_rate": per,
@@ -324,7 +327,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg1", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="zipf-1m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--spmv-cap", type=int, default=0)
@@ -333,14 +336,16 @@ def main():
                     help="also run a 256 MiB plain copy kernel (counter calibration)")
     ap.add_argument("--mixed", action="store_true", help="FP32 matrix / FP64 state")
     ap.add_argument("--verbose", action="store_true", help="print the solver's iteration log")
-    ap.add_argument("--batch-count", type=int, default=0, help="cfg3: LPs in the batch")
-    ap.add_argument("--cpu-budget", type=float, default=20.0, help="cfg3: CPU sample seconds")
-    ap.add_argument("--no-plain", action="store_true", help="cfg3: skip the plain-loop leg")
-    ap.add_argument("--no-fp64-batch", action="store_true", help="cfg3: skip the FP64 batch leg")
-    ap.add_argument("--rates", default="", help="cfg4: comma-separated arrival rates")
-    ap.add_argument("--serve-duration", type=float, default=30.0, help="cfg4: arrival window s")
-    ap.add_argument("--serve-drain", type=float, default=30.0, help="cfg4: drain cap s")
-    ap.add_argument("--serve-max-lps", type=int, default=512, help="cfg4: LPs in flight")
+    ap.add_argument("--batch-count", type=int, default=0, help="batch: LPs in the batch")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="batch: CPU sample seconds")
+    ap.add_argument("--no-plain", action="store_true", help="batch: skip the plain-loop leg")
+    ap.add_argument("--batch-cap-at", type=int, default=0,
+                    help="batch: row-block capacity of the batched A' (0: as A)")
+    ap.add_argument("--no-fp64-batch", action="store_true", help="batch: skip the FP64 batch leg")
+    ap.add_argument("--rates", default="", help="serving: comma-separated arrival rates")
+    ap.add_argument("--serve-duration", type=float, default=30.0, help="serving: arrival window s")
+    ap.add_argument("--serve-drain", type=float, default=30.0, help="serving: drain cap s")
+    ap.add_argument("--serve-max-lps", type=int, default=512, help="serving: LPs in flight")
     ap.add_argument("--compare-precision", type=float, default=0.0, metavar="EPS",
                     help="solve to EPS in FP64 and in mixed precision and report both")
     args = ap.parse_args()
@@ -350,9 +355,9 @@ def main():
         sys.exit(2)
 
     import dpdlp
-    if args.workload == "cfg3":
+    if args.workload == "batch-512":
         return bench_batch(args, dpdlp)
-    if args.workload == "cfg4":
+    if args.workload == "serve-poisson":
         return bench_serve(args, dpdlp)
     w = WORKLOADS[args.workload]
     t = time.perf_counter()
@@ -433,8 +438,8 @@ def main():
                                  "slice of its own XCD (L2-resident)",
                   dominant_frac=(nnz / (dom[2] * 1e-3) / 1e9) / gceil if gceil > 0 else None)
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "cfg1_fp64_hbm_traffic.json")
-    if args.workload == "cfg1" and not args.mixed and os.path.exists(tfile):
+    tfile = os.path.join(ROOT, "profiles", "zipf1m_fp64_hbm_traffic.json")
+    if args.workload == "zipf-1m" and not args.mixed and os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)
         traffic = tj.get("bytes_per_launch", {}).get(dom[0])

@@ -330,7 +330,7 @@ static hipError_t launch_b(const dpdlp_batch_dev_t *b, int slot0, hipStream_t s)
   const dpdlp_bblk_t *list = T ? b->bAt : b->bA;
   int n = T ? b->nbAt : b->nbA;
   if (n <= 0) return hipSuccess;
-  switch (b->cap) {
+  switch (T ? b->cap_At : b->cap) {
   case 1024:
     k_bspmv<KIND, 1024><<<n, DPDLP_BS, 0, s>>>(b->lps, b->ctl, list, b->part, b->stride, slot0);
     break;

@@ -267,6 +267,8 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
   if (b->o.max_lps <= 0) b->o.max_lps = 1024;
   if (b->o.spmv_cap != 1024 && b->o.spmv_cap != 2048 && b->o.spmv_cap != 4096)
     b->o.spmv_cap = 4096;
+  if (b->o.spmv_cap_At != 1024 && b->o.spmv_cap_At != 2048 && b->o.spmv_cap_At != 4096)
+    b->o.spmv_cap_At = b->o.spmv_cap;
   if (b->prm.termination_evaluation_frequency <= 0) {
     dpdlp_set_error("termination_evaluation_frequency must be positive");
     pthread_mutex_destroy(&b->mu);
@@ -494,6 +496,8 @@ void dpdlp_i_batch_reap(dpdlp_batch_t *b, size_t keep_bytes) {
   }
 }
 
+int dpdlp_i_batch_cap_At(const dpdlp_batch_t *b) { return b->o.spmv_cap_At; }
+
 char *dpdlp_i_batch_get_mem(dpdlp_batch_t *b, size_t need, size_t *cap) {
   char *p = NULL;
   take_arena(b, need, &p, cap);
@@ -538,9 +542,10 @@ void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int i) {
  * joins at the next window start. */
 int dpdlp_i_batch_prepare_issue(dpdlp_batch_t *b, int i, dpdlp_resident_t *R,
                                 int take_ownership, long long tag, void *stream) {
-  if (R->cap != b->o.spmv_cap || R->At_cap != b->o.spmv_cap || R->A_P != 1 || R->At_P != 1 || R->h_row_perm) {
-    dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
-                    b->o.spmv_cap);
+  if (R->cap != b->o.spmv_cap || R->At_cap != b->o.spmv_cap_At || R->A_P != 1 || R->At_P != 1 ||
+      R->h_row_perm) {
+    dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_caps(prob, %d, %d, 1, 1)",
+                    b->o.spmv_cap, b->o.spmv_cap_At);
     dpdlp_i_batch_unreserve(b, i);
     return -1;
   }
@@ -805,6 +810,7 @@ static int rebuild_lists(dpdlp_batch_t *b) {
     d->part = b->d_part;
     d->stride = (int)need_stride;
     d->cap = b->o.spmv_cap;
+    d->cap_At = b->o.spmv_cap_At;
   }
   b->dirty = 0;
   b->epoch++;
@@ -1482,12 +1488,16 @@ int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parame
   else
     dpdlp_batch_default_opts(&local);
   if (local.max_lps < count) local.max_lps = count;
+  const int cap_At = (local.spmv_cap_At == 1024 || local.spmv_cap_At == 2048 ||
+                      local.spmv_cap_At == 4096)
+                         ? local.spmv_cap_At
+                         : local.spmv_cap;
   for (int i = 0; i < count; ++i) results[i] = NULL;
   for (int i = 0; i < count; ++i)
-    if (!res[i] || res[i]->cap != local.spmv_cap || res[i]->At_cap != local.spmv_cap || res[i]->A_P != 1 || res[i]->At_P != 1 ||
-        res[i]->h_row_perm) {
-      dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_ex(prob, %d, 1, 1)",
-                      local.spmv_cap);
+    if (!res[i] || res[i]->cap != local.spmv_cap || res[i]->At_cap != cap_At || res[i]->A_P != 1 ||
+        res[i]->At_P != 1 || res[i]->h_row_perm) {
+      dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_caps(prob, %d, %d, 1, 1)",
+                      local.spmv_cap, cap_At);
       return -1;
     }
   dpdlp_batch_t *b = dpdlp_batch_create(params, &local);

@@ -47,8 +47,9 @@ typedef struct {
 } dpdlp_uploader_t;
 
 /* dpdlp_upload_ex through an uploader */
-struct dpdlp_resident *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, int panels_A,
-                                           int panels_At, const dpdlp_uploader_t *u);
+struct dpdlp_resident *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, int spmv_cap_At,
+                                           int panels_A, int panels_At,
+                                           const dpdlp_uploader_t *u);
 /* frees the host side of r and returns its device blob (and its size) */
 char *dpdlp_i_resident_release(struct dpdlp_resident *r, size_t *cap);
 

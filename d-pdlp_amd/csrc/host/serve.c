@@ -212,7 +212,7 @@ static void *lane_main(void *arg) {
     dpdlp_default_opts(&o);
     o.quiet = 1;
     o.precision = sp->precision;
-    dpdlp_resident_t *R = dpdlp_i_upload_into(S->pool[S->req_pool[k]], 0, 0, 0, &up);
+    dpdlp_resident_t *R = dpdlp_i_upload_into(S->pool[S->req_pool[k]], 0, 0, 0, 0, &up);
     cupdlpx_result_t *r = R ? dpdlp_solve_resident(R, &prm, &o) : NULL;
     if (R) {
       size_t cap = 0;
@@ -292,7 +292,8 @@ static void *loader_main(void *arg) {
       if (stop) break;
     }
     if (slot < 0) break;
-    dpdlp_resident_t *R = dpdlp_i_upload_into(S->pool[S->req_pool[k]], sp->spmv_cap, 1, 1, &up);
+    dpdlp_resident_t *R = dpdlp_i_upload_into(S->pool[S->req_pool[k]], sp->spmv_cap,
+                                              dpdlp_i_batch_cap_At(S->b), 1, 1, &up);
     if (!R || dpdlp_i_batch_prepare(S->b, slot, R, 1, k, S->stream)) {
       if (!R) dpdlp_i_batch_unreserve(S->b, slot);
       if (R) {

@@ -157,8 +157,8 @@ void dpdlp_default_opts(dpdlp_solve_opts_t *o) {
   o->time_to_1e6_sec = -1.0;
 }
 
-static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt,
-                                     int order, const dpdlp_uploader_t *u);
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int cap_At, int PA,
+                                     int PAt, int order, const dpdlp_uploader_t *u);
 
 /* frees the host side of r and hands back its device blob */
 char *dpdlp_i_resident_release(dpdlp_resident_t *r, size_t *cap) {
@@ -175,9 +175,9 @@ char *dpdlp_i_resident_release(dpdlp_resident_t *r, size_t *cap) {
   return b;
 }
 
-dpdlp_resident_t *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, int panels_A,
-                                      int panels_At, const dpdlp_uploader_t *u) {
-  return upload_with(p, spmv_cap, panels_A, panels_At,
+dpdlp_resident_t *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, int spmv_cap_At,
+                                      int panels_A, int panels_At, const dpdlp_uploader_t *u) {
+  return upload_with(p, spmv_cap, spmv_cap_At, panels_A, panels_At,
                      (panels_A > 0 || panels_At > 0) ? 0 : -1, u);
 }
 
@@ -264,7 +264,7 @@ static hipError_t sink_close(usink_t *k) {
 
 
 dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
-  return upload_with(p, spmv_cap, 0, 0, -1, NULL);
+  return upload_with(p, spmv_cap, 0, 0, 0, -1, NULL);
 }
 
 dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panels_A,
@@ -275,7 +275,22 @@ dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panel
     return NULL;
   }
   /* explicit layouts (the batch's plain one) keep the row order */
-  return upload_with(p, spmv_cap, panels_A, panels_At,
+  return upload_with(p, spmv_cap, 0, panels_A, panels_At,
+                     (panels_A > 0 || panels_At > 0) ? 0 : -1, NULL);
+}
+
+dpdlp_resident_t *dpdlp_upload_caps(const lp_problem_t *p, int spmv_cap, int spmv_cap_At,
+                                    int panels_A, int panels_At) {
+  if ((panels_A && panels_A != 1 && panels_A != 2 && panels_A != 4 && panels_A != 8) ||
+      (panels_At && panels_At != 1 && panels_At != 2 && panels_At != 4 && panels_At != 8)) {
+    dpdlp_set_error("panel counts must be 0 (automatic), 1, 2, 4 or 8");
+    return NULL;
+  }
+  if (spmv_cap_At && spmv_cap_At != 1024 && spmv_cap_At != 2048 && spmv_cap_At != 4096) {
+    dpdlp_set_error("row-block capacities must be 0 (automatic), 1024, 2048 or 4096");
+    return NULL;
+  }
+  return upload_with(p, spmv_cap, spmv_cap_At, panels_A, panels_At,
                      (panels_A > 0 || panels_At > 0) ? 0 : -1, NULL);
 }
 
@@ -302,7 +317,7 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
   double bestA = 1e30, bestT = 1e30;
   int PA = 1, PT = 1;
   for (int i = 0; i < 4; ++i) {
-    dpdlp_resident_t *r = upload_with(p, spmv_cap, cand[i], cand[i], -1, NULL);
+    dpdlp_resident_t *r = upload_with(p, spmv_cap, 0, cand[i], cand[i], -1, NULL);
     if (!r) continue;
     double ta, tt;
     time_both(r, &ta, &tt);
@@ -320,7 +335,7 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
     chosen[0] = PA;
     chosen[1] = PT;
   }
-  return upload_with(p, spmv_cap, PA, PT, -1, NULL);
+  return upload_with(p, spmv_cap, 0, PA, PT, -1, NULL);
 }
 
 /* row lengths skewed enough for the row renumbering to pay: the longest row
@@ -388,8 +403,8 @@ static void free_perm_copies(dpdlp_resident_t *r) {
   r->h_perm_v = r->h_perm_lc = r->h_perm_uc = NULL;
 }
 
-static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA, int PAt,
-                                     int order, const dpdlp_uploader_t *u) {
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int cap_At, int PA,
+                                     int PAt, int order, const dpdlp_uploader_t *u) {
   double t0 = dpdlp_wall();
   usink_t K;
   memset(&K, 0, sizeof(K));
@@ -540,6 +555,7 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int PA
    * 4096-nonzero blocks (its rows, the columns of A, are short and many) */
   r->At_cap = r->cap;
   if (spmv_cap <= 0 && r->At_P == 1 && nnz >= 4000000) r->At_cap = 4096;
+  if (cap_At > 0) r->At_cap = cap_At;
 
   /* A': an untiled A' is transposed on the device after the upload (the
    * host only counts columns for its row pointers); A' tiles are cut on the
@@ -1255,7 +1271,7 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     dpdlp_set_error("termination_evaluation_frequency must be positive");
     return NULL;
   }
-  if (opts->use_graph && R->A_P == 1 && R->At_P == 1 && R->At_cap == R->cap &&
+  if (opts->use_graph && R->A_P == 1 && R->At_P == 1 && !R->h_row_perm &&
       opts->trace_points <= 0 &&
       !opts->profile_phases) {
     /* the windowed engine with a single LP: per-iteration parameters live in
@@ -1266,6 +1282,7 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     bo.precision = opts->precision;
     bo.mixed_switch_tol = opts->mixed_switch_tol;
     bo.spmv_cap = R->cap;
+    bo.spmv_cap_At = R->At_cap;
     bo.use_graph = 1;
     cupdlpx_result_t *r1 = NULL;
     if (dpdlp_solve_batch(&R, 1, params, &bo, &r1) != 1) return NULL;

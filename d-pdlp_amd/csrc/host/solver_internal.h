@@ -86,5 +86,7 @@ void dpdlp_i_batch_abandon(dpdlp_batch_t *b, int slot);
 void dpdlp_i_batch_reap(dpdlp_batch_t *b, size_t keep_bytes);
 /* device memory from / back to the batch's pool of recycled allocations */
 char *dpdlp_i_batch_get_mem(dpdlp_batch_t *b, size_t need, size_t *cap);
+/* the row-block capacity the batch's LPs use for A' */
+int dpdlp_i_batch_cap_At(const dpdlp_batch_t *b);
 void dpdlp_i_batch_put_mem(dpdlp_batch_t *b, char *p, size_t cap);
 size_t dpdlp_i_batch_pool_bytes(dpdlp_batch_t *b);

@@ -63,6 +63,7 @@ def _load() -> C.CDLL:
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
         "dpdlp_resident_transpose": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
         "dpdlp_upload_ex": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int]),
+        "dpdlp_upload_caps": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
         "dpdlp_resident_bytes": (C.c_size_t, [C.c_void_p]),
         "dpdlp_batch_member_spec": (None, [C.c_int, C.c_uint64, C.c_int, C.c_int,
                                            C.POINTER(GenSpec)]),
@@ -314,9 +315,13 @@ def solve_plain(problem: Problem, params: PdhgParameters | None = None) -> Resul
 class Resident:
     """A problem uploaded to HBM once (dpdlp_upload) and solved many times."""
 
-    def __init__(self, problem: Problem, spmv_cap=0, tune=False, panels=None):
+    def __init__(self, problem: Problem, spmv_cap=0, tune=False, panels=None, spmv_cap_At=0):
         self.panels = None
-        if panels is not None:
+        if spmv_cap_At:
+            pa, pt = panels if panels is not None else (0, 0)
+            self.h = lib.dpdlp_upload_caps(problem.ptr, spmv_cap, spmv_cap_At, pa, pt)
+            self.panels = None if panels is None else tuple(panels)
+        elif panels is not None:
             self.h = lib.dpdlp_upload_ex(problem.ptr, spmv_cap, panels[0], panels[1])
             self.panels = tuple(panels)
         elif tune:
@@ -373,12 +378,14 @@ class Resident:
 
 
 def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=4096, mixed_switch_tol=None,
-               profile_phases=0, use_graph=0, solo_lps=None):
+               profile_phases=0, use_graph=0, solo_lps=None, spmv_cap_At=None):
     o = BatchOpts()
     lib.dpdlp_batch_default_opts(C.byref(o))
     o.max_lps = max_lps
     o.precision = precision
     o.spmv_cap = spmv_cap
+    if spmv_cap_At is not None:
+        o.spmv_cap_At = spmv_cap_At
     o.profile_phases = profile_phases
     o.use_graph = use_graph
     if solo_lps is not None:

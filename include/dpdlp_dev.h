@@ -118,7 +118,8 @@ typedef struct {
   const int *rA, *rAt, *rN, *rM, *rNM; /* per LP [lo, hi) into the lists */
   double *part;
   int stride;
-  int cap;
+  int cap;    /* row-block capacity of the A lists */
+  int cap_At; /* of the A' lists */
 } dpdlp_batch_dev_t;
 
 /* Batched steps: every launch covers all LPs whose control flag is set. */

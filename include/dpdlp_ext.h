@@ -73,6 +73,11 @@ void dpdlp_resident_free(dpdlp_resident_t *res);
  * panels 1, 1 and the batch's capacity. */
 dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *prob, int spmv_cap, int panels_A,
                                   int panels_At);
+/* dpdlp_upload_ex with a row-block capacity of its own for A' (1024, 2048,
+ * 4096; 0 = the rule dpdlp_upload_ex applies).  A batch whose spmv_cap_At
+ * differs from its spmv_cap takes LPs uploaded this way. */
+dpdlp_resident_t *dpdlp_upload_caps(const lp_problem_t *prob, int spmv_cap, int spmv_cap_At,
+                                    int panels_A, int panels_At);
 /* Device bytes a resident problem holds (matrix, transpose, plans, vectors). */
 size_t dpdlp_resident_bytes(const dpdlp_resident_t *res);
 
@@ -141,6 +146,8 @@ typedef struct {
                               and replay it while the batch keeps its shape */
   int solo_lps;            /* windows with at most this many iterating LPs use
                               their single-LP launches (default 4; 0: never) */
+  int spmv_cap_At;         /* row-block capacity of every LP's A' (0: the
+                              same as spmv_cap) */
   /* outputs of dpdlp_solve_batch */
   long long windows, syncs, launches, graph_replays;
   int phase_samples;

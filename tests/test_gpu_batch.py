@@ -247,3 +247,65 @@ def test_degenerate_shapes_batch_equals_single_and_oracle(dp, orc):
         assert mix[k].status == ref["status"], (k, mix[k].status, ref["status"])
         assert abs(mix[k].primal_obj - ref["primal_objective_value"]) <= 1e-6 * max(
             1.0, abs(ref["primal_objective_value"])), k
+
+
+def boxed_lp_with_dense_column(dp, m, n, seed):
+    """Poisson(8)+1 random entries per row plus column 0 in every row (an A'
+    row longer than a 1024-nonzero block); every variable in [0, 10], rows
+    feasible at a random box point, so the LP is feasible and bounded."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(m):
+        k = min(n - 1, int(rng.poisson(8.0)) + 1)
+        cs = np.concatenate(([0], 1 + rng.choice(n - 1, size=k, replace=False)))
+        rows.extend([i] * len(cs))
+        cols.extend(np.sort(cs).tolist())
+    rows, cols = np.array(rows), np.array(cols)
+    vals = rng.standard_normal(len(rows))
+    rp = np.concatenate(([0], np.cumsum(np.bincount(rows, minlength=m))))
+    x0 = rng.uniform(0.0, 10.0, n)
+    ax = np.bincount(rows, weights=vals * x0[cols], minlength=m)
+    eq = rng.random(m) < 0.3
+    lb = np.where(eq, ax, ax - rng.uniform(0.0, 1.0, m))
+    ub = np.where(eq, ax, np.inf)
+    return dp.Problem.from_csr(m, n, rp, cols, vals, c=rng.standard_normal(n), con_lb=lb,
+                               con_ub=ub, var_lb=np.zeros(n), var_ub=np.full(n, 10.0))
+
+
+@pytest.mark.parametrize("caps", [(4096, 2048), (2048, 1024)])
+def test_separate_At_capacity_alone_equals_mix_equals_single(dp, caps):
+    """A' planned and launched with a smaller row-block capacity than A:
+    batch alone == batch mix == the single-LP path on the same layout, bit
+    for bit, including an A' row longer than its blocks."""
+    cap, cap_at = caps
+    probs = [p for p, _ in uneven_lps(dp, SIZES[:4], seed0=3300)]
+    probs.append(boxed_lp_with_dense_column(dp, 1500, 2600, seed=7))
+    res = [dp.Resident(p, spmv_cap=cap, panels=(1, 1), spmv_cap_At=cap_at) for p in probs]
+    params = dp.default_params(presolve=False)
+    for prec in (dp.PREC_FP64, dp.PREC_MIXED):
+        mix = dp.solve_batch(res, params, spmv_cap=cap, spmv_cap_At=cap_at, precision=prec)
+        for k, r in enumerate(res):
+            one = dp.solve_batch([r], params, spmv_cap=cap, spmv_cap_At=cap_at,
+                                 precision=prec)[0]
+            assert same(mix[k], one), (caps, prec, k)
+            assert mix[k].status == "OPTIMAL", (caps, prec, k, mix[k].status)
+            if prec == dp.PREC_FP64:
+                single, _ = r.solve(params)
+                assert same(mix[k], single), (caps, k)
+    # a batch refuses LPs whose A' layout differs from its own
+    with pytest.raises(RuntimeError):
+        dp.solve_batch(res[:1], params, spmv_cap=cap, spmv_cap_At=cap)
+
+
+def test_graph_option_on_renumbered_upload(dp, monkeypatch):
+    """use_graph on a problem whose upload renumbered its rows (the windowed
+    engine takes the caller's row order only) runs the direct loop and gives
+    its result."""
+    monkeypatch.setenv("DPDLP_ROW_ORDER", "1")
+    prob, _ = dp.generate(1500, 3500, 30000, row_rule=1, seed=11)
+    r = dp.Resident(prob)
+    params = dp.default_params(presolve=False)
+    a, _ = r.solve(params, use_graph=1)
+    b, _ = r.solve(params)
+    assert a.status == "OPTIMAL"
+    assert same(a, b)

@@ -1,8 +1,8 @@
 """Full-size checks through properties that do not depend on the size: the
-cfg1 LP (m = 1e6, n = 2e6, nnz = 2e7, Zipf rows) solved through the C API
+large LP (m = 1e6, n = 2e6, nnz = 2e7, Zipf rows) solved through the C API
 must return a solution whose feasibility, reduced costs and objective hold
 when recomputed independently on the host (scipy), and batch members of the
-largest cfg3 sizes must give the same bits in a mix as alone."""
+largest batch-workload sizes must give the same bits in a mix as alone."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -10,7 +10,7 @@ import scipy.sparse as sp
 pytestmark = pytest.mark.gpu
 
 
-def test_cfg1_solution_properties(dp):
+def test_large_zipf_lp_solution_properties(dp):
     prob, opt = dp.generate(1_000_000, 2_000_000, 20_000_000, row_rule=1, pm1_frac=0.1,
                             seed=1000)
     a = prob.arrays()
