@@ -1307,9 +1307,34 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
     }
   }
   double rescale_time = 0.0;
-  if (dpdlp_i_ctx_setup(c, R, params, opts->precision == DPDLP_PREC_MIXED, c->s, NULL, NULL,
-                        &rescale_time))
+  if (dpdlp_i_ctx_setup_issue(c, R, params, opts->precision == DPDLP_PREC_MIXED, c->s, NULL,
+                              NULL))
     goto fail;
+  /* the power iteration's start vector (utils.cu:80-82) is drawn on the host
+   * while the device scales the problem */
+  if (nnz != 0) {
+    h_q = (double *)malloc((size_t)m * sizeof(double));
+    if (!h_q) {
+      dpdlp_set_error("host allocation failed");
+      goto fail;
+    }
+    dpdlp_normal_fill(1u, m, h_q);
+    if (R->h_row_perm) { /* draw i belongs to original row i */
+      double *t = (double *)malloc((size_t)m * sizeof(double));
+      if (!t) {
+        dpdlp_set_error("host allocation failed");
+        goto fail;
+      }
+      for (int i = 0; i < m; ++i) t[i] = h_q[R->h_row_perm[i]];
+      free(h_q);
+      h_q = t;
+    }
+  }
+  if (hipStreamSynchronize(c->s) != hipSuccess) {
+    dpdlp_set_error("device sync failed");
+    goto fail;
+  }
+  if (dpdlp_i_ctx_setup_complete(c, R, params, &rescale_time)) goto fail;
   double *pi_eig = c->pi_eig, *pi_next = c->pi_next, *pi_dual = c->pi_dual;
 
   /* ---- step size: power iteration (pdlp_core_op.cu:638-656) ---- */
@@ -1318,16 +1343,6 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
   if (nnz == 0) {
     h->step = 1.0;
   } else {
-    h_q = (double *)malloc((size_t)m * sizeof(double));
-    if (!h_q) goto fail;
-    dpdlp_normal_fill(1u, m, h_q);
-    if (R->h_row_perm) { /* draw i belongs to original row i */
-      double *t = (double *)malloc((size_t)m * sizeof(double));
-      if (!t) goto fail;
-      for (int i = 0; i < m; ++i) t[i] = h_q[R->h_row_perm[i]];
-      free(h_q);
-      h_q = t;
-    }
     HIPCHK(hipMemcpy(pi_eig, h_q, (size_t)m * sizeof(double), hipMemcpyHostToDevice));
     double sigma_sq = 1.0;
     int done = 0, batch = 8;

@@ -367,3 +367,46 @@ def test_mps_number_formats_parse_exactly(dp, tmp_path):
     want = np.array([float(t) for t in forms])
     assert vals.shape == want.shape
     assert np.array_equal(vals.view(np.int64), want.view(np.int64))
+
+
+STD_NORMALS = r"""
+#include <random>
+#include <cstdio>
+#include <cstdlib>
+int main(int argc, char** argv) {
+  std::mt19937 gen(std::atoi(argv[1]));
+  std::normal_distribution<double> dist(0.0, 1.0);
+  const int n = std::atoi(argv[2]);
+  for (int i = 0; i < n; ++i) std::printf("%a\n", dist(gen));
+  return 0;
+}
+"""
+
+
+def test_power_iteration_start_vector_is_the_std_library_draw(dp, tmp_path):
+    """utils.cu:29-30,80-82: the library's start vector (engine run once,
+    logarithms on worker threads) equals std::mt19937 + std::
+    normal_distribution<double> bit for bit, for odd and even lengths and on
+    1, 3 and 8 host threads."""
+    src, exe = tmp_path / "g.cc", tmp_path / "g"
+    src.write_text(STD_NORMALS)
+    subprocess.run(["g++", "-O2", "-o", str(exe), str(src)], check=True)
+    lib = C.CDLL(dp.LIB_PATH)
+    fill = lib.dpdlp_normal_fill
+    fill.argtypes = [C.c_uint32, C.c_int, C.POINTER(C.c_double)]
+    longest = 200_001
+    for seed in (1, 7):
+        out = subprocess.run([str(exe), str(seed), str(longest)], check=True,
+                             capture_output=True, text=True).stdout.split()
+        ref = np.array([float.fromhex(s) for s in out])
+        for threads in ("1", "3", "8"):
+            os.environ["DPDLP_HOST_THREADS"] = threads
+            try:
+                for count in (0, 1, 2, 3, 16384 * 2 + 1, longest):
+                    got = np.full(count + 1, 7.0)
+                    fill(seed, count, got.ctypes.data_as(C.POINTER(C.c_double)))
+                    assert np.array_equal(got[:count].view(np.int64),
+                                          ref[:count].view(np.int64)), (seed, threads, count)
+                    assert got[count] == 7.0  # nothing written past the end
+            finally:
+                del os.environ["DPDLP_HOST_THREADS"]
