@@ -88,6 +88,10 @@ double dpdlp_wall(void);
  * are taken in increasing order) and returns when all have finished. */
 int dpdlp_host_threads(void);
 void dpdlp_par_for(int ntask, void (*fn)(void *ctx, int task), void *ctx);
+/* caps dpdlp_host_threads() for the calling thread only (0: no cap): the
+ * serving loader and large-LP lane run their loops on their own thread, so
+ * the thread that steps the batch keeps its core */
+void dpdlp_host_threads_cap(int max_threads);
 
 /* util.c: stable CSR transpose on the host */
 int dpdlp_host_transpose(int m, int n, const int *rp, const int *ci, const double *v,

@@ -171,6 +171,10 @@ static void *lane_main(void *arg) {
   const dpdlp_serve_spec_t *sp = S->spec;
   uctx_t uc;
   dpdlp_uploader_t up;
+  /* host loops stay on this thread: worker threads would take the cores the
+   * batch's stepping thread needs (200 arrivals/s: 39.7 served LPs/s with
+   * 16 workers here and in the loader, 44.7 with none) */
+  dpdlp_host_threads_cap(1);
   if (uctx_init(&uc, &up, S->b)) {
     pthread_mutex_lock(&S->mu);
     S->loader_error = 1;
@@ -247,6 +251,7 @@ static void *loader_main(void *arg) {
   double budget = sp->hbm_budget_gb * 1e9;
   uctx_t uc;
   dpdlp_uploader_t up;
+  dpdlp_host_threads_cap(1); /* as the large-LP lane */
   if (uctx_init(&uc, &up, S->b)) {
     pthread_mutex_lock(&S->mu);
     S->loader_error = 1;

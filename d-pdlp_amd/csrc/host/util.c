@@ -32,7 +32,12 @@ double dpdlp_wall(void) {
   return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
 }
 
+static __thread int tl_cap;
+
+void dpdlp_host_threads_cap(int max_threads) { tl_cap = max_threads > 0 ? max_threads : 0; }
+
 int dpdlp_host_threads(void) {
+  if (tl_cap == 1) return 1;
   const char *env = getenv("DPDLP_HOST_THREADS");
   if (!env || !*env) env = getenv("OMP_NUM_THREADS");
   long t = env && *env ? atol(env) : 0;
@@ -43,6 +48,7 @@ int dpdlp_host_threads(void) {
   }
   if (t < 1) t = 1;
   if (t > 32) t = 32;
+  if (tl_cap > 0 && t > tl_cap) t = tl_cap;
   return (int)t;
 }
 

@@ -410,3 +410,29 @@ def test_power_iteration_start_vector_is_the_std_library_draw(dp, tmp_path):
                     assert got[count] == 7.0  # nothing written past the end
             finally:
                 del os.environ["DPDLP_HOST_THREADS"]
+
+
+def test_host_worker_cap_is_per_thread(dp):
+    """dpdlp_host_threads_cap limits the calling thread's workers only (the
+    serving loader and large-LP lane use it); 0 lifts the cap."""
+    import threading
+    lib = C.CDLL(dp.LIB_PATH)
+    lib.dpdlp_host_threads.restype = C.c_int
+    lib.dpdlp_host_threads_cap.argtypes = [C.c_int]
+    os.environ["DPDLP_HOST_THREADS"] = "8"
+    try:
+        assert lib.dpdlp_host_threads() == 8
+        lib.dpdlp_host_threads_cap(3)
+        assert lib.dpdlp_host_threads() == 3
+        other = []
+        t = threading.Thread(target=lambda: other.append(lib.dpdlp_host_threads()))
+        t.start()
+        t.join()
+        assert other == [8]
+        lib.dpdlp_host_threads_cap(1)
+        assert lib.dpdlp_host_threads() == 1
+        lib.dpdlp_host_threads_cap(0)
+        assert lib.dpdlp_host_threads() == 8
+    finally:
+        lib.dpdlp_host_threads_cap(0)
+        del os.environ["DPDLP_HOST_THREADS"]
