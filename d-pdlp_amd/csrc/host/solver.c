@@ -1352,16 +1352,29 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
       if (k > batch) k = batch;
       HIPCHK(dpdlp_dev_power_steps(st, pi_eig, pi_next, pi_dual, k, c->s));
       if (sync_scal(c, DPDLP_SC_PI_HIST + 4 * k)) goto fail;
+      double r_first = 0.0, r_last = 0.0;
       for (int i = 0; i < k; ++i) {
         const double *hh = c->h_scal + DPDLP_SC_PI_HIST + 4 * i;
         sigma_sq = hh[0];
         pi_iters++;
-        if (dpdlp_blue(hh + 1) < params->sv_tol) {
+        const double r = dpdlp_blue(hh + 1);
+        if (i == 0) r_first = r;
+        r_last = r;
+        if (r < params->sv_tol) {
           done = 1;
           break;
         }
       }
+      /* the next batch of speculative steps: doubling up to 32, or, once the
+       * residuals fall geometrically, the steps they predict to the
+       * tolerance plus two (steps past the first converged one are wasted;
+       * sigma is always read at that step) */
       batch = batch * 2 > 32 ? 32 : batch * 2;
+      if (!done && k >= 4 && r_last > 0.0 && r_first > r_last) {
+        const double rate = log(r_last / r_first) / (k - 1); /* < 0 */
+        const double need = log(params->sv_tol / r_last) / rate;
+        if (need >= 0.0 && need < 30.0) batch = (int)need + 2;
+      }
     }
     h->step = 0.998 / sqrt(sigma_sq);
   }
