@@ -1,4 +1,22 @@
 /*
+ * The declarations in this header keep the interface of the cuPDLPx / D-PDLP
+ * reference header of the same name, which carries this notice:
+ *
+ *   Copyright 2025 Haihao Lu
+ *
+ *   Licensed under the Apache License, Version 2.0 (the "License");
+ *   you may not use this file except in compliance with the License.
+ *   You may obtain a copy of the License at
+ *
+ *       http://www.apache.org/licenses/LICENSE-2.0
+ *
+ *   Unless required by applicable law or agreed to in writing, software
+ *   distributed under the License is distributed on an "AS IS" BASIS,
+ *   WITHOUT WARRANTIES OR CONDITIONS OF ANY KIND, either express or implied.
+ *   See the License for the specific language governing permissions and
+ *   limitations under the License.
+ */
+/*
  * Public data types of the single-GPU PDHG (PDLP) solver.
  *
  * Drop-in replacement for include/cupdlpx_types.h of the reference

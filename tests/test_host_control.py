@@ -107,41 +107,52 @@ def test_restart_sequence(dp, ctl):
         assert h.fpe_last == fpe      # the trial value is remembered every time
 
 
-def _pid(rp, st, pd, dd):
-    """The PID step written out from pdlp_core_op.cu:593-619."""
-    ratio = st["rel_d"] / st["rel_p"]
-    if 1e-16 < pd < 1e12 and 1e-16 < dd < 1e12 and 1e-8 < ratio < 1e8:
-        err = math.log(dd) - math.log(pd) - math.log(st["omega"])
-        st["pw_err_sum"] = st["pw_err_sum"] * rp.i_smooth + err
-        derr = err - st["pw_last_err"]
-        st["omega"] *= math.exp(rp.k_p * err + rp.k_i * st["pw_err_sum"] + rp.k_d * derr)
-        st["pw_last_err"] = err
-    else:
-        st["omega"] = st["best_omega"]
-        st["pw_err_sum"] = 0.0
-        st["pw_last_err"] = 0.0
-    g = abs(math.log10(st["rel_d"] / st["rel_p"]))
-    if g < st["best_gap"]:
-        st["best_gap"] = g
-        st["best_omega"] = st["omega"]
-
-
 def test_pid_primal_weight_sequence(dp, ctl):
-    rp = dp.default_params().restart_params      # k_p 0.99, k_i 0.01, k_d 0, i_smooth 0.3
-    st = dict(omega=1.0, best_omega=1.0, pw_err_sum=0.0, pw_last_err=0.0, best_gap=math.inf)
-    h = HState(**st)
-    seq = [  # (rel_p, rel_d, primal distance, dual distance)
-        (1e-2, 1e-3, 2.0, 8.0),
-        (5e-3, 4e-3, 1.5, 0.5),
-        (1e-3, 1e-3, 3.0, 3.0),
-        (1e-3, 1e-12, 1.0, 1.0),    # ratio 1e-9: guard -> best weight, sums reset
-        (1e-4, 2e-4, 1e-17, 1.0),   # primal distance below 1e-16: guard
-        (2e-4, 1e-4, 0.7, 2.2),
+    """pdlp_core_op.cu:593-619 with the default gains (k_p 0.99, k_i 0.01,
+    k_d 0, i_smooth 0.3), every expected value worked out by hand:
+      1. omega 1, d_p 2, d_d 8: e = ln 8 - ln 2 - ln 1 = ln 4, S = ln 4,
+         omega = exp(0.99 ln 4 + 0.01 ln 4) = 4; gap |log10(1e-3 / 1e-2)| = 1
+         beats +inf, so best = (1, 4)
+      2. omega 4, d_p 1, d_d 4: e = 0, S = 0.3 ln 4, omega = 4 exp(0.003 ln 4)
+         = 4^1.003; ratio 1, gap 0 beats 1: best = (0, 4^1.003)
+      3. ratio 1e-9 (outside (1e-8, 1e8)): guard, omega = best = 4^1.003,
+         S = e_prev = 0; gap 9 does not beat 0
+      4. d_p = 1e-17 (<= 1e-16): guard again, nothing changes
+      5. d_p 1, d_d = 2 omega: e = ln 2, S = ln 2, omega = 2 * 4^1.003;
+         gap log10(2) does not beat 0"""
+    rp = dp.default_params().restart_params
+    assert (rp.k_p, rp.k_i, rp.k_d, rp.i_smooth) == (0.99, 0.01, 0.0, 0.3)
+    h = HState(omega=1.0, best_omega=1.0, pw_err_sum=0.0, pw_last_err=0.0, best_gap=math.inf)
+    w1003 = 4.0 ** 1.003
+    seq = [  # (rel_p, rel_d, d_p, d_d) -> (omega, best_omega, S, e_prev, best_gap)
+        ((1e-2, 1e-3, 2.0, 8.0), (4.0, 4.0, math.log(4.0), math.log(4.0), 1.0)),
+        ((1e-3, 1e-3, 1.0, 4.0), (w1003, w1003, 0.3 * math.log(4.0), 0.0, 0.0)),
+        ((1e-3, 1e-12, 1.0, 1.0), (w1003, w1003, 0.0, 0.0, 0.0)),
+        ((1e-4, 1e-4, 1e-17, 1.0), (w1003, w1003, 0.0, 0.0, 0.0)),
+        ((1e-4, 2e-4, 1.0, 2.0 * w1003), (2.0 * w1003, w1003, math.log(2.0), math.log(2.0), 0.0)),
     ]
-    for rel_p, rel_d, pd, dd in seq:
+    for (rel_p, rel_d, pd, dd), want in seq:
         h.rel_p, h.rel_d = rel_p, rel_d
-        st["rel_p"], st["rel_d"] = rel_p, rel_d
         ctl.ctl_pid_update(C.byref(h), C.byref(rp), pd, dd)
-        _pid(rp, st, pd, dd)
-        for k in ("omega", "best_omega", "pw_err_sum", "pw_last_err", "best_gap"):
-            assert math.isclose(getattr(h, k), st[k], rel_tol=1e-15, abs_tol=1e-300), (k, rel_p)
+        got = (h.omega, h.best_omega, h.pw_err_sum, h.pw_last_err, h.best_gap)
+        for g, w in zip(got, want):
+            assert math.isclose(g, w, rel_tol=1e-14, abs_tol=1e-15), (rel_p, got, want)
+
+
+def test_pid_gains_are_not_interchangeable(dp, ctl):
+    """Gains k_p 1/2, k_i 1/4, k_d 1/8, i_smooth 1/2 from omega 1: restart 1
+    with e1 = 2 (d_p 1, d_d e^2) gives S = 2 and omega = exp(1 + 0.5 + 0.25)
+    = e^1.75; restart 2 with e2 = -1 (d_d = omega / e) gives S = 1 - 1 = 0,
+    derivative -3 and omega = e^1.75 exp(-0.5 + 0 - 0.375) = e^0.875.
+    Swapping k_p and k_i would give e^1.125, k_p and k_d e^0.125."""
+    prm = dp.default_params()
+    rp = prm.restart_params
+    rp.k_p, rp.k_i, rp.k_d, rp.i_smooth = 0.5, 0.25, 0.125, 0.5
+    h = HState(omega=1.0, best_omega=1.0, pw_err_sum=0.0, pw_last_err=0.0, best_gap=math.inf)
+    h.rel_p, h.rel_d = 1e-3, 1e-3
+    ctl.ctl_pid_update(C.byref(h), C.byref(rp), 1.0, math.exp(2.0))
+    assert math.isclose(h.omega, math.exp(1.75), rel_tol=1e-14)
+    assert math.isclose(h.pw_err_sum, 2.0, rel_tol=1e-14)
+    ctl.ctl_pid_update(C.byref(h), C.byref(rp), 1.0, h.omega / math.e)
+    assert math.isclose(h.omega, math.exp(0.875), rel_tol=1e-14)
+    assert abs(h.pw_err_sum) <= 1e-15 and math.isclose(h.pw_last_err, -1.0, rel_tol=1e-14)

@@ -311,8 +311,17 @@ def test_infeasible_unbounded_and_limits(orc):
 def test_polish_reaches_tolerance(dp, orc):
     prob, opt = _gen_arrays(dp, 80, 200, 1500, 5)
     r, _ = orc.solve(prob.ptr, orc.default_params(presolve=False, feasibility_polishing=True))
-    assert r["status"] == "OPTIMAL"
-    assert r["relative_primal_residual"] <= 1e-6 or r["feasibility_iteration"] > 0
+    r0, _ = orc.solve(prob.ptr, orc.default_params(presolve=False))
+    assert r["status"] == r0["status"] == "OPTIMAL" and r["total_count"] == r0["total_count"]
+    # unpolished: feasible only to the 1e-4 termination tolerance
+    assert r0["relative_primal_residual"] > 1e-6 and r0["relative_dual_residual"] > 1e-6
+    # both polishes succeed: each residual reaches eps_polish = 1e-6
+    assert r["relative_primal_residual"] <= 1e-6 and r["relative_dual_residual"] <= 1e-6
+    assert r["feasibility_iteration"] > 0
+    # the gap is recomputed from the polished objectives (pdlp_core_op.cu:923-928)
+    pobj, dobj = r["primal_objective_value"], r["dual_objective_value"]
+    assert r["objective_gap"] == abs(pobj - dobj)
+    assert r["relative_objective_gap"] == abs(pobj - dobj) / (1 + abs(pobj) + abs(dobj))
     assert r["feasibility_polishing_time"] >= 0.0
 
 
