@@ -13,9 +13,8 @@ the mean wall time of a step (seconds, lower is better).  Kernel roofline: the
 two fused phase kernels (SpMV + PDHG update + Halpern) are timed live with HIP
 events on the solve's stream during the timed steps.
 
-CPU baseline: the OpenMP build of the FP64 oracle on the host cores, on a
-bounded sample of the same LP (setup + a fixed number of iterations), projected
-to the GPU run's iteration and power-iteration counts.
+CPU baseline: the OpenMP build of the FP64 oracle on the host cores solving the
+same LP completely (median of 3 solves; end-to-end and loop-only times).
 """
 import argparse
 import json
@@ -298,28 +297,69 @@ def phase_bytes(m, n, nnz, sv=8, sx=8):
     return p1, p2
 
 
-def cpu_baseline(problem, gpu_iters, gpu_pi_iters, sample_iters):
+def host_cores():
+    """Physical cores and logical CPUs of this host (lscpu), for the record."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    kv = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        cores = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (KeyError, ValueError):
+        cores = None
+    return {"physical_cores": cores, "logical_cpus": int(kv.get("CPU(s)", "0") or 0),
+            "model": kv.get("Model name")}
+
+
+def cpu_baseline(problem, repeats, gpu_result):
+    """Complete solves of the same LP by the OpenMP build of the FP64 oracle
+    (scaling, transpose, power iteration, PDHG loop to 1e-4, result), median of
+    `repeats`: end-to-end wall time and the loop time (the reference's solve
+    time, which starts after setup).  The oracle's status, iterations and
+    objective are reported beside the GPU's."""
+    import statistics
     import oracle_py as orc
     L = orc.lib(omp=True)
     threads = L.orc_num_threads()
-    pi_sample = 20
-    prm = orc.default_params(presolve=False, sv_max_iter=pi_sample,
-                             termination_criteria__iteration_limit=sample_iters)
-    t = time.perf_counter()
-    orc.solve(problem.ptr, prm, omp=True)
-    wall = time.perf_counter() - t
+    prm = orc.default_params(presolve=False)
+    walls, loops, last = [], [], None
+    for k in range(repeats):
+        t = time.perf_counter()
+        last, _ = orc.solve(problem.ptr, prm, omp=True)
+        walls.append(time.perf_counter() - t)
+        loops.append(orc.last_timings(omp=True)["loop"])
+        print(f"# cpu baseline solve {k}: {last['status']} iters={last['total_count']} "
+              f"wall={walls[-1]:.2f}s loop={loops[-1]:.2f}s", file=sys.stderr, flush=True)
     tm = orc.last_timings(omp=True)
-    t_iter = tm["loop"] / sample_iters
-    t_pi = tm["power"] / pi_sample
-    setup = tm["rescale"] + tm["transpose"]
-    projected = setup + gpu_pi_iters * t_pi + gpu_iters * t_iter
-    return dict(value=projected, unit="s", cores=threads, kind="port",
-                sample=(f"OpenMP FP64 oracle ({threads} threads) on the same LP, {wall:.1f} s: "
-                        f"scaling+transpose {setup:.2f} s, {pi_sample} power iterations "
-                        f"({t_pi * 1e3:.1f} ms each), {sample_iters} PDHG iterations "
-                        f"({t_iter * 1e3:.1f} ms each); projected to the GPU run's "
-                        f"{gpu_pi_iters} power + {gpu_iters} PDHG iterations"),
-                ms_per_iter=t_iter * 1e3, setup_s=setup)
+    r, o = gpu_result
+    obj = last["primal_objective_value"]
+    return dict(value=statistics.median(walls), unit="s", cores=threads, kind="port",
+                sample=(f"{repeats} complete solves of the same LP by the OpenMP FP64 oracle "
+                        f"({threads} threads) to eps 1e-4; value = median end-to-end wall"),
+                loop_s=statistics.median(loops), walls_s=walls, loops_s=loops,
+                setup_s=tm["rescale"] + tm["transpose"], power_s=tm["power"],
+                host=host_cores(),
+                oracle=dict(status=last["status"], iterations=last["total_count"], objective=obj,
+                            gpu_objective_rel_diff=abs(r.primal_obj - obj) / max(1.0, abs(obj)),
+                            gpu_iterations=r.iterations))
+
+
+def gather_sweep(dpdlp, gathers=20_000_000, reps=10):
+    """Random 8-byte gathers with the SpMV's index stream at growing working
+    sets: each XCD reading only its own slice (32 KiB .. 256 MiB per XCD), and
+    every workgroup reading one shared vector (the two SpMVs' vectors)."""
+    out = {"xcd_local": {}, "global": {}}
+    for sz in (32 << 10, 256 << 10, 2 << 20, 4 << 20, 16 << 20, 256 << 20):
+        out["xcd_local"][str(sz)] = dpdlp.lib.dpdlp_gather_calibration_ex(sz, gathers, reps, 1, 2048)
+    for sz in (2 << 20, 8_000_000, 16_000_000, 256 << 20):
+        out["global"][str(sz)] = dpdlp.lib.dpdlp_gather_calibration_ex(sz, gathers, reps, 0, 2048)
+    return out
 
 
 def main():
@@ -329,7 +369,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="zipf-1m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-iters", type=int, default=60)
+    ap.add_argument("--cpu-repeats", type=int, default=3,
+                    help="complete OpenMP oracle solves for the CPU baseline (median)")
     ap.add_argument("--spmv-cap", type=int, default=0)
     ap.add_argument("--no-tune", action="store_true", help="panel counts from the size rule")
     ap.add_argument("--calib-copy", action="store_true",
@@ -429,14 +470,27 @@ def main():
         cms = dpdlp.lib.dpdlp_copy_calibration(nbytes, 5)
         copy_GBps = 2 * nbytes / (cms * 1e-3) / 1e9
     # what bounds the phases: every nonzero gathers one vector entry per
-    # launch; the ceiling is the measured rate of L2-resident random gathers
-    gceil = dpdlp.lib.dpdlp_gather_calibration(2 << 20, 20_000_000, 10)
+    # launch; the ceiling is the measured rate of random gathers with the
+    # same index stream over the same working set (phase 1 gathers y from
+    # one m-vector; phase 2 gathers x, each XCD from its panel's n/P slice)
+    sweep = gather_sweep(dpdlp)
+    pA = R.panels[0]
+    ws1 = dict(kind="global", bytes=8 * m)
+    ws2 = dict(kind="xcd_local", bytes=8 * n // pA) if pA > 1 else dict(kind="global", bytes=8 * n)
+
+    def ceiling(ws):
+        return dpdlp.lib.dpdlp_gather_calibration_ex(ws["bytes"], nnz, 10,
+                                                     1 if ws["kind"] == "xcd_local" else 0, 2048)
+    c1, c2 = ceiling(ws1), ceiling(ws2)
+    g1, g2 = nnz / (p1 * 1e-3) / 1e9, nnz / (p2 * 1e-3) / 1e9
     gather = dict(gathers_per_launch=nnz,
-                  phase1_G_per_s=nnz / (p1 * 1e-3) / 1e9, phase2_G_per_s=nnz / (p2 * 1e-3) / 1e9,
-                  ceiling_G_per_s=gceil if gceil > 0 else None,
-                  ceiling_kernel="random 8-byte gathers, every workgroup within a 2 MiB "
-                                 "slice of its own XCD (L2-resident)",
-                  dominant_frac=(nnz / (dom[2] * 1e-3) / 1e9) / gceil if gceil > 0 else None)
+                  phase1_G_per_s=g1, phase1_working_set=ws1, phase1_ceiling_G_per_s=c1,
+                  phase1_frac=g1 / c1 if c1 > 0 else None,
+                  phase2_G_per_s=g2, phase2_working_set=ws2, phase2_ceiling_G_per_s=c2,
+                  phase2_frac=g2 / c2 if c2 > 0 else None,
+                  ceiling_kernel="k_gather_calib: coalesced index stream, 2048 indices per "
+                                 "workgroup, 8 gathers in flight per thread",
+                  sweep_G_per_s=sweep)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "zipf1m_fp64_hbm_traffic.json")
     if args.workload == "zipf-1m" and not args.mixed and os.path.exists(tfile):
@@ -445,8 +499,8 @@ def main():
         traffic = tj.get("bytes_per_launch", {}).get(dom[0])
     cpu = None
     if not args.no_cpu:
-        print("# cpu baseline sample ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(prob, r.iterations, o.power_iterations, args.cpu_iters)
+        print("# cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(prob, args.cpu_repeats, last)
     out = {
         "metric": "time-to-1e-4 KKT & SpMV HBM GB/s; served LPs/s + p99 latency vs plain loop",
         "value": step_s,

@@ -72,14 +72,15 @@ __device__ __forceinline__ double wave_max(double v) {
 
 /*
  * Reduces NS sums and NM maxima over the 256-thread workgroup and writes the
- * block's partials to part[slot * stride + blockIdx.x] (sums first, then
- * maxima).  Every thread must call it.  `lds` needs (NS+NM)*4 doubles.
+ * block's partials to part[slot * stride + index] (sums first, then maxima;
+ * block_partials: index = blockIdx.x).  Every thread must call it.  `lds`
+ * needs (NS+NM)*4 doubles.
  */
 template <int NS, int NM>
-__device__ __forceinline__ void block_partials(double (&s)[NS > 0 ? NS : 1],
-                                               double (&mx)[NM > 0 ? NM : 1],
-                                               double *lds, double *part,
-                                               int stride, int slot0) {
+__device__ __forceinline__ void block_partials_at(double (&s)[NS > 0 ? NS : 1],
+                                                  double (&mx)[NM > 0 ? NM : 1],
+                                                  double *lds, double *part, int stride,
+                                                  int slot0, int index) {
   const int lane = threadIdx.x & (DPDLP_WAVE - 1);
   const int wid = threadIdx.x >> 6;
 #pragma unroll
@@ -98,8 +99,16 @@ __device__ __forceinline__ void block_partials(double (&s)[NS > 0 ? NS : 1],
     const double *q = lds + k * 4;
     double v = (k < NS) ? ((q[0] + q[1]) + (q[2] + q[3]))
                         : fmax(fmax(q[0], q[1]), fmax(q[2], q[3]));
-    part[(size_t)(slot0 + k) * stride + blockIdx.x] = v;
+    part[(size_t)(slot0 + k) * stride + index] = v;
   }
+}
+
+template <int NS, int NM>
+__device__ __forceinline__ void block_partials(double (&s)[NS > 0 ? NS : 1],
+                                               double (&mx)[NM > 0 ? NM : 1],
+                                               double *lds, double *part,
+                                               int stride, int slot0) {
+  block_partials_at<NS, NM>(s, mx, lds, part, stride, slot0, blockIdx.x);
 }
 
 __device__ __forceinline__ double lfin(double v) { return isfinite(v) ? v : 0.0; }

@@ -17,6 +17,8 @@
  * is compiled without FMA contraction so it rounds exactly like the FP64
  * oracle; only SpMV and reduction summation order differ from it.
  */
+#include <stdlib.h>
+
 #include "dev_common.h"
 #include "dpdlp_dev.h"
 
@@ -133,6 +135,100 @@ __global__ __launch_bounds__(DPDLP_BS) void k_xp_epi(const double *__restrict__ 
   }
 }
 
+/* Column-panel tile SpMV in one pass.  Workgroup w = b*P + k sums the entries
+ * of row block b that lie in panel k (as k_xp_partial) and writes its partial
+ * row sums to xpart[k*rows + row]; then it takes a ticket on row block b.
+ * Each launch adds exactly P tickets per block, so the counter, which is never
+ * reset, reads (P-1) mod P for exactly one workgroup of the block: the last
+ * to finish.  That workgroup adds the P partials of every row of the block in
+ * panel order 0..P-1 (the same sums, in the same order, as k_xp_epi) and runs
+ * the epilogue, whose per-workgroup reduction partials go to index b.  No
+ * workgroup waits for another, and the result does not depend on which one
+ * arrives last.  The agent-scope fences make the partials written on the
+ * other XCDs (whose L2s are not coherent with this one's) visible: release
+ * before the ticket, acquire after it. */
+template <typename VT, typename XT, class Epi, int CAP>
+__global__ __launch_bounds__(DPDLP_BS) void k_xp_fused(
+    const int *__restrict__ blk, const int *__restrict__ toff,
+    const unsigned short *__restrict__ rofs, const int *__restrict__ pci,
+    const VT *__restrict__ pv, const XT *__restrict__ x, double *xpart,
+    unsigned *__restrict__ tick, int rows, int P, Epi epi, double *__restrict__ part,
+    int stride, int slot0) {
+  constexpr int UNR = 8;
+  constexpr int NS = Epi::NS, NM = Epi::NM;
+  __shared__ double prod[CAP];
+  __shared__ int srp[DPDLP_PANEL_RMAX + 1];
+  __shared__ double ssum[4];
+  __shared__ double red[(NS + NM > 0 ? NS + NM : 1) * 4];
+  __shared__ int is_last;
+  const int tid = threadIdx.x;
+  const int w = blockIdx.x, b = w / P, k = w - b * P;
+  const int r0 = blk[b], r1 = blk[b + 1];
+  const int R = r1 - r0;
+  const int t0 = toff[w], t1 = toff[w + 1];
+  const int nz = t1 - t0;
+  double *out = xpart + (size_t)k * rows;
+  if (R == 1) {
+    double acc = row_sum_wg<VT, XT>(pci, pv, x, t0, t1);
+    acc = wave_sum(acc);
+    if ((tid & 63) == 0) ssum[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) out[r0] = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
+  } else {
+    const unsigned short *ro = rofs + (size_t)P * (r0 + b) + (size_t)k * (R + 1);
+    for (int t = tid; t <= R; t += DPDLP_BS) srp[t] = ro[t];
+    for (int base = tid; base < nz; base += DPDLP_BS * UNR) {
+      int cidx[UNR];
+      VT vv[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        int q = base + u * DPDLP_BS;
+        if (q < nz) {
+          cidx[u] = ntload(pci + t0 + q);
+          vv[u] = ntload(pv + t0 + q);
+        }
+      }
+      XT xx[UNR];
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        int q = base + u * DPDLP_BS;
+        if (q < nz) xx[u] = x[cidx[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < UNR; ++u) {
+        int q = base + u * DPDLP_BS;
+        if (q < nz) prod[q] = (double)vv[u] * (double)xx[u];
+      }
+    }
+    __syncthreads();
+    rows_reduce(prod, srp, R, [&](int r, double v) { out[r0 + r] = v; });
+  }
+  __threadfence(); /* release this workgroup's partial row sums */
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned old = atomicAdd(tick + b, 1u);
+    is_last = ((old + 1u) % (unsigned)P) == 0u;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence(); /* acquire the other panels' partials */
+  double s[NS > 0 ? NS : 1];
+  double mx[NM > 0 ? NM : 1];
+#pragma unroll
+  for (int q = 0; q < (NS > 0 ? NS : 1); ++q) s[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < (NM > 0 ? NM : 1); ++q) mx[q] = 0.0;
+  for (int r = tid; r < R; r += DPDLP_BS) {
+    const int i = r0 + r;
+    double a = xpart[i];
+    for (int q = 1; q < P; ++q) a += xpart[(size_t)q * rows + i];
+    epi(i, a, s, mx);
+  }
+  if (NS + NM > 0) {
+    if (epi.partials) block_partials_at<NS, NM>(s, mx, red, part, stride, slot0, b);
+  }
+}
+
 /* dst[t] = src[perm[t]] */
 template <typename T>
 __global__ void k_permute(const T *__restrict__ src, const int *__restrict__ perm,
@@ -142,9 +238,25 @@ __global__ void k_permute(const T *__restrict__ src, const int *__restrict__ per
     dst[i] = src[perm[i]];
 }
 
+/* The one-pass tiled SpMV runs when the matrix has tickets and the
+ * environment sets DPDLP_TILE_ONEPASS=1; otherwise the two-kernel form runs.
+ * Both give the same row sums.  The two-kernel form is the default: on cfg1's
+ * A (4 panels) the one-pass kernel took 282-288 us against 167 us, because
+ * the agent-scope release every workgroup needs (buffer_wbl2) writes back
+ * its XCD's L2 and costs more than the partial-sum round trip it saves. */
+static bool tile_onepass(const dpdlp_csr_t *A) {
+  static int enabled = -1;
+  if (enabled < 0) {
+    const char *e = getenv("DPDLP_TILE_ONEPASS");
+    enabled = (e && e[0] == '1') ? 1 : 0;
+  }
+  return A->P > 1 && A->tick != nullptr && enabled;
+}
+
 /* partial-sum count written by an SpMV with a reducing epilogue */
 static inline int spmv_nparts(const dpdlp_csr_t *A) {
-  return A->P > 1 ? ew_grid(A->rows) : A->nblk;
+  if (A->P > 1) return tile_onepass(A) ? A->nblk : ew_grid(A->rows);
+  return A->nblk;
 }
 
 template <typename VT, typename XT, class Epi>
@@ -152,6 +264,27 @@ static hipError_t launch_spmv_t(const dpdlp_csr_t *A, const int *ci, const VT *v
                                 const XT *x, const Epi &epi, double *part, int stride,
                                 int slot0, hipStream_t s) {
   if (A->rows <= 0) return hipSuccess;
+  if (tile_onepass(A)) {
+    int grid = A->nblk * A->P;
+    switch (A->cap) {
+    case 1024:
+      k_xp_fused<VT, XT, Epi, 1024><<<grid, DPDLP_BS, 0, s>>>(
+          A->blk, A->toff, A->rofs, A->pci, pv, x, A->xpart, A->tick, A->rows, A->P, epi, part,
+          stride, slot0);
+      break;
+    case 4096:
+      k_xp_fused<VT, XT, Epi, 4096><<<grid, DPDLP_BS, 0, s>>>(
+          A->blk, A->toff, A->rofs, A->pci, pv, x, A->xpart, A->tick, A->rows, A->P, epi, part,
+          stride, slot0);
+      break;
+    default:
+      k_xp_fused<VT, XT, Epi, 2048><<<grid, DPDLP_BS, 0, s>>>(
+          A->blk, A->toff, A->rofs, A->pci, pv, x, A->xpart, A->tick, A->rows, A->P, epi, part,
+          stride, slot0);
+      break;
+    }
+    return hipGetLastError();
+  }
   if (A->P > 1) {
     int grid = A->nblk * A->P;
     switch (A->cap) {
@@ -685,26 +818,37 @@ __global__ __launch_bounds__(DPDLP_BS) void k_reduce_final(int op, const double 
   }
 }
 
-/* random 8-byte gathers, L per thread, every workgroup from the slice of x
- * that belongs to its XCD (workgroup w runs on XCD w mod 8): the rate of
- * L2-resident random gathers, the ceiling of the SpMVs' gather streams */
+/* Gather-rate calibration with the SpMV's access pattern: workgroup w streams
+ * its chunk of `chunk` column indices coalesced (consecutive threads read
+ * consecutive indices, eight loads in flight per thread, non-temporal, as
+ * spmv_block does) and gathers x[base + idx] for each, where base is the
+ * slice of its own XCD ((w mod 8) * slice; workgroup w runs on XCD w mod 8)
+ * when xcd_local is set, else 0.  Each thread writes the sum of its gathers. */
 __global__ __launch_bounds__(DPDLP_BS) void k_gather_calib(const int *__restrict__ idx,
                                                           const double *__restrict__ x,
-                                                          double *__restrict__ y, long long rows,
-                                                          int L, long long slice) {
-  const long long i = (long long)blockIdx.x * DPDLP_BS + threadIdx.x;
-  if (i >= rows) return;
-  const int *p = idx + i * L;
-  const double *xs = x + (long long)(blockIdx.x & 7) * slice;
+                                                          double *__restrict__ y, long long count,
+                                                          int chunk, long long slice,
+                                                          int xcd_local) {
+  constexpr int UNR = 8;
+  const long long c0 = (long long)blockIdx.x * chunk;
+  const double *xs = x + (xcd_local ? (long long)(blockIdx.x & 7) * slice : 0);
+  long long lim = count - c0;
+  if (lim > chunk) lim = chunk;
   double acc = 0.0;
-  for (int k = 0; k + 9 < L; k += 10) {
-    double v[10];
+  for (int base = threadIdx.x; base < lim; base += DPDLP_BS * UNR) {
+    int ci[UNR];
 #pragma unroll
-    for (int u = 0; u < 10; ++u) v[u] = xs[__builtin_nontemporal_load(p + k + u)];
+    for (int u = 0; u < UNR; ++u) {
+      const int k = base + u * DPDLP_BS;
+      ci[u] = k < lim ? __builtin_nontemporal_load(idx + c0 + k) : -1;
+    }
+    double v[UNR];
 #pragma unroll
-    for (int u = 0; u < 10; ++u) acc += v[u];
+    for (int u = 0; u < UNR; ++u) v[u] = ci[u] >= 0 ? xs[ci[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) acc += v[u];
   }
-  y[i] = acc;
+  y[(long long)blockIdx.x * DPDLP_BS + threadIdx.x] = acc;
 }
 
 /* warm start: v = start * D * s into init, cur and pdhg (pdlp_core_op.cu:174-201) */
@@ -1036,11 +1180,13 @@ int dpdlp_dev_reduce(int op, const double *x, const double *y, long long n, doub
   return (int)hipGetLastError();
 }
 
-int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long rows, int L,
-                           long long slice, void *stream) {
-  if (rows <= 0 || L % 10 != 0) return (int)hipErrorInvalidValue;
-  long long g = (rows + DPDLP_BS - 1) / DPDLP_BS;
-  k_gather_calib<<<(unsigned)g, DPDLP_BS, 0, ST(stream)>>>(idx, x, y, rows, L, slice);
+int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long count,
+                           int chunk, long long slice, int xcd_local, void *stream) {
+  if (count <= 0 || chunk <= 0) return (int)hipErrorInvalidValue;
+  long long g = (count + chunk - 1) / chunk;
+  if (g > 0x7fffffffLL) return (int)hipErrorInvalidValue;
+  k_gather_calib<<<(unsigned)g, DPDLP_BS, 0, ST(stream)>>>(idx, x, y, count, chunk, slice,
+                                                           xcd_local);
   return (int)hipGetLastError();
 }
 

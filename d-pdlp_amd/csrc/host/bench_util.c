@@ -28,6 +28,10 @@ double dpdlp_time_spmv(dpdlp_resident_t *r, int transpose, int reps, int use32) 
   double *xpart = NULL;
   long long xl = dpdlp_resident_xpart_len(r);
   if (xl > 0 && hipMalloc((void **)&xpart, (size_t)xl * sizeof(double)) != hipSuccess) return -1;
+  if (xl > 0 && hipMemset(xpart, 0, (size_t)xl * sizeof(double)) != hipSuccess) {
+    hipFree(xpart);
+    return -1;
+  }
   dpdlp_resident_csr(r, transpose, &A, xpart);
   hipStream_t s = NULL;
   hipEvent_t e0 = NULL, e1 = NULL;
@@ -94,12 +98,22 @@ int dpdlp_resident_transpose(const dpdlp_resident_t *r, int *t_rp, int *t_ci, do
 
 int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, double *yh,
                         int use32) {
-  if (!r || !xh || !yh) return -1;
+  return dpdlp_resident_spmv_ex(r, transpose, xh, yh, use32, 1, 0);
+}
+
+int dpdlp_resident_spmv_ex(dpdlp_resident_t *r, int transpose, const double *xh, double *yh,
+                           int use32, int reps, int two_kernel) {
+  if (!r || !xh || !yh || reps < 1) return -1;
   dpdlp_csr_t A;
   double *xpart = NULL;
   long long xl = dpdlp_resident_xpart_len(r);
   if (xl > 0 && hipMalloc((void **)&xpart, (size_t)xl * sizeof(double)) != hipSuccess) return -1;
+  if (xl > 0 && hipMemset(xpart, 0, (size_t)xl * sizeof(double)) != hipSuccess) {
+    hipFree(xpart);
+    return -1;
+  }
   dpdlp_resident_csr(r, transpose, &A, xpart);
+  if (two_kernel) A.tick = NULL;
   double *x = NULL, *y = NULL, *xp = NULL;
   float *x32 = NULL, *v32 = NULL, *pv32 = NULL;
   int rc = -1;
@@ -127,9 +141,11 @@ int dpdlp_resident_spmv(dpdlp_resident_t *r, int transpose, const double *xh, do
       if (dpdlp_dev_make_f32(A.pv, pv32, A.nnz, s)) goto out;
       A.pv32 = pv32;
     }
-    if (dpdlp_dev_spmv_f32(&A, x32, y, s)) goto out;
+    for (int k = 0; k < reps; ++k)
+      if (dpdlp_dev_spmv_f32(&A, x32, y, s)) goto out;
   } else {
-    if (dpdlp_dev_spmv(&A, x, y, s)) goto out;
+    for (int k = 0; k < reps; ++k)
+      if (dpdlp_dev_spmv(&A, x, y, s)) goto out;
   }
   if (hipStreamSynchronize(s) != hipSuccess) goto out;
   if (A.rows && hipMemcpy(yh, y, (size_t)A.rows * 8, hipMemcpyDeviceToHost) != hipSuccess)
@@ -209,39 +225,41 @@ out:
   return rc;
 }
 
-double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps) {
-  const int L = 20;
-  long long slice = slice_bytes / 8, rows = gathers / L;
-  if (slice < 1 || rows < 1 || reps < 1) return -1.0;
-  int *h = (int *)malloc((size_t)(rows * L) * sizeof(int));
+double dpdlp_gather_calibration_ex(long long slice_bytes, long long gathers, int reps,
+                                   int xcd_local, int chunk) {
+  long long slice = slice_bytes / 8;
+  if (slice < 1 || slice > 0x7fffffffLL || gathers < 1 || reps < 1 || chunk < 1) return -1.0;
+  long long xlen = xcd_local ? 8 * slice : slice;
+  long long blocks = (gathers + chunk - 1) / chunk;
+  int *h = (int *)malloc((size_t)gathers * sizeof(int));
   int *idx = NULL;
   double *x = NULL, *y = NULL;
   hipEvent_t e0 = NULL, e1 = NULL;
   double rate = -1.0;
   if (!h) return -1.0;
   uint64_t st = 0x9e3779b97f4a7c15ull; /* xorshift64: fixed random columns */
-  for (long long k = 0; k < rows * L; ++k) {
+  for (long long k = 0; k < gathers; ++k) {
     st ^= st << 13;
     st ^= st >> 7;
     st ^= st << 17;
     h[k] = (int)(st % (uint64_t)slice);
   }
-  if (hipMalloc((void **)&idx, (size_t)(rows * L) * sizeof(int)) != hipSuccess) goto out;
-  if (hipMalloc((void **)&x, (size_t)(8 * slice) * sizeof(double)) != hipSuccess) goto out;
-  if (hipMalloc((void **)&y, (size_t)rows * sizeof(double)) != hipSuccess) goto out;
-  if (hipMemcpy(idx, h, (size_t)(rows * L) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(x, 0, (size_t)(8 * slice) * sizeof(double)) != hipSuccess)
+  if (hipMalloc((void **)&idx, (size_t)gathers * sizeof(int)) != hipSuccess) goto out;
+  if (hipMalloc((void **)&x, (size_t)xlen * sizeof(double)) != hipSuccess) goto out;
+  if (hipMalloc((void **)&y, (size_t)blocks * 256 * sizeof(double)) != hipSuccess) goto out;
+  if (hipMemcpy(idx, h, (size_t)gathers * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(x, 0, (size_t)xlen * sizeof(double)) != hipSuccess)
     goto out;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto out;
-  if (dpdlp_dev_gather_calib(idx, x, y, rows, L, slice, NULL)) goto out;
+  if (dpdlp_dev_gather_calib(idx, x, y, gathers, chunk, slice, xcd_local, NULL)) goto out;
   if (hipEventRecord(e0, NULL) != hipSuccess) goto out;
   for (int k = 0; k < reps; ++k)
-    if (dpdlp_dev_gather_calib(idx, x, y, rows, L, slice, NULL)) goto out;
+    if (dpdlp_dev_gather_calib(idx, x, y, gathers, chunk, slice, xcd_local, NULL)) goto out;
   if (hipEventRecord(e1, NULL) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) goto out;
   {
     float t = 0.f;
     if (hipEventElapsedTime(&t, e0, e1) == hipSuccess && t > 0.f)
-      rate = (double)(rows * L) * reps / ((double)t * 1e-3) / 1e9;
+      rate = (double)gathers * reps / ((double)t * 1e-3) / 1e9;
   }
 out:
   hipDeviceSynchronize();
@@ -252,6 +270,10 @@ out:
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   return rate;
+}
+
+double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps) {
+  return dpdlp_gather_calibration_ex(slice_bytes, gathers, reps, 1, 2048);
 }
 
 double dpdlp_copy_calibration(long long bytes, int reps) {

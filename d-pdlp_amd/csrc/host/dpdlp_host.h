@@ -54,10 +54,12 @@ struct dpdlp_resident *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, 
 char *dpdlp_i_resident_release(struct dpdlp_resident *r, size_t *cap);
 
 /* Device descriptors of a resident problem's unscaled matrices (xpart must be
- * provided by the caller when P > 1). */
+ * provided by the caller when P > 1, dpdlp_resident_xpart_len doubles,
+ * zeroed before its first use: it ends with the tiled SpMVs' tickets). */
 void dpdlp_resident_csr(const struct dpdlp_resident *r, int transpose, void *csr_out,
                         double *xpart);
-/* Doubles of partial-sum scratch the column-panel SpMVs of r need. */
+/* Doubles of scratch the column-panel SpMVs of r need: the partial row sums,
+ * then one 4-byte ticket per row block of each tiled matrix. */
 long long dpdlp_resident_xpart_len(const struct dpdlp_resident *r);
 
 /* logging.c: the reference's console formats (src/utils.cu:191-501) */

@@ -124,23 +124,37 @@ static int pick_panels(int cols) {
   return P;
 }
 
+/* Partial row sums of the tiled SpMV: P vectors of the tiled matrix's rows,
+ * shared by A and A' (one SpMV at a time). */
+static long long xpart_sums_len(const dpdlp_resident_t *r) {
+  long long a = r->A_P > 1 ? (long long)r->A_P * r->m : 0;
+  long long b = r->At_P > 1 ? (long long)r->At_P * r->n : 0;
+  return a > b ? a : b;
+}
+
 void dpdlp_resident_csr(const dpdlp_resident_t *r, int transpose, void *out, double *xpart) {
   dpdlp_csr_t *A = (dpdlp_csr_t *)out;
   memset(A, 0, sizeof(*A));
+  /* the tickets follow the partial sums: A's row blocks first, then A''s */
+  unsigned *tick = xpart ? (unsigned *)(xpart + xpart_sums_len(r)) : NULL;
   if (!transpose) {
     *A = (dpdlp_csr_t){r->m, r->n, r->nnz, r->A_rp, r->A_ci, r->A_v, NULL, r->A_blk, r->A_nblk,
                        r->cap, r->A_P, r->A_toff, r->A_rofs, r->A_pci, r->A_pv, NULL, xpart};
+    if (r->A_P > 1) A->tick = tick;
   } else {
     *A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
                        r->At_nblk, r->At_cap, r->At_P, r->At_toff, r->At_rofs, r->At_pci, r->At_pv,
                        NULL, xpart};
+    if (r->At_P > 1 && tick) A->tick = tick + (r->A_P > 1 ? r->A_nblk : 0);
   }
 }
 
 long long dpdlp_resident_xpart_len(const dpdlp_resident_t *r) {
-  long long a = r->A_P > 1 ? (long long)r->A_P * r->m : 0;
-  long long b = r->At_P > 1 ? (long long)r->At_P * r->n : 0;
-  return a > b ? a : b;
+  long long sums = xpart_sums_len(r);
+  if (sums == 0) return 0;
+  long long tickets = (r->A_P > 1 ? (long long)r->A_nblk : 0) +
+                      (r->At_P > 1 ? (long long)r->At_nblk : 0);
+  return sums + (tickets + 1) / 2; /* two 4-byte tickets per double */
 }
 
 static int pick_cap(long long nnz, int rows) {

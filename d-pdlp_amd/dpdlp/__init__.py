@@ -57,10 +57,14 @@ def _load() -> C.CDLL:
         "dpdlp_generate": (P, [C.POINTER(GenSpec), _dp]),
         "dpdlp_copy_calibration": (C.c_double, [C.c_longlong, C.c_int]),
         "dpdlp_gather_calibration": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int]),
+        "dpdlp_gather_calibration_ex": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int,
+                                                     C.c_int, C.c_int]),
         "dpdlp_reduce_host": (C.c_int, [C.c_int, _dp, _dp, C.c_longlong, _dp]),
         "dpdlp_scaling_host": (C.c_int, [P, C.POINTER(PdhgParameters)] + [_dp] * 9),
         "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),
         "dpdlp_resident_spmv": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int]),
+        "dpdlp_resident_spmv_ex": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_int,
+                                             C.c_int]),
         "dpdlp_resident_transpose": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
         "dpdlp_upload_ex": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int]),
         "dpdlp_upload_caps": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
@@ -344,10 +348,11 @@ class Resident:
         rp = lib.dpdlp_solve_resident(self.h, pp or C.byref(default_params()), C.byref(o))
         return result_from_ptr(rp), o
 
-    def spmv(self, x, transpose=False, use32=False):
+    def spmv(self, x, transpose=False, use32=False, reps=1, two_kernel=False):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.n if transpose else self.m)
-        rc = lib.dpdlp_resident_spmv(self.h, int(transpose), _ptr(x), _ptr(y), int(use32))
+        rc = lib.dpdlp_resident_spmv_ex(self.h, int(transpose), _ptr(x), _ptr(y), int(use32),
+                                        int(reps), int(two_kernel))
         if rc != 0:
             raise RuntimeError("spmv failed: " + last_error())
         return y
@@ -527,6 +532,24 @@ def batch_member(k, base_seed=3000, m_lo=500, m_hi=200_000):
     """Member k of the uneven batch workload: (Problem, known optimum)."""
     spec = GenSpec()
     lib.dpdlp_batch_member_spec(k, base_seed, m_lo, m_hi, C.byref(spec))
+    opt = C.c_double(0.0)
+    ptr = lib.dpdlp_generate(C.byref(spec), C.byref(opt))
+    return Problem(ptr), opt.value
+
+
+def serve_pool_member(k, pool_small=64, member_seed=3000, m_lo=500, m_hi=200_000,
+                      large_m=1_000_000, large_nnz=10_000_000):
+    """Entry k of the serving request pool (dpdlp_serve_run builds the same
+    pool): members 0..pool_small-1 of the uneven batch workload, then the
+    large requests (batch-generator draws at m = large_m with
+    large_nnz / large_m - 1 as the Poisson row-length mean)."""
+    spec = GenSpec()
+    if k < pool_small:
+        lib.dpdlp_batch_member_spec(k, member_seed, m_lo, m_hi, C.byref(spec))
+    else:
+        lib.dpdlp_batch_member_spec(k - pool_small, member_seed + 100000, large_m, large_m,
+                                    C.byref(spec))
+        spec.poisson_lambda = max(large_nnz / large_m - 1.0, 0.5)
     opt = C.c_double(0.0)
     ptr = lib.dpdlp_generate(C.byref(spec), C.byref(opt))
     return Problem(ptr), opt.value

@@ -24,8 +24,9 @@ extern "C" {
  * Workgroup w of the tiled SpMV handles tile w; the hardware dispatches
  * workgroups round-robin over the 8 XCDs, so every XCD gathers from only one
  * panel of the vector, which stays resident in its 4 MB L2.  Per-panel row
- * sums go to xpart[k*rows + row] and a second kernel adds the P partials in
- * panel order and applies the epilogue. */
+ * sums go to xpart[k*rows + row]; the last of a row block's P workgroups to
+ * finish (counted on tick[b]) adds the P partials in panel order and applies
+ * the epilogue.  Without tickets a second kernel does that. */
 typedef struct {
   int rows, cols, nnz;
   const int *rp;     /* [rows+1] */
@@ -43,6 +44,9 @@ typedef struct {
   const double *pv;           /* [nnz] tiled FP64 values */
   const float *pv32;          /* [nnz] tiled FP32 values or NULL */
   double *xpart;              /* [P*rows] partial row sums (scratch) */
+  unsigned *tick;             /* [nblk] last-arriver tickets of the one-pass
+                                 tiled SpMV (zeroed once, never reset), or NULL
+                                 for the two-kernel form */
 } dpdlp_csr_t;
 
 /* One LP's device state (scaled space).  Field meaning follows the
@@ -255,11 +259,12 @@ int dpdlp_dev_reduce_blocks(long long n);
 int dpdlp_dev_reduce(int op, const double *x, const double *y, long long n, double *part,
                      double *out, void *stream);
 
-/* y[i] = sum of x[slice_base + idx[i*L + k]], k < L (L a multiple of 10),
- * where workgroup w reads the slice (w mod 8) * slice of x: random
- * L2-resident gathers (gather-rate calibration). */
-int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long rows, int L,
-                           long long slice, void *stream);
+/* Gather-rate calibration (the SpMVs' gather ceiling): workgroup w streams
+ * idx[w*chunk .. (w+1)*chunk) coalesced and gathers x[base + idx[k]], base =
+ * (w mod 8) * slice when xcd_local (each XCD reads only its own slice), else
+ * 0; y gets one partial sum per thread (ceil(count/chunk) * 256 doubles). */
+int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long count,
+                           int chunk, long long slice, int xcd_local, void *stream);
 /* Plain device copy of `bytes` (multiple of 32): the reference point that
  * scales the FETCH_SIZE / WRITE_SIZE counters to bytes. */
 int dpdlp_dev_copy_calib(const void *src, void *dst, long long bytes, void *stream);

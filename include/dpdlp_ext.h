@@ -96,6 +96,12 @@ double dpdlp_time_spmv(dpdlp_resident_t *res, int transpose, int reps, int use32
  * unscaled matrix, with host input and output (for parity tests). */
 int dpdlp_resident_spmv(dpdlp_resident_t *res, int transpose, const double *x, double *y,
                         int use32);
+/* dpdlp_resident_spmv run `reps` times on the same scratch (the tiled SpMV's
+ * tickets advance across the launches), the last result returned;
+ * two_kernel = 1 runs a tiled matrix with the two-kernel form instead of the
+ * one-pass kernel. */
+int dpdlp_resident_spmv_ex(dpdlp_resident_t *res, int transpose, const double *x, double *y,
+                           int use32, int reps, int two_kernel);
 
 /* The device scaling of a solve (Ruiz, Pock-Chambolle, bound/objective as
  * params ask) of p in its given row order, copied back: scaled A values (CSR
@@ -108,9 +114,15 @@ int dpdlp_scaling_host(const lp_problem_t *p, const pdhg_parameters_t *params, d
  * *out (y is read for the dot product only); for tests. */
 int dpdlp_reduce_host(int op, const double *x, const double *y, long long n, double *out);
 
-/* Rate (G gathers/s) of `gathers` random 8-byte reads in which every
- * workgroup reads only the slice_bytes slice of its own XCD (L2-resident
- * for slice_bytes up to a few MB): the ceiling of an SpMV's gathers. */
+/* Rate (G gathers/s) of `gathers` random 8-byte reads with the SpMV's access
+ * pattern (coalesced index stream, `chunk` indices per workgroup, eight
+ * gathers in flight per thread).  xcd_local: every workgroup reads only the
+ * slice_bytes slice of its own XCD (an 8 * slice_bytes vector, each XCD's L2
+ * holding its slice when it fits); else all workgroups read one slice_bytes
+ * vector.  The ceiling of an SpMV's gathers at that working-set size. */
+double dpdlp_gather_calibration_ex(long long slice_bytes, long long gathers, int reps,
+                                   int xcd_local, int chunk);
+/* dpdlp_gather_calibration_ex(slice_bytes, gathers, reps, 1, 2048) */
 double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps);
 
 /* Copies `bytes` device-to-device with a plain kernel `reps` times; returns the

@@ -1,4 +1,6 @@
 """GPU parity tests of the core path: SpMV, full solves vs the oracle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -144,6 +146,48 @@ def test_spmv_column_panels(dp, panels, cap):
     assert np.max(np.abs(z - refz)) <= 1e-12 * max(1.0, np.max(np.abs(refz)))
     y32 = R.spmv(x, use32=True)
     assert np.max(np.abs(y32 - ref)) <= 1e-5 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("panels", ["2,2", "4,4", "8,8"], indirect=True)
+def test_tiled_spmv_one_pass_equals_two_kernels(dp, panels, monkeypatch):
+    """The one-pass tiled SpMV (the last of a row block's P workgroups to
+    finish adds the panel partials; DPDLP_TILE_ONEPASS=1) gives the same bits
+    as the two-kernel form, on the first launch and after many launches on the
+    same tickets, with rows longer than a tile, empty tiles and blocks of up to
+    1024 rows.  The library reads the switch once per process, so this test
+    runs the one-pass form in a child process."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path[:0] = sys.argv[1:]\n"
+        "import test_gpu_core as t, dpdlp\n"
+        "t._one_pass_vs_two_kernels(dpdlp)\n")
+    here = os.path.dirname(os.path.abspath(__file__))
+    root = os.path.dirname(here)
+    env = dict(os.environ, DPDLP_TILE_ONEPASS="1")
+    r = subprocess.run([sys.executable, "-c", code, here, os.path.join(root, "d-pdlp_amd")],
+                       env=env, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def _one_pass_vs_two_kernels(dp):
+    rng = np.random.default_rng(29)
+    m, n = 6000, 5000
+    lens = np.minimum((rng.pareto(1.2, m) * 2).astype(int), 4900)
+    rp, ci, v = random_csr(m, n, lens, seed=31, long_rows=((7, 4999), (4000, 3000)))
+    prob = dp.Problem.from_csr(m, n, rp, ci, v, c=np.zeros(n))
+    R = dp.Resident(prob, spmv_cap=1024)
+    for transpose, length in ((False, n), (True, m)):
+        x = rng.standard_normal(length)
+        two = R.spmv(x, transpose=transpose, two_kernel=True)
+        one = R.spmv(x, transpose=transpose)
+        many = R.spmv(x, transpose=transpose, reps=37)
+        assert np.array_equal(one, two) and np.array_equal(many, two), transpose
+        for use32 in (True,):
+            a32 = R.spmv(x, transpose=transpose, use32=True, reps=5)
+            b32 = R.spmv(x, transpose=transpose, use32=True, two_kernel=True)
+            assert np.array_equal(a32, b32)
+    R.free()
 
 
 @pytest.mark.parametrize("panels", ["8,4"], indirect=True)

@@ -61,3 +61,81 @@ def test_largest_batch_members_alone_equal_mix(dp):
         assert np.array_equal(alone.x, mix[i].x) and np.array_equal(alone.y, mix[i].y)
     for r in res:
         r.free()
+
+
+def test_large_zipf_lp_matches_openmp_oracle(dp, orc):
+    """The m = 1e6, n = 2e6, nnz = 2e7 Zipf LP (seed 1000) solved on the device
+    through the C API's automatic layout (rows renumbered by length, A on XCD
+    column panels) against the OpenMP build of the FP64 oracle on the host (its
+    loops over rows and columns are split across threads; each row's sum is
+    still taken in order): same status and iteration count, objective within
+    1e-6, and x_cur / y_cur within 1e-8 at trace points through the first
+    restart (T = 200) and beyond."""
+    from helpers import rel
+    prob, opt = dp.generate(1_000_000, 2_000_000, 20_000_000, row_rule=1, pm1_frac=0.1,
+                            seed=1000)
+    iters = [0, 10, 100, 200, 201, 400, 600]
+    params = dp.default_params(presolve=False)
+    res, opts, (tx, ty) = dp.solve(prob, params, trace_iters=iters)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(presolve=False), trace_iters=iters,
+                                 n=prob.n, m=prob.m, omp=True)
+    print(f"device {res.status} T={res.iterations} obj={res.primal_obj!r}; oracle "
+          f"{ref['status']} T={ref['total_count']} obj={ref['primal_objective_value']!r}",
+          flush=True)
+    assert res.status == ref["status"] == "OPTIMAL"
+    assert res.iterations == ref["total_count"]
+    obj = ref["primal_objective_value"]
+    assert abs(res.primal_obj - obj) <= 1e-6 * max(1.0, abs(obj))
+    k = min(len(tx), len(ox))
+    assert k == len(iters)
+    worst = 0.0
+    for i in range(k):
+        ex, ey = rel(tx[i], ox[i]), rel(ty[i], oy[i])
+        worst = max(worst, ex, ey)
+        assert ex < 1e-8 and ey < 1e-8, (iters[i], ex, ey)
+    print(f"worst iterate deviation over {iters}: {worst:.2e}", flush=True)
+    assert rel(res.x, ref["x"]) < 1e-8 and rel(res.y, ref["y"]) < 1e-8
+
+
+def test_zipf4m_mixed_precision(dp):
+    """The m = 4e6, n = 8e6, nnz = 1.2e8 LP (seed 2000) with FP32 matrix values
+    and gathers, FP64 state and an FP64 finish, to 1e-4: OPTIMAL, a solution
+    whose bounds, constraint violation, dual signs and reduced costs hold when
+    recomputed with scipy, the objective near the known optimum; and the FP64
+    device solve of the same LP for the error of the mixed result."""
+    prob, opt = dp.generate(4_000_000, 8_000_000, 120_000_000, row_rule=1, seed=2000)
+    a = prob.arrays()
+    params = dp.default_params(presolve=False)
+    R = dp.Resident(prob)
+    mixed, om = R.solve(params, precision=dp.PREC_MIXED)
+    fp64, of = R.solve(params)
+    R.free()
+    print(f"mixed {mixed.status} T={mixed.iterations} (fp32 {om.iterations_fp32}) "
+          f"loop {om.loop_time_sec:.2f}s; fp64 {fp64.status} T={fp64.iterations} "
+          f"loop {of.loop_time_sec:.2f}s", flush=True)
+    assert mixed.status == fp64.status == "OPTIMAL"
+    assert om.iterations_fp32 > 0
+    for r in (mixed, fp64):
+        assert r.rel_primal < 1e-4 and r.rel_dual < 1e-4 and r.rel_gap < 1e-4
+    A = sp.csr_matrix((a["vals"], a["col_ind"], a["row_ptr"]), shape=(prob.m, prob.n))
+    x, y, rc = mixed.x, mixed.y, mixed.reduced_cost
+    c, lc, uc, lv, uv = a["c"], a["con_lb"], a["con_ub"], a["var_lb"], a["var_ub"]
+    assert np.all(x >= lv - 1e-9 * (1 + np.abs(np.where(np.isfinite(lv), lv, 0))))
+    assert np.all(x <= uv + 1e-9 * (1 + np.abs(np.where(np.isfinite(uv), uv, 0))))
+    ax = A @ x
+    viol = ax - np.clip(ax, lc, uc)
+    bnorm = np.linalg.norm(np.concatenate([lc[np.isfinite(lc)], uc[np.isfinite(uc)]]))
+    assert np.linalg.norm(viol) <= 1e-3 * (1 + bnorm)
+    ge = np.isfinite(lc) & ~np.isfinite(uc)
+    le = ~np.isfinite(lc) & np.isfinite(uc)
+    assert np.all(y[ge] >= 0) and np.all(y[le] <= 0)
+    assert np.linalg.norm(rc - (c - A.T @ y)) <= 1e-9 * (1 + np.linalg.norm(c))
+    obj = float(c @ x)
+    assert abs(mixed.primal_obj - obj) <= 1e-9 * max(1.0, abs(obj))
+    assert abs(obj - opt) <= 1e-3 * max(1.0, abs(opt))
+    # error of the mixed result against the FP64 device solve (both at 1e-4)
+    dobj = abs(mixed.primal_obj - fp64.primal_obj) / max(1.0, abs(fp64.primal_obj))
+    ex = np.linalg.norm(mixed.x - fp64.x) / np.linalg.norm(fp64.x)
+    ey = np.linalg.norm(mixed.y - fp64.y) / np.linalg.norm(fp64.y)
+    print(f"mixed vs fp64: objective {dobj:.2e}, x {ex:.2e}, y {ey:.2e}", flush=True)
+    assert dobj <= 1e-4

@@ -86,3 +86,58 @@ def test_large_lane_results_equal_alone(dp, precision):
     assert rep["completed"] == rep["arrived"] > 20
     for k in range(rep["arrived"]):
         assert same(res[k], ref[idx[k]]), (k, idx[k])
+
+
+def test_large_request_through_the_mix(dp):
+    """An m = 1e6, nnz = 1e7 request (the serving workload's large class)
+    arrives among small ones, goes to the large-LP lane and overlaps the batch
+    on the device; its result equals the same LP solved alone on the
+    single-LP path (automatic layout, as the lane uploads it) bit for bit,
+    and every small result equals its solve alone."""
+    params = dp.default_params(presolve=False)
+    pool = dict(POOL, pool_small=3, pool_large=1, large_frac=0.25, large_m=1_000_000,
+                large_nnz=10_000_000)
+    rep, lat, idx, res = dp.serve(params, want_results=True, rate=40.0, duration_sec=0.2,
+                                  drain_sec=120.0, seed=21, max_lps=4, precision=0, **pool)
+    print(f"serve with large lane: arrived {rep['arrived']} pool indices {list(idx)}", flush=True)
+    assert rep["completed"] == rep["arrived"] and rep["optimal"] == rep["arrived"]
+    large = [k for k in range(rep["arrived"]) if idx[k] == 3]
+    assert large, "no large request in this stream"
+    prob, opt = dp.serve_pool_member(3, pool_small=3, member_seed=POOL["member_seed"],
+                                     m_lo=POOL["m_lo"], m_hi=POOL["m_hi"],
+                                     large_m=1_000_000, large_nnz=10_000_000)
+    assert prob.m == 1_000_000 and abs(prob.nnz - 10_000_000) < 1_000_000
+    R = dp.Resident(prob)
+    alone, _ = R.solve(params)
+    for k in large:
+        assert same(res[k], alone), k
+    assert abs(alone.primal_obj - opt) <= 1e-3 * max(1.0, abs(opt))
+    small_ref = alone_results(dp, params, 0)
+    for k in range(rep["arrived"]):
+        if idx[k] < 3:
+            assert same(res[k], small_ref[idx[k]]), (k, idx[k])
+
+
+def test_served_request_matches_oracle(dp, orc):
+    """A served small request against the FP64 oracle on the same LP: same
+    status and iteration count, objective within 1e-6, x and y within 1e-8."""
+    from helpers import rel
+    params = dp.default_params(presolve=False)
+    rep, lat, idx, res = dp.serve(params, want_results=True, rate=200.0, duration_sec=0.05,
+                                  drain_sec=60.0, seed=31, max_lps=4, precision=0, **POOL)
+    assert rep["completed"] == rep["arrived"] > 0
+    checked = set()
+    for k in range(rep["arrived"]):
+        j = int(idx[k])
+        if j in checked:
+            continue
+        checked.add(j)
+        prob, _ = dp.batch_member(j, POOL["member_seed"], POOL["m_lo"], POOL["m_hi"])
+        ref, _ = orc.solve(prob.ptr, orc.default_params(presolve=False))
+        r = res[k]
+        assert r.status == ref["status"] == "OPTIMAL"
+        assert r.iterations == ref["total_count"], (j, r.iterations, ref["total_count"])
+        obj = ref["primal_objective_value"]
+        assert abs(r.primal_obj - obj) <= 1e-6 * max(1.0, abs(obj))
+        assert rel(r.x, ref["x"]) < 1e-8 and rel(r.y, ref["y"]) < 1e-8, j
+    assert len(checked) >= 2
