@@ -536,6 +536,7 @@ def main():
         "copy_kernel_GBps": copy_GBps,
         "cpu_baseline": cpu,
         "xcd_panels": R.panels,
+        "layout": {"panels": R.panels, "row_block_caps": R.caps, "rows_renumbered": R.renumbered},
         "generate_s": gen_s,
         "upload_s": up_s,
     }

@@ -22,6 +22,15 @@ void dpdlp_resident_info(const dpdlp_resident_t *r, int *m, int *n, int *nnz, in
   if (nblk_At) *nblk_At = r->At_nblk;
 }
 
+void dpdlp_resident_layout(const dpdlp_resident_t *r, int *panels_A, int *panels_At, int *cap_A,
+                           int *cap_At, int *renumbered) {
+  if (panels_A) *panels_A = r->A_P;
+  if (panels_At) *panels_At = r->At_P;
+  if (cap_A) *cap_A = r->cap;
+  if (cap_At) *cap_At = r->At_cap;
+  if (renumbered) *renumbered = r->h_row_perm != NULL;
+}
+
 double dpdlp_time_spmv(dpdlp_resident_t *r, int transpose, int reps, int use32) {
   if (!r || reps <= 0) return -1.0;
   dpdlp_csr_t A;

@@ -139,12 +139,13 @@ void dpdlp_resident_csr(const dpdlp_resident_t *r, int transpose, void *out, dou
   unsigned *tick = xpart ? (unsigned *)(xpart + xpart_sums_len(r)) : NULL;
   if (!transpose) {
     *A = (dpdlp_csr_t){r->m, r->n, r->nnz, r->A_rp, r->A_ci, r->A_v, NULL, r->A_blk, r->A_nblk,
-                       r->cap, r->A_P, r->A_toff, r->A_rofs, r->A_pci, r->A_pv, NULL, xpart};
+                       r->cap, r->A_P, r->A_toff, r->A_rofs, r->A_pci, r->A_pv, NULL, xpart,
+                       NULL};
     if (r->A_P > 1) A->tick = tick;
   } else {
     *A = (dpdlp_csr_t){r->n, r->m, r->nnz, r->At_rp, r->At_ci, r->At_v, NULL, r->At_blk,
                        r->At_nblk, r->At_cap, r->At_P, r->At_toff, r->At_rofs, r->At_pci, r->At_pv,
-                       NULL, xpart};
+                       NULL, xpart, NULL};
     if (r->At_P > 1 && tick) A->tick = tick + (r->A_P > 1 ? r->A_nblk : 0);
   }
 }
@@ -407,6 +408,36 @@ static int row_order_by_length(int m, const int *rp, int *perm) {
   return 0;
 }
 
+/* Deals the rows of a length order (perm[rank] = row) round-robin into P
+ * contiguous ranges of width W = ceil(m / P), the column panels of A'
+ * (panels.c), a full range being skipped: perm[new] = row afterwards.  Every
+ * panel then holds an equal share of the long rows, and within a panel the
+ * rows still run from longest to shortest, so the dual entries that each
+ * XCD gathers most often share cache lines of its own slice. */
+static int interleave_panels(int m, int P, int *perm) {
+  if (P <= 1 || m <= 0) return 0;
+  const long long W = ((long long)m + P - 1) / P;
+  int *out = (int *)malloc((size_t)m * sizeof(int));
+  long long fill[8] = {0};
+  if (!out) {
+    dpdlp_set_error("host allocation failed");
+    return -1;
+  }
+  int k = 0;
+  for (int r = 0; r < m; ++r) {
+    for (;;) {
+      const long long capk = k < P - 1 ? W : (long long)m - (long long)(P - 1) * W;
+      if (fill[k] < capk) break;
+      k = (k + 1) % P;
+    }
+    out[k * W + fill[k]++] = perm[r];
+    k = (k + 1) % P;
+  }
+  memcpy(perm, out, (size_t)m * sizeof(int));
+  free(out);
+  return 0;
+}
+
 static void free_perm_copies(dpdlp_resident_t *r) {
   free(r->h_perm_rp);
   free(r->h_perm_ci);
@@ -502,10 +533,32 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
    * rows are renumbered by decreasing length, so the dual entries that A'
    * gathers most often share cache lines; vectors indexed by rows are
    * permuted at upload and the dual result back at the end. */
+  const int renumber = order > 0 || (order < 0 && rows_skewed(m, rp));
+
+  /* XCD column-panel tiles.  Without a panel count from the caller, A'
+   * stays untiled when the rows are renumbered (A' 128 us untiled at
+   * m = 1e6, nnz = 2e7 Zipf rows).  When A' is tiled over renumbered rows,
+   * the length order is dealt round-robin over its column panels
+   * (interleave_panels): in plain length order, panels of equal width would
+   * hand the first panel most of the nonzeros (256 / 487 us on 2 / 4
+   * panels). */
+  r->A_P = PA > 0 ? PA : pick_panels(n);
+  r->At_P = PAt > 0 ? PAt : (renumber ? 1 : pick_panels(m));
+  if (PA <= 0) {
+    const char *env = getenv("DPDLP_PANELS");
+    if (env) {
+      int a = 0, b = 0;
+      if (sscanf(env, "%d,%d", &a, &b) == 2) {
+        if (a == 1 || a == 2 || a == 4 || a == 8) r->A_P = a;
+        if (b == 1 || b == 2 || b == 4 || b == 8) r->At_P = b;
+      }
+    }
+  }
+
   const int *rp_u = rp, *ci_u = ci;
   const double *v_u = p->constraint_matrix_values, *lc_u = p->constraint_lower_bound,
                *uc_u = p->constraint_upper_bound;
-  if (order > 0 || (order < 0 && rows_skewed(m, rp))) {
+  if (renumber) {
     int *perm = (int *)malloc((size_t)(m ? m : 1) * sizeof(int));
     int *rp2 = (int *)malloc((size_t)(m + 1) * sizeof(int));
     int *ci2 = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
@@ -523,6 +576,7 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
       goto ufail;
     }
     if (row_order_by_length(m, rp, perm)) goto ufail;
+    if (interleave_panels(m, r->At_P, perm)) goto ufail;
     rp2[0] = 0;
     for (int i = 0; i < m; ++i) {
       int o = perm[i];
@@ -547,23 +601,6 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
     uc_u = uc2;
   }
 
-  /* XCD column-panel tiles.  A' stays untiled when the rows were renumbered:
-   * its columns (the rows of A) then run from longest to shortest, and
-   * panels of equal width would hand the first panel most of the nonzeros
-   * (m = 1e6, nnz = 2e7 Zipf rows: A' 128 us untiled, 256 / 487 us on 2 / 4
-   * panels; m = 4e6, nnz = 1.2e8: 0.80 ms against 3.3 ms on 4). */
-  r->A_P = PA > 0 ? PA : pick_panels(n);
-  r->At_P = PAt > 0 ? PAt : (r->h_row_perm ? 1 : pick_panels(m));
-  if (PA <= 0) {
-    const char *env = getenv("DPDLP_PANELS");
-    if (env) {
-      int a = 0, b = 0;
-      if (sscanf(env, "%d,%d", &a, &b) == 2) {
-        if (a == 1 || a == 2 || a == 4 || a == 8) r->A_P = a;
-        if (b == 1 || b == 2 || b == 4 || b == 8) r->At_P = b;
-      }
-    }
-  }
 
   /* an untiled A' of a large problem with an automatic block size takes
    * 4096-nonzero blocks (its rows, the columns of A, are short and many) */

@@ -51,6 +51,7 @@ def _load() -> C.CDLL:
         "dpdlp_solve_resident": (R, [C.c_void_p, C.POINTER(PdhgParameters), O]),
         "dpdlp_resident_free": (None, [C.c_void_p]),
         "dpdlp_resident_info": (None, [C.c_void_p, _ip, _ip, _ip, _ip, _ip]),
+        "dpdlp_resident_layout": (None, [C.c_void_p, _ip, _ip, _ip, _ip, _ip]),
         "dpdlp_last_error": (C.c_char_p, []),
         "dpdlp_write_vector": (C.c_int, [C.c_char_p, _dp, C.c_int]),
         "dpdlp_time_spmv": (C.c_double, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
@@ -341,6 +342,12 @@ class Resident:
                                 C.byref(bt))
         self.m, self.n, self.nnz, self.nblk_A, self.nblk_At = (
             m.value, n.value, nnz.value, ba.value, bt.value)
+        pa, pt, ca, ct, rn = (C.c_int() for _ in range(5))
+        lib.dpdlp_resident_layout(self.h, C.byref(pa), C.byref(pt), C.byref(ca), C.byref(ct),
+                                  C.byref(rn))
+        self.panels = (pa.value, pt.value)
+        self.caps = (ca.value, ct.value)
+        self.renumbered = bool(rn.value)
 
     def solve(self, params=None, **opt_kw):
         o, _ = make_opts(n=self.n, m=self.m, **opt_kw)

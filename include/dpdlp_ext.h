@@ -137,6 +137,11 @@ int dpdlp_resident_transpose(const dpdlp_resident_t *res, int *t_rp, int *t_ci, 
 /* Problem sizes of a resident problem (m, n, nnz, row blocks of A and A'). */
 void dpdlp_resident_info(const dpdlp_resident_t *res, int *m, int *n, int *nnz,
                          int *nblk_A, int *nblk_At);
+/* The layout a resident problem was uploaded with: column-panel counts and
+ * row-block capacities of A and A', and whether its rows were renumbered by
+ * length (any output may be NULL). */
+void dpdlp_resident_layout(const dpdlp_resident_t *res, int *panels_A, int *panels_At, int *cap_A,
+                           int *cap_At, int *renumbered);
 
 /* ---- batched solving: many LPs advancing together ---- */
 

@@ -190,8 +190,11 @@ def _one_pass_vs_two_kernels(dp):
     R.free()
 
 
-@pytest.mark.parametrize("panels", ["8,4"], indirect=True)
-def test_parity_trace_with_panels(dp, orc, panels):
+@pytest.mark.parametrize("panels", ["8,4", "2,2"], indirect=True)
+def test_parity_trace_with_panels(dp, orc, panels, row_order):
+    """Tiled A and A' against the oracle, in the caller's row order and with
+    the rows renumbered by length, where a tiled A' deals the length order
+    round-robin over its column panels."""
     prob, opt = dp.generate(2000, 5000, 50000, row_rule=0, seed=2)
     iters = [0, 10, 100, 400, 1000]
     res, o, (tx, ty) = dp.solve(prob, dp.default_params(presolve=False), trace_iters=iters)

@@ -22,6 +22,7 @@ def _decls(header):
     txt = open(header).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     txt = re.sub(r"//.*", "", txt)
+    txt = re.sub(r"^\s*#.*$", "", txt, flags=re.M)   # preprocessor lines
     names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{]*\)\s*;", txt)
     return [n for n in names if n not in ("if", "for", "while", "sizeof", "return")]
 
