@@ -310,3 +310,9 @@ out:
   if (e1) hipEventDestroy(e1);
   return ms;
 }
+
+int dpdlp_resident_device_csr(const dpdlp_resident_t *r, int transpose, void *csr_out) {
+  if (!r || !csr_out) return -1;
+  dpdlp_resident_csr(r, transpose, csr_out, NULL);
+  return 0;
+}

@@ -184,6 +184,18 @@ int dpdlp_dev_spmv(const dpdlp_csr_t *A, const double *x, double *y, void *strea
 /* y = A x with FP32 values and FP32 x, FP64 accumulation. */
 int dpdlp_dev_spmv_f32(const dpdlp_csr_t *A, const float *x, double *y, void *stream);
 
+/* y = A x (FP64) without allocating or taking scratch from the caller.  No
+ * counterpart in the reference (whose cusparseSpMV call sites,
+ * pdlp_core_op.cu:501,534, run on a buffer allocated per solve); it is for
+ * callers that cannot allocate device memory (graph capture, serving loops).
+ * A tiled matrix (P > 1) uses a fixed per-GPU scratch instead of A->xpart and
+ * A->tick: P * rows <= DPDLP_NOALLOC_SUMS and nblk <= DPDLP_NOALLOC_TICKETS,
+ * else hipErrorInvalidValue with a message in dpdlp_last_error().  Calls that
+ * overlap on one GPU race on that scratch. */
+#define DPDLP_NOALLOC_SUMS (4 << 20)
+#define DPDLP_NOALLOC_TICKETS (1 << 16)
+int dpdlp_dev_spmv_noalloc(const dpdlp_csr_t *A, const double *x, double *y, void *stream);
+
 /* Phase 1 of one PDHG step: A'y_cur, primal step with projection and
  * reflection, Halpern update of x_cur.  Replaces the SpMV at
  * pdlp_core_op.cu:501 + kernels :393-418 + the x half of :445-462.

@@ -96,6 +96,10 @@ double dpdlp_time_spmv(dpdlp_resident_t *res, int transpose, int reps, int use32
  * unscaled matrix, with host input and output (for parity tests). */
 int dpdlp_resident_spmv(dpdlp_resident_t *res, int transpose, const double *x, double *y,
                         int use32);
+/* The device descriptor (a dpdlp_csr_t of include/dpdlp_dev.h) of a
+ * resident problem's unscaled A (transpose = 0) or A' (1), with no scratch
+ * (xpart and tick NULL), for callers of the dpdlp_dev_* SpMVs.  0 or -1. */
+int dpdlp_resident_device_csr(const dpdlp_resident_t *res, int transpose, void *csr_out);
 /* dpdlp_resident_spmv run `reps` times on the same scratch (the tiled SpMV's
  * tickets advance across the launches), the last result returned;
  * two_kernel = 1 runs a tiled matrix with the two-kernel form instead of the
