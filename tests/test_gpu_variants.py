@@ -265,3 +265,18 @@ def test_major_variant_plus_two_rule_on_device(dp):
         assert r.iterations == lim
         assert np.allclose(r.x, want_x, rtol=1e-12, atol=1e-15), (lim, r.x)
         assert np.allclose(r.y, want_y, rtol=1e-12, atol=1e-15), (lim, r.y)
+
+
+@pytest.mark.gpu
+def test_repeated_solves_in_one_process_are_identical(dp):
+    """Each solve reseeds the power iteration's start vector, so solving the
+    same LP twice in one process gives the same power-iteration count, step
+    size and result bits (the reference's second solve in a process would
+    continue its generator and may differ)."""
+    prob, _ = dp.generate(600, 1500, 12000, row_rule=0, seed=77)
+    params = dp.default_params(presolve=False)
+    r1, o1, _ = dp.solve(prob, params)
+    r2, o2, _ = dp.solve(prob, params)
+    assert o1.power_iterations == o2.power_iterations and o1.step_size == o2.step_size
+    assert r1.iterations == r2.iterations
+    assert np.array_equal(r1.x, r2.x) and np.array_equal(r1.y, r2.y)

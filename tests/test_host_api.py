@@ -437,3 +437,24 @@ def test_host_worker_cap_is_per_thread(dp):
     finally:
         lib.dpdlp_host_threads_cap(0)
         del os.environ["DPDLP_HOST_THREADS"]
+
+
+def test_start_vector_restarts_from_the_seed_every_call(dp):
+    """Every solve draws its power-iteration start vector from a fresh
+    mt19937(1) (solver.c and batch.c call dpdlp_normal_fill(1, m, ...)): two
+    draws in one process are the same bits, and a shorter draw is a prefix of a
+    longer one (no state carried between calls).  The reference keeps one
+    process-wide generator (utils.cu:29-30), so only its first solve in a
+    process starts from this vector."""
+    lib = C.CDLL(dp.LIB_PATH)
+    fill = lib.dpdlp_normal_fill
+    fill.argtypes = [C.c_uint32, C.c_int, C.POINTER(C.c_double)]
+    P = C.POINTER(C.c_double)
+    a, b, c = np.zeros(5000), np.zeros(5000), np.zeros(1234)
+    fill(1, 5000, a.ctypes.data_as(P))
+    fill(1, 1234, c.ctypes.data_as(P))
+    fill(1, 5000, b.ctypes.data_as(P))
+    assert np.array_equal(a.view(np.int64), b.view(np.int64))
+    assert np.array_equal(c.view(np.int64), a[:1234].view(np.int64))
+    src = open(os.path.join(ROOT, "d-pdlp_amd", "csrc", "host", "solver.c")).read()
+    assert "dpdlp_normal_fill(1u, m, h_q)" in src
