@@ -39,7 +39,7 @@ WORKLOADS = {
                  desc="batch of 512 LPs: m log-uniform [500, 2e5], n = m*U[1.5,3], "
                       "Poisson(8)+1 nnz/row, seeds 3000..3511, eps 1e-4"),
     "serve-poisson": dict(rates=(50.0, 200.0, 800.0), large_frac=0.05, large_m=1_000_000,
-                 large_nnz=10_000_000, budget_gb=200.0, pool_small=64,
+                 large_nnz=10_000_000, budget_gb=200.0, pool_small=512, pool_large=4,
                  desc="serving: Poisson arrivals, LPs from the batch generator + 5% "
                       "m=1e6 nnz=1e7, 200 GB admission, eps 1e-4"),
 }
@@ -101,13 +101,30 @@ def bench_batch(args, dpdlp):
     gen_s = time.perf_counter() - t
     nnz = sum(p.nnz for p, _ in probs)
     print(f"# generated {count} LPs, nnz={nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
+    params = dpdlp.default_params(presolve=False)
+    # per size class, the row-block layout (capacity, rows per block) of A and
+    # A' for which the batched phase kernels run fastest, cached by device
+    tune = None
+    lays = None
+    if not args.no_autotune:
+        cache = args.tune_cache or os.path.join(os.environ.get("TMPDIR", "/tmp"),
+                                                "dpdlp_batch_layouts.txt")
+        lays, rep = dpdlp.batch_autotune([p for p, _ in probs], params,
+                                         precision=dpdlp.PREC_MIXED, cache_path=cache)
+        hist = {}
+        for L in lays:
+            hist[str(L)] = hist.get(str(L), 0) + 1
+        tune = dict(report=rep, layouts=hist, cache=cache)
+        print(f"# autotune: {rep}", file=sys.stderr, flush=True)
     t = time.perf_counter()
     plain_res = [dpdlp.Resident(p) for p, _ in probs] if not args.no_plain else []
     cap_at = args.batch_cap_at or BATCH_CAP
-    batch_res = [dpdlp.Resident(p, spmv_cap=BATCH_CAP, panels=(1, 1), spmv_cap_At=cap_at)
-                 for p, _ in probs]
+    if lays is not None:
+        batch_res = [dpdlp.Resident(p, layout=L) for (p, _), L in zip(probs, lays)]
+    else:
+        batch_res = [dpdlp.Resident(p, spmv_cap=BATCH_CAP, panels=(1, 1), spmv_cap_At=cap_at)
+                     for p, _ in probs]
     up_s = time.perf_counter() - t
-    params = dpdlp.default_params(presolve=False)
     for k in range(args.warmup):
         if plain_res:
             plain_res[0].solve(params)
@@ -141,6 +158,18 @@ def bench_batch(args, dpdlp):
         print(f"# batch mixed step {k}: {el:.2f}s {count / el:.1f} LPs/s {cnt}", file=sys.stderr,
               flush=True)
     mixed, mixed_ph, mixed_cnt = res, ph, cnt
+    untuned_s = None
+    if lays is not None and not args.no_untuned:
+        # the same batch with every LP on the default 4096-nonzero, 256-row blocks
+        ures = [dpdlp.Resident(p, spmv_cap=BATCH_CAP, panels=(1, 1)) for p, _ in probs]
+        t0 = time.perf_counter()
+        dpdlp.solve_batch(ures, params, max_lps=count, spmv_cap=BATCH_CAP,
+                          spmv_cap_At=BATCH_CAP, precision=dpdlp.PREC_MIXED)
+        untuned_s = time.perf_counter() - t0
+        for r in ures:
+            r.free()
+        print(f"# batch mixed, untuned layout: {untuned_s:.2f}s {count / untuned_s:.1f} LPs/s",
+              file=sys.stderr, flush=True)
     fp64, fp64_s, fp64_cnt = None, None, None
     if not args.no_fp64_batch:
         fp64, fp64_s, fp64_cnt, _ = run_batch(dpdlp.PREC_FP64, 0)
@@ -185,6 +214,10 @@ def bench_batch(args, dpdlp):
             "speedup_vs_plain_loop": None if plain is None else plain_s / fp64_s,
             "counters": fp64_cnt},
         "batch_counters": mixed_cnt,
+        "autotune": tune,
+        "untuned_layout": None if untuned_s is None else {
+            "layout": [BATCH_CAP, 256, BATCH_CAP, 256], "s": untuned_s,
+            "LPs_per_s": count / untuned_s},
         "objective_rel_diff": {"mixed_vs_plain": obj_err(mixed, plain),
                                "fp64_batch_vs_plain": obj_err(fp64, plain)},
         "phase_ms_per_sample": {"primal": mixed_ph["primal_ms"] / n_s,
@@ -213,7 +246,7 @@ def bench_serve(args, dpdlp):
                 params, rate=rate, duration_sec=args.serve_duration, drain_sec=args.serve_drain,
                 seed=4000, mode=mode, precision=dpdlp.PREC_MIXED if mode == 0 else dpdlp.PREC_FP64,
                 large_frac=w["large_frac"], large_m=w["large_m"], large_nnz=w["large_nnz"],
-                pool_small=w["pool_small"],
+                pool_small=w["pool_small"], pool_large=w["pool_large"],
                 hbm_budget_gb=w["budget_gb"], max_lps=args.serve_max_lps)
             rep["wall_s"] = time.perf_counter() - t
             # per request class (pool entries past the small pool are the
@@ -383,8 +416,13 @@ def main():
     ap.add_argument("--batch-cap-at", type=int, default=0,
                     help="batch: row-block capacity of the batched A' (0: as A)")
     ap.add_argument("--no-fp64-batch", action="store_true", help="batch: skip the FP64 batch leg")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="batch: every LP on 4096-nonzero blocks instead of the autotuned layouts")
+    ap.add_argument("--no-untuned", action="store_true",
+                    help="batch: skip the comparison run on the untuned layout")
+    ap.add_argument("--tune-cache", default="", help="batch: layout cache file of the autotuner")
     ap.add_argument("--rates", default="", help="serving: comma-separated arrival rates")
-    ap.add_argument("--serve-duration", type=float, default=30.0, help="serving: arrival window s")
+    ap.add_argument("--serve-duration", type=float, default=60.0, help="serving: arrival window s")
     ap.add_argument("--serve-drain", type=float, default=30.0, help="serving: drain cap s")
     ap.add_argument("--serve-max-lps", type=int, default=512, help="serving: LPs in flight")
     ap.add_argument("--compare-precision", type=float, default=0.0, metavar="EPS",

@@ -153,14 +153,15 @@ __global__ __launch_bounds__(DPDLP_BS) void k_bspmv(const dpdlp_blp_t *__restric
   const int *rp = T ? L.At_rp : L.A_rp;
   const int *ci = T ? L.At_ci : L.A_ci;
   const bool f32 = (KIND == K_PRIMAL || KIND == K_DUAL) && c.use32;
+  const int cap = T ? L.cap_At : L.cap_A;
   if (f32) {
     const float *v = T ? L.At_v32 : L.A_v32;
     const float *x = KIND == K_PRIMAL ? L.y_cur32 : L.x_ref32;
-    spmv_block<float, float, CAP>(rp, ci, v, x, bk.a, bk.b, epi, s, mx, prod, srp, ssum);
+    spmv_block<float, float, CAP>(rp, ci, v, x, bk.a, bk.b, epi, s, mx, prod, srp, ssum, cap);
   } else {
     const double *v = T ? L.At_v : L.A_v;
     spmv_block<double, double, CAP>(rp, ci, v, gathered<KIND>(L), bk.a, bk.b, epi, s, mx, prod,
-                                    srp, ssum);
+                                    srp, ssum, cap);
   }
   if (NS + NM > 0) {
     if (epi.partials) block_partials<NS, NM>(s, mx, red, part, stride, slot0);

@@ -124,12 +124,15 @@ template <class Epi> struct EpiPre<Epi, decltype((void)sizeof(typename Epi::Pre)
 /* One CSR-stream row block [r0, r1): stream, gather, stage in LDS, reduce each
  * row (rows_reduce) and call epi(row, sum, s, mx) once per row.  The caller
  * owns the LDS arrays (prod[CAP], srp[257], ssum[256]) and the per-thread
- * partial accumulators s[], mx[].  Every thread of the workgroup must call it. */
+ * partial accumulators s[], mx[].  A block of more than `cap` nonzeros (the
+ * capacity its matrix was planned with, at most CAP: then it is one long row)
+ * is summed by the whole workgroup without staging.  Every thread of the
+ * workgroup must call it. */
 template <typename VT, typename XT, int CAP, class Epi>
 __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int *__restrict__ ci,
                                            const VT *__restrict__ val, const XT *__restrict__ x,
                                            int r0, int r1, const Epi &epi, double *s, double *mx,
-                                           double *prod, int *srp, double *ssum) {
+                                           double *prod, int *srp, double *ssum, int cap = CAP) {
   constexpr int UNR = 8;
   using PF = EpiPre<Epi>;
   const int tid = threadIdx.x;
@@ -138,7 +141,7 @@ __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int
   if (tid < R) pre = PF::load(epi, r0 + tid);
   const int p0 = rp[r0], p1 = rp[r1];
   const int nz = p1 - p0;
-  if (nz > CAP) {
+  if (nz > cap || nz > CAP) {
     /* one long row, whole workgroup */
     double acc = row_sum_wg<VT, XT>(ci, val, x, p0, p1);
     acc = wave_sum(acc);

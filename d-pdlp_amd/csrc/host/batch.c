@@ -386,6 +386,8 @@ static void fill_blp(dpdlp_blp_t *L, const ctx_t *c) {
   L->At_v = st->At.v;
   L->A_v32 = st->A.v32;
   L->At_v32 = st->At.v32;
+  L->cap_A = st->A.cap;
+  L->cap_At = st->At.cap;
   L->lv = st->lv;
   L->uv = st->uv;
   L->c = st->c;
@@ -542,10 +544,14 @@ void dpdlp_i_batch_unreserve(dpdlp_batch_t *b, int i) {
  * joins at the next window start. */
 int dpdlp_i_batch_prepare_issue(dpdlp_batch_t *b, int i, dpdlp_resident_t *R,
                                 int take_ownership, long long tag, void *stream) {
-  if (R->cap != b->o.spmv_cap || R->At_cap != b->o.spmv_cap_At || R->A_P != 1 || R->At_P != 1 ||
+  /* any untiled layout whose row blocks fit the batch's kernels: capacities
+   * at most the batch's (its kernels stage that many products in LDS) */
+  if (R->cap > b->o.spmv_cap || R->At_cap > b->o.spmv_cap_At || R->A_P != 1 || R->At_P != 1 ||
       R->h_row_perm) {
-    dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_caps(prob, %d, %d, 1, 1)",
-                    b->o.spmv_cap, b->o.spmv_cap_At);
+    dpdlp_set_error("batch LPs need untiled A and A' in the caller's row order with row-block "
+                    "capacities at most %d and %d (dpdlp_upload_caps(prob, %d, %d, 1, 1) or "
+                    "dpdlp_upload_batch_layout)",
+                    b->o.spmv_cap, b->o.spmv_cap_At, b->o.spmv_cap, b->o.spmv_cap_At);
     dpdlp_i_batch_unreserve(b, i);
     return -1;
   }
@@ -1494,10 +1500,12 @@ int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parame
                          : local.spmv_cap;
   for (int i = 0; i < count; ++i) results[i] = NULL;
   for (int i = 0; i < count; ++i)
-    if (!res[i] || res[i]->cap != local.spmv_cap || res[i]->At_cap != cap_At || res[i]->A_P != 1 ||
+    if (!res[i] || res[i]->cap > local.spmv_cap || res[i]->At_cap > cap_At || res[i]->A_P != 1 ||
         res[i]->At_P != 1 || res[i]->h_row_perm) {
-      dpdlp_set_error("batch LPs must be uploaded with dpdlp_upload_caps(prob, %d, %d, 1, 1)",
-                      local.spmv_cap, cap_At);
+      dpdlp_set_error("batch LPs need untiled A and A' in the caller's row order with row-block "
+                      "capacities at most %d and %d (dpdlp_upload_caps(prob, %d, %d, 1, 1) or "
+                      "dpdlp_upload_batch_layout)",
+                      local.spmv_cap, cap_At, local.spmv_cap, cap_At);
       return -1;
     }
   dpdlp_batch_t *b = dpdlp_batch_create(params, &local);

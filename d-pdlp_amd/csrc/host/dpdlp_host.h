@@ -15,6 +15,7 @@ extern "C" {
 struct dpdlp_resident {
   int m, n, nnz, cap; /* cap: row-block capacity of A */
   int At_cap;         /* of A' */
+  int rows_A, rows_At; /* most rows per untiled row block of A, A' */
   int *A_rp, *A_ci, *At_rp, *At_ci, *A_blk, *At_blk;
   int A_nblk, At_nblk;
   double *A_v, *At_v;

@@ -10,6 +10,11 @@
 /* Consecutive rows per block: at most DPDLP_BLOCK_ROWS rows and cap nonzeros; a row
  * longer than cap gets a block of its own. */
 int dpdlp_plan_blocks(int rows, const int *rp, int cap, int *blk) {
+  return dpdlp_plan_blocks_rows(rows, rp, cap, DPDLP_BLOCK_ROWS, blk);
+}
+
+int dpdlp_plan_blocks_rows(int rows, const int *rp, int cap, int rmax, int *blk) {
+  if (rmax < 1 || rmax > DPDLP_BLOCK_ROWS) rmax = DPDLP_BLOCK_ROWS;
   int nb = 0, r = 0;
   while (r < rows) {
     int start = r;
@@ -18,7 +23,7 @@ int dpdlp_plan_blocks(int rows, const int *rp, int cap, int *blk) {
       r++;
     } else {
       long long nz = 0;
-      while (r < rows && r - start < DPDLP_BLOCK_ROWS) {
+      while (r < rows && r - start < rmax) {
         long long l = (long long)rp[r + 1] - rp[r];
         if (l > cap || nz + l > cap) break;
         nz += l;

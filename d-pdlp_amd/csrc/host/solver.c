@@ -172,7 +172,8 @@ void dpdlp_default_opts(dpdlp_solve_opts_t *o) {
   o->time_to_1e6_sec = -1.0;
 }
 
-static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int cap_At, int PA,
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int cap_At,
+                                     int rows_A, int rows_At, int PA,
                                      int PAt, int order, const dpdlp_uploader_t *u);
 
 /* frees the host side of r and hands back its device blob */
@@ -192,7 +193,7 @@ char *dpdlp_i_resident_release(dpdlp_resident_t *r, size_t *cap) {
 
 dpdlp_resident_t *dpdlp_i_upload_into(const lp_problem_t *p, int spmv_cap, int spmv_cap_At,
                                       int panels_A, int panels_At, const dpdlp_uploader_t *u) {
-  return upload_with(p, spmv_cap, spmv_cap_At, panels_A, panels_At,
+  return upload_with(p, spmv_cap, spmv_cap_At, 0, 0, panels_A, panels_At,
                      (panels_A > 0 || panels_At > 0) ? 0 : -1, u);
 }
 
@@ -279,7 +280,7 @@ static hipError_t sink_close(usink_t *k) {
 
 
 dpdlp_resident_t *dpdlp_upload(const lp_problem_t *p, int spmv_cap) {
-  return upload_with(p, spmv_cap, 0, 0, 0, -1, NULL);
+  return upload_with(p, spmv_cap, 0, 0, 0, 0, 0, -1, NULL);
 }
 
 dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panels_A,
@@ -290,7 +291,7 @@ dpdlp_resident_t *dpdlp_upload_ex(const lp_problem_t *p, int spmv_cap, int panel
     return NULL;
   }
   /* explicit layouts (the batch's plain one) keep the row order */
-  return upload_with(p, spmv_cap, 0, panels_A, panels_At,
+  return upload_with(p, spmv_cap, 0, 0, 0, panels_A, panels_At,
                      (panels_A > 0 || panels_At > 0) ? 0 : -1, NULL);
 }
 
@@ -305,8 +306,27 @@ dpdlp_resident_t *dpdlp_upload_caps(const lp_problem_t *p, int spmv_cap, int spm
     dpdlp_set_error("row-block capacities must be 0 (automatic), 1024, 2048 or 4096");
     return NULL;
   }
-  return upload_with(p, spmv_cap, spmv_cap_At, panels_A, panels_At,
+  return upload_with(p, spmv_cap, spmv_cap_At, 0, 0, panels_A, panels_At,
                      (panels_A > 0 || panels_At > 0) ? 0 : -1, NULL);
+}
+
+dpdlp_resident_t *dpdlp_upload_batch_layout(const lp_problem_t *p, const dpdlp_batch_layout_t *L) {
+  if (!L) {
+    dpdlp_set_error("no layout");
+    return NULL;
+  }
+  const int caps[2] = {L->cap_A, L->cap_At}, rows[2] = {L->rows_A, L->rows_At};
+  for (int k = 0; k < 2; ++k) {
+    if (caps[k] != 1024 && caps[k] != 2048 && caps[k] != 4096) {
+      dpdlp_set_error("row-block capacities must be 1024, 2048 or 4096");
+      return NULL;
+    }
+    if (rows[k] < 1 || rows[k] > DPDLP_BLOCK_ROWS) {
+      dpdlp_set_error("rows per block must be 1..%d", DPDLP_BLOCK_ROWS);
+      return NULL;
+    }
+  }
+  return upload_with(p, L->cap_A, L->cap_At, L->rows_A, L->rows_At, 1, 1, 0, NULL);
 }
 
 size_t dpdlp_resident_bytes(const dpdlp_resident_t *r) {
@@ -332,7 +352,7 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
   double bestA = 1e30, bestT = 1e30;
   int PA = 1, PT = 1;
   for (int i = 0; i < 4; ++i) {
-    dpdlp_resident_t *r = upload_with(p, spmv_cap, 0, cand[i], cand[i], -1, NULL);
+    dpdlp_resident_t *r = upload_with(p, spmv_cap, 0, 0, 0, cand[i], cand[i], -1, NULL);
     if (!r) continue;
     double ta, tt;
     time_both(r, &ta, &tt);
@@ -350,7 +370,7 @@ dpdlp_resident_t *dpdlp_upload_tuned(const lp_problem_t *p, int spmv_cap, int *c
     chosen[0] = PA;
     chosen[1] = PT;
   }
-  return upload_with(p, spmv_cap, 0, PA, PT, -1, NULL);
+  return upload_with(p, spmv_cap, 0, 0, 0, PA, PT, -1, NULL);
 }
 
 /* row lengths skewed enough for the row renumbering to pay: the longest row
@@ -448,7 +468,8 @@ static void free_perm_copies(dpdlp_resident_t *r) {
   r->h_perm_v = r->h_perm_lc = r->h_perm_uc = NULL;
 }
 
-static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int cap_At, int PA,
+static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int cap_At,
+                                     int rows_A, int rows_At, int PA,
                                      int PAt, int order, const dpdlp_uploader_t *u) {
   double t0 = dpdlp_wall();
   usink_t K;
@@ -607,6 +628,9 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   r->At_cap = r->cap;
   if (spmv_cap <= 0 && r->At_P == 1 && nnz >= 4000000) r->At_cap = 4096;
   if (cap_At > 0) r->At_cap = cap_At;
+  /* most rows per untiled row block (the lane-group width of the reduction) */
+  r->rows_A = rows_A > 0 && rows_A <= DPDLP_BLOCK_ROWS ? rows_A : DPDLP_BLOCK_ROWS;
+  r->rows_At = rows_At > 0 && rows_At <= DPDLP_BLOCK_ROWS ? rows_At : DPDLP_BLOCK_ROWS;
 
   /* A': an untiled A' is transposed on the device after the upload (the
    * host only counts columns for its row pointers); A' tiles are cut on the
@@ -632,7 +656,7 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
     const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
     int cap = which ? r->At_cap : r->cap;
     int nb = P > 1 ? dpdlp_plan_blocks_panels(rows, cols, hrp, hci, cap, P, DPDLP_PANEL_RMAX, blk)
-                   : dpdlp_plan_blocks(rows, hrp, cap, blk);
+                   : dpdlp_plan_blocks_rows(rows, hrp, cap, which ? r->rows_At : r->rows_A, blk);
     int *h = (int *)malloc((size_t)(nb + 1) * sizeof(int));
     if (!h) {
       dpdlp_set_error("host allocation failed");

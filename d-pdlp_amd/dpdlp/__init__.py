@@ -22,7 +22,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libcupdlpx-dist.so")
 CLI_PATH = os.path.join(PKG_ROOT, "lib", "cupdlpx-dist")
 
 from ._types import *  # noqa: F401,F403
-from ._types import (_dp, _ip, _Csr, _Dense, BatchOpts, CupdlpxResult, GenSpec, LpProblem,
+from ._types import (_dp, _ip, _Csr, _Dense, AutotuneReport, BatchLayout, BatchOpts,
+                     CupdlpxResult, GenSpec, LpProblem,
                      ServeReport, ServeSpec,
                      MatrixDesc, PdhgParameters, SolveOpts, TERMINATION, MATRIX_COO,
                      MATRIX_CSC, MATRIX_CSR, MATRIX_DENSE, PREC_FP64)
@@ -69,6 +70,11 @@ def _load() -> C.CDLL:
         "dpdlp_resident_transpose": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
         "dpdlp_upload_ex": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int]),
         "dpdlp_upload_caps": (C.c_void_p, [P, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "dpdlp_upload_batch_layout": (C.c_void_p, [P, C.POINTER(BatchLayout)]),
+        "dpdlp_batch_size_class": (C.c_int, [P]),
+        "dpdlp_batch_autotune": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(PdhgParameters),
+                                          C.c_int, C.c_char_p, C.POINTER(BatchLayout),
+                                          C.POINTER(AutotuneReport)]),
         "dpdlp_resident_bytes": (C.c_size_t, [C.c_void_p]),
         "dpdlp_batch_member_spec": (None, [C.c_int, C.c_uint64, C.c_int, C.c_int,
                                            C.POINTER(GenSpec)]),
@@ -320,9 +326,13 @@ def solve_plain(problem: Problem, params: PdhgParameters | None = None) -> Resul
 class Resident:
     """A problem uploaded to HBM once (dpdlp_upload) and solved many times."""
 
-    def __init__(self, problem: Problem, spmv_cap=0, tune=False, panels=None, spmv_cap_At=0):
+    def __init__(self, problem: Problem, spmv_cap=0, tune=False, panels=None, spmv_cap_At=0,
+                 layout=None):
         self.panels = None
-        if spmv_cap_At:
+        if layout is not None:
+            lay = layout if isinstance(layout, BatchLayout) else BatchLayout(*layout)
+            self.h = lib.dpdlp_upload_batch_layout(problem.ptr, C.byref(lay))
+        elif spmv_cap_At:
             pa, pt = panels if panels is not None else (0, 0)
             self.h = lib.dpdlp_upload_caps(problem.ptr, spmv_cap, spmv_cap_At, pa, pt)
             self.panels = None if panels is None else tuple(panels)
@@ -533,6 +543,22 @@ def serve(params=None, max_requests=0, want_results=False, **spec_kw):
     if want_results:
         results = [result_from_ptr(res[i]) if res[i] else None for i in range(n)]
     return report, lat[:n], idx[:n], results
+
+
+def batch_autotune(problems, params=None, precision=PREC_FP64, cache_path=None):
+    """dpdlp_batch_autotune: per size class, the row-block layout of A and A'
+    whose batched phase kernels run fastest; returns ([(cap_A, rows_A,
+    cap_At, rows_At) per problem], report dict)."""
+    n = len(problems)
+    arr = (C.POINTER(LpProblem) * max(n, 1))(*[p.ptr for p in problems])
+    out = (BatchLayout * max(n, 1))()
+    rep = AutotuneReport()
+    pp = params if params is not None else default_params()
+    path = cache_path.encode() if cache_path else None
+    if lib.dpdlp_batch_autotune(arr, n, C.byref(pp), precision, path, out, C.byref(rep)) != 0:
+        raise RuntimeError("autotune failed: " + last_error())
+    lays = [(o.cap_A, o.rows_A, o.cap_At, o.rows_At) for o in out[:n]]
+    return lays, {k: getattr(rep, k) for k, _ in AutotuneReport._fields_}
 
 
 def batch_member(k, base_seed=3000, m_lo=500, m_hi=200_000):

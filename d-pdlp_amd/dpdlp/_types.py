@@ -155,3 +155,15 @@ class ServeReport(C.Structure):
                 ("max_in_flight", C.c_int), ("peak_gb", C.c_double),
                 ("pool_gen_sec", C.c_double), ("windows", C.c_longlong),
                 ("syncs", C.c_longlong)]
+
+
+class BatchLayout(C.Structure):
+    """dpdlp_batch_layout_t"""
+    _fields_ = [("cap_A", C.c_int), ("rows_A", C.c_int), ("cap_At", C.c_int),
+                ("rows_At", C.c_int)]
+
+
+class AutotuneReport(C.Structure):
+    """dpdlp_autotune_report_t"""
+    _fields_ = [("classes", C.c_int), ("tuned", C.c_int), ("cached", C.c_int),
+                ("seconds", C.c_double)]

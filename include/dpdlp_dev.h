@@ -88,6 +88,8 @@ typedef struct {
   float *x_ref32, *y_cur32;
   double *pi_eig, *pi_next, *pi_dual;
   double *scal;
+  int cap_A, cap_At; /* the LP's row-block capacities: a single-row block
+                        longer than this takes the whole-workgroup sum */
 } dpdlp_blp_t;
 
 /* Per-LP control of one batched step, written by the host. */
@@ -157,6 +159,10 @@ enum {
  * of their own, other blocks hold <= 256 rows and <= cap nonzeros.  Writes
  * blk[0..nblk] and returns nblk. */
 int dpdlp_plan_blocks(int rows, const int *rp_host, int cap, int *blk_out);
+/* The same with at most rmax (1..DPDLP_BLOCK_ROWS) rows per block: fewer rows
+ * per block give each row a wider lane group in the row reduction
+ * (256 / rows, rounded down to a power of two, at most 64 lanes). */
+int dpdlp_plan_blocks_rows(int rows, const int *rp_host, int cap, int rmax, int *blk_out);
 /* rows per row block of dpdlp_plan_blocks: one per thread of the SpMV's
  * 256-thread workgroup */
 #define DPDLP_BLOCK_ROWS 256

@@ -216,6 +216,43 @@ void dpdlp_batch_free(dpdlp_batch_t *b);
 int dpdlp_solve_batch(dpdlp_resident_t *const *res, int count, const pdhg_parameters_t *params,
                       dpdlp_batch_opts_t *o, cupdlpx_result_t **results);
 
+/* ---- batch layout autotuner ---- */
+
+/* SpMV row-block layout of one batched LP (untiled A and A'): row-block
+ * capacity (1024, 2048 or 4096 nonzeros) and most rows per block (1..256;
+ * the row reduction gives each row of a block of R rows a group of 256 / R
+ * lanes, rounded down to a power of two, at most 64), for A and for A'. */
+typedef struct {
+  int cap_A, rows_A, cap_At, rows_At;
+} dpdlp_batch_layout_t;
+
+/* Uploads an LP for a batch with this layout.  A batch takes LPs whose
+ * capacities are at most its spmv_cap (A) and spmv_cap_At (A'). */
+dpdlp_resident_t *dpdlp_upload_batch_layout(const lp_problem_t *prob,
+                                            const dpdlp_batch_layout_t *layout);
+
+/* The autotuner's size class of an LP: its nonzeros in factors of 4 and the
+ * mean row lengths of A and A' in factors of 2, as one integer. */
+int dpdlp_batch_size_class(const lp_problem_t *prob);
+
+typedef struct {
+  int classes;    /* size classes among the LPs */
+  int tuned;      /* classes timed by this call */
+  int cached;     /* classes read from the cache */
+  double seconds; /* time the call took */
+} dpdlp_autotune_report_t;
+
+/* Picks a layout for every LP: per size class, the capacity and rows per
+ * block (1024/2048/4096 x 64/128/256) for which the batch's own dual (A x)
+ * and primal (A' y) phase kernels run fastest on up to 16 members of the
+ * class solved as one batch for 200 iterations (params otherwise; precision
+ * as the batch will run).  Classes found in cache_path (a text file; NULL:
+ * no cache) for this device are not timed again; timed ones are appended.
+ * layouts[i] receives LP i's layout.  Returns 0 or -1. */
+int dpdlp_batch_autotune(const lp_problem_t *const *probs, int count,
+                         const pdhg_parameters_t *params, int precision, const char *cache_path,
+                         dpdlp_batch_layout_t *layouts, dpdlp_autotune_report_t *report);
+
 /* ---- serving: a Poisson request stream with continuous batching ---- */
 
 typedef struct {

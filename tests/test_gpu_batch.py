@@ -292,9 +292,9 @@ def test_separate_At_capacity_alone_equals_mix_equals_single(dp, caps):
             if prec == dp.PREC_FP64:
                 single, _ = r.solve(params)
                 assert same(mix[k], single), (caps, k)
-    # a batch refuses LPs whose A' layout differs from its own
+    # a batch refuses LPs whose row blocks are larger than its kernels' capacity
     with pytest.raises(RuntimeError):
-        dp.solve_batch(res[:1], params, spmv_cap=cap, spmv_cap_At=cap)
+        dp.solve_batch(res[:1], params, spmv_cap=cap // 2, spmv_cap_At=cap_at)
 
 
 def test_graph_option_on_renumbered_upload(dp, monkeypatch):
@@ -309,3 +309,38 @@ def test_graph_option_on_renumbered_upload(dp, monkeypatch):
     b, _ = r.solve(params)
     assert a.status == "OPTIMAL"
     assert same(a, b)
+
+
+def test_autotuned_layouts_mix_equals_alone(dp, orc, tmp_path):
+    """The batch autotuner picks a row-block layout per size class (timing the
+    batched phase kernels on a sample of each class), caches it by device and
+    class, and a batch of LPs with different per-class layouts gives each LP
+    the bits of that LP solved alone with its layout, and the oracle's
+    objective."""
+    probs = [dp.batch_member(k, 3200, 300, 6000)[0] for k in range(10)]
+    params = dp.default_params(presolve=False)
+    cache = str(tmp_path / "layouts.txt")
+    lays, rep = dp.batch_autotune(probs, params, precision=dp.PREC_FP64, cache_path=cache)
+    print("autotune", rep, sorted(set(lays)), flush=True)
+    assert rep["classes"] >= 2 and rep["tuned"] == rep["classes"] and rep["cached"] == 0
+    for L in lays:
+        assert L[0] in (1024, 2048, 4096) and L[2] in (1024, 2048, 4096)
+        assert L[1] in (64, 128, 256) and L[3] in (64, 128, 256)
+    lays2, rep2 = dp.batch_autotune(probs, params, precision=dp.PREC_FP64, cache_path=cache)
+    assert lays2 == lays and rep2["cached"] == rep["classes"] and rep2["tuned"] == 0
+    # plus an LP whose dense column (an A' row of 1500 entries) is longer than
+    # its 1024-nonzero blocks but fits the batch's 4096: it is summed as a long
+    # row alone and in the mix
+    probs.append(boxed_lp_with_dense_column(dp, 1500, 2600, seed=9))
+    lays.append((1024, 256, 1024, 64))
+    res = [dp.Resident(p, layout=L) for p, L in zip(probs, lays)]
+    mix = dp.solve_batch(res, params, spmv_cap=4096, spmv_cap_At=4096)
+    for k, r in enumerate(res):
+        alone = dp.solve_batch([r], params, spmv_cap=4096, spmv_cap_At=4096)[0]
+        assert mix[k].status == alone.status == "OPTIMAL"
+        assert np.array_equal(mix[k].x, alone.x) and np.array_equal(mix[k].y, alone.y), k
+        ref, _ = orc.solve(probs[k].ptr, orc.default_params(presolve=False))
+        obj = ref["primal_objective_value"]
+        assert abs(mix[k].primal_obj - obj) <= 1e-6 * max(1.0, abs(obj)), k
+    for r in res:
+        r.free()

@@ -458,3 +458,29 @@ def test_start_vector_restarts_from_the_seed_every_call(dp):
     assert np.array_equal(c.view(np.int64), a[:1234].view(np.int64))
     src = open(os.path.join(ROOT, "d-pdlp_amd", "csrc", "host", "solver.c")).read()
     assert "dpdlp_normal_fill(1u, m, h_q)" in src
+
+
+def test_batch_size_classes_and_layout_cache(dp, tmp_path):
+    """The autotuner's size classes (nonzeros in factors of 4, mean row
+    lengths of A and A' in factors of 2) and its cache: when every class of
+    the LPs is in the cache for this device and precision, the layouts come
+    from the cache and nothing is timed (so this runs without a GPU, whose
+    device signature is then "unknown")."""
+    A = np.array([[1.0, 2.0, 0.0], [0.0, 1.0, 1.0]])
+    p_small = dp.Problem.from_dense(A, c=np.ones(3))            # nnz 4, 2 per row, 4/3 per col
+    cls_small = dp.lib.dpdlp_batch_size_class(p_small.ptr)
+    assert cls_small == 1 * 256 + 1 * 16 + 0
+    p_big, _ = dp.batch_member(3)                                # m 3607, n 10760, nnz 32342
+    cls_big = dp.lib.dpdlp_batch_size_class(p_big.ptr)
+    assert cls_big == 7 * 256 + 3 * 16 + 1
+    cache = tmp_path / "layouts.txt"
+    cache.write_text(
+        f"dpdlp-layout-v1 unknown:fp64 {cls_small} 1024 64 2048 128 1.0 2.0\n"
+        f"dpdlp-layout-v1 unknown:mixed {cls_big} 1024 64 1024 64 1.0 2.0\n"
+        f"dpdlp-layout-v1 other:fp64 {cls_big} 1024 64 1024 64 1.0 2.0\n"
+        f"dpdlp-layout-v1 unknown:fp64 {cls_big} 2048 256 4096 128 1.0 2.0\n")
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU is present: its device signature is not 'unknown'")
+    lays, rep = dp.batch_autotune([p_small, p_big, p_small], cache_path=str(cache))
+    assert rep["classes"] == 2 and rep["cached"] == 2 and rep["tuned"] == 0
+    assert lays == [(1024, 64, 2048, 128), (2048, 256, 4096, 128), (1024, 64, 2048, 128)]
