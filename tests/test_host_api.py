@@ -484,3 +484,9 @@ def test_batch_size_classes_and_layout_cache(dp, tmp_path):
     lays, rep = dp.batch_autotune([p_small, p_big, p_small], cache_path=str(cache))
     assert rep["classes"] == 2 and rep["cached"] == 2 and rep["tuned"] == 0
     assert lays == [(1024, 64, 2048, 128), (2048, 256, 4096, 128), (1024, 64, 2048, 128)]
+    # no LPs: nothing to tune; bad arguments: an error, no crash
+    lays, rep = dp.batch_autotune([], cache_path=str(cache))
+    assert lays == [] and rep["classes"] == 0
+    assert dp.lib.dpdlp_batch_autotune(None, 0, None, 0, None, None, None) == -1
+    assert "bad arguments" in dp.last_error()
+    assert dp.lib.dpdlp_batch_size_class(None) == -1
