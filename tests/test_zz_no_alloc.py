@@ -92,3 +92,75 @@ def test_noalloc_spmv_rejects_matrices_beyond_its_scratch(dp):
     rc = dp.lib.dpdlp_dev_spmv_noalloc(C.byref(csr), None, None, None)
     assert rc != 0
     assert "exceed the fixed scratch" in dp.last_error()
+
+
+def test_noalloc_spmv_replays_from_a_graph(dp):
+    """The no-allocation SpMV captured into a hipGraph (its ticket clearing is
+    a fill kernel, so the graph holds only kernel nodes) and replayed five
+    times, with an eager SpMV of another matrix on the same scratch and a
+    synchronisation between replays: every replay gives the eager result."""
+    _setup(dp)
+    hip = _hip()
+    vp = C.c_void_p
+    hip.hipStreamCreate.argtypes = [C.POINTER(vp)]
+    hip.hipStreamDestroy.argtypes = [vp]
+    hip.hipStreamSynchronize.argtypes = [vp]
+    hip.hipStreamBeginCapture.argtypes = [vp, C.c_int]
+    hip.hipStreamEndCapture.argtypes = [vp, C.POINTER(vp)]
+    hip.hipGraphInstantiate.argtypes = [C.POINTER(vp), vp, vp, vp, C.c_size_t]
+    hip.hipGraphLaunch.argtypes = [vp, vp]
+    hip.hipGraphExecDestroy.argtypes = [vp]
+    hip.hipGraphDestroy.argtypes = [vp]
+    hip.hipMemsetAsync.argtypes = [vp, C.c_int, C.c_size_t, vp]
+    rng = np.random.default_rng(43)
+    mats = []
+    for m, n, panels in ((20000, 30000, (4, 1)), (4000, 6000, (8, 1))):
+        lens = np.minimum((rng.pareto(1.3, m) * 4).astype(int), n - 1)
+        rp, ci, v = random_csr(m, n, lens, seed=m + 1)
+        R = dp.Resident(dp.Problem.from_csr(m, n, rp, ci, v, c=np.zeros(n)), spmv_cap=2048,
+                        panels=panels)
+        csr = DevCsr()
+        assert dp.lib.dpdlp_resident_device_csr(R.h, 0, C.byref(csr)) == 0
+        mats.append((R, csr))
+    (R1, A1), (R2, A2) = mats
+    x1 = rng.standard_normal(A1.cols)
+    x2 = rng.standard_normal(A2.cols)
+    want1 = _noalloc_spmv(dp, hip, A1, x1)
+    want2 = _noalloc_spmv(dp, hip, A2, x2)
+    assert np.array_equal(want1, R1.spmv(x1))
+    s = vp()
+    assert hip.hipStreamCreate(C.byref(s)) == 0
+    bufs = {}
+    for name, size in (("x", x1.nbytes), ("y", 8 * A1.rows)):
+        p = vp()
+        assert hip.hipMalloc(C.byref(p), size) == 0
+        bufs[name] = p
+    graph, exe = vp(), vp()
+    try:
+        assert hip.hipMemcpy(bufs["x"], x1.ctypes.data, x1.nbytes, H2D) == 0
+        assert hip.hipStreamBeginCapture(s, 2) == 0          # relaxed capture
+        rc = dp.lib.dpdlp_dev_spmv_noalloc(C.byref(A1), C.cast(bufs["x"], _dp),
+                                           C.cast(bufs["y"], _dp), s)
+        assert hip.hipStreamEndCapture(s, C.byref(graph)) == 0
+        assert rc == 0, dp.last_error()
+        assert hip.hipGraphInstantiate(C.byref(exe), graph, None, None, 0) == 0
+        for k in range(5):
+            assert hip.hipMemsetAsync(bufs["y"], 0, 8 * A1.rows, s) == 0   # outside the graph
+            assert hip.hipGraphLaunch(exe, s) == 0
+            assert hip.hipStreamSynchronize(s) == 0
+            got = np.empty(A1.rows)
+            assert hip.hipMemcpy(got.ctypes.data, bufs["y"], got.nbytes, D2H) == 0
+            assert np.array_equal(got, want1), k
+            # eager work between replays: another tiled SpMV on the same scratch
+            assert np.array_equal(_noalloc_spmv(dp, hip, A2, x2), want2), k
+    finally:
+        if exe:
+            hip.hipGraphExecDestroy(exe)
+        if graph:
+            hip.hipGraphDestroy(graph)
+        hip.hipStreamSynchronize(s)
+        for p in bufs.values():
+            hip.hipFree(p)
+        hip.hipStreamDestroy(s)
+        R1.free()
+        R2.free()
