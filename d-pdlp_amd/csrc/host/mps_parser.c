@@ -23,28 +23,35 @@
  * dereferences NULL); lines longer than 4095 bytes are read whole instead of
  * ending the file.
  *
- * The input is read whole into memory.  The body of a COLUMNS section (the
- * bulk of a file) is parsed by host worker threads: it is cut at line starts
- * into chunks; each chunk's lines are split into tokens, their numbers
- * converted and their row names looked up (the row map is complete and read
- * only by then) into runs of entries, one run per column name change; one
- * thread then walks the chunks in file order, numbering the columns and
- * appending the entries, so the result is the one a line-by-line read gives.
+ * A plain input is read whole into memory; a gzipped one is inflated by a
+ * producer thread while the parser consumes it (see text_open).  The body of
+ * a COLUMNS section (the bulk of a file) is parsed by host worker threads:
+ * it is cut at line starts into chunks; each chunk's lines are split into
+ * tokens, their numbers converted and their row names looked up (the row map
+ * is complete and read only by then) into runs of entries, one run per
+ * column name change; one thread then walks the chunks in file order,
+ * numbering the columns and appending the entries, so the result is the one
+ * a line-by-line read gives.
  * DPDLP_HOST_THREADS sets the thread count, DPDLP_MPS_CHUNK the smallest
- * chunk in bytes (default 1 MiB).
+ * chunk in bytes (default 1 MiB), DPDLP_MPS_WINDOW the window of a streamed
+ * input (default 16 MiB); DPDLP_MPS_STREAM=0 reads a gzipped input whole
+ * first.
  */
-#define _POSIX_C_SOURCE 200809L
+#define _GNU_SOURCE
 #include "mps_parser.h"
 #include "cupdlpx.h"
 #include "dpdlp_host.h"
 
 #include <ctype.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdbool.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
 #include <zlib.h>
 
 enum { S_NONE, S_ROWS, S_COLUMNS, S_RHS, S_RANGES, S_BOUNDS, S_OBJSENSE, S_END };
@@ -224,7 +231,126 @@ static double mps_atof(const char *s) {
   return neg ? -v : v;
 }
 
-/* ---- the whole input, NUL-terminated: plain stdio or zlib ---- */
+/* ---- the input text.  A plain file is read whole before parsing.  A ".gz"
+ * file is inflated by a producer thread into an address range reserved up
+ * front (deflate expands at most 1032:1, so the compressed size bounds it),
+ * publishing each piece under a mutex, while the parser consumes whole lines
+ * behind it: the serial inflate then overlaps the threaded parsing instead
+ * of preceding it.  The reader never looks past the published length, and
+ * the producer never writes below it. ---- */
+typedef struct {
+  gzFile gz;
+  char *buf;
+  size_t cap, len; /* len: published bytes (under mu) */
+  int eof, stop;   /* eof: all published, buf[len] = NUL; stop: reader gave up */
+  pthread_mutex_t mu;
+  pthread_cond_t cv;
+  pthread_t th;
+} gzsrc_t;
+
+#define GZ_PIECE (2u << 20)
+
+static void *gz_produce(void *arg) {
+  gzsrc_t *S = (gzsrc_t *)arg;
+  size_t len = 0;
+  for (;;) {
+    pthread_mutex_lock(&S->mu);
+    int stop = S->stop;
+    pthread_mutex_unlock(&S->mu);
+    size_t want = S->cap - 1 - len;
+    if (want > GZ_PIECE) want = GZ_PIECE;
+    int r = stop || want == 0 ? 0 : gzread(S->gz, S->buf + len, (unsigned)want);
+    size_t got = r > 0 ? (size_t)r : 0;
+    len += got;
+    if (got == 0) S->buf[len] = '\0';
+    pthread_mutex_lock(&S->mu);
+    S->len = len;
+    if (got == 0) S->eof = 1;
+    pthread_cond_broadcast(&S->cv);
+    pthread_mutex_unlock(&S->mu);
+    if (got == 0) return NULL;
+  }
+}
+
+typedef struct {
+  char *buf;
+  size_t len, pos; /* readable text is buf[0, len); unread from pos */
+  int eof;         /* len is final */
+  gzsrc_t *src;    /* streamed input, or NULL: the text is whole */
+  size_t map_len;  /* the reservation of a streamed input */
+} text_t;
+
+/* waits until at least want bytes are readable or the input has ended */
+static void text_wait(text_t *t, size_t want) {
+  gzsrc_t *S = t->src;
+  if (!S || t->eof) return;
+  pthread_mutex_lock(&S->mu);
+  while (S->len < want && !S->eof) pthread_cond_wait(&S->cv, &S->mu);
+  t->len = S->len;
+  t->eof = S->eof;
+  pthread_mutex_unlock(&S->mu);
+}
+
+static void text_close(text_t *t) {
+  if (t->src) {
+    gzsrc_t *S = t->src;
+    pthread_mutex_lock(&S->mu);
+    S->stop = 1;
+    pthread_mutex_unlock(&S->mu);
+    pthread_join(S->th, NULL);
+    gzclose(S->gz);
+    pthread_mutex_destroy(&S->mu);
+    pthread_cond_destroy(&S->cv);
+    munmap(t->buf, t->map_len);
+    free(S);
+  } else {
+    free(t->buf);
+  }
+  memset(t, 0, sizeof(*t));
+}
+
+/* a ".gz" input streamed by a producer thread; -1 if the reservation or the
+ * thread cannot be had (the caller then reads the file whole) */
+static int gz_stream_open(const char *filename, text_t *t) {
+  struct stat st;
+  if (stat(filename, &st) != 0 || st.st_size < 0) return -1;
+  size_t cap = (size_t)st.st_size * 1040u + (16u << 20);
+  gzsrc_t *S = (gzsrc_t *)calloc(1, sizeof(*S));
+  if (!S) return -1;
+  void *b = mmap(NULL, cap, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE,
+                 -1, 0);
+  if (b == MAP_FAILED) {
+    free(S);
+    return -1;
+  }
+  S->gz = gzopen(filename, "rb");
+  if (!S->gz) {
+    munmap(b, cap);
+    free(S);
+    return -2;
+  }
+  gzbuffer(S->gz, 1u << 20);
+  S->buf = (char *)b;
+  S->cap = cap;
+  pthread_mutex_init(&S->mu, NULL);
+  pthread_cond_init(&S->cv, NULL);
+  if (pthread_create(&S->th, NULL, gz_produce, S) != 0) {
+    gzclose(S->gz);
+    pthread_mutex_destroy(&S->mu);
+    pthread_cond_destroy(&S->cv);
+    munmap(b, cap);
+    free(S);
+    return -1;
+  }
+  t->buf = S->buf;
+  t->len = t->pos = 0;
+  t->eof = 0;
+  t->src = S;
+  t->map_len = cap;
+  return 0;
+}
+
+/* the whole input, NUL-terminated: plain stdio or zlib */
 static char *load_text(const char *filename, size_t *len_out) {
   size_t fl = strlen(filename), len = 0, cap = 4u << 20;
   gzFile gz = NULL;
@@ -279,23 +405,43 @@ static char *load_text(const char *filename, size_t *len_out) {
   return buf;
 }
 
-typedef struct {
-  char *buf;
-  size_t len, pos; /* unread text is buf[pos, len) */
-} text_t;
+/* opens the input: 0, or -1 if it cannot be opened, -2 if it cannot be read */
+static int text_open(const char *filename, text_t *t) {
+  memset(t, 0, sizeof(*t));
+  size_t fl = strlen(filename);
+  const char *env = getenv("DPDLP_MPS_STREAM");
+  int stream = !(env && env[0] == '0');
+  if (stream && fl > 3 && strcmp(filename + fl - 3, ".gz") == 0) {
+    int rc = gz_stream_open(filename, t);
+    if (rc == 0) return 0;
+    if (rc == -2) return -1;
+  }
+  t->buf = load_text(filename, &t->len);
+  if (!t->buf) return t->len == 0 ? -1 : -2;
+  t->eof = 1;
+  return 0;
+}
 
 /* next line, NUL-terminated in place without its newline; NULL at the end */
 static char *next_line(text_t *t) {
-  if (t->pos >= t->len) return NULL;
-  char *line = t->buf + t->pos;
-  char *nl = (char *)memchr(line, '\n', t->len - t->pos);
-  if (nl) {
-    *nl = '\0';
-    t->pos = (size_t)(nl - t->buf) + 1;
-  } else {
-    t->pos = t->len; /* the last line has no newline; buf[len] is NUL */
+  for (;;) {
+    if (t->pos < t->len) {
+      char *line = t->buf + t->pos;
+      char *nl = (char *)memchr(line, '\n', t->len - t->pos);
+      if (nl) {
+        *nl = '\0';
+        t->pos = (size_t)(nl - t->buf) + 1;
+        return line;
+      }
+      if (t->eof) {
+        t->pos = t->len; /* the last line has no newline; buf[len] is NUL */
+        return line;
+      }
+    } else if (t->eof) {
+      return NULL;
+    }
+    text_wait(t, t->len + 1);
   }
-  return line;
 }
 
 static inline int is_delim(char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r'; }
@@ -799,12 +945,11 @@ static void apply_bound(mps_t *P, const srec_t *q) {
   }
 }
 
-/* The body of the section whose header was just read (it runs from the
- * text's read position to the next header line, or the end): cut into
- * chunks, parsed by the worker threads, merged or applied in file order; the
- * read position is left at the header that ends it. */
-static int section_body(mps_t *P, text_t *T, int sec) {
-  char *b = T->buf + T->pos, *end = T->buf + T->len;
+/* One window [b, end) of whole lines of a section body: the part before
+ * the first header line in it is cut into chunks, parsed by the worker
+ * threads and merged or applied in file order.  *stop is set to that
+ * header (or end if there is none) and *found to whether there was one. */
+static int body_window(mps_t *P, char *b, char *end, int sec, char **stop, int *found) {
   const char *env = getenv("DPDLP_MPS_CHUNK");
   long long min_chunk = env ? atoll(env) : (1LL << 20);
   if (min_chunk < 1) min_chunk = 1;
@@ -817,9 +962,11 @@ static int section_body(mps_t *P, text_t *T, int sec) {
   cut_chunks(b, end, nch, ch);
   dpdlp_par_for(nch, chunk_find_header, &job);
   char *e = end;
+  *found = 0;
   for (int k = 0; k < nch; ++k)
     if (ch[k].hdr) {
       e = ch[k].hdr;
+      *found = 1;
       break;
     }
   want = (long long)(e - b) / min_chunk;
@@ -854,8 +1001,40 @@ static int section_body(mps_t *P, text_t *T, int sec) {
     free(ch[k].rec);
   }
   free(ch);
-  T->pos = (size_t)(e - T->buf);
+  *stop = e;
   return rc;
+}
+
+/* The body of the section whose header was just read (it runs from the
+ * text's read position to the next header line, or the end).  A whole text
+ * is one window; a streamed one is taken in windows of whole lines as they
+ * arrive (DPDLP_MPS_WINDOW bytes, default 16 MiB), each parsed while the
+ * producer inflates the next.  The read position is left at the header that
+ * ends the body. */
+static int section_body(mps_t *P, text_t *T, int sec) {
+  const char *env = getenv("DPDLP_MPS_WINDOW");
+  long long wv = env ? atoll(env) : (16LL << 20);
+  size_t win = wv < 1 ? 1 : (size_t)wv;
+  size_t want = T->pos + win;
+  for (;;) {
+    text_wait(T, want);
+    char *b = T->buf + T->pos, *end = T->buf + T->len;
+    if (!T->eof) {
+      char *ln = b < end ? (char *)memrchr(b, '\n', (size_t)(end - b)) : NULL;
+      if (!ln) { /* not one whole line yet */
+        want = T->len + win;
+        continue;
+      }
+      end = ln + 1;
+    }
+    if (b == end) return 0; /* the end of the text */
+    char *e;
+    int found;
+    if (body_window(P, b, end, sec, &e, &found)) return -1;
+    T->pos = (size_t)(e - T->buf);
+    if (found || (T->eof && T->pos >= T->len)) return 0;
+    want = T->pos + win;
+  }
 }
 
 static void mps_free(mps_t *P) {
@@ -986,10 +1165,10 @@ static int build_csr(const mps_t *P, int m, int *rp, int *ci, double *v) {
 
 lp_problem_t *read_mps_file(const char *filename) {
   if (!filename) return NULL;
-  text_t T = {NULL, 0, 0};
-  T.buf = load_text(filename, &T.len);
-  if (!T.buf) {
-    if (T.len == 0) fprintf(stderr, "ERROR: Could not open file %s\n", filename);
+  text_t T;
+  int orc = text_open(filename, &T);
+  if (orc) {
+    if (orc == -1) fprintf(stderr, "ERROR: Could not open file %s\n", filename);
     return NULL;
   }
   mps_t P;
@@ -1035,7 +1214,7 @@ lp_problem_t *read_mps_file(const char *filename) {
       break;
     }
   }
-  free(T.buf);
+  text_close(&T);
   if (P.err) {
     fprintf(stderr, "ERROR: Failed to parse MPS file.\n");
     mps_free(&P);

@@ -283,6 +283,49 @@ def test_chunked_reader_equals_line_by_line(dp, tmp_path, monkeypatch, seed):
         assert a.get("obj_const", 0.0) == ref["obj_const"]
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_streamed_gzip_reader_equals_line_by_line(dp, tmp_path, monkeypatch, seed):
+    """A gzipped file is inflated by a producer thread while the sections are
+    parsed in windows of whole lines behind it: every window size (one line
+    per window up to the whole file), thread count and chunk size, and the
+    whole-file path (DPDLP_MPS_STREAM=0), must give the line-by-line result."""
+    import gzip
+    text = random_mps(seed + 100, ncol=400, nrow=80)
+    path = tmp_path / f"r{seed}.mps.gz"
+    with gzip.open(path, "wb") as f:
+        f.write(text.encode())
+    ref = reference_read(text)
+    assert ref is not None and len(ref["vals"]) > 0
+    cases = ((1, 1 << 30, 1, "1"), (4, 1, 37, "1"), (3, 17, 1000, "1"), (8, 64, 1 << 24, "1"),
+             (2, 400, 5000, "0"))
+    for threads, chunk, window, stream in cases:
+        monkeypatch.setenv("DPDLP_HOST_THREADS", str(threads))
+        monkeypatch.setenv("DPDLP_MPS_CHUNK", str(chunk))
+        monkeypatch.setenv("DPDLP_MPS_WINDOW", str(window))
+        monkeypatch.setenv("DPDLP_MPS_STREAM", stream)
+        p = dp.Problem.from_mps(str(path))
+        a = p.arrays()
+        assert (p.m, p.n) == (ref["m"], ref["n"]), (threads, chunk, window)
+        _same(a, ref)
+        assert a.get("obj_const", 0.0) == ref["obj_const"]
+
+
+def test_streamed_gzip_reader_errors(dp, tmp_path, monkeypatch):
+    """Errors of a streamed input: a value pair before any column name fails
+    in a late window (the producer is stopped and joined), and a missing
+    .gz file is refused."""
+    import gzip
+    body = "".join(f" r1 {i}.0\n" for i in range(5000))
+    path = tmp_path / "bad.mps.gz"
+    with gzip.open(path, "wt") as f:
+        f.write("NAME X\nROWS\n N obj\n L r1\nCOLUMNS\n" + body + "RHS\nENDATA\n")
+    monkeypatch.setenv("DPDLP_MPS_WINDOW", "512")
+    with pytest.raises(ValueError):
+        dp.Problem.from_mps(str(path))
+    with pytest.raises(ValueError):
+        dp.Problem.from_mps(str(tmp_path / "missing.mps.gz"))
+
+
 def test_chunked_reader_rejects_pair_before_any_column(dp, tmp_path, monkeypatch):
     """A continuation line before any column name fails (mps_parser.c:616-621),
     also when it is the first line of a later chunk."""

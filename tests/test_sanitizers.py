@@ -101,10 +101,17 @@ def test_mps_reader_threads_under_thread_sanitizer(tmp_path):
     files = []
     for seed in range(4):
         f = tmp_path / f"rnd{seed}.mps"
-        f.write_text(tmp_mod.random_mps(seed, ncol=2000, nrow=300))
+        text = tmp_mod.random_mps(seed, ncol=2000, nrow=300)
+        f.write_text(text)
         files.append(str(f))
+        # the gzipped copy is inflated by a producer thread while the parser
+        # consumes 2 KiB windows behind it
+        g = tmp_path / f"rnd{seed}.mps.gz"
+        with gzip.open(g, "wt") as fh:
+            fh.write(text)
+        files.append(str(g))
     env = dict(os.environ, DPDLP_HOST_THREADS="6", DPDLP_MPS_CHUNK="256",
-               TSAN_OPTIONS="halt_on_error=1")
+               DPDLP_MPS_WINDOW="2048", TSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([exe, "--mps-only"] + files, capture_output=True, text=True, env=env,
                        timeout=300)
     report = r.stdout + r.stderr
