@@ -180,6 +180,65 @@ __global__ void k_tr_finish(const int *__restrict__ rp, int m, const double *__r
 
 inline long long ntiles_of(long long nnz) { return (nnz + RS_TILE - 1) / RS_TILE; }
 
+/* Column-panel tiles of a device CSR matrix under a given row-block plan,
+ * the layout dpdlp_build_panels makes on the host: one workgroup per row
+ * block.  Block b (rows r0..r1, R = r1 - r0) fills its own nonzero range
+ * [rp[r0], rp[r1]) tile by tile (panel k = column / W); within a tile the
+ * entries keep row order and, within a row, CSR order.  The threads count
+ * every row's entries per panel into LDS, one thread per panel turns the
+ * counts into exclusive offsets down the rows (the rofs entries: R + 1 per
+ * tile), and each row's thread then writes its entries to their slots. */
+constexpr int BP_RMAX = DPDLP_PANEL_RMAX;
+
+__global__ __launch_bounds__(DPDLP_BS) void k_build_panels(
+    const int *__restrict__ rp, const int *__restrict__ ci, const int *__restrict__ blk, int nblk,
+    int P, long long W, int *__restrict__ toff, unsigned short *__restrict__ rofs,
+    int *__restrict__ perm, int *__restrict__ pci) {
+  __shared__ int cnt[BP_RMAX * 8];
+  __shared__ int tot[8];
+  const int b = blockIdx.x;
+  const int r0 = blk[b], r1 = blk[b + 1], R = r1 - r0;
+  if (R > BP_RMAX) return; /* the planner never makes such a block */
+  for (int i = threadIdx.x; i < R; i += blockDim.x) {
+    int c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int e = rp[r0 + i]; e < rp[r0 + i + 1]; ++e) c[(int)(ci[e] / W)]++;
+    for (int k = 0; k < P; ++k) cnt[i * P + k] = c[k];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < P) {
+    const int k = threadIdx.x;
+    int run = 0;
+    for (int i = 0; i < R; ++i) {
+      const int v = cnt[i * P + k];
+      cnt[i * P + k] = run;
+      run += v;
+    }
+    tot[k] = run;
+  }
+  __syncthreads();
+  int start[8];
+  start[0] = rp[r0];
+  for (int k = 1; k < P; ++k) start[k] = start[k - 1] + tot[k - 1];
+  if ((int)threadIdx.x < P) toff[(size_t)b * P + threadIdx.x] = start[threadIdx.x];
+  if (b == nblk - 1 && threadIdx.x == 0) toff[(size_t)nblk * P] = rp[r1];
+  unsigned short *ro = rofs + (size_t)P * ((size_t)r0 + b);
+  for (int q = threadIdx.x; q < P * (R + 1); q += blockDim.x) {
+    const int k = q / (R + 1), i = q - k * (R + 1);
+    ro[q] = (unsigned short)(i < R ? cnt[i * P + k] : tot[k]);
+  }
+  for (int i = threadIdx.x; i < R; i += blockDim.x) {
+    int fill[8];
+    for (int k = 0; k < P; ++k) fill[k] = start[k] + cnt[i * P + k];
+    for (int e = rp[r0 + i]; e < rp[r0 + i + 1]; ++e) {
+      const int c = ci[e];
+      const int k = (int)(c / W);
+      perm[fill[k]] = e;
+      pci[fill[k]] = c;
+      fill[k]++;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -217,6 +276,19 @@ int dpdlp_dev_transpose(int m, int n, long long nnz, const int *rp, const int *c
     vin = vout;
   }
   k_tr_finish<<<dpdlp_ew_grid(nnz), DPDLP_BS, 0, s>>>(rp, m, v, t_ci, t_v, nnz);
+  return (int)hipGetLastError();
+}
+
+int dpdlp_dev_build_panels(int rows, int cols, const int *rp, const int *ci, const int *blk,
+                           int nblk, int P, int *toff, unsigned short *rofs, int *perm, int *pci,
+                           void *stream) {
+  (void)rows;
+  if (P < 1 || P > 8 || nblk < 0) return (int)hipErrorInvalidValue;
+  hipStream_t s = (hipStream_t)stream;
+  if (nblk == 0) return (int)hipMemsetAsync(toff, 0, sizeof(int), s);
+  long long W = ((long long)cols + P - 1) / P;
+  if (W < 1) W = 1;
+  k_build_panels<<<(unsigned)nblk, DPDLP_BS, 0, s>>>(rp, ci, blk, nblk, P, W, toff, rofs, perm, pci);
   return (int)hipGetLastError();
 }
 

@@ -116,17 +116,10 @@ static void pc_task(void *arg, int t) {
   }
 }
 
-int dpdlp_plan_blocks_panels(int rows, int cols, const int *rp, const int *ci, int cap, int P,
-                             int rmax, int *blk) {
-  long long W = ((long long)cols + P - 1) / P;
-  if (W < 1) W = 1;
-  /* the rows' panel counts, counted by the worker threads (counted while
-   * planning if there is no memory for them) */
-  unsigned short *rc = rows > 0 ? (unsigned short *)malloc((size_t)rows * P * sizeof(unsigned short)) : NULL;
-  if (rc) {
-    pc_job_t J = {rp, ci, W, P, rows, 4 * dpdlp_host_threads(), rc};
-    dpdlp_par_for(J.ntask, pc_task, &J);
-  }
+/* greedy row blocks over the rows' panel counts rc[r * P + k] (or counted
+ * from rp/ci while planning when rc is NULL) */
+static int plan_from_counts(int rows, const int *rp, const int *ci, long long W,
+                            const unsigned short *rc, int cap, int P, int rmax, int *blk) {
   int tile[8], rowc[8];
   int nb = 0, r = 0;
   while (r < rows) {
@@ -153,6 +146,73 @@ int dpdlp_plan_blocks_panels(int rows, int cols, const int *rp, const int *ci, i
     blk[nb++] = start;
   }
   blk[nb] = rows;
+  return nb;
+}
+
+int dpdlp_plan_blocks_panels(int rows, int cols, const int *rp, const int *ci, int cap, int P,
+                             int rmax, int *blk) {
+  long long W = ((long long)cols + P - 1) / P;
+  if (W < 1) W = 1;
+  /* the rows' panel counts, counted by the worker threads (counted while
+   * planning if there is no memory for them) */
+  unsigned short *rc = rows > 0 ? (unsigned short *)malloc((size_t)rows * P * sizeof(unsigned short)) : NULL;
+  if (rc) {
+    pc_job_t J = {rp, ci, W, P, rows, 4 * dpdlp_host_threads(), rc};
+    dpdlp_par_for(J.ntask, pc_task, &J);
+  }
+  int nb = plan_from_counts(rows, rp, ci, W, rc, cap, P, rmax, blk);
+  free(rc);
+  return nb;
+}
+
+/* Panel counts of the transpose A' (n rows) read from A (m x n, CSR)
+ * without transposing: A' row j, panel k of A''s columns (A's rows
+ * [k W, (k + 1) W)) counts the entries of A's column j in those rows.  The
+ * worker threads add into 32-bit counters (relaxed atomics: the sums do not
+ * depend on the order), saturated to 16 bits afterwards as the planner
+ * wants. */
+typedef struct {
+  const int *rp, *ci;
+  int m, n, P, ntask;
+  long long W;
+  unsigned *c32;
+  unsigned short *rc;
+} tc_job_t;
+
+static void tc_count(void *arg, int t) {
+  tc_job_t *J = (tc_job_t *)arg;
+  int r0 = (int)((long long)J->m * t / J->ntask), r1 = (int)((long long)J->m * (t + 1) / J->ntask);
+  for (int i = r0; i < r1; ++i) {
+    int k = (int)(i / J->W);
+    for (int e = J->rp[i]; e < J->rp[i + 1]; ++e)
+      __atomic_fetch_add(&J->c32[(size_t)J->ci[e] * J->P + k], 1u, __ATOMIC_RELAXED);
+  }
+}
+
+static void tc_saturate(void *arg, int t) {
+  tc_job_t *J = (tc_job_t *)arg;
+  size_t len = (size_t)J->n * J->P;
+  size_t a = len * (size_t)t / (size_t)J->ntask, b = len * (size_t)(t + 1) / (size_t)J->ntask;
+  for (size_t q = a; q < b; ++q) J->rc[q] = (unsigned short)(J->c32[q] > 65535u ? 65535u : J->c32[q]);
+}
+
+int dpdlp_plan_blocks_panels_T(int m, int n, const int *rp, const int *ci, int cap, int P,
+                               int rmax, int *blk) {
+  if (P < 1 || P > 8) return -1;
+  long long W = ((long long)m + P - 1) / P;
+  if (W < 1) W = 1;
+  unsigned short *rc = (unsigned short *)malloc((size_t)(n > 0 ? n : 1) * P * sizeof(unsigned short));
+  unsigned *c32 = (unsigned *)calloc((size_t)(n > 0 ? n : 1) * P, sizeof(unsigned));
+  if (!rc || !c32) {
+    free(rc);
+    free(c32);
+    return -1;
+  }
+  tc_job_t J = {rp, ci, m, n, P, 4 * dpdlp_host_threads(), W, c32, rc};
+  dpdlp_par_for(J.ntask, tc_count, &J);
+  dpdlp_par_for(J.ntask, tc_saturate, &J);
+  free(c32);
+  int nb = plan_from_counts(n, NULL, NULL, W, rc, cap, P, rmax, blk);
   free(rc);
   return nb;
 }

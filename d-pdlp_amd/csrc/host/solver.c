@@ -503,8 +503,7 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
       return NULL;
     }
   dpdlp_resident_t *r = (dpdlp_resident_t *)calloc(1, sizeof(*r));
-  int *t_rp = NULL, *t_ci = NULL, *blk = NULL;
-  double *t_v = NULL;
+  int *t_rp = NULL, *blk = NULL;
   if (!r) return NULL;
   r->m = m;
   r->n = n;
@@ -632,31 +631,29 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   r->rows_A = rows_A > 0 && rows_A <= DPDLP_BLOCK_ROWS ? rows_A : DPDLP_BLOCK_ROWS;
   r->rows_At = rows_At > 0 && rows_At <= DPDLP_BLOCK_ROWS ? rows_At : DPDLP_BLOCK_ROWS;
 
-  /* A': an untiled A' is transposed on the device after the upload (the
-   * host only counts columns for its row pointers); A' tiles are cut on the
-   * host from a host transpose */
-  const int dev_tr = r->At_P == 1;
+  /* A' is transposed on the device after the upload (the host only counts
+   * columns for its row pointers); its column-panel tiles, when it has
+   * them, are planned from A's host CSR and cut on the device from the
+   * device transpose */
   t_rp = (int *)malloc((size_t)(n + 1) * sizeof(int));
   blk = (int *)malloc((size_t)((m > n ? m : n) + 2) * sizeof(int));
-  if (!dev_tr) {
-    t_ci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
-    t_v = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
-  }
-  if (!t_rp || !blk || (!dev_tr && (!t_ci || !t_v))) {
+  if (!t_rp || !blk) {
     dpdlp_set_error("host allocation failed");
     goto ufail;
   }
-  if (dev_tr)
-    dpdlp_host_col_ptr(m, n, rp_u, ci_u, t_rp);
-  else if (dpdlp_host_transpose(m, n, rp_u, ci_u, v_u, t_rp, t_ci, t_v))
-    goto ufail;
+  dpdlp_host_col_ptr(m, n, rp_u, ci_u, t_rp);
   /* row-block plans (host copies kept for the batch's work lists) */
   for (int which = 0; which < 2; ++which) {
-    int P = which ? r->At_P : r->A_P, rows = which ? n : m, cols = which ? m : n;
-    const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
+    int P = which ? r->At_P : r->A_P, rows = which ? n : m;
+    const int *hrp = which ? t_rp : rp_u;
     int cap = which ? r->At_cap : r->cap;
-    int nb = P > 1 ? dpdlp_plan_blocks_panels(rows, cols, hrp, hci, cap, P, DPDLP_PANEL_RMAX, blk)
-                   : dpdlp_plan_blocks_rows(rows, hrp, cap, which ? r->rows_At : r->rows_A, blk);
+    int nb = P <= 1 ? dpdlp_plan_blocks_rows(rows, hrp, cap, which ? r->rows_At : r->rows_A, blk)
+             : which ? dpdlp_plan_blocks_panels_T(m, n, rp_u, ci_u, cap, P, DPDLP_PANEL_RMAX, blk)
+                     : dpdlp_plan_blocks_panels(m, n, rp_u, ci_u, cap, P, DPDLP_PANEL_RMAX, blk);
+    if (nb < 0) {
+      dpdlp_set_error("host allocation failed (panel plan)");
+      goto ufail;
+    }
     int *h = (int *)malloc((size_t)(nb + 1) * sizeof(int));
     if (!h) {
       dpdlp_set_error("host allocation failed");
@@ -695,17 +692,26 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   UPCHK(up(&K, (void **)&r->A_ci, ci_u, (size_t)nnz * sizeof(int)));
   UPCHK(up(&K, (void **)&r->A_v, v_u, (size_t)nnz * sizeof(double)));
   UPCHK(up(&K, (void **)&r->At_rp, t_rp, (size_t)(n + 1) * sizeof(int)));
-  UPCHK(up(&K, (void **)&r->At_ci, t_ci, (size_t)nnz * sizeof(int)));
-  UPCHK(up(&K, (void **)&r->At_v, t_v, (size_t)nnz * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->At_ci, NULL, (size_t)nnz * sizeof(int)));
+  UPCHK(up(&K, (void **)&r->At_v, NULL, (size_t)nnz * sizeof(double)));
   UPCHK(up(&K, (void **)&r->A_blk, r->h_A_blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
   UPCHK(up(&K, (void **)&r->At_blk, r->h_At_blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
 
-  for (int which = 0; which < 2; ++which) {
-    int P = which ? r->At_P : r->A_P;
+  if (r->At_P > 1) {
+    /* A''s tiles: device arrays carved here, filled after the transpose */
+    size_t nb = (size_t)r->At_nblk, P = (size_t)r->At_P;
+    UPCHK(up(&K, (void **)&r->At_toff, NULL, (nb * P + 1) * sizeof(int)));
+    UPCHK(up(&K, (void **)&r->At_rofs, NULL, P * ((size_t)n + nb) * sizeof(unsigned short)));
+    UPCHK(up(&K, (void **)&r->At_perm, NULL, (size_t)nnz * sizeof(int)));
+    UPCHK(up(&K, (void **)&r->At_pci, NULL, (size_t)nnz * sizeof(int)));
+    UPCHK(up(&K, (void **)&r->At_pv, NULL, (size_t)nnz * sizeof(double)));
+  }
+  for (int which = 0; which < 1; ++which) {
+    int P = r->A_P;
     if (P <= 1) continue;
-    int rows = which ? n : m, cols = which ? m : n, nb = which ? r->At_nblk : r->A_nblk;
-    const int *hrp = which ? t_rp : rp_u, *hci = which ? t_ci : ci_u;
-    const int *blkh = which ? r->h_At_blk : r->h_A_blk;
+    int rows = m, cols = n, nb = r->A_nblk;
+    const int *hrp = rp_u, *hci = ci_u;
+    const int *blkh = r->h_A_blk;
     int *toff = (int *)malloc(((size_t)nb * P + 1) * sizeof(int));
     unsigned short *rofs =
         (unsigned short *)malloc((size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
@@ -747,10 +753,13 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   UPCHK(up(&K, (void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
   UPCHK(sink_close(&K));
   if (r->A_P > 1 && nnz > 0) UPCHK((hipError_t)dpdlp_dev_permute(r->A_v, r->A_perm, r->A_pv, nnz, K.s));
-  if (r->At_P > 1 && nnz > 0)
-    UPCHK((hipError_t)dpdlp_dev_permute(r->At_v, r->At_perm, r->At_pv, nnz, K.s));
-  if (!dev_tr || nnz == 0) UPCHK(hipStreamSynchronize(K.s));
-  if (dev_tr && nnz > 0) {
+  if (nnz == 0) {
+    if (r->At_P > 1)
+      UPCHK((hipError_t)dpdlp_dev_build_panels(n, m, r->At_rp, r->At_ci, r->At_blk, r->At_nblk,
+                                               r->At_P, r->At_toff, r->At_rofs, r->At_perm,
+                                               r->At_pci, K.s));
+    UPCHK(hipStreamSynchronize(K.s));
+  } else {
     size_t sb = dpdlp_dev_transpose_scratch_bytes(nnz), scap = sb;
     char *sc = NULL;
     if (u)
@@ -763,6 +772,12 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
     }
     hipError_t e = (hipError_t)dpdlp_dev_transpose(m, n, nnz, r->A_rp, r->A_ci, r->A_v,
                                                    r->At_ci, r->At_v, sc, K.s);
+    if (e == hipSuccess && r->At_P > 1)
+      e = (hipError_t)dpdlp_dev_build_panels(n, m, r->At_rp, r->At_ci, r->At_blk, r->At_nblk,
+                                             r->At_P, r->At_toff, r->At_rofs, r->At_perm,
+                                             r->At_pci, K.s);
+    if (e == hipSuccess && r->At_P > 1)
+      e = (hipError_t)dpdlp_dev_permute(r->At_v, r->At_perm, r->At_pv, nnz, K.s);
     if (e == hipSuccess) e = hipStreamSynchronize(K.s);
     if (u)
       u->put_blob(u->ctx, sc, scap);
@@ -771,16 +786,12 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
     UPCHK(e);
   }
   free(t_rp);
-  free(t_ci);
-  free(t_v);
   free(blk);
   free_perm_copies(r);
   r->upload_time = dpdlp_wall() - t0;
   return r;
 ufail:
   free(t_rp);
-  free(t_ci);
-  free(t_v);
   free(blk);
   dpdlp_resident_free(r);
   return NULL;

@@ -178,6 +178,19 @@ int dpdlp_build_panels(int rows, int cols, const int *rp, const int *ci, const i
 int dpdlp_plan_blocks_panels(int rows, int cols, const int *rp, const int *ci, int cap, int P,
                              int rmax, int *blk);
 #define DPDLP_PANEL_RMAX 1024
+/* The row blocks of the tiled transpose A' (n rows, columns = A's m rows)
+ * planned from A itself (m x n CSR on the host), without transposing: the
+ * same plan dpdlp_plan_blocks_panels gives on A''s host CSR.  Returns nblk,
+ * or -1 if P is not 1..8 or host memory runs out. */
+int dpdlp_plan_blocks_panels_T(int m, int n, const int *rp, const int *ci, int cap, int P,
+                               int rmax, int *blk);
+/* dpdlp_build_panels on the device: rp, ci (device CSR of the matrix being
+ * tiled, rows x cols) and blk (a plan of dpdlp_plan_blocks_panels, blocks
+ * of at most DPDLP_PANEL_RMAX rows) in device memory; the outputs likewise.
+ * Stream-ordered, no allocation. */
+int dpdlp_dev_build_panels(int rows, int cols, const int *rp, const int *ci, const int *blk,
+                           int nblk, int P, int *toff, unsigned short *rofs, int *perm, int *pci,
+                           void *stream);
 /* dst[t] = src[perm[t]] (tiled copies of values after scaling). */
 int dpdlp_dev_permute(const double *src, const int *perm, double *dst, long long count,
                       void *stream);

@@ -1,4 +1,8 @@
-"""Shared test helpers: small LPs with known answers and relative errors."""
+"""Shared test helpers: small LPs with known answers and relative errors, and
+the ctypes view of device CSR descriptors and the HIP runtime."""
+import ctypes as C
+import os
+
 import numpy as np
 
 
@@ -41,3 +45,36 @@ def csr_matvec(rp, ci, v, x):
     rows = np.repeat(np.arange(m), np.diff(rp))
     np.add.at(y, rows, v * x[ci])
     return y
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class DevCsr(C.Structure):
+    """dpdlp_csr_t of include/dpdlp_dev.h"""
+    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int), ("rp", _ip),
+                ("ci", _ip), ("v", _dp), ("v32", C.c_void_p), ("blk", _ip), ("nblk", C.c_int),
+                ("cap", C.c_int), ("P", C.c_int), ("toff", _ip), ("rofs", C.c_void_p),
+                ("pci", _ip), ("pv", _dp), ("pv32", C.c_void_p), ("xpart", _dp),
+                ("tick", C.c_void_p)]
+
+
+def hip_runtime():
+    h = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipDeviceSynchronize.argtypes = []
+    return h
+
+
+H2D, D2H = 1, 2
+
+
+def device_array(hip, ptr, count, dtype):
+    """A host copy of `count` elements at a device address."""
+    out = np.empty(count, dtype=dtype)
+    if count:
+        assert hip.hipMemcpy(out.ctypes.data, C.cast(ptr, C.c_void_p), out.nbytes, D2H) == 0
+    return out

@@ -1,10 +1,11 @@
 """GPU parity tests of the core path: SpMV, full solves vs the oracle."""
+import ctypes as C
 import os
 
 import numpy as np
 import pytest
 
-from helpers import capi_example, csr_matvec, random_csr, rel
+from helpers import DevCsr, capi_example, csr_matvec, device_array, hip_runtime, random_csr, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -245,6 +246,82 @@ def test_device_transpose_matches_stable_sort(dp, case):
     R.free()
 
 
+def _host_tiles(dp, rows, cols, rp, ci, v, P, cap):
+    """Block plan and column-panel tiles cut on the host (the planner and
+    cutter over a host CSR): blk, toff, rofs, tiled columns, tiled values."""
+    Pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    nnz = len(ci)
+    blk = np.zeros(rows + 2, np.int32)
+    nb = dp.lib.dpdlp_plan_blocks_panels(rows, cols, Pi(rp), Pi(ci), cap, P, 1024, Pi(blk))
+    toff = np.zeros(nb * P + 1, np.int32)
+    rofs = np.zeros(P * (rows + nb), np.uint16)
+    perm = np.zeros(max(1, nnz), np.int32)
+    pci = np.zeros(max(1, nnz), np.int32)
+    assert dp.lib.dpdlp_build_panels(rows, cols, Pi(rp), Pi(ci), Pi(blk), nb, P, Pi(toff),
+                                     rofs.ctypes.data_as(C.POINTER(C.c_ushort)), Pi(perm),
+                                     Pi(pci)) == 0
+    return blk[:nb + 1], toff, rofs, pci[:nnz], v[perm[:nnz]]
+
+
+@pytest.mark.parametrize("P", [2, 4, 8])
+@pytest.mark.parametrize("case", ["skewed", "dense_column", "sparse"])
+def test_device_cut_transpose_tiles_equal_host_cut(dp, case, P):
+    """A' column-panel tiles are planned from A's host CSR (no host
+    transpose) and cut on the device from the device transpose: the same
+    block plan, tile offsets, tile-local row offsets, tiled columns and tiled
+    values as the host planner and cutter give on the stable transpose --
+    with long rows of A split over panels, a dense column of A (an A' row
+    above the block capacity, a block of its own), and empty rows and
+    columns -- and the same products."""
+    rng = np.random.default_rng(71 + P)
+    if case == "skewed":
+        m, n = 3000, 2500
+        lens = np.minimum((rng.pareto(1.3, m) * 3).astype(int), n - 1)
+        rp, ci, v = random_csr(m, n, lens, seed=5, long_rows=((0, n - 1), (17, 1800)))
+    elif case == "dense_column":
+        m, n = 6000, 400
+        lens = rng.integers(1, 6, m)
+        rp, ci, v = random_csr(m, n, lens, seed=6)
+        rows = np.repeat(np.arange(m), np.diff(rp))
+        keep = ci != 7
+        rows, ci, v = rows[keep], ci[keep], v[keep]
+        rows = np.concatenate([rows, np.arange(m)])     # column 7 in every row
+        ci = np.concatenate([ci, np.full(m, 7, np.int32)])
+        v = np.concatenate([v, rng.standard_normal(m)])
+        o = np.lexsort((ci, rows))
+        rows, ci, v = rows[o], ci[o].astype(np.int32), v[o]
+        rp = np.zeros(m + 1, np.int32)
+        np.cumsum(np.bincount(rows, minlength=m), out=rp[1:])
+    else:
+        m, n = 2000, 3000
+        lens = rng.integers(0, 3, m) * (rng.random(m) < 0.5)
+        rp, ci, v = random_csr(m, n, lens, seed=7)
+    prob = dp.Problem.from_csr(m, n, rp, ci, v, c=np.zeros(n))
+    R = dp.Resident(prob, spmv_cap=1024, panels=(1, P))
+    assert R.panels == (1, P)
+    dp.lib.dpdlp_resident_device_csr.restype = C.c_int
+    dp.lib.dpdlp_resident_device_csr.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    csr = DevCsr()
+    assert dp.lib.dpdlp_resident_device_csr(R.h, 1, C.byref(csr)) == 0
+    assert (csr.rows, csr.cols, csr.P) == (n, m, P)
+    hip = hip_runtime()
+    t_rp, t_ci, t_v = _stable_transpose(rp, ci, v, n)
+    blk, toff, rofs, pci, pv = _host_tiles(dp, n, m, t_rp, t_ci, t_v, P, csr.cap)
+    nb = len(blk) - 1
+    assert csr.nblk == nb
+    assert np.array_equal(device_array(hip, csr.blk, nb + 1, np.int32), blk)
+    assert np.array_equal(device_array(hip, csr.toff, nb * P + 1, np.int32), toff)
+    assert np.array_equal(device_array(hip, csr.rofs, P * (n + nb), np.uint16), rofs)
+    assert np.array_equal(device_array(hip, csr.pci, len(ci), np.int32), pci)
+    assert np.array_equal(device_array(hip, csr.pv, len(ci), np.float64), pv)
+    yv = rng.standard_normal(m)
+    z = R.spmv(yv, transpose=True)
+    refz = np.zeros(n)
+    np.add.at(refz, ci, v * np.repeat(yv, np.diff(rp)))
+    assert np.max(np.abs(z - refz)) <= 1e-12 * max(1.0, np.max(np.abs(refz)))
+    R.free()
+
+
 def test_device_reductions(dp):
     """Standalone device reductions against exactly rounded host references:
     nrm2 (Blue's scaled sums) without overflow at 1e200 or underflow at
@@ -276,7 +353,6 @@ def test_device_reductions(dp):
 
 
 def _oracle_rescale(orc, prob, **kw):
-    import ctypes as C
     prm = orc.default_params(**kw)
     m, n, nnz = prob.m, prob.n, prob.nnz
     out = dict(val=np.zeros(nnz), c=np.zeros(n), lv=np.zeros(n), uv=np.zeros(n), lc=np.zeros(m),
@@ -338,7 +414,6 @@ def test_power_iteration_step_size(dp, orc, case):
     step 0.998 / sigma_max with sigma_max of the closed form (diagonal: max
     |a_ii|; rank one u v': |u| |v|) or of numpy's SVD, and the same step and
     iteration count as the oracle's power iteration from the same start."""
-    import ctypes as C
     rng = np.random.default_rng(12)
     if case == "diagonal":
         d = np.array([3.0, -7.0, 2.0, 0.5, 6.0])

@@ -490,3 +490,41 @@ def test_batch_size_classes_and_layout_cache(dp, tmp_path):
     assert dp.lib.dpdlp_batch_autotune(None, 0, None, 0, None, None, None) == -1
     assert "bad arguments" in dp.last_error()
     assert dp.lib.dpdlp_batch_size_class(None) == -1
+
+
+@pytest.mark.parametrize("P", [1, 2, 3, 4, 8])
+def test_transpose_panel_plan_from_a_equals_plan_on_the_transpose(dp, P):
+    """The row blocks of a tiled A' are planned from A's CSR without a host
+    transpose (panel counts of A''s rows added up from A's rows): the same
+    blocks as the planner over the stable transpose gives, for every cap
+    (long A' rows alone in their block) and row limit, with empty rows and
+    columns and a column dense enough (70000 entries) to saturate the
+    16-bit panel counts."""
+    rng = np.random.default_rng(P)
+    Pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+    cases = []
+    m, n = 4000, 3000
+    lens = np.minimum((rng.pareto(1.2, m) * 3).astype(int), n) * (rng.random(m) < 0.8)
+    rp = np.zeros(m + 1, np.int32)
+    np.cumsum(lens, out=rp[1:])
+    ci = np.concatenate([np.sort(rng.choice(n, L, replace=False)) for L in lens]).astype(np.int32)
+    cases.append((m, n, rp, ci))
+    m2, n2 = 70000, 50
+    ci2 = np.zeros(m2, np.int32)                       # every row: column 0 only
+    cases.append((m2, n2, np.arange(m2 + 1, dtype=np.int32), ci2))
+    for m, n, rp, ci in cases:
+        rows = np.repeat(np.arange(m, dtype=np.int32), np.diff(rp))
+        order = np.argsort(ci, kind="stable")
+        t_rp = np.zeros(n + 1, np.int32)
+        np.cumsum(np.bincount(ci, minlength=n), out=t_rp[1:])
+        t_ci = np.ascontiguousarray(rows[order])
+        for cap in (8, 64, 1024, 4096):
+            for rmax in (1024, 16):
+                a = np.zeros(n + 2, np.int32)
+                b = np.zeros(n + 2, np.int32)
+                na = dp.lib.dpdlp_plan_blocks_panels(n, m, Pi(t_rp), Pi(t_ci), cap, P, rmax, Pi(a))
+                nb = dp.lib.dpdlp_plan_blocks_panels_T(m, n, Pi(rp), Pi(ci), cap, P, rmax, Pi(b))
+                assert na == nb and np.array_equal(a[:na + 1], b[:nb + 1]), (m, cap, rmax)
+    assert dp.lib.dpdlp_plan_blocks_panels_T(2, 2, Pi(np.zeros(3, np.int32)),
+                                             Pi(np.zeros(1, np.int32)), 64, 9, 16,
+                                             Pi(np.zeros(4, np.int32))) == -1

@@ -4,12 +4,11 @@ device buffers, it gives the same bits as dpdlp_dev_spmv with the caller's
 scratch, on two calls in a row (the second smaller, with a different panel
 count, so it must clear the tickets the first left behind)."""
 import ctypes as C
-import os
 
 import numpy as np
 import pytest
 
-from helpers import random_csr
+from helpers import D2H, H2D, DevCsr, hip_runtime as _hip, random_csr
 
 pytestmark = pytest.mark.gpu
 
@@ -17,25 +16,6 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
 
-class DevCsr(C.Structure):
-    """dpdlp_csr_t of include/dpdlp_dev.h"""
-    _fields_ = [("rows", C.c_int), ("cols", C.c_int), ("nnz", C.c_int), ("rp", _ip),
-                ("ci", _ip), ("v", _dp), ("v32", C.c_void_p), ("blk", _ip), ("nblk", C.c_int),
-                ("cap", C.c_int), ("P", C.c_int), ("toff", _ip), ("rofs", C.c_void_p),
-                ("pci", _ip), ("pv", _dp), ("pv32", C.c_void_p), ("xpart", _dp),
-                ("tick", C.c_void_p)]
-
-
-def _hip():
-    h = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
-    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-    h.hipFree.argtypes = [C.c_void_p]
-    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    h.hipDeviceSynchronize.argtypes = []
-    return h
-
-
-H2D, D2H = 1, 2
 
 
 def _noalloc_spmv(dp, hip, csr, x):
