@@ -697,54 +697,18 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   UPCHK(up(&K, (void **)&r->A_blk, r->h_A_blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
   UPCHK(up(&K, (void **)&r->At_blk, r->h_At_blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
 
-  if (r->At_P > 1) {
-    /* A''s tiles: device arrays carved here, filled after the transpose */
-    size_t nb = (size_t)r->At_nblk, P = (size_t)r->At_P;
-    UPCHK(up(&K, (void **)&r->At_toff, NULL, (nb * P + 1) * sizeof(int)));
-    UPCHK(up(&K, (void **)&r->At_rofs, NULL, P * ((size_t)n + nb) * sizeof(unsigned short)));
-    UPCHK(up(&K, (void **)&r->At_perm, NULL, (size_t)nnz * sizeof(int)));
-    UPCHK(up(&K, (void **)&r->At_pci, NULL, (size_t)nnz * sizeof(int)));
-    UPCHK(up(&K, (void **)&r->At_pv, NULL, (size_t)nnz * sizeof(double)));
-  }
-  for (int which = 0; which < 1; ++which) {
-    int P = r->A_P;
+  /* column-panel tiles: device arrays carved here and cut on the device
+   * (k_build_panels) once A is there, A''s after the transpose */
+  for (int which = 0; which < 2; ++which) {
+    size_t P = (size_t)(which ? r->At_P : r->A_P), nb = (size_t)(which ? r->At_nblk : r->A_nblk);
+    size_t rows = (size_t)(which ? n : m);
     if (P <= 1) continue;
-    int rows = m, cols = n, nb = r->A_nblk;
-    const int *hrp = rp_u, *hci = ci_u;
-    const int *blkh = r->h_A_blk;
-    int *toff = (int *)malloc(((size_t)nb * P + 1) * sizeof(int));
-    unsigned short *rofs =
-        (unsigned short *)malloc((size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
-    int *perm = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
-    int *pci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
-    int ok = toff && rofs && perm && pci;
-    if (ok) ok = dpdlp_build_panels(rows, cols, hrp, hci, blkh, nb, P, toff, rofs, perm, pci) == 0;
-    if (ok) {
-      hipError_t e = hipSuccess;
-      int **dt = which ? &r->At_toff : &r->A_toff, **dp = which ? &r->At_perm : &r->A_perm;
-      int **dc = which ? &r->At_pci : &r->A_pci;
-      unsigned short **dr = which ? &r->At_rofs : &r->A_rofs;
-      double **dv = which ? &r->At_pv : &r->A_pv;
-      if (e == hipSuccess) e = up(&K, (void **)dt, toff, ((size_t)nb * P + 1) * sizeof(int));
-      if (e == hipSuccess)
-        e = up(&K, (void **)dr, rofs,
-               (size_t)P * ((size_t)rows + nb) * sizeof(unsigned short));
-      if (e == hipSuccess) e = up(&K, (void **)dp, perm, (size_t)nnz * sizeof(int));
-      if (e == hipSuccess) e = up(&K, (void **)dc, pci, (size_t)nnz * sizeof(int));
-      /* the tiled values, pv[t] = v[perm[t]], are gathered on the device
-       * once the CSR values have arrived */
-      if (e == hipSuccess) e = up(&K, (void **)dv, NULL, (size_t)nnz * sizeof(double));
-      ok = e == hipSuccess;
-      if (!ok) dpdlp_set_error("HIP error uploading panel tiles: %s", hipGetErrorString(e));
-    }
-    free(toff);
-    free(rofs);
-    free(perm);
-    free(pci);
-    if (!ok) {
-      dpdlp_set_error("building column-panel tiles failed");
-      goto ufail;
-    }
+    UPCHK(up(&K, (void **)(which ? &r->At_toff : &r->A_toff), NULL, (nb * P + 1) * sizeof(int)));
+    UPCHK(up(&K, (void **)(which ? &r->At_rofs : &r->A_rofs), NULL,
+             P * (rows + nb) * sizeof(unsigned short)));
+    UPCHK(up(&K, (void **)(which ? &r->At_perm : &r->A_perm), NULL, (size_t)nnz * sizeof(int)));
+    UPCHK(up(&K, (void **)(which ? &r->At_pci : &r->A_pci), NULL, (size_t)nnz * sizeof(int)));
+    UPCHK(up(&K, (void **)(which ? &r->At_pv : &r->A_pv), NULL, (size_t)nnz * sizeof(double)));
   }
   UPCHK(up(&K, (void **)&r->c, p->objective_vector, (size_t)n * sizeof(double)));
   UPCHK(up(&K, (void **)&r->lv, p->variable_lower_bound, (size_t)n * sizeof(double)));
@@ -752,7 +716,11 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   UPCHK(up(&K, (void **)&r->lc, lc_u, (size_t)m * sizeof(double)));
   UPCHK(up(&K, (void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
   UPCHK(sink_close(&K));
-  if (r->A_P > 1 && nnz > 0) UPCHK((hipError_t)dpdlp_dev_permute(r->A_v, r->A_perm, r->A_pv, nnz, K.s));
+  if (r->A_P > 1) {
+    UPCHK((hipError_t)dpdlp_dev_build_panels(m, n, r->A_rp, r->A_ci, r->A_blk, r->A_nblk, r->A_P,
+                                             r->A_toff, r->A_rofs, r->A_perm, r->A_pci, K.s));
+    if (nnz > 0) UPCHK((hipError_t)dpdlp_dev_permute(r->A_v, r->A_perm, r->A_pv, nnz, K.s));
+  }
   if (nnz == 0) {
     if (r->At_P > 1)
       UPCHK((hipError_t)dpdlp_dev_build_panels(n, m, r->At_rp, r->At_ci, r->At_blk, r->At_nblk,

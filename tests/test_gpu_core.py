@@ -265,14 +265,14 @@ def _host_tiles(dp, rows, cols, rp, ci, v, P, cap):
 
 @pytest.mark.parametrize("P", [2, 4, 8])
 @pytest.mark.parametrize("case", ["skewed", "dense_column", "sparse"])
-def test_device_cut_transpose_tiles_equal_host_cut(dp, case, P):
-    """A' column-panel tiles are planned from A's host CSR (no host
-    transpose) and cut on the device from the device transpose: the same
-    block plan, tile offsets, tile-local row offsets, tiled columns and tiled
-    values as the host planner and cutter give on the stable transpose --
-    with long rows of A split over panels, a dense column of A (an A' row
-    above the block capacity, a block of its own), and empty rows and
-    columns -- and the same products."""
+def test_device_cut_tiles_equal_host_cut(dp, case, P):
+    """Column-panel tiles are cut on the device: A's from the uploaded CSR,
+    A''s (planned from A's host CSR, with no host transpose) from the device
+    transpose.  Both have the same block plan, tile offsets, tile-local row
+    offsets, tiled columns and tiled values as the host planner and cutter
+    give on A and on the stable transpose -- with long rows of A split over
+    panels, a dense column of A (an A' row above the block capacity, a block
+    of its own), and empty rows and columns -- and the same products."""
     rng = np.random.default_rng(71 + P)
     if case == "skewed":
         m, n = 3000, 2500
@@ -297,23 +297,28 @@ def test_device_cut_transpose_tiles_equal_host_cut(dp, case, P):
         lens = rng.integers(0, 3, m) * (rng.random(m) < 0.5)
         rp, ci, v = random_csr(m, n, lens, seed=7)
     prob = dp.Problem.from_csr(m, n, rp, ci, v, c=np.zeros(n))
-    R = dp.Resident(prob, spmv_cap=1024, panels=(1, P))
-    assert R.panels == (1, P)
+    R = dp.Resident(prob, spmv_cap=1024, panels=(P, P))
+    assert R.panels == (P, P)
     dp.lib.dpdlp_resident_device_csr.restype = C.c_int
     dp.lib.dpdlp_resident_device_csr.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
-    csr = DevCsr()
-    assert dp.lib.dpdlp_resident_device_csr(R.h, 1, C.byref(csr)) == 0
-    assert (csr.rows, csr.cols, csr.P) == (n, m, P)
     hip = hip_runtime()
     t_rp, t_ci, t_v = _stable_transpose(rp, ci, v, n)
-    blk, toff, rofs, pci, pv = _host_tiles(dp, n, m, t_rp, t_ci, t_v, P, csr.cap)
-    nb = len(blk) - 1
-    assert csr.nblk == nb
-    assert np.array_equal(device_array(hip, csr.blk, nb + 1, np.int32), blk)
-    assert np.array_equal(device_array(hip, csr.toff, nb * P + 1, np.int32), toff)
-    assert np.array_equal(device_array(hip, csr.rofs, P * (n + nb), np.uint16), rofs)
-    assert np.array_equal(device_array(hip, csr.pci, len(ci), np.int32), pci)
-    assert np.array_equal(device_array(hip, csr.pv, len(ci), np.float64), pv)
+    for transpose, mat in ((0, (m, n, rp, ci, v)), (1, (n, m, t_rp, t_ci, t_v))):
+        rows, cols = mat[0], mat[1]
+        csr = DevCsr()
+        assert dp.lib.dpdlp_resident_device_csr(R.h, transpose, C.byref(csr)) == 0
+        assert (csr.rows, csr.cols, csr.P) == (rows, cols, P)
+        blk, toff, rofs, pci, pv = _host_tiles(dp, *mat, P, csr.cap)
+        nb = len(blk) - 1
+        assert csr.nblk == nb
+        assert np.array_equal(device_array(hip, csr.blk, nb + 1, np.int32), blk)
+        assert np.array_equal(device_array(hip, csr.toff, nb * P + 1, np.int32), toff)
+        assert np.array_equal(device_array(hip, csr.rofs, P * (rows + nb), np.uint16), rofs)
+        assert np.array_equal(device_array(hip, csr.pci, len(ci), np.int32), pci)
+        assert np.array_equal(device_array(hip, csr.pv, len(ci), np.float64), pv)
+    x = rng.standard_normal(n)
+    ref = csr_matvec(rp, ci, v, x)
+    assert np.max(np.abs(R.spmv(x) - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
     yv = rng.standard_normal(m)
     z = R.spmv(yv, transpose=True)
     refz = np.zeros(n)
