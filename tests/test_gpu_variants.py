@@ -128,6 +128,32 @@ def test_cli_end_to_end(dp, tmp_path):
     assert r.returncode == 0
 
 
+def test_cli_reads_gzipped_mps(dp, tmp_path):
+    """The command line on a gzipped file (inflated on a producer thread while
+    the sections are parsed, here in 64-byte windows): the output files take
+    the name up to the first '.', and the summary and solutions equal those
+    of the plain file's run bit for bit apart from the runtime line."""
+    import gzip
+    mps = os.path.join(ROOT, "tests", "golden", "capi_example.mps")
+    gz = tmp_path / "capi_example.mps.gz"
+    with open(mps, "rb") as f, gzip.open(gz, "wb") as g:
+        g.write(f.read())
+    outs = {}
+    for name, path in (("plain", mps), ("gz", str(gz))):
+        d = tmp_path / name
+        d.mkdir()
+        env = dict(os.environ, DPDLP_MPS_WINDOW="64")
+        r = subprocess.run([dp.CLI_PATH, "--no_presolve", path, str(d)], capture_output=True,
+                           text=True, timeout=120, env=env)
+        assert r.returncode == 0, r.stderr
+        summ = (d / "capi_example_summary.txt").read_text().splitlines()
+        outs[name] = ([l for l in summ if not l.startswith("Runtime")],
+                      (d / "capi_example_primal_solution.txt").read_text(),
+                      (d / "capi_example_dual_solution.txt").read_text())
+    assert outs["gz"] == outs["plain"]
+    assert outs["gz"][0][0] == "Termination Reason: OPTIMAL"
+
+
 @pytest.mark.parametrize("panels", ["1,1", "8,4"])
 def test_mixed_precision_reaches_fp64_accuracy(dp, orc, panels, monkeypatch):
     """FP32 matrix / gathers with FP64 state: the FP64 finish reaches the same
