@@ -176,7 +176,7 @@ void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o) {
   memset(o, 0, sizeof(*o));
   o->max_lps = 1024;
   o->precision = DPDLP_PREC_FP64;
-  o->mixed_switch_tol = 1e-4 * 0.1;
+  o->mixed_switch_tol = 1e-6;
   o->spmv_cap = 4096;
   o->solo_lps = 4;
 }

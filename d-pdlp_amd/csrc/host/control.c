@@ -194,5 +194,6 @@ int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps) 
     return 1;
   }
   int window = h->total / 4 > 200 ? h->total / 4 : 200;
-  return !(h->m32_best < 100.0 * eps && h->total - h->m32_best_T >= window);
+  int since = h->total - h->m32_best_T;
+  return !((h->m32_best < 100.0 * eps && since >= window) || since >= 2 * window);
 }

@@ -42,10 +42,11 @@ void ctl_check_polish_termination(hstate_t *h, const termination_criteria_t *tc,
 int ctl_should_restart(hstate_t *h, const restart_parameters_t *rp, int freq);
 void ctl_pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, double dd);
 /* Mixed precision: 1 while FP32 iterations should continue.  FP32 ends once
- * the max relative KKT error is below switch_tol, or once it is within 100x
- * of the target eps and the best value seen in FP32 has not improved by 10%
- * for max(200, T/4) iterations (the FP32 rounding floor of the residuals is
- * near the target). */
+ * the max relative KKT error is below switch_tol, or once the best value
+ * seen in FP32 has not improved by 10% for W = max(200, T/4) iterations
+ * while it is within 100x of the target eps (the FP32 rounding floor of the
+ * residuals is near the target), or for 2W iterations wherever it is (a
+ * floor far above the target: FP32 then never gets there). */
 int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps);
 
 #ifdef __cplusplus

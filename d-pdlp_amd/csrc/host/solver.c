@@ -167,7 +167,7 @@ static int pick_cap(long long nnz, int rows) {
 void dpdlp_default_opts(dpdlp_solve_opts_t *o) {
   memset(o, 0, sizeof(*o));
   o->precision = DPDLP_PREC_FP64;
-  o->mixed_switch_tol = 1e-4 * 0.1;
+  o->mixed_switch_tol = 1e-6;
   o->time_to_1e4_sec = -1.0;
   o->time_to_1e6_sec = -1.0;
 }

@@ -21,7 +21,8 @@ enum { DPDLP_PREC_FP64 = 0, DPDLP_PREC_MIXED = 1 };
 typedef struct {
   int precision;           /* DPDLP_PREC_FP64 (default) or DPDLP_PREC_MIXED */
   double mixed_switch_tol; /* mixed: iterate in FP64 once max(rel_p, rel_d,
-                              rel_gap) < this (default 1e-5 * 10) */
+                              rel_gap) < this (default 1e-6), or once FP32
+                              progress stalls */
   int spmv_cap;            /* 0 = automatic, else 1024 / 2048 / 4096 */
   int quiet;               /* 1: no "Solution Summary" print */
   int use_graph;           /* 1: replay the inner loop from a hipGraph */
@@ -157,7 +158,8 @@ void dpdlp_resident_layout(const dpdlp_resident_t *res, int *panels_A, int *pane
 typedef struct {
   int max_lps;             /* slot capacity (default 1024) */
   int precision;           /* DPDLP_PREC_FP64 (default) or DPDLP_PREC_MIXED */
-  double mixed_switch_tol; /* mixed: FP64 from this max relative KKT on */
+  double mixed_switch_tol; /* mixed: FP64 from this max relative KKT on
+                              (default 1e-6), or once FP32 progress stalls */
   int spmv_cap;            /* row-block capacity of every LP (default 4096) */
   int profile_phases;      /* 1: HIP events around the two phase kernels of
                               the first iteration of every 4th batched

@@ -156,3 +156,37 @@ def test_pid_gains_are_not_interchangeable(dp, ctl):
     ctl.ctl_pid_update(C.byref(h), C.byref(rp), 1.0, h.omega / math.e)
     assert math.isclose(h.omega, math.exp(0.875), rel_tol=1e-14)
     assert abs(h.pw_err_sum) <= 1e-15 and math.isclose(h.pw_last_err, -1.0, rel_tol=1e-14)
+
+
+def _keep(ctl, h, T, kkt, switch_tol, eps):
+    h.total = T
+    return ctl.ctl_mixed_keep_fp32(C.byref(h), kkt, switch_tol, eps)
+
+
+def test_mixed_precision_switch_rule(ctl):
+    """When a mixed-precision solve leaves FP32 for good, by hand: below the
+    switch tolerance at once; a 10% improvement of the best FP32 value (kkt
+    <= 0.9 best) restarts the stall clock; a stall of W = max(200, T/4)
+    iterations ends FP32 when the best is within 100x of the target, and a
+    stall of 2W wherever it is (so no LP stays in FP32 above its floor)."""
+    ctl.ctl_mixed_keep_fp32.argtypes = [C.POINTER(HState), C.c_double, C.c_double, C.c_double]
+    ctl.ctl_mixed_keep_fp32.restype = C.c_int
+    h = HState(m32_best=math.inf, m32_best_T=0)
+    assert _keep(ctl, h, 10, 5e-7, 1e-6, 1e-8) == 0            # below switch_tol
+    # far from the target (eps = 1e-6: 100 eps = 1e-4)
+    h = HState(m32_best=math.inf, m32_best_T=0)
+    assert _keep(ctl, h, 100, 1e-3, 1e-6, 1e-6) == 1           # first value: the best
+    assert (h.m32_best, h.m32_best_T) == (1e-3, 100)
+    assert _keep(ctl, h, 200, 0.95e-3, 1e-6, 1e-6) == 1        # not 10% better; 100 < W = 200
+    assert h.m32_best_T == 100
+    assert _keep(ctl, h, 300, 1e-3, 1e-6, 1e-6) == 1           # 200 >= W but best > 100 eps
+    assert _keep(ctl, h, 499, 1e-3, 1e-6, 1e-6) == 1           # 399 < 2W = 400
+    assert _keep(ctl, h, 500, 1e-3, 1e-6, 1e-6) == 0           # 400 >= 2W: floor far above
+    # near the target (eps = 1e-4: 100 eps = 1e-2)
+    h = HState(m32_best=math.inf, m32_best_T=0)
+    assert _keep(ctl, h, 1000, 5e-3, 1e-6, 1e-4) == 1
+    assert _keep(ctl, h, 1100, 4.5e-3, 1e-6, 1e-4) == 1        # exactly 0.9 best: improved
+    assert (h.m32_best, h.m32_best_T) == (4.5e-3, 1100)
+    assert _keep(ctl, h, 1400, 4.2e-3, 1e-6, 1e-4) == 1        # W = max(200, 350) = 350 > 300
+    assert _keep(ctl, h, 1465, 4.2e-3, 1e-6, 1e-4) == 1        # W = 366 > 365
+    assert _keep(ctl, h, 1466, 4.2e-3, 1e-6, 1e-4) == 0        # W = 366 = 366: near, stalled
