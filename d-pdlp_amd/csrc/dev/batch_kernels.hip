@@ -40,6 +40,7 @@ template <> struct BEpi<K_PRIMAL> {
     /* FP32 rounds write only the FP32 copies the next SpMV gathers */
     e.x_ref = c.use32 ? nullptr : L.x_ref;
     e.x_ref32 = c.use32 ? L.x_ref32 : nullptr;
+    e.aty0 = nullptr; e.delta32 = 0; /* a batch keeps plain shadows */
     e.tau = c.tau; e.w = c.w; e.rho = c.rho; e.major = c.major; e.fpe = c.fpe; e.partials = c.fpe;
     return e;
   }
@@ -50,6 +51,7 @@ template <> struct BEpi<K_DUAL> {
     T e;
     e.y_cur = L.y_cur; e.y_init = L.y_init; e.lc = L.lc; e.uc = L.uc;
     e.y_pdhg = L.y_pdhg; e.dy = L.dy; e.y_cur32 = c.use32 ? L.y_cur32 : nullptr;
+    e.ax0 = nullptr; e.delta32 = 0;
     e.sigma = c.sigma; e.w = c.w; e.rho = c.rho; e.major = c.major; e.fpe = c.fpe;
     e.partials = c.fpe;
     return e;

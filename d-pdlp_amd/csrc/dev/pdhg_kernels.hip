@@ -386,7 +386,7 @@ static hipError_t finalize(const dpdlp_state_t *st, int nparts, int slot0, int c
 /* restart copies (pdlp_core_op.cu:621-632 and :1290-1310) */
 __global__ void k_restart_copy(double *x_init, double *x_cur, const double *x_pdhg,
                                double *y_init, double *y_cur, const double *y_pdhg,
-                               float *y_cur32, int n, int m, int part) {
+                               float *y_cur32, int anchored, int n, int m, int part) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < (long long)n + m;
        i += (long long)gridDim.x * blockDim.x) {
     if (i < n) {
@@ -401,10 +401,17 @@ __global__ void k_restart_copy(double *x_init, double *x_cur, const double *x_pd
         double v = y_pdhg[k];
         y_init[k] = v;
         y_cur[k] = v;
-        if (y_cur32) y_cur32[k] = (float)v;
+        if (y_cur32) y_cur32[k] = anchored ? 0.0f : (float)v; /* y_cur = y_init */
       }
     }
   }
+}
+
+/* y_cur32 = y_cur - y_init (anchored shadows) */
+__global__ void k_delta_f32(const double *a, const double *b, float *out, int count) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = (float)(a[i] - b[i]);
 }
 
 /* ray projections (utils.cu:533-559) + max|dx|, max|dy| partials */
@@ -900,6 +907,9 @@ int dpdlp_dev_primal_phase(const dpdlp_state_t *st, double tau, double w, double
   e.z = st->dual_slack;
   e.dx = st->dx;
   e.x_ref32 = use32 ? st->x_ref32 : nullptr;
+  const bool anchored = use32 && st->ax0 && st->aty0;
+  e.aty0 = anchored ? st->aty0 : nullptr;
+  e.delta32 = anchored;
   e.tau = tau;
   e.w = w;
   e.rho = rho;
@@ -923,6 +933,9 @@ int dpdlp_dev_dual_phase(const dpdlp_state_t *st, double sigma, double w, double
   e.y_pdhg = st->y_pdhg;
   e.dy = st->dy;
   e.y_cur32 = use32 ? st->y_cur32 : nullptr;
+  const bool anchored = use32 && st->ax0 && st->aty0;
+  e.ax0 = anchored ? st->ax0 : nullptr;
+  e.delta32 = anchored;
   e.sigma = sigma;
   e.w = w;
   e.rho = rho;
@@ -1046,9 +1059,23 @@ int dpdlp_dev_infeasibility(const dpdlp_state_t *st, void *stream) {
 
 int dpdlp_dev_restart_copy(const dpdlp_state_t *st, int part, void *stream) {
   k_restart_copy<<<ew_grid((long long)st->n + st->m), DPDLP_BS, 0, ST(stream)>>>(
-      st->x_init, st->x_cur, st->x_pdhg, st->y_init, st->y_cur, st->y_pdhg, st->y_cur32, st->n,
-      st->m, part);
+      st->x_init, st->x_cur, st->x_pdhg, st->y_init, st->y_cur, st->y_pdhg, st->y_cur32,
+      st->ax0 && st->aty0, st->n, st->m, part);
   return (int)hipGetLastError();
+}
+
+int dpdlp_dev_anchor_products(const dpdlp_state_t *st, void *stream) {
+  hipStream_t s = ST(stream);
+  if (!st->ax0 || !st->aty0) return (int)hipErrorInvalidValue;
+  EpiStore ea{st->ax0, 0};
+  RET(launch_spmv(&st->A, st->x_init, ea, nullptr, 0, 0, s));
+  EpiStore et{st->aty0, 0};
+  RET(launch_spmv(&st->At, st->y_init, et, nullptr, 0, 0, s));
+  if (st->y_cur32 && st->m > 0) {
+    k_delta_f32<<<ew_grid(st->m), DPDLP_BS, 0, s>>>(st->y_cur, st->y_init, st->y_cur32, st->m);
+    RET(hipGetLastError());
+  }
+  return 0;
 }
 
 int dpdlp_dev_power_init(const dpdlp_state_t *st, double *eig, double *next, void *stream) {

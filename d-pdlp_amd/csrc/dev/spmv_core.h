@@ -199,15 +199,22 @@ struct EpiPrimal {
   const double *x_init, *c, *lv, *uv;
   double *x_ref, *x_pdhg, *z, *dx;
   float *x_ref32;
+  /* anchored FP32 shadows: aty0 = A' y_init is added to the FP32 product of
+   * the gathered y_cur - y_init; delta32: x_ref32 holds x_ref - x_init */
+  const double *aty0;
+  int delta32;
   double tau, w, rho;
   int major, fpe, partials;
   /* the column's operands, loadable before its A'y is known */
   struct Pre {
-    double xc, cj, l, u, xi;
+    double xc, cj, l, u, xi, a0;
   };
-  __device__ Pre pre(int j) const { return Pre{x_cur[j], c[j], lv[j], uv[j], x_init[j]}; }
+  __device__ Pre pre(int j) const {
+    return Pre{x_cur[j], c[j], lv[j], uv[j], x_init[j], aty0 ? aty0[j] : 0.0};
+  }
   __device__ void apply(int j, double aty, const Pre &q, double *s, double *) const {
     double xc = q.xc;
+    if (aty0) aty += q.a0;
     double t = xc - tau * (q.cj - aty);
     double p = fmax(q.l, fmin(t, q.u));
     double xr;
@@ -218,7 +225,7 @@ struct EpiPrimal {
     xr = 2.0 * p - xc;
     /* the dual phase gathers exactly one of the two copies */
     if (x_ref) x_ref[j] = xr;
-    if (x_ref32) x_ref32[j] = (float)xr;
+    if (x_ref32) x_ref32[j] = (float)(delta32 ? xr - q.xi : xr);
     if (fpe) {
       double d = xr - xc;
       dx[j] = d;
@@ -240,14 +247,21 @@ struct EpiDual {
   const double *y_init, *lc, *uc;
   double *y_pdhg, *dy;
   float *y_cur32;
+  /* anchored FP32 shadows: ax0 = A x_init is added to the FP32 product of
+   * the gathered x_ref - x_init; delta32: y_cur32 holds y_cur - y_init */
+  const double *ax0;
+  int delta32;
   double sigma, w, rho;
   int major, fpe, partials;
   struct Pre {
-    double yc, l, u, yi;
+    double yc, l, u, yi, a0;
   };
-  __device__ Pre pre(int i) const { return Pre{y_cur[i], lc[i], uc[i], y_init[i]}; }
+  __device__ Pre pre(int i) const {
+    return Pre{y_cur[i], lc[i], uc[i], y_init[i], ax0 ? ax0[i] : 0.0};
+  }
   __device__ void apply(int i, double ax, const Pre &q, double *s, double *) const {
     double yc = q.yc;
+    if (ax0) ax += q.a0;
     double t = yc / sigma - ax;
     double p = fmax(-q.u, fmin(t, -q.l));
     double yr;
@@ -266,7 +280,7 @@ struct EpiDual {
     double rr = rho * yr + (1.0 - rho) * yc;
     double yn = w * rr + (1.0 - w) * q.yi;
     y_cur[i] = yn;
-    if (y_cur32) y_cur32[i] = (float)yn;
+    if (y_cur32) y_cur32[i] = (float)(delta32 ? yn - q.yi : yn);
   }
   __device__ void operator()(int i, double ax, double *s, double *mx) const {
     apply(i, ax, pre(i), s, mx);

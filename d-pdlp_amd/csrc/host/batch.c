@@ -1099,7 +1099,8 @@ static int process_eval(dpdlp_batch_t *b, int i) {
   if (q->c.use32 &&
       !ctl_mixed_keep_fp32(h, kkt, b->o.mixed_switch_tol,
                            fmax(p->termination_criteria.eps_optimal_relative,
-                                p->termination_criteria.eps_feasible_relative)))
+                                p->termination_criteria.eps_feasible_relative),
+                           0))
     q->c.use32 = 0;
   ctl_check_termination(h, &p->termination_criteria);
   if (h->reason != TERMINATION_REASON_UNSPECIFIED) return finish(b, i);

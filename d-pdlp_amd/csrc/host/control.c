@@ -186,14 +186,18 @@ void ctl_pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, doub
 }
 
 
-int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps) {
+int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps, int anchored) {
   if (kkt < switch_tol) return 0;
   if (kkt <= 0.9 * h->m32_best) {
     h->m32_best = kkt;
     h->m32_best_T = h->total;
     return 1;
   }
-  int window = h->total / 4 > 200 ? h->total / 4 : 200;
+  /* anchored shadows have no rounding floor at the iterates' scale: only
+   * a much longer stall is taken for one */
+  int window = anchored ? (h->total / 2 > 800 ? h->total / 2 : 800)
+                        : (h->total / 4 > 200 ? h->total / 4 : 200);
+  int far = anchored ? window + window / 2 : 2 * window;
   int since = h->total - h->m32_best_T;
-  return !((h->m32_best < 100.0 * eps && since >= window) || since >= 2 * window);
+  return !((h->m32_best < 100.0 * eps && since >= window) || since >= far);
 }

@@ -46,8 +46,11 @@ void ctl_pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, doub
  * seen in FP32 has not improved by 10% for W = max(200, T/4) iterations
  * while it is within 100x of the target eps (the FP32 rounding floor of the
  * residuals is near the target), or for 2W iterations wherever it is (a
- * floor far above the target: FP32 then never gets there). */
-int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps);
+ * floor far above the target: FP32 then never gets there).  With anchored
+ * shadows (FP32 products of the distance from the last restart point) the
+ * rounding has no floor at the iterates' scale, and plateaus of the method
+ * itself must not end FP32: W = max(800, T/2) and the far stall 3W/2. */
+int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps, int anchored);
 
 #ifdef __cplusplus
 }

@@ -897,6 +897,7 @@ int dpdlp_i_feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *p
   P.dy = dzero(c, m);
   P.x_ref32 = NULL;
   P.y_cur32 = NULL;
+  P.ax0 = P.aty0 = NULL;
   if (!P.c || !P.x_init || !P.x_cur || !P.x_pdhg || !P.x_ref || !P.primal_product || !P.y_init ||
       !P.y_cur || !P.y_pdhg || !P.y_ref || !P.dual_product || !P.dual_slack || !P.primal_slack ||
       !P.dx || !P.dy)
@@ -938,6 +939,7 @@ int dpdlp_i_feasibility_polish(ctx_t *c, hstate_t *h, const pdhg_parameters_t *p
   D.dy = dzero(c, m);
   D.x_ref32 = NULL;
   D.y_cur32 = NULL;
+  D.ax0 = D.aty0 = NULL;
   if (!D.y_init || !D.y_cur || !D.y_pdhg || !D.y_ref || !D.dual_product || !D.dual_slack ||
       !D.x_init || !D.x_cur || !D.x_pdhg || !D.x_ref || !D.primal_product || !D.primal_slack ||
       !D.dx || !D.dy)
@@ -1390,6 +1392,20 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
   }
   if (dpdlp_i_ctx_setup_complete(c, R, params, &rescale_time)) goto fail;
   double *pi_eig = c->pi_eig, *pi_next = c->pi_next, *pi_dual = c->pi_dual;
+  /* mixed precision: anchored FP32 shadows.  The FP32 SpMVs multiply
+   * x_ref - x_init and y_cur - y_init, and the FP64 products of the anchors
+   * (refreshed at every restart) are added back, so the FP32 rounding scales
+   * with the distance from the last restart point instead of with the
+   * iterates themselves. */
+  if (c->mixed && n > 0 && m > 0) {
+    st->ax0 = (double *)dmalloc(c, (size_t)m * sizeof(double));
+    st->aty0 = (double *)dmalloc(c, (size_t)n * sizeof(double));
+    if (!st->ax0 || !st->aty0) {
+      dpdlp_set_error("device allocation failed (anchor products)");
+      goto fail;
+    }
+    HIPCHK(dpdlp_dev_anchor_products(st, c->s));
+  }
 
   /* ---- step size: power iteration (pdlp_core_op.cu:638-656) ---- */
   double t_pi0 = dpdlp_wall();
@@ -1480,7 +1496,8 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
       if (opts->time_to_1e6_sec < 0 && kkt < 1e-6) opts->time_to_1e6_sec = h->t_cum;
       if (c->use32 && !ctl_mixed_keep_fp32(h, kkt, opts->mixed_switch_tol,
                                            fmax(tc->eps_optimal_relative,
-                                                tc->eps_feasible_relative)))
+                                                tc->eps_feasible_relative),
+                                           st->ax0 != NULL))
         c->use32 = 0;
       ctl_check_termination(h, tc);
       if (params->verbose)
@@ -1495,6 +1512,7 @@ cupdlpx_result_t *dpdlp_solve_resident(dpdlp_resident_t *R, const pdhg_parameter
         ctl_pid_update(h, &params->restart_params, dpdlp_blue(S + DPDLP_SL_RESD + 5),
                        dpdlp_blue(S + DPDLP_SL_RESP + 4));
         HIPCHK(dpdlp_dev_restart_copy(st, 3, c->s));
+        if (c->use32 && st->ax0) HIPCHK(dpdlp_dev_anchor_products(st, c->s));
         h->inner = 0;
         h->fpe_last = INFINITY;
       }

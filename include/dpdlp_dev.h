@@ -59,6 +59,11 @@ typedef struct {
   double *x_init, *x_cur, *x_pdhg, *x_ref, *dual_product, *dual_slack, *dx;
   double *y_init, *y_cur, *y_pdhg, *y_ref, *primal_product, *primal_slack, *dy;
   float *x_ref32, *y_cur32; /* FP32 gather shadows (mixed precision) or NULL */
+  /* Anchored shadows (mixed precision, single-LP solves): A x_init and
+   * A' y_init in FP64, refreshed whenever the anchors move; the shadows then
+   * hold x_ref - x_init and y_cur - y_init, and each FP32 product adds its
+   * anchor's.  NULL: the shadows hold x_ref and y_cur. */
+  double *ax0, *aty0;
   double *part;   /* partial sums: DPDLP_SLOTS x part_stride */
   int part_stride;
   double *scal;   /* DPDLP_SCAL_LEN doubles of raw reduction results */
@@ -248,6 +253,9 @@ int dpdlp_dev_residual(const dpdlp_state_t *st, const dpdlp_state_t *orig,
 int dpdlp_dev_infeasibility(const dpdlp_state_t *st, void *stream);
 /* Restart copies of pdlp_core_op.cu:621-632 (part: 3 both, 1 x, 2 y). */
 int dpdlp_dev_restart_copy(const dpdlp_state_t *st, int part, void *stream);
+/* ax0 = A x_init and aty0 = A' y_init (FP64 SpMVs), and the shadow y_cur32 =
+ * y_cur - y_init, for a state with anchored shadows. */
+int dpdlp_dev_anchor_products(const dpdlp_state_t *st, void *stream);
 /* Power iteration (utils.cu:66-171), `iters` steps starting at history index
  * 0; q0 must already be in `eig` (m).  Workspace: next (m), dual (n). */
 int dpdlp_dev_power_init(const dpdlp_state_t *st, double *eig, double *next,

@@ -77,15 +77,30 @@ def test_drain_cap_reports_unfinished(dp):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_large_lane_results_equal_alone(dp, precision):
     """Requests above the large-LP threshold go to the single-LP lane beside
-    the batch; every result still equals the LP solved alone."""
+    the batch; every result still equals the LP solved alone on the path
+    that served it: a lane request equals the single-LP solve of the LP
+    (automatic layout, as the lane uploads it), a batched one the batch of
+    that LP alone.  (In FP64 the two paths agree bit for bit; in mixed
+    precision the single-LP path anchors its FP32 shadows and the batch
+    does not.)"""
     params = dp.default_params(presolve=False)
     ref = alone_results(dp, params, precision)
+    lane = {}
+    for k in range(POOL["pool_small"]):
+        prob, _ = dp.batch_member(k, POOL["member_seed"], POOL["m_lo"], POOL["m_hi"])
+        if prob.nnz >= 9000:
+            lane[k] = dp.Resident(prob).solve(params, precision=precision)[0]
+    assert lane and len(lane) < POOL["pool_small"], "the pool must feed both paths"
     rep, lat, idx, res = dp.serve(params, want_results=True, rate=300.0, duration_sec=0.2,
                                   drain_sec=60.0, seed=12, max_lps=6, precision=precision,
                                   large_lane_nnz=9000, **POOL)
     assert rep["completed"] == rep["arrived"] > 20
     for k in range(rep["arrived"]):
-        assert same(res[k], ref[idx[k]]), (k, idx[k])
+        want = lane.get(int(idx[k]), ref[idx[k]])
+        assert same(res[k], want), (k, idx[k])
+    if precision == 0:
+        for j, r in lane.items():
+            assert same(r, ref[j]), j
 
 
 def test_large_request_through_the_mix(dp):

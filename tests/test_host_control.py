@@ -158,9 +158,9 @@ def test_pid_gains_are_not_interchangeable(dp, ctl):
     assert abs(h.pw_err_sum) <= 1e-15 and math.isclose(h.pw_last_err, -1.0, rel_tol=1e-14)
 
 
-def _keep(ctl, h, T, kkt, switch_tol, eps):
+def _keep(ctl, h, T, kkt, switch_tol, eps, anchored=0):
     h.total = T
-    return ctl.ctl_mixed_keep_fp32(C.byref(h), kkt, switch_tol, eps)
+    return ctl.ctl_mixed_keep_fp32(C.byref(h), kkt, switch_tol, eps, anchored)
 
 
 def test_mixed_precision_switch_rule(ctl):
@@ -168,8 +168,10 @@ def test_mixed_precision_switch_rule(ctl):
     switch tolerance at once; a 10% improvement of the best FP32 value (kkt
     <= 0.9 best) restarts the stall clock; a stall of W = max(200, T/4)
     iterations ends FP32 when the best is within 100x of the target, and a
-    stall of 2W wherever it is (so no LP stays in FP32 above its floor)."""
-    ctl.ctl_mixed_keep_fp32.argtypes = [C.POINTER(HState), C.c_double, C.c_double, C.c_double]
+    stall of 2W wherever it is (so no LP stays in FP32 above its floor);
+    anchored shadows make W four times longer."""
+    ctl.ctl_mixed_keep_fp32.argtypes = [C.POINTER(HState), C.c_double, C.c_double, C.c_double,
+                                        C.c_int]
     ctl.ctl_mixed_keep_fp32.restype = C.c_int
     h = HState(m32_best=math.inf, m32_best_T=0)
     assert _keep(ctl, h, 10, 5e-7, 1e-6, 1e-8) == 0            # below switch_tol
@@ -190,3 +192,12 @@ def test_mixed_precision_switch_rule(ctl):
     assert _keep(ctl, h, 1400, 4.2e-3, 1e-6, 1e-4) == 1        # W = max(200, 350) = 350 > 300
     assert _keep(ctl, h, 1465, 4.2e-3, 1e-6, 1e-4) == 1        # W = 366 > 365
     assert _keep(ctl, h, 1466, 4.2e-3, 1e-6, 1e-4) == 0        # W = 366 = 366: near, stalled
+    # anchored shadows: W = max(800, T/2), far stall 3W/2
+    h = HState(m32_best=math.inf, m32_best_T=0)
+    assert _keep(ctl, h, 1000, 5e-3, 1e-6, 1e-4, 1) == 1       # near: 5e-3 < 100 eps
+    assert _keep(ctl, h, 1998, 4.9e-3, 1e-6, 1e-4, 1) == 1     # 998 < W = 999
+    assert _keep(ctl, h, 1999, 4.9e-3, 1e-6, 1e-4, 1) == 0     # 999 >= W = 999
+    h = HState(m32_best=math.inf, m32_best_T=0)
+    assert _keep(ctl, h, 1000, 1e-3, 1e-6, 1e-6, 1) == 1       # far: 1e-3 > 100 eps
+    assert _keep(ctl, h, 3994, 1e-3, 1e-6, 1e-6, 1) == 1       # 2994 < 3W/2 = 1997 + 998
+    assert _keep(ctl, h, 3995, 1e-3, 1e-6, 1e-6, 1) == 0       # 2995 >= 1997 + 998
