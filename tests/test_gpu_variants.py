@@ -173,6 +173,32 @@ def test_mixed_precision_reaches_fp64_accuracy(dp, orc, panels, monkeypatch):
 
 
 
+@pytest.mark.parametrize("panels", ["1,1", "4,2"])
+def test_anchored_mixed_precision_has_no_fp32_floor(dp, panels, monkeypatch):
+    """Single-LP mixed precision gathers x_ref - x_init and y_cur - y_init in
+    FP32 and adds the FP64 products of the anchors: to 1e-7, with the FP64
+    switch tolerance set to 0, every iteration stays in FP32, the iteration
+    count is within 5 % of the FP64 run's, and the result is the FP64 one to
+    well below the tolerance."""
+    monkeypatch.setenv("DPDLP_PANELS", panels)
+    prob, opt = dp.generate(20000, 50000, 400000, row_rule=0, seed=52)
+    kw = dict(presolve=False, termination_criteria__eps_optimal_relative=1e-7,
+              termination_criteria__eps_feasible_relative=1e-7)
+    R = dp.Resident(prob)
+    ref, _ = R.solve(dp.default_params(**kw))
+    res, o = R.solve(dp.default_params(**kw), precision=dp.PREC_MIXED, mixed_switch_tol=0.0)
+    print(f"fp64 T={ref.iterations}; mixed T={res.iterations} fp32 {o.iterations_fp32}; "
+          f"objective {abs(res.primal_obj - ref.primal_obj) / abs(ref.primal_obj):.2e}, "
+          f"x {rel(res.x, ref.x):.2e}, y {rel(res.y, ref.y):.2e}", flush=True)
+    assert res.status == ref.status == "OPTIMAL"
+    assert o.iterations_fp32 == res.iterations
+    assert abs(res.iterations - ref.iterations) <= 0.05 * ref.iterations
+    assert res.rel_primal < 1e-7 and res.rel_dual < 1e-7 and res.rel_gap < 1e-7
+    assert abs(res.primal_obj - ref.primal_obj) <= 1e-8 * abs(ref.primal_obj)
+    assert rel(res.x, ref.x) < 1e-4 and rel(res.y, ref.y) < 1e-4
+    R.free()
+
+
 def test_skewed_rows_renumbered_parity(dp, orc):
     """Zipf row lengths trigger the row renumbering of the upload (rows by
     decreasing length): iterates (in the caller's order), duals, status and
