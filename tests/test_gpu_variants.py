@@ -199,6 +199,34 @@ def test_anchored_mixed_precision_has_no_fp32_floor(dp, panels, monkeypatch):
     R.free()
 
 
+def test_anchored_mixed_precision_warm_start_and_polish(dp):
+    """Anchored FP32 shadows start from a warm start (the first anchors are
+    the start point, so their FP64 products are taken before the first
+    iteration) and hand over to FP64 feasibility polishing: the mixed solve
+    begins at the same iterate as the FP64 one, reaches the tolerance, and
+    polishes to the same polish tolerance."""
+    prob, opt = dp.generate(3000, 7000, 60000, row_rule=0, seed=53)
+    rng = np.random.default_rng(3)
+    prob.set_start(rng.uniform(0, 5, prob.n), rng.standard_normal(prob.m))
+    kw = dict(presolve=False, termination_criteria__eps_optimal_relative=1e-6,
+              termination_criteria__eps_feasible_relative=1e-6)
+    iters = [0, 1]
+    r64, o64, (x64, y64) = dp.solve(prob, dp.default_params(**kw), trace_iters=iters)
+    r32, o32, (x32, y32) = dp.solve(prob, dp.default_params(**kw), trace_iters=iters,
+                                    precision=dp.PREC_MIXED)
+    assert np.array_equal(x64[0], x32[0]) and np.array_equal(y64[0], y32[0])
+    assert rel(x32[1], x64[1]) < 1e-6 and rel(y32[1], y64[1]) < 1e-6
+    assert r64.status == r32.status == "OPTIMAL" and o32.iterations_fp32 > 0
+    assert r32.rel_primal < 1e-6 and r32.rel_dual < 1e-6 and r32.rel_gap < 1e-6
+    assert abs(r32.primal_obj - r64.primal_obj) <= 1e-6 * max(1.0, abs(r64.primal_obj))
+    # polishing after a mixed solve to the default 1e-4 (the 1e-6 polish
+    # tolerance is then not met yet, so both polish loops run, in FP64)
+    kwp = dict(presolve=False, feasibility_polishing=True)
+    p32, _, _ = dp.solve(prob, dp.default_params(**kwp), precision=dp.PREC_MIXED)
+    assert p32.status == "OPTIMAL" and p32.raw["feasibility_polishing_time"] > 0
+    assert p32.rel_primal <= 1e-6 and p32.rel_dual <= 1e-6
+
+
 def test_skewed_rows_renumbered_parity(dp, orc):
     """Zipf row lengths trigger the row renumbering of the upload (rows by
     decreasing length): iterates (in the caller's order), duals, status and
