@@ -146,6 +146,14 @@ static int planners(void) {
       CHECK(toff && rofs && perm && pci);
       CHECK(dpdlp_build_panels(m, n, rp, ci, blk, nb, P, toff, rofs, perm, pci) == 0);
       CHECK(toff[(size_t)nb * P] == nnz);
+      /* A''s panel plan from A (no transpose) equals the plan on A' */
+      int *bt = malloc((size_t)(n + 2) * sizeof(int)), *bT = malloc((size_t)(n + 2) * sizeof(int));
+      CHECK(bt && bT);
+      int na = dpdlp_plan_blocks_panels(n, m, trp, tci, 1024, P, DPDLP_PANEL_RMAX, bt);
+      int nT = dpdlp_plan_blocks_panels_T(m, n, rp, ci, 1024, P, DPDLP_PANEL_RMAX, bT);
+      CHECK(na == nT && memcmp(bt, bT, (size_t)(na + 1) * sizeof(int)) == 0);
+      free(bt);
+      free(bT);
       free(toff);
       free(rofs);
       free(perm);
