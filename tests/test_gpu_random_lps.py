@@ -93,3 +93,18 @@ def test_random_lp_matches_oracle(dp, orc, seed, monkeypatch):
     # (measured up to 2.6e-7 at T = 3000), so 1e-6 there
     tol = 1e-8 if ref["status"] == "OPTIMAL" else 1e-6
     assert rel(res.x, ref["x"]) < tol and rel(res.y, ref["y"]) < tol
+
+
+def test_random_lps_batched_equal_single(dp):
+    """The same random LPs in one FP64 batch (every bound and row kind, some
+    stopping at the iteration limit): each result is bit for bit the
+    single-LP solve of that LP on the same layout."""
+    probs = [random_lp(dp, 9000 + s) for s in range(18)]
+    params = dp.default_params(presolve=False, termination_criteria__iteration_limit=3000)
+    res = [dp.Resident(p, spmv_cap=1024, panels=(1, 1)) for p in probs]
+    mix = dp.solve_batch(res, params, spmv_cap=1024)
+    for k, r in enumerate(res):
+        one, _ = r.solve(params)
+        assert (mix[k].status, mix[k].iterations) == (one.status, one.iterations), k
+        assert np.array_equal(mix[k].x, one.x) and np.array_equal(mix[k].y, one.y), k
+        assert mix[k].primal_obj == one.primal_obj, k
