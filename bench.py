@@ -516,18 +516,26 @@ def main():
     ws1 = dict(kind="global", bytes=8 * m)
     ws2 = dict(kind="xcd_local", bytes=8 * n // pA) if pA > 1 else dict(kind="global", bytes=8 * n)
 
-    def ceiling(ws):
-        return dpdlp.lib.dpdlp_gather_calibration_ex(ws["bytes"], nnz, 10,
-                                                     1 if ws["kind"] == "xcd_local" else 0, 2048)
-    c1, c2 = ceiling(ws1), ceiling(ws2)
+    def ceiling(ws, values):
+        return dpdlp.lib.dpdlp_gather_calibration_values(
+            ws["bytes"], nnz, 10, 1 if ws["kind"] == "xcd_local" else 0, 2048, values)
+    c1, c2 = ceiling(ws1, 0), ceiling(ws2, 0)
+    # the same gathers with the FP64 values streamed beside the indices: the
+    # ceiling of a pass that reads the matrix as the SpMV does (12 B/nonzero)
+    v1, v2 = ceiling(ws1, 1), ceiling(ws2, 1)
     g1, g2 = nnz / (p1 * 1e-3) / 1e9, nnz / (p2 * 1e-3) / 1e9
     gather = dict(gathers_per_launch=nnz,
                   phase1_G_per_s=g1, phase1_working_set=ws1, phase1_ceiling_G_per_s=c1,
                   phase1_frac=g1 / c1 if c1 > 0 else None,
+                  phase1_ceiling_with_values_G_per_s=v1,
+                  phase1_frac_with_values=g1 / v1 if v1 > 0 else None,
                   phase2_G_per_s=g2, phase2_working_set=ws2, phase2_ceiling_G_per_s=c2,
                   phase2_frac=g2 / c2 if c2 > 0 else None,
+                  phase2_ceiling_with_values_G_per_s=v2,
+                  phase2_frac_with_values=g2 / v2 if v2 > 0 else None,
                   ceiling_kernel="k_gather_calib: coalesced index stream, 2048 indices per "
-                                 "workgroup, 8 gathers in flight per thread",
+                                 "workgroup, 8 gathers in flight per thread; *_with_values: "
+                                 "FP64 values streamed beside the indices",
                   sweep_G_per_s=sweep)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "zipf1m_fp64_hbm_traffic.json")

@@ -830,8 +830,12 @@ __global__ __launch_bounds__(DPDLP_BS) void k_reduce_final(int op, const double 
  * consecutive indices, eight loads in flight per thread, non-temporal, as
  * spmv_block does) and gathers x[base + idx] for each, where base is the
  * slice of its own XCD ((w mod 8) * slice; workgroup w runs on XCD w mod 8)
- * when xcd_local is set, else 0.  Each thread writes the sum of its gathers. */
+ * when xcd_local is set, else 0.  With WITHV the FP64 values are streamed
+ * beside the indices (the SpMV's 12 bytes per nonzero) and each gather is
+ * multiplied by its value.  Each thread writes the sum of its products. */
+template <bool WITHV>
 __global__ __launch_bounds__(DPDLP_BS) void k_gather_calib(const int *__restrict__ idx,
+                                                          const double *__restrict__ val,
                                                           const double *__restrict__ x,
                                                           double *__restrict__ y, long long count,
                                                           int chunk, long long slice,
@@ -844,16 +848,19 @@ __global__ __launch_bounds__(DPDLP_BS) void k_gather_calib(const int *__restrict
   double acc = 0.0;
   for (int base = threadIdx.x; base < lim; base += DPDLP_BS * UNR) {
     int ci[UNR];
+    double vv[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       const int k = base + u * DPDLP_BS;
       ci[u] = k < lim ? __builtin_nontemporal_load(idx + c0 + k) : -1;
+      vv[u] = 1.0;
+      if (WITHV && k < lim) vv[u] = __builtin_nontemporal_load(val + c0 + k);
     }
     double v[UNR];
 #pragma unroll
     for (int u = 0; u < UNR; ++u) v[u] = ci[u] >= 0 ? xs[ci[u]] : 0.0;
 #pragma unroll
-    for (int u = 0; u < UNR; ++u) acc += v[u];
+    for (int u = 0; u < UNR; ++u) acc += WITHV ? vv[u] * v[u] : v[u];
   }
   y[(long long)blockIdx.x * DPDLP_BS + threadIdx.x] = acc;
 }
@@ -1207,14 +1214,24 @@ int dpdlp_dev_reduce(int op, const double *x, const double *y, long long n, doub
   return (int)hipGetLastError();
 }
 
-int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long count,
-                           int chunk, long long slice, int xcd_local, void *stream) {
+int dpdlp_dev_gather_calib_values(const int *idx, const double *val, const double *x,
+                                  double *y, long long count, int chunk, long long slice,
+                                  int xcd_local, void *stream) {
   if (count <= 0 || chunk <= 0) return (int)hipErrorInvalidValue;
   long long g = (count + chunk - 1) / chunk;
   if (g > 0x7fffffffLL) return (int)hipErrorInvalidValue;
-  k_gather_calib<<<(unsigned)g, DPDLP_BS, 0, ST(stream)>>>(idx, x, y, count, chunk, slice,
-                                                           xcd_local);
+  if (val)
+    k_gather_calib<true><<<(unsigned)g, DPDLP_BS, 0, ST(stream)>>>(idx, val, x, y, count, chunk,
+                                                                   slice, xcd_local);
+  else
+    k_gather_calib<false><<<(unsigned)g, DPDLP_BS, 0, ST(stream)>>>(idx, val, x, y, count, chunk,
+                                                                    slice, xcd_local);
   return (int)hipGetLastError();
+}
+
+int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long count,
+                           int chunk, long long slice, int xcd_local, void *stream) {
+  return dpdlp_dev_gather_calib_values(idx, NULL, x, y, count, chunk, slice, xcd_local, stream);
 }
 
 int dpdlp_dev_copy_calib(const void *src, void *dst, long long bytes, void *stream) {

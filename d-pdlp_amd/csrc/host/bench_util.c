@@ -234,15 +234,15 @@ out:
   return rc;
 }
 
-double dpdlp_gather_calibration_ex(long long slice_bytes, long long gathers, int reps,
-                                   int xcd_local, int chunk) {
+double dpdlp_gather_calibration_values(long long slice_bytes, long long gathers, int reps,
+                                       int xcd_local, int chunk, int with_values) {
   long long slice = slice_bytes / 8;
   if (slice < 1 || slice > 0x7fffffffLL || gathers < 1 || reps < 1 || chunk < 1) return -1.0;
   long long xlen = xcd_local ? 8 * slice : slice;
   long long blocks = (gathers + chunk - 1) / chunk;
   int *h = (int *)malloc((size_t)gathers * sizeof(int));
   int *idx = NULL;
-  double *x = NULL, *y = NULL;
+  double *x = NULL, *y = NULL, *val = NULL;
   hipEvent_t e0 = NULL, e1 = NULL;
   double rate = -1.0;
   if (!h) return -1.0;
@@ -256,14 +256,20 @@ double dpdlp_gather_calibration_ex(long long slice_bytes, long long gathers, int
   if (hipMalloc((void **)&idx, (size_t)gathers * sizeof(int)) != hipSuccess) goto out;
   if (hipMalloc((void **)&x, (size_t)xlen * sizeof(double)) != hipSuccess) goto out;
   if (hipMalloc((void **)&y, (size_t)blocks * 256 * sizeof(double)) != hipSuccess) goto out;
+  if (with_values) {
+    if (hipMalloc((void **)&val, (size_t)gathers * sizeof(double)) != hipSuccess) goto out;
+    if (hipMemset(val, 0, (size_t)gathers * sizeof(double)) != hipSuccess) goto out;
+  }
   if (hipMemcpy(idx, h, (size_t)gathers * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(x, 0, (size_t)xlen * sizeof(double)) != hipSuccess)
     goto out;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) goto out;
-  if (dpdlp_dev_gather_calib(idx, x, y, gathers, chunk, slice, xcd_local, NULL)) goto out;
+  if (dpdlp_dev_gather_calib_values(idx, val, x, y, gathers, chunk, slice, xcd_local, NULL))
+    goto out;
   if (hipEventRecord(e0, NULL) != hipSuccess) goto out;
   for (int k = 0; k < reps; ++k)
-    if (dpdlp_dev_gather_calib(idx, x, y, gathers, chunk, slice, xcd_local, NULL)) goto out;
+    if (dpdlp_dev_gather_calib_values(idx, val, x, y, gathers, chunk, slice, xcd_local, NULL))
+      goto out;
   if (hipEventRecord(e1, NULL) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) goto out;
   {
     float t = 0.f;
@@ -276,9 +282,15 @@ out:
   if (idx) hipFree(idx);
   if (x) hipFree(x);
   if (y) hipFree(y);
+  if (val) hipFree(val);
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   return rate;
+}
+
+double dpdlp_gather_calibration_ex(long long slice_bytes, long long gathers, int reps,
+                                   int xcd_local, int chunk) {
+  return dpdlp_gather_calibration_values(slice_bytes, gathers, reps, xcd_local, chunk, 0);
 }
 
 double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps) {

@@ -61,6 +61,8 @@ def _load() -> C.CDLL:
         "dpdlp_gather_calibration": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int]),
         "dpdlp_gather_calibration_ex": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int,
                                                      C.c_int, C.c_int]),
+        "dpdlp_gather_calibration_values": (C.c_double, [C.c_longlong, C.c_longlong, C.c_int,
+                                                         C.c_int, C.c_int, C.c_int]),
         "dpdlp_reduce_host": (C.c_int, [C.c_int, _dp, _dp, C.c_longlong, _dp]),
         "dpdlp_scaling_host": (C.c_int, [P, C.POINTER(PdhgParameters)] + [_dp] * 9),
         "dpdlp_plan_blocks": (C.c_int, [C.c_int, _ip, C.c_int, _ip]),

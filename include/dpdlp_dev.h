@@ -304,6 +304,12 @@ int dpdlp_dev_reduce(int op, const double *x, const double *y, long long n, doub
  * 0; y gets one partial sum per thread (ceil(count/chunk) * 256 doubles). */
 int dpdlp_dev_gather_calib(const int *idx, const double *x, double *y, long long count,
                            int chunk, long long slice, int xcd_local, void *stream);
+/* The same with the FP64 values val[count] streamed beside the indices (the
+ * SpMV's 12 bytes per nonzero) and multiplied into the gathers; val NULL is
+ * dpdlp_dev_gather_calib. */
+int dpdlp_dev_gather_calib_values(const int *idx, const double *val, const double *x,
+                                  double *y, long long count, int chunk, long long slice,
+                                  int xcd_local, void *stream);
 /* Plain device copy of `bytes` (multiple of 32): the reference point that
  * scales the FETCH_SIZE / WRITE_SIZE counters to bytes. */
 int dpdlp_dev_copy_calib(const void *src, void *dst, long long bytes, void *stream);

@@ -127,6 +127,11 @@ int dpdlp_reduce_host(int op, const double *x, const double *y, long long n, dou
  * vector.  The ceiling of an SpMV's gathers at that working-set size. */
 double dpdlp_gather_calibration_ex(long long slice_bytes, long long gathers, int reps,
                                    int xcd_local, int chunk);
+/* The same with the FP64 value of each gather streamed beside its index
+ * when with_values (the full 12-byte-per-nonzero stream of an FP64 SpMV, so
+ * the rate is the ceiling of a row-block pass that also reads its values). */
+double dpdlp_gather_calibration_values(long long slice_bytes, long long gathers, int reps,
+                                       int xcd_local, int chunk, int with_values);
 /* dpdlp_gather_calibration_ex(slice_bytes, gathers, reps, 1, 2048) */
 double dpdlp_gather_calibration(long long slice_bytes, long long gathers, int reps);
 
