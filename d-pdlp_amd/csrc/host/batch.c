@@ -51,10 +51,11 @@
 
 /* slot stages: FREE -> RESERVED (being set up, possibly on another thread)
  * -> READY (set up, waiting for a window start) -> PI -> PDHG -> FREE */
-enum { S_FREE = 0, S_RESERVED, S_READY, S_PI, S_PDHG, S_ZOMBIE };
+enum { S_FREE = 0, S_RESERVED, S_READY, S_PI, S_PDHG, S_ZOMBIE, S_PERSIST };
 /* S_ZOMBIE: retired, but the work lists still name the slot (its items
  * return at once: the slot's control records are zero); it becomes FREE at
- * the next list rebuild */
+ * the next list rebuild.  S_PERSIST: the LP runs whole in the persistent
+ * kernel (one workgroup, its own loop on the device), outside the work lists */
 
 
 enum { NLIST = 5 };
@@ -72,6 +73,7 @@ typedef struct {
   double sigma_sq;
   int do_restart;
   int ev_on, ev_rc, ev_inf, ev_final, fresh;
+  int persist; /* 1: in the persistent group (stage S_PERSIST until it retires) */
 } bslot_t;
 
 typedef struct {
@@ -163,6 +165,17 @@ struct dpdlp_batch {
   double pev_bytes[32][2];
   int npev_created, npev_used, psamples;
   double p_ms[2], p_bytes[2];
+  /* persistent group: descriptors of its LPs (device and pinned host copy),
+   * the slot of each, the launch in flight on s_pp (ev_pp: kernel and the
+   * descriptors' copy back complete), ev_pp_go: set-up work on s done */
+  hipStream_t s_pp;
+  hipEvent_t ev_pp, ev_pp_go;
+  dpdlp_plp_t *d_plp, *h_plp;
+  int *pp_slot;
+  int pp_n, pp_inflight;
+  pdhg_parameters_t *d_prm;
+  double clk_hz;
+  long long pp_launches;
 };
 
 static int stage_of(const bslot_t *q) { return __atomic_load_n(&q->stage, __ATOMIC_ACQUIRE); }
@@ -179,6 +192,7 @@ void dpdlp_batch_default_opts(dpdlp_batch_opts_t *o) {
   o->mixed_switch_tol = 1e-6;
   o->spmv_cap = 4096;
   o->solo_lps = 4;
+  o->persist_budget = 250;
 }
 
 static int gcd(int a, int b) {
@@ -194,6 +208,7 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   if (!b) return;
   if (b->s) hipStreamSynchronize(b->s);
   if (b->s_pi) hipStreamSynchronize(b->s_pi);
+  if (b->s_pp) hipStreamSynchronize(b->s_pp);
   for (int i = 0; i < b->cap && b->sl; ++i) {
     bslot_t *q = &b->sl[i];
     if (stage_of(q) == S_FREE) continue;
@@ -218,12 +233,13 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   for (int i = 0; b->sl && i < b->cap; ++i)
     if (b->sl[i].ev_ready) hipEventDestroy(b->sl[i].ev_ready);
   free(b->sl);
-  void *dev[] = {b->d_lps, b->d_ctl_eval, b->d_ctl_win, b->d_scal,
-                 b->d_list, b->d_rng,     b->d_part,    b->d_normals};
+  void *dev[] = {b->d_lps,  b->d_ctl_eval, b->d_ctl_win, b->d_scal, b->d_list,
+                 b->d_rng,  b->d_part,     b->d_normals, b->d_plp,  b->d_prm};
   for (size_t k = 0; k < sizeof(dev) / sizeof(dev[0]); ++k)
     if (dev[k]) hipFree(dev[k]);
   void *host[] = {b->h_lps,  b->h_ctl_eval, b->h_ctl_win[0], b->h_ctl_win[1],
-                  b->h_scal, b->h_rb,       b->h_list,       b->h_rng};
+                  b->h_scal, b->h_rb,       b->h_list,       b->h_rng,
+                  b->h_plp};
   for (size_t k = 0; k < sizeof(host) / sizeof(host[0]); ++k)
     if (host[k]) hipHostFree(host[k]);
   for (int k = 0; k < 2; ++k)
@@ -236,6 +252,10 @@ void dpdlp_batch_free(dpdlp_batch_t *b) {
   if (b->s_pi) hipStreamDestroy(b->s_pi);
   if (b->ev_ctl) hipEventDestroy(b->ev_ctl);
   if (b->ev_pi) hipEventDestroy(b->ev_pi);
+  if (b->s_pp) hipStreamDestroy(b->s_pp);
+  if (b->ev_pp) hipEventDestroy(b->ev_pp);
+  if (b->ev_pp_go) hipEventDestroy(b->ev_pp_go);
+  free(b->pp_slot);
   pthread_mutex_destroy(&b->mu);
   free(b);
 }
@@ -269,6 +289,7 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
     b->o.spmv_cap = 4096;
   if (b->o.spmv_cap_At != 1024 && b->o.spmv_cap_At != 2048 && b->o.spmv_cap_At != 4096)
     b->o.spmv_cap_At = b->o.spmv_cap;
+  if (b->o.persist_budget <= 0) b->o.persist_budget = 250;
   if (b->prm.termination_evaluation_frequency <= 0) {
     dpdlp_set_error("termination_evaluation_frequency must be positive");
     pthread_mutex_destroy(&b->mu);
@@ -312,6 +333,24 @@ dpdlp_batch_t *dpdlp_batch_create(const pdhg_parameters_t *params,
   CCHK(hipMalloc((void **)&b->d_rng, C * 2 * NLIST * sizeof(int)));
   CCHK(hipHostMalloc((void **)&b->h_rng, C * 2 * NLIST * sizeof(int), 0));
   memset(b->h_lps, 0, C * sizeof(dpdlp_blp_t));
+  if (b->o.persist_max_nnz > 0) {
+    CCHK(hipStreamCreateWithFlags(&b->s_pp, hipStreamNonBlocking));
+    CCHK(hipEventCreateWithFlags(&b->ev_pp, hipEventDisableTiming));
+    CCHK(hipEventCreateWithFlags(&b->ev_pp_go, hipEventDisableTiming));
+    CCHK(hipMalloc((void **)&b->d_plp, C * sizeof(dpdlp_plp_t)));
+    CCHK(hipHostMalloc((void **)&b->h_plp, C * sizeof(dpdlp_plp_t), 0));
+    CCHK(hipMalloc((void **)&b->d_prm, sizeof(pdhg_parameters_t)));
+    CCHK(hipMemcpy(b->d_prm, &b->prm, sizeof(pdhg_parameters_t), hipMemcpyHostToDevice));
+    int dev = 0, khz = 0;
+    CCHK(hipGetDevice(&dev));
+    CCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    b->clk_hz = khz > 0 ? 1e3 * khz : 1e8;
+    b->pp_slot = (int *)calloc(C, sizeof(int));
+    if (!b->pp_slot) {
+      dpdlp_batch_free(b);
+      return NULL;
+    }
+  }
   if (b->o.profile_phases) {
     for (int k = 0; k < 32; ++k) {
       for (int q = 0; q < 3; ++q) CCHK(hipEventCreate(&b->pev[k][q]));
@@ -653,7 +692,15 @@ static int activate_ready(dpdlp_batch_t *b) {
         dpdlp_set_error("power-iteration start failed on the device");
         return -1;
       }
-      set_stage(q, S_PI);
+      if (b->o.persist_max_nnz > 0 && R->nnz <= b->o.persist_max_nnz &&
+          R->cap <= DPDLP_PERSIST_CAP && R->At_cap <= DPDLP_PERSIST_CAP) {
+        /* the whole solve runs in the persistent kernel (FP64) */
+        q->persist = 1;
+        q->c.use32 = 0;
+        set_stage(q, S_PERSIST);
+      } else {
+        set_stage(q, S_PI);
+      }
     } else {
       start_loop(b, q);
       q->fresh = 0; /* iteration 0 is evaluated with the others below */
@@ -1061,7 +1108,7 @@ static int finish(dpdlp_batch_t *b, int i) {
   b->stage_off[g] += need;
   /* the arena moves to the pending entry; the slot's other allocations (if
    * polishing made any) are freed now; no stream synchronisation here */
-  {
+  if (!q->persist) {
     const dpdlp_resident_t *R = q->R;
     b->zombie_items += R->A_nblk + R->At_nblk + dpdlp_ew_grid(R->n) + dpdlp_ew_grid(R->m) +
                        dpdlp_ew_grid((long long)R->n + R->m);
@@ -1343,6 +1390,84 @@ static int launch_window(dpdlp_batch_t *b) {
   return 0;
 }
 
+/* ---- persistent group ---- */
+
+/* Completes the persistent launch in flight (block: wait for it).  LPs that
+ * terminated retire (result copies queued on the batch stream, after the
+ * launch); the others keep their descriptors for the next launch. */
+static int pp_complete(dpdlp_batch_t *b, int block) {
+  if (!b->pp_inflight) return 0;
+  if (block) {
+    BCHK(hipEventSynchronize(b->ev_pp));
+  } else {
+    hipError_t e = hipEventQuery(b->ev_pp);
+    if (e == hipErrorNotReady) return 0;
+    BCHK(e);
+  }
+  b->pp_inflight = 0;
+  BCHK(hipStreamWaitEvent(b->s, b->ev_pp, 0));
+  int keep = 0;
+  for (int k = 0; k < b->pp_n; ++k) {
+    const int i = b->pp_slot[k];
+    bslot_t *q = &b->sl[i];
+    if (b->h_plp[k].stage == DPDLP_PS_DONE) {
+      q->h = b->h_plp[k].h;
+      q->sigma_sq = b->h_plp[k].sigma_sq;
+      q->pi_iters = b->h_plp[k].pi_iters;
+      if (finish(b, i)) return -1;
+    } else {
+      if (keep != k) {
+        b->h_plp[keep] = b->h_plp[k];
+        b->pp_slot[keep] = i;
+      }
+      keep++;
+    }
+  }
+  b->pp_n = keep;
+  return 0;
+}
+
+/* Adds the group's new LPs and launches the next round of the group on its
+ * own stream (after the set-up and power-iteration starts queued on the
+ * batch stream). */
+static int pp_launch(dpdlp_batch_t *b) {
+  if (b->pp_inflight) return 0;
+  for (int i = 0; i < b->nhw; ++i) {
+    bslot_t *q = &b->sl[i];
+    if (stage_of(q) != S_PERSIST || q->persist != 1) continue;
+    dpdlp_plp_t *D = &b->h_plp[b->pp_n];
+    memset(D, 0, sizeof(*D));
+    D->L = b->h_lps[i];
+    D->A_blk = q->c.st.A.blk;
+    D->At_blk = q->c.st.At.blk;
+    D->A_nblk = q->c.st.A.nblk;
+    D->At_nblk = q->c.st.At.nblk;
+    D->k = q->c.k;
+    D->sigma_sq = 1.0;
+    D->stage = DPDLP_PS_PI;
+    b->pp_slot[b->pp_n++] = i;
+    q->persist = 2;
+  }
+  if (b->pp_n == 0) return 0;
+  const size_t bytes = (size_t)b->pp_n * sizeof(dpdlp_plp_t);
+  BCHK(hipEventRecord(b->ev_pp_go, b->s));
+  BCHK(hipStreamWaitEvent(b->s_pp, b->ev_pp_go, 0));
+  BCHK(hipMemcpyAsync(b->d_plp, b->h_plp, bytes, hipMemcpyHostToDevice, b->s_pp));
+  /* at most persist_budget iterations (power-iteration steps and PDHG
+   * iterations) per LP: each launch ends within a bounded time, and the host
+   * retires finished LPs and admits new ones between launches */
+  if (dpdlp_pdev_run(b->d_plp, b->pp_n, b->d_prm, b->clk_hz, b->o.persist_budget, b->s_pp)) {
+    dpdlp_set_error("persistent kernel launch failed");
+    return -1;
+  }
+  BCHK(hipMemcpyAsync(b->h_plp, b->d_plp, bytes, hipMemcpyDeviceToHost, b->s_pp));
+  BCHK(hipEventRecord(b->ev_pp, b->s_pp));
+  b->pp_inflight = 1;
+  b->pp_launches++;
+  b->launches++;
+  return 0;
+}
+
 static int n_active(const dpdlp_batch_t *b) {
   int k = 0;
   for (int i = 0; i < b->nhw; ++i) k += active(&b->sl[i]);
@@ -1355,6 +1480,11 @@ int dpdlp_batch_step(dpdlp_batch_t *b) {
   pthread_mutex_lock(&b->mu);
   if (b->reserve_failed && b->zombie_items > 0) b->dirty = 1;
   pthread_mutex_unlock(&b->mu);
+  if (b->o.persist_max_nnz > 0) {
+    /* the persistent group runs beside the windows; with no window to run,
+     * wait for its launch instead of polling */
+    if (pp_complete(b, n_active(b) == 0) || pp_launch(b)) return -1;
+  }
   if (n_active(b) == 0) {
     if (b->npend[0] || b->npend[1]) {
       BCHK(hipStreamSynchronize(b->s));

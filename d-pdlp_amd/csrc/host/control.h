@@ -9,29 +9,11 @@
 #pragma once
 
 #include "cupdlpx_types.h"
+#include "dpdlp_dev.h"
 
 #ifdef __cplusplus
 extern "C" {
 #endif
-
-typedef struct {
-  double step, omega, best_omega, pw_err_sum, pw_last_err, best_gap;
-  int total, inner, is_major;
-  double fpe, fpe0, fpe_last;
-  double abs_p, rel_p, abs_d, rel_d, pobj, dobj, gap, rel_gap;
-  double max_p_ray_inf, max_d_ray_inf, p_ray_lin_obj, d_ray_obj;
-  termination_reason_t reason;
-  double t_cum;
-  int pend_rst, pend_maj;
-  double m32_best; /* mixed precision: best max relative KKT seen in FP32 */
-  int m32_best_T;  /* iteration at which it was reached */
-} hstate_t;
-
-
-/* per-LP constants: scaling factors, objective constant, unscaled norms */
-typedef struct {
-  double beta, gamma, c0, obj_norm, bound_norm;
-} lpk_t;
 
 double dpdlp_blue(const double *three_sums);
 void ctl_residual(hstate_t *h, const lpk_t *k, const double *S, norm_type_t norm);
@@ -51,6 +33,10 @@ void ctl_pid_update(hstate_t *h, const restart_parameters_t *rp, double pd, doub
  * rounding has no floor at the iterates' scale, and plateaus of the method
  * itself must not end FP32: W = max(800, T/2) and the far stall 3W/2. */
 int ctl_mixed_keep_fp32(hstate_t *h, double kkt, double switch_tol, double eps, int anchored);
+double ctl_blue(const double *three_sums);
+double ctl_fpe_value(const double *G, double omega, double step);
+int ctl_print_frequency(int iter);
+void ctl_init_state(hstate_t *h, const lpk_t *k, const pdhg_parameters_t *params, double step);
 
 #ifdef __cplusplus
 }

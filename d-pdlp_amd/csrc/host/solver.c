@@ -1221,14 +1221,7 @@ fail:
 
 void dpdlp_i_init_hstate(hstate_t *h, const ctx_t *c, const pdhg_parameters_t *params,
                          double step) {
-  memset(h, 0, sizeof(*h));
-  h->step = step;
-  h->omega = params->bound_objective_rescaling ? 1.0 : (c->k.obj_norm + 1.0) / (c->k.bound_norm + 1.0);
-  h->best_omega = h->omega;
-  h->best_gap = INFINITY;
-  h->fpe_last = INFINITY;
-  h->reason = TERMINATION_REASON_UNSPECIFIED;
-  h->m32_best = INFINITY;
+  ctl_init_state(h, &c->k, params, step);
 }
 
 void dpdlp_i_result_scalars(cupdlpx_result_t *res, const ctx_t *c, const hstate_t *h,

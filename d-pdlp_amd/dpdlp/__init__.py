@@ -402,10 +402,14 @@ class Resident:
 
 
 def batch_opts(max_lps=1024, precision=PREC_FP64, spmv_cap=4096, mixed_switch_tol=None,
-               profile_phases=0, use_graph=0, solo_lps=None, spmv_cap_At=None):
+               profile_phases=0, use_graph=0, solo_lps=None, spmv_cap_At=None,
+               persist_max_nnz=0, persist_budget=None):
     o = BatchOpts()
     lib.dpdlp_batch_default_opts(C.byref(o))
     o.max_lps = max_lps
+    o.persist_max_nnz = persist_max_nnz
+    if persist_budget is not None:
+        o.persist_budget = persist_budget
     o.precision = precision
     o.spmv_cap = spmv_cap
     if spmv_cap_At is not None:

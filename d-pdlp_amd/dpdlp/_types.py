@@ -128,7 +128,8 @@ class BatchOpts(C.Structure):
     """dpdlp_batch_opts_t (include/dpdlp_ext.h)."""
     _fields_ = [("max_lps", C.c_int), ("precision", C.c_int), ("mixed_switch_tol", C.c_double),
                 ("spmv_cap", C.c_int), ("profile_phases", C.c_int), ("use_graph", C.c_int),
-                ("solo_lps", C.c_int), ("spmv_cap_At", C.c_int),
+                ("solo_lps", C.c_int), ("spmv_cap_At", C.c_int), ("persist_max_nnz", C.c_int),
+                ("persist_budget", C.c_int),
                 ("windows", C.c_longlong), ("syncs", C.c_longlong), ("launches", C.c_longlong),
                 ("graph_replays", C.c_longlong),
                 ("phase_samples", C.c_int), ("primal_ms", C.c_double),

@@ -12,6 +12,8 @@
 
 #include <stddef.h>
 
+#include "cupdlpx_types.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -143,6 +145,75 @@ int dpdlp_bdev_fpe(const dpdlp_batch_dev_t *b, void *stream);
 int dpdlp_bdev_residual(const dpdlp_batch_dev_t *b, void *stream);
 int dpdlp_bdev_infeasibility(const dpdlp_batch_dev_t *b, void *stream);
 int dpdlp_bdev_power_step(const dpdlp_batch_dev_t *b, void *stream);
+
+/* ---- per-LP controller state (the reference's scalar loop state) ---- */
+
+/* The decisions of the reference loop (solver.cu:56-102) read and write
+ * this: step size and primal weight with the PID terms
+ * (pdlp_core_op.cu:593-619), iteration counters, fixed-point errors of the
+ * restart test (utils.cu:265-293), the last evaluation's residuals and
+ * objectives (utils.cu:639-719) and infeasibility rays (utils.cu:721-813),
+ * and the termination reason.  The host controller (batch and single-LP
+ * drivers) and the persistent kernel keep the same struct. */
+typedef struct {
+  double step, omega, best_omega, pw_err_sum, pw_last_err, best_gap;
+  int total, inner, is_major;
+  double fpe, fpe0, fpe_last;
+  double abs_p, rel_p, abs_d, rel_d, pobj, dobj, gap, rel_gap;
+  double max_p_ray_inf, max_d_ray_inf, p_ray_lin_obj, d_ray_obj;
+  termination_reason_t reason;
+  double t_cum;
+  int pend_rst, pend_maj;
+  double m32_best; /* mixed precision: best max relative KKT seen in FP32 */
+  int m32_best_T;  /* iteration at which it was reached */
+} hstate_t;
+
+/* per-LP constants: scaling factors, objective constant, unscaled norms */
+typedef struct {
+  double beta, gamma, c0, obj_norm, bound_norm;
+} lpk_t;
+
+/* ---- persistent solving: one workgroup runs one LP's whole loop ---- */
+
+/* Stages of an LP in the persistent kernel. */
+enum { DPDLP_PS_PI = 0, DPDLP_PS_PDHG = 1, DPDLP_PS_DONE = 2 };
+
+/* One LP of a persistent launch (device memory; the kernel reads and writes
+ * it).  L: the LP's device arrays (untiled A and A', row-block capacity at
+ * most DPDLP_PERSIST_CAP); A_blk/At_blk: its row-block plans (device);
+ * pi_eig/pi_next (in L) hold the power iteration's normalised start vector
+ * (dpdlp_dev_power_init) when stage is DPDLP_PS_PI.  The raw reduction slots
+ * live in L.scal as in the batch.  The loop's time (h.t_cum, checked against
+ * the time limit at every evaluation) is read from the device's constant-rate
+ * wall clock, clk_hz ticks per second. */
+typedef struct {
+  dpdlp_blp_t L;
+  const int *A_blk, *At_blk;
+  int A_nblk, At_nblk;
+  lpk_t k;
+  hstate_t h;
+  double sigma_sq;
+  long long clk0; /* device wall clock when the PDHG loop started */
+  int stage;
+  int pi_iters;
+  int do_restart;
+  int iters_run; /* PDHG iterations and power-iteration steps of the last launch */
+} dpdlp_plp_t;
+
+/* the largest row-block capacity the persistent kernel stages in LDS */
+#define DPDLP_PERSIST_CAP 4096
+
+/* Runs up to `budget` iterations (power-iteration steps and PDHG
+ * iterations together) of each of the `count` LPs in lps (device array;
+ * params: the solve parameters, also in device memory),
+ * one workgroup per LP, with the reference's per-LP decisions (evaluation
+ * cadence, residuals, infeasibility rays, termination, restart test, PID
+ * weight) made on the device: the loop of solver.cu:56-102 after the power
+ * iteration of utils.cu:66-171, FP64 throughout.  An LP whose stage is
+ * DPDLP_PS_DONE is skipped; one that terminates leaves h.reason set and its
+ * stage DPDLP_PS_DONE.  Stream-ordered, no allocation. */
+int dpdlp_pdev_run(dpdlp_plp_t *lps, int count, const pdhg_parameters_t *params,
+                   double clk_hz, int budget, void *stream);
 
 /* Scalar / partial slot map (raw values; the host combines them). */
 enum {

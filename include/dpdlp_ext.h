@@ -176,6 +176,14 @@ typedef struct {
                               their single-LP launches (default 4; 0: never) */
   int spmv_cap_At;         /* row-block capacity of every LP's A' (0: the
                               same as spmv_cap) */
+  int persist_max_nnz;     /* LPs with 1..persist_max_nnz nonzeros run their
+                              whole solve (power iteration and PDHG loop, FP64)
+                              in the persistent kernel, one workgroup each,
+                              beside the windows of the others (0: none; the
+                              default) */
+  int persist_budget;      /* iterations per LP and persistent launch: an LP
+                              that finished retires when its launch ends
+                              (default 250) */
   /* outputs of dpdlp_solve_batch */
   long long windows, syncs, launches, graph_replays;
   int phase_samples;
