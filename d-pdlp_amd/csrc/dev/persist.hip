@@ -7,12 +7,13 @@
  * pi_next), then the PDHG loop of solver.cu:56-102 with the reference's
  * evaluation cadence, residuals (utils.cu:639-719), infeasibility rays
  * (utils.cu:721-813), termination test, restart test and PID primal weight
- * (pdlp_core_op.cu:577-636).  Every SpMV walks the LP's own row-block plan
- * with the same row-block body and fused epilogues as the batched and
- * single-LP kernels (spmv_core.h); the scalar decisions are the host
- * controller's code compiled for the device (control_impl.h).  Reductions
- * combine per-thread partial sums over the whole pass in a fixed order, so an
- * LP's iterates do not depend on which other LPs share the launch.
+ * (pdlp_core_op.cu:577-636).  Every SpMV is a pass of the LP's CSR matrix
+ * with one thread per row, each row summed in storage order, and the same
+ * fused epilogues as the batched and single-LP kernels (spmv_core.h); the
+ * scalar decisions are the host controller's code compiled for the device
+ * (control_impl.h).  Reductions combine per-thread partial sums over the
+ * whole pass in a fixed order, so an LP's iterates do not depend on which
+ * other LPs share the launch.
  *
  * Workgroups never wait on each other, so any number of LPs may share a
  * launch.  Each launch runs at most `budget` iterations per LP and saves the
@@ -27,30 +28,78 @@
 
 namespace {
 
-constexpr int PCAP = DPDLP_PERSIST_CAP;
-
 struct PShared {
-  double prod[PCAP];
-  int srp[DPDLP_BS + 1];
-  double ssum[DPDLP_BS];
   double red[9 * 4];
   /* per-iteration control, written by thread 0 */
-  double tau, sigma, w, rho, neg, inv;
+  double tau, sigma, w, rho;
   int major, fpe, fpe_to, restart, stop, ev_on, ev_rc, ev_inf, ev_final;
   int stage; /* the LP's stage; thread 0 writes it back to the descriptor */
 };
 
-/* one SpMV pass over an LP's row blocks with a fused epilogue; the epilogue's
- * per-thread partial sums accumulate over the whole pass */
+/* LDS copy of the gathered vector when it has at most this many entries */
+constexpr int XS_CAP = 6144;
+
+/* One SpMV pass y = M x over all rows of a CSR matrix (cols columns) with a
+ * fused epilogue epi(row, sum, s, mx).  Thread t owns rows t, t + 256, ...;
+ * it sums each row sequentially in storage order (the oracle's order).  Four
+ * rows are in flight per thread and their entries are read four at a time
+ * (16 index and value loads, then 16 gathers, then the sums).  A gathered
+ * vector of at most XS_CAP entries is first copied to LDS, so the gathers of
+ * small LPs cost no memory round trip.  The epilogue's per-thread partial
+ * sums accumulate over the pass. */
 template <class Epi>
-__device__ __forceinline__ void pass(const int *rp, const int *ci, const double *v,
-                                     const double *x, const int *blk, int nblk, int cap,
-                                     const Epi &epi, double *s, double *mx, PShared &sh) {
-  for (int j = 0; j < nblk; ++j) {
-    spmv_block<double, double, PCAP>(rp, ci, v, x, blk[j], blk[j + 1], epi, s, mx, sh.prod,
-                                     sh.srp, sh.ssum, cap);
-    __syncthreads(); /* LDS is reused by the next block */
+__device__ __forceinline__ void pass(const int *__restrict__ rp, const int *__restrict__ ci,
+                                     const double *__restrict__ v, const double *x, int rows,
+                                     int cols, const Epi &epi, double *s, double *mx,
+                                     double *xs) {
+  constexpr int R4 = 4, K4 = 4;
+  const bool lds = cols <= XS_CAP;
+  if (lds) {
+    for (int i = threadIdx.x; i < cols; i += DPDLP_BS) xs[i] = x[i];
+    __syncthreads();
   }
+  const double *xg = lds ? xs : x;
+  for (int r0 = threadIdx.x; r0 < rows; r0 += R4 * DPDLP_BS) {
+    int a[R4], e[R4];
+    double acc[R4];
+    int len = 0;
+#pragma unroll
+    for (int u = 0; u < R4; ++u) {
+      const int r = r0 + u * DPDLP_BS;
+      a[u] = r < rows ? rp[r] : 0;
+      e[u] = r < rows ? rp[r + 1] : 0;
+      acc[u] = 0.0;
+      len = max(len, e[u] - a[u]);
+    }
+    for (int k = 0; k < len; k += K4) {
+      int c[R4][K4];
+      double w[R4][K4], g[R4][K4];
+#pragma unroll
+      for (int u = 0; u < R4; ++u)
+#pragma unroll
+        for (int j = 0; j < K4; ++j) {
+          const int q = a[u] + k + j;
+          const bool in = q < e[u];
+          c[u][j] = in ? ci[q] : 0;
+          w[u][j] = in ? v[q] : 0.0;
+        }
+#pragma unroll
+      for (int u = 0; u < R4; ++u)
+#pragma unroll
+        for (int j = 0; j < K4; ++j) g[u][j] = a[u] + k + j < e[u] ? xg[c[u][j]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < R4; ++u)
+#pragma unroll
+        for (int j = 0; j < K4; ++j)
+          if (a[u] + k + j < e[u]) acc[u] += w[u][j] * g[u][j];
+    }
+#pragma unroll
+    for (int u = 0; u < R4; ++u) {
+      const int r = r0 + u * DPDLP_BS;
+      if (r < rows) epi(r, acc[u], s, mx);
+    }
+  }
+  __syncthreads(); /* the pass's outputs are read by the next pass */
 }
 
 /* workgroup totals of NS sums and NM maxima into S[slot0 ...] (sums, then
@@ -73,6 +122,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
                                                      const pdhg_parameters_t *__restrict__ prmp,
                                                      double clk_hz, int budget) {
   __shared__ PShared sh;
+  __shared__ double xs[XS_CAP];
   dpdlp_plp_t *P = lps + blockIdx.x;
   if (P->stage == DPDLP_PS_DONE) return;
   const pdhg_parameters_t &prm = *prmp;
@@ -91,13 +141,13 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
     double s[4], mx[1];
     {
       EpiStore e{L.pi_dual, 0};
-      pass(L.At_rp, L.At_ci, L.At_v, L.pi_next, P->At_blk, P->At_nblk, L.cap_At, e, s, mx, sh);
+      pass(L.At_rp, L.At_ci, L.At_v, L.pi_next, n, m, e, s, mx, xs);
     }
     zero(s, 4);
     mx[0] = 0.0;
     {
       EpiPow e{L.pi_eig, L.pi_next, 1};
-      pass(L.A_rp, L.A_ci, L.A_v, L.pi_dual, P->A_blk, P->A_nblk, L.cap_A, e, s, mx, sh);
+      pass(L.A_rp, L.A_ci, L.A_v, L.pi_dual, m, n, e, s, mx, xs);
     }
     wg_total<4, 0>(s, mx, sh, S, DPDLP_SL_PI);
     /* residual AA'q - sigma^2 q, then q <- AA'q / |AA'q| for the next step */
@@ -158,7 +208,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
         EpiResP e;
         e.pprod = L.primal_product; e.lc = L.lc; e.uc = L.uc; e.D_r = L.D_r; e.y = L.y_pdhg;
         e.y_init = L.y_init; e.dist = dist; e.partials = 1;
-        pass(L.A_rp, L.A_ci, L.A_v, L.x_pdhg, P->A_blk, P->A_nblk, L.cap_A, e, s, mx, sh);
+        pass(L.A_rp, L.A_ci, L.A_v, L.x_pdhg, m, n, e, s, mx, xs);
         wg_total<7, 1>(s, mx, sh, S, DPDLP_SL_RESP);
       }
       {
@@ -168,7 +218,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
         EpiResD e;
         e.dprod = L.dual_product; e.c = L.c; e.z = L.dual_slack; e.D_c = L.D_c;
         e.x_obj = L.x_pdhg; e.x_init = L.x_init; e.dist = dist; e.partials = 1;
-        pass(L.At_rp, L.At_ci, L.At_v, L.y_pdhg, P->At_blk, P->At_nblk, L.cap_At, e, s, mx, sh);
+        pass(L.At_rp, L.At_ci, L.At_v, L.y_pdhg, n, m, e, s, mx, xs);
         wg_total<8, 1>(s, mx, sh, S, DPDLP_SL_RESD);
       }
       if (sh.ev_inf) {
@@ -202,7 +252,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
           EpiInfP e;
           e.pprod = L.primal_product; e.pslack = L.primal_slack; e.lc = L.lc; e.uc = L.uc;
           e.D_r = L.D_r; e.dy = L.dy; e.partials = 1;
-          pass(L.A_rp, L.A_ci, L.A_v, L.dx, P->A_blk, P->A_nblk, L.cap_A, e, s, mx, sh);
+          pass(L.A_rp, L.A_ci, L.A_v, L.dx, m, n, e, s, mx, xs);
           wg_total<1, 1>(s, mx, sh, S, DPDLP_SL_INF + 2);
         }
         {
@@ -210,7 +260,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
           EpiInfD e;
           e.dprod = L.dual_product; e.dslack = L.dual_slack; e.lv = L.lv; e.uv = L.uv;
           e.D_c = L.D_c; e.c = L.c; e.dx = L.dx; e.partials = 1;
-          pass(L.At_rp, L.At_ci, L.At_v, L.dy, P->At_blk, P->At_nblk, L.cap_At, e, s, mx, sh);
+          pass(L.At_rp, L.At_ci, L.At_v, L.dy, n, m, e, s, mx, xs);
           wg_total<2, 1>(s, mx, sh, S, DPDLP_SL_INF + 4);
         }
       }
@@ -294,7 +344,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
       e.x_ref32 = nullptr; e.aty0 = nullptr; e.delta32 = 0;
       e.tau = sh.tau; e.w = sh.w; e.rho = sh.rho; e.major = sh.major; e.fpe = sh.fpe;
       e.partials = sh.fpe;
-      pass(L.At_rp, L.At_ci, L.At_v, L.y_cur, P->At_blk, P->At_nblk, L.cap_At, e, s, mx, sh);
+      pass(L.At_rp, L.At_ci, L.At_v, L.y_cur, n, m, e, s, mx, xs);
       if (sh.fpe) wg_total<3, 0>(s, mx, sh, S, DPDLP_SL_FPE);
     }
     {
@@ -304,14 +354,14 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
       e.dy = L.dy; e.y_cur32 = nullptr; e.ax0 = nullptr; e.delta32 = 0;
       e.sigma = sh.sigma; e.w = sh.w; e.rho = sh.rho; e.major = sh.major; e.fpe = sh.fpe;
       e.partials = sh.fpe;
-      pass(L.A_rp, L.A_ci, L.A_v, L.x_ref, P->A_blk, P->A_nblk, L.cap_A, e, s, mx, sh);
+      pass(L.A_rp, L.A_ci, L.A_v, L.x_ref, m, n, e, s, mx, xs);
       if (sh.fpe) wg_total<3, 0>(s, mx, sh, S, DPDLP_SL_FPE + 3);
     }
     if (sh.fpe) {
       /* fixed-point error finish (pdlp_core_op.cu:666-692): A'dy and <A'dy, dx> */
       double s[1] = {0.0}, mx[1] = {0.0};
       EpiFpe e{L.dual_product, L.dx, 1};
-      pass(L.At_rp, L.At_ci, L.At_v, L.dy, P->At_blk, P->At_nblk, L.cap_At, e, s, mx, sh);
+      pass(L.At_rp, L.At_ci, L.At_v, L.dy, n, m, e, s, mx, xs);
       wg_total<1, 0>(s, mx, sh, S, DPDLP_SL_FPE + 6);
       if (tid < 7) {
         const double v = S[DPDLP_SL_FPE + tid];
