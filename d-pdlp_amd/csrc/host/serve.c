@@ -51,6 +51,7 @@ void dpdlp_serve_default_spec(dpdlp_serve_spec_t *s) {
   s->mode = 0;
   s->spmv_cap = 4096;
   s->large_lane_nnz = 4000000;
+  s->persist_max_nnz = 0;
 }
 
 /* SplitMix64 stream for arrival gaps and request draws */
@@ -455,6 +456,7 @@ int dpdlp_serve_run(const dpdlp_serve_spec_t *spec, const pdhg_parameters_t *par
     bo.max_lps = sp.max_lps;
     bo.precision = sp.precision;
     bo.spmv_cap = sp.spmv_cap;
+    bo.persist_max_nnz = sp.persist_max_nnz;
     b = dpdlp_batch_create(&prm, &bo);
     if (!b) goto out;
     S.spec = &sp;

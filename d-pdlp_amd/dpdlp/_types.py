@@ -144,7 +144,8 @@ class ServeSpec(C.Structure):
                 ("large_frac", C.c_double), ("large_m", C.c_int), ("large_nnz", C.c_longlong),
                 ("m_lo", C.c_int), ("m_hi", C.c_int), ("member_seed", C.c_uint64),
                 ("hbm_budget_gb", C.c_double), ("max_lps", C.c_int), ("precision", C.c_int),
-                ("mode", C.c_int), ("spmv_cap", C.c_int), ("large_lane_nnz", C.c_longlong)]
+                ("mode", C.c_int), ("spmv_cap", C.c_int), ("large_lane_nnz", C.c_longlong),
+                ("persist_max_nnz", C.c_int)]
 
 
 class ServeReport(C.Structure):

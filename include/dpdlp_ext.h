@@ -290,6 +290,9 @@ typedef struct {
   long long large_lane_nnz; /* requests with at least this many nonzeros are
                                solved by the single-LP path on a second
                                thread and stream, beside the batch (0: off) */
+  int persist_max_nnz;  /* requests with at most this many nonzeros run whole
+                           in the batch's persistent kernel (the batch option
+                           of the same name; 0: off, the default) */
 } dpdlp_serve_spec_t;
 
 typedef struct {

@@ -118,3 +118,30 @@ def test_persistent_polish_and_mixed_precision_batch(dp, orc):
                            precision=dp.PREC_MIXED)
     for k, p in enumerate(probs):
         check_oracle(orc, p, mixed[k])
+
+
+def test_served_small_requests_run_persistent(dp, orc):
+    """Serving with a persistent cut: requests at or below it run whole on the
+    device beside the windowed batch; every served result equals the same LP
+    solved alone with the same cut, and the persistent ones match the oracle."""
+    pool = dict(pool_small=6, pool_large=0, large_frac=0.0, m_lo=300, m_hi=3000,
+                spmv_cap=1024, member_seed=3600)
+    params = dp.default_params(presolve=False)
+    cut = 9000
+    probs = [dp.batch_member(k, pool["member_seed"], pool["m_lo"], pool["m_hi"])[0]
+             for k in range(pool["pool_small"])]
+    ref = [dp.solve_batch([dp.Resident(p, spmv_cap=1024, panels=(1, 1))], params,
+                          spmv_cap=1024, persist_max_nnz=cut)[0] for p in probs]
+    rep, lat, idx, res = dp.serve(params, want_results=True, rate=300.0, duration_sec=0.2,
+                                  drain_sec=60.0, seed=11, max_lps=5, precision=0,
+                                  persist_max_nnz=cut, **pool)
+    assert rep["arrived"] > 20 and rep["completed"] == rep["arrived"]
+    assert rep["optimal"] == rep["arrived"] and rep["max_in_flight"] <= 5
+    kinds = set()
+    for k in range(rep["arrived"]):
+        assert same(res[k], ref[idx[k]]), (k, idx[k])
+        kinds.add(probs[idx[k]].nnz <= cut)
+    assert kinds == {True, False}
+    for k, p in enumerate(probs):
+        if p.nnz <= cut:
+            check_oracle(orc, p, ref[k])
