@@ -241,13 +241,16 @@ def bench_serve(args, dpdlp):
     for rate in rates:
         row = {"rate": rate}
         for mode, name in ((0, "batched"), (1, "plain_loop")):
+            if mode == 1 and args.no_plain:
+                continue
             t = time.perf_counter()
             rep, lat, idx, _ = dpdlp.serve(
                 params, rate=rate, duration_sec=args.serve_duration, drain_sec=args.serve_drain,
                 seed=4000, mode=mode, precision=dpdlp.PREC_MIXED if mode == 0 else dpdlp.PREC_FP64,
                 large_frac=w["large_frac"], large_m=w["large_m"], large_nnz=w["large_nnz"],
                 pool_small=w["pool_small"], pool_large=w["pool_large"],
-                hbm_budget_gb=w["budget_gb"], max_lps=args.serve_max_lps)
+                hbm_budget_gb=w["budget_gb"], max_lps=args.serve_max_lps,
+                persist_max_nnz=args.persist_max_nnz if mode == 0 else 0)
             rep["wall_s"] = time.perf_counter() - t
             # per request class (pool entries past the small pool are the
             # m = 1e6 requests): arrival -> result latency, inf = unfinished
@@ -273,10 +276,10 @@ def bench_serve(args, dpdlp):
         "dtype": "f32-matrix/f64", "data": "synthetic (seeded generator, feasible by construction)",
         "config": {"workload": f"serve-poisson: {w['desc']}", "rates": rates,
                    "arrival_window_s": args.serve_duration, "drain_cap_s": args.serve_drain,
-                   "max_lps": args.serve_max_lps},
+                   "max_lps": args.serve_max_lps, "persist_max_nnz": args.persist_max_nnz},
         "per_rate": per,
         "speedup_vs_plain_loop_at_top_rate": top["batched"]["lps_per_sec"] /
-        max(top["plain_loop"]["lps_per_sec"], 1e-9),
+        max(top["plain_loop"]["lps_per_sec"], 1e-9) if "plain_loop" in top else None,
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(clean(out)))
@@ -425,6 +428,9 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=60.0, help="serving: arrival window s")
     ap.add_argument("--serve-drain", type=float, default=30.0, help="serving: drain cap s")
     ap.add_argument("--serve-max-lps", type=int, default=512, help="serving: LPs in flight")
+    ap.add_argument("--persist-max-nnz", type=int, default=0,
+                    help="batch and serving: LPs with at most this many nonzeros run whole in "
+                         "the persistent kernel (0: none)")
     ap.add_argument("--compare-precision", type=float, default=0.0, metavar="EPS",
                     help="solve to EPS in FP64 and in mixed precision and report both")
     args = ap.parse_args()

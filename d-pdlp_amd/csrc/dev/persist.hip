@@ -29,6 +29,10 @@
 namespace {
 
 struct PShared {
+  hstate_t h; /* the LP's controller state during the launch (thread 0) */
+  double sigma_sq;
+  long long clk0;
+  int pi_iters, do_restart;
   double red[9 * 4];
   /* per-iteration control, written by thread 0 */
   double tau, sigma, w, rho;
@@ -63,14 +67,18 @@ __device__ __forceinline__ void pass(const int *__restrict__ rp, const int *__re
     int a[R4], e[R4];
     double acc[R4];
     int len = 0;
+    typename EpiPre<Epi>::T pre[R4];
 #pragma unroll
     for (int u = 0; u < R4; ++u) {
       const int r = r0 + u * DPDLP_BS;
       a[u] = r < rows ? rp[r] : 0;
       e[u] = r < rows ? rp[r + 1] : 0;
       acc[u] = 0.0;
-      len = max(len, e[u] - a[u]);
+      /* the row's epilogue operands that do not depend on its sum */
+      if (r < rows) pre[u] = EpiPre<Epi>::load(epi, r);
     }
+#pragma unroll
+    for (int u = 0; u < R4; ++u) len = max(len, e[u] - a[u]);
     for (int k = 0; k < len; k += K4) {
       int c[R4][K4];
       double w[R4][K4], g[R4][K4];
@@ -96,7 +104,7 @@ __device__ __forceinline__ void pass(const int *__restrict__ rp, const int *__re
 #pragma unroll
     for (int u = 0; u < R4; ++u) {
       const int r = r0 + u * DPDLP_BS;
-      if (r < rows) epi(r, acc[u], s, mx);
+      if (r < rows) EpiPre<Epi>::apply(epi, r, acc[u], pre[u], s, mx);
     }
   }
   __syncthreads(); /* the pass's outputs are read by the next pass */
@@ -126,7 +134,14 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
   dpdlp_plp_t *P = lps + blockIdx.x;
   if (P->stage == DPDLP_PS_DONE) return;
   const pdhg_parameters_t &prm = *prmp;
-  if (threadIdx.x == 0) sh.stage = P->stage;
+  if (threadIdx.x == 0) {
+    sh.stage = P->stage;
+    sh.h = P->h;
+    sh.sigma_sq = P->sigma_sq;
+    sh.clk0 = P->clk0;
+    sh.pi_iters = P->pi_iters;
+    sh.do_restart = P->do_restart;
+  }
   __syncthreads();
   const dpdlp_blp_t &L = P->L;
   const int tid = threadIdx.x;
@@ -163,13 +178,13 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
     }
     wg_total<3, 0>(r3, rm, sh, S, DPDLP_SL_PI + 4);
     if (tid == 0) {
-      P->sigma_sq = S[DPDLP_SL_PI];
-      P->pi_iters++;
-      if (dctl_blue(S + DPDLP_SL_PI + 4) < prm.sv_tol || P->pi_iters >= prm.sv_max_iter) {
-        double step = 0.998 / sqrt(P->sigma_sq);
-        dctl_init_state(&P->h, &P->k, &prm, step);
-        P->do_restart = 0;
-        P->clk0 = wall_clock64();
+      sh.sigma_sq = S[DPDLP_SL_PI];
+      sh.pi_iters++;
+      if (dctl_blue(S + DPDLP_SL_PI + 4) < prm.sv_tol || sh.pi_iters >= prm.sv_max_iter) {
+        double step = 0.998 / sqrt(sh.sigma_sq);
+        dctl_init_state(&sh.h, &P->k, &prm, step);
+        sh.do_restart = 0;
+        sh.clk0 = wall_clock64();
         sh.stage = DPDLP_PS_PDHG;
       }
     }
@@ -179,7 +194,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
 
   /* ---- PDHG loop (solver.cu:56-102) ---- */
   while (sh.stage == DPDLP_PS_PDHG && used < budget) {
-    hstate_t *h = &P->h;
+    hstate_t *h = &sh.h;
     /* evaluation point at the top of iteration T = h->total */
     if (tid == 0) {
       int on = 0, rc = 0, inf = 0, fin = 0;
@@ -274,13 +289,13 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
           stop = 1;
         } else {
           if (sh.ev_inf) dctl_infeasibility(h, &P->k, S);
-          h->t_cum = (double)(wall_clock64() - P->clk0) / clk_hz;
+          h->t_cum = (double)(wall_clock64() - sh.clk0) / clk_hz;
           dctl_check_termination(h, tc);
           if (h->reason != TERMINATION_REASON_UNSPECIFIED) {
             stop = 1;
           } else if (sh.ev_rc) {
-            P->do_restart = dctl_should_restart(h, &prm.restart_params, freq);
-            if (P->do_restart) {
+            sh.do_restart = dctl_should_restart(h, &prm.restart_params, freq);
+            if (sh.do_restart) {
               dctl_pid_update(h, &prm.restart_params, dctl_blue(S + DPDLP_SL_RESD + 5),
                               dctl_blue(S + DPDLP_SL_RESP + 4));
               h->inner = 0;
@@ -298,7 +313,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
     /* one iteration (solver.cu:85-102): this iteration's parameters */
     if (tid == 0) {
       const int is_major = ((h->total + 1) % freq) == 0;
-      const int dr = P->do_restart;
+      const int dr = sh.do_restart;
       const int major_variant =
           is_major || ((h->total + 2) % dctl_print_frequency(h->total + 2)) == 0;
       const int fpe = is_major || dr;
@@ -317,7 +332,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
       h->is_major = is_major;
       h->inner++;
       h->total++;
-      P->do_restart = 0;
+      sh.do_restart = 0;
     }
     __syncthreads();
     if (sh.restart) {
@@ -374,6 +389,11 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
   }
   if (tid == 0) {
     P->stage = sh.stage;
+    P->h = sh.h;
+    P->sigma_sq = sh.sigma_sq;
+    P->clk0 = sh.clk0;
+    P->pi_iters = sh.pi_iters;
+    P->do_restart = sh.do_restart;
     P->iters_run = used;
   }
 }

@@ -1112,6 +1112,9 @@ static int finish(dpdlp_batch_t *b, int i) {
     const dpdlp_resident_t *R = q->R;
     b->zombie_items += R->A_nblk + R->At_nblk + dpdlp_ew_grid(R->n) + dpdlp_ew_grid(R->m) +
                        dpdlp_ew_grid((long long)R->n + R->m);
+  } else {
+    /* no list items, but the slot is reclaimed by the same rebuild */
+    b->zombie_items += 1;
   }
   q->c.arena = NULL;
   q->c.s = NULL;
