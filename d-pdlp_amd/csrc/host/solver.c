@@ -503,7 +503,8 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
       return NULL;
     }
   dpdlp_resident_t *r = (dpdlp_resident_t *)calloc(1, sizeof(*r));
-  int *t_rp = NULL, *blk = NULL;
+  int *t_rp = NULL, *blk = NULL, *t_ci = NULL;
+  double *t_v = NULL;
   if (!r) return NULL;
   r->m = m;
   r->n = n;
@@ -634,22 +635,35 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   /* A' is transposed on the device after the upload (the host only counts
    * columns for its row pointers); its column-panel tiles, when it has
    * them, are planned from A's host CSR and cut on the device from the
-   * device transpose */
+   * device transpose.  Uploads through an uploader (serving: the device is
+   * busy solving other LPs, and the device transpose and tile cutting
+   * delay their launches) transpose and cut tiled matrices on the host
+   * instead. */
+  const int host_At = u != NULL && r->At_P > 1, host_A = u != NULL && r->A_P > 1;
   t_rp = (int *)malloc((size_t)(n + 1) * sizeof(int));
   blk = (int *)malloc((size_t)((m > n ? m : n) + 2) * sizeof(int));
-  if (!t_rp || !blk) {
+  if (host_At) {
+    t_ci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+    t_v = (double *)malloc((size_t)(nnz ? nnz : 1) * sizeof(double));
+  }
+  if (!t_rp || !blk || (host_At && (!t_ci || !t_v))) {
     dpdlp_set_error("host allocation failed");
     goto ufail;
   }
-  dpdlp_host_col_ptr(m, n, rp_u, ci_u, t_rp);
+  if (host_At) {
+    if (dpdlp_host_transpose(m, n, rp_u, ci_u, v_u, t_rp, t_ci, t_v)) goto ufail;
+  } else {
+    dpdlp_host_col_ptr(m, n, rp_u, ci_u, t_rp);
+  }
   /* row-block plans (host copies kept for the batch's work lists) */
   for (int which = 0; which < 2; ++which) {
     int P = which ? r->At_P : r->A_P, rows = which ? n : m;
     const int *hrp = which ? t_rp : rp_u;
     int cap = which ? r->At_cap : r->cap;
     int nb = P <= 1 ? dpdlp_plan_blocks_rows(rows, hrp, cap, which ? r->rows_At : r->rows_A, blk)
-             : which ? dpdlp_plan_blocks_panels_T(m, n, rp_u, ci_u, cap, P, DPDLP_PANEL_RMAX, blk)
-                     : dpdlp_plan_blocks_panels(m, n, rp_u, ci_u, cap, P, DPDLP_PANEL_RMAX, blk);
+             : !which ? dpdlp_plan_blocks_panels(m, n, rp_u, ci_u, cap, P, DPDLP_PANEL_RMAX, blk)
+             : host_At ? dpdlp_plan_blocks_panels(n, m, t_rp, t_ci, cap, P, DPDLP_PANEL_RMAX, blk)
+                       : dpdlp_plan_blocks_panels_T(m, n, rp_u, ci_u, cap, P, DPDLP_PANEL_RMAX, blk);
     if (nb < 0) {
       dpdlp_set_error("host allocation failed (panel plan)");
       goto ufail;
@@ -692,8 +706,8 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   UPCHK(up(&K, (void **)&r->A_ci, ci_u, (size_t)nnz * sizeof(int)));
   UPCHK(up(&K, (void **)&r->A_v, v_u, (size_t)nnz * sizeof(double)));
   UPCHK(up(&K, (void **)&r->At_rp, t_rp, (size_t)(n + 1) * sizeof(int)));
-  UPCHK(up(&K, (void **)&r->At_ci, NULL, (size_t)nnz * sizeof(int)));
-  UPCHK(up(&K, (void **)&r->At_v, NULL, (size_t)nnz * sizeof(double)));
+  UPCHK(up(&K, (void **)&r->At_ci, t_ci, (size_t)nnz * sizeof(int)));
+  UPCHK(up(&K, (void **)&r->At_v, t_v, (size_t)nnz * sizeof(double)));
   UPCHK(up(&K, (void **)&r->A_blk, r->h_A_blk, (size_t)(r->A_nblk + 1) * sizeof(int)));
   UPCHK(up(&K, (void **)&r->At_blk, r->h_At_blk, (size_t)(r->At_nblk + 1) * sizeof(int)));
 
@@ -703,12 +717,43 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
     size_t P = (size_t)(which ? r->At_P : r->A_P), nb = (size_t)(which ? r->At_nblk : r->A_nblk);
     size_t rows = (size_t)(which ? n : m);
     if (P <= 1) continue;
-    UPCHK(up(&K, (void **)(which ? &r->At_toff : &r->A_toff), NULL, (nb * P + 1) * sizeof(int)));
-    UPCHK(up(&K, (void **)(which ? &r->At_rofs : &r->A_rofs), NULL,
-             P * (rows + nb) * sizeof(unsigned short)));
-    UPCHK(up(&K, (void **)(which ? &r->At_perm : &r->A_perm), NULL, (size_t)nnz * sizeof(int)));
-    UPCHK(up(&K, (void **)(which ? &r->At_pci : &r->A_pci), NULL, (size_t)nnz * sizeof(int)));
-    UPCHK(up(&K, (void **)(which ? &r->At_pv : &r->A_pv), NULL, (size_t)nnz * sizeof(double)));
+    int *toff = NULL, *perm = NULL, *pci = NULL;
+    unsigned short *rofs = NULL;
+    if (which ? host_At : host_A) {
+      /* cut on the host (dpdlp_build_panels) and copied with the rest */
+      toff = (int *)malloc((nb * P + 1) * sizeof(int));
+      rofs = (unsigned short *)malloc(P * (rows + nb) * sizeof(unsigned short));
+      perm = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+      pci = (int *)malloc((size_t)(nnz ? nnz : 1) * sizeof(int));
+      int ok = toff && rofs && perm && pci &&
+               dpdlp_build_panels((int)rows, which ? m : n, which ? t_rp : rp_u,
+                                  which ? t_ci : ci_u, which ? r->h_At_blk : r->h_A_blk,
+                                  (int)nb, (int)P, toff, rofs, perm, pci) == 0;
+      if (!ok) {
+        free(toff);
+        free(rofs);
+        free(perm);
+        free(pci);
+        dpdlp_set_error("building column-panel tiles failed");
+        goto ufail;
+      }
+    }
+    hipError_t e =
+        up(&K, (void **)(which ? &r->At_toff : &r->A_toff), toff, (nb * P + 1) * sizeof(int));
+    if (e == hipSuccess)
+      e = up(&K, (void **)(which ? &r->At_rofs : &r->A_rofs), rofs,
+             P * (rows + nb) * sizeof(unsigned short));
+    if (e == hipSuccess)
+      e = up(&K, (void **)(which ? &r->At_perm : &r->A_perm), perm, (size_t)nnz * sizeof(int));
+    if (e == hipSuccess)
+      e = up(&K, (void **)(which ? &r->At_pci : &r->A_pci), pci, (size_t)nnz * sizeof(int));
+    if (e == hipSuccess)
+      e = up(&K, (void **)(which ? &r->At_pv : &r->A_pv), NULL, (size_t)nnz * sizeof(double));
+    free(toff);
+    free(rofs);
+    free(perm);
+    free(pci);
+    UPCHK(e);
   }
   UPCHK(up(&K, (void **)&r->c, p->objective_vector, (size_t)n * sizeof(double)));
   UPCHK(up(&K, (void **)&r->lv, p->variable_lower_bound, (size_t)n * sizeof(double)));
@@ -717,11 +762,17 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
   UPCHK(up(&K, (void **)&r->uc, uc_u, (size_t)m * sizeof(double)));
   UPCHK(sink_close(&K));
   if (r->A_P > 1) {
-    UPCHK((hipError_t)dpdlp_dev_build_panels(m, n, r->A_rp, r->A_ci, r->A_blk, r->A_nblk, r->A_P,
-                                             r->A_toff, r->A_rofs, r->A_perm, r->A_pci, K.s));
+    if (!host_A)
+      UPCHK((hipError_t)dpdlp_dev_build_panels(m, n, r->A_rp, r->A_ci, r->A_blk, r->A_nblk,
+                                               r->A_P, r->A_toff, r->A_rofs, r->A_perm, r->A_pci,
+                                               K.s));
     if (nnz > 0) UPCHK((hipError_t)dpdlp_dev_permute(r->A_v, r->A_perm, r->A_pv, nnz, K.s));
   }
-  if (nnz == 0) {
+  if (host_At) {
+    /* A' and its tiles came from the host: only the tiled values are gathered */
+    if (nnz > 0) UPCHK((hipError_t)dpdlp_dev_permute(r->At_v, r->At_perm, r->At_pv, nnz, K.s));
+    UPCHK(hipStreamSynchronize(K.s));
+  } else if (nnz == 0) {
     if (r->At_P > 1)
       UPCHK((hipError_t)dpdlp_dev_build_panels(n, m, r->At_rp, r->At_ci, r->At_blk, r->At_nblk,
                                                r->At_P, r->At_toff, r->At_rofs, r->At_perm,
@@ -754,12 +805,16 @@ static dpdlp_resident_t *upload_with(const lp_problem_t *p, int spmv_cap, int ca
     UPCHK(e);
   }
   free(t_rp);
+  free(t_ci);
+  free(t_v);
   free(blk);
   free_perm_copies(r);
   r->upload_time = dpdlp_wall() - t0;
   return r;
 ufail:
   free(t_rp);
+  free(t_ci);
+  free(t_v);
   free(blk);
   dpdlp_resident_free(r);
   return NULL;
