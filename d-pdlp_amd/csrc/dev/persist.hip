@@ -7,9 +7,10 @@
  * pi_next), then the PDHG loop of solver.cu:56-102 with the reference's
  * evaluation cadence, residuals (utils.cu:639-719), infeasibility rays
  * (utils.cu:721-813), termination test, restart test and PID primal weight
- * (pdlp_core_op.cu:577-636).  Every SpMV is a pass of the LP's CSR matrix
- * with one thread per row, each row summed in storage order, and the same
- * fused epilogues as the batched and single-LP kernels (spmv_core.h); the
+ * (pdlp_core_op.cu:577-636).  Every SpMV is a pass of the LP's CSR matrix in
+ * which each wave streams its quarter of the entries and each lane sums a
+ * row in storage order, with the same fused epilogues as the batched and
+ * single-LP kernels (spmv_core.h); the
  * scalar decisions are the host controller's code compiled for the device
  * (control_impl.h).  Reductions combine per-thread partial sums over the
  * whole pass in a fixed order, so an LP's iterates do not depend on which
@@ -38,74 +39,132 @@ struct PShared {
   double tau, sigma, w, rho;
   int major, fpe, fpe_to, restart, stop, ev_on, ev_rc, ev_inf, ev_final;
   int stage; /* the LP's stage; thread 0 writes it back to the descriptor */
+  int cutA[5], cutAt[5]; /* each wave's rows of A and of A' */
 };
 
 /* LDS copy of the gathered vector when it has at most this many entries */
 constexpr int XS_CAP = 6144;
+/* a wave's block: at most this many entries (staged in LDS) and rows */
+constexpr int WB_NZ = 512, WB_ROWS = 128;
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+/* Rows [0, rows) of a CSR matrix cut into 4 ranges of about equal nonzeros,
+ * one per wave: cut[w] = first row whose start offset reaches w * nnz / 4. */
+__device__ __forceinline__ void wave_ranges(const int *__restrict__ rp, int rows, int *cut) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    cut[0] = 0;
+    cut[4] = rows;
+  } else if (t < 4) {
+    const long long nnz = rp[rows];
+    const long long target = nnz * t / 4;
+    int lo = 0, hi = rows;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (rp[mid] < target) lo = mid + 1;
+      else hi = mid;
+    }
+    cut[t] = lo;
+  }
+}
 
 /* One SpMV pass y = M x over all rows of a CSR matrix (cols columns) with a
- * fused epilogue epi(row, sum, s, mx).  Thread t owns rows t, t + 256, ...;
- * it sums each row sequentially in storage order (the oracle's order).  Four
- * rows are in flight per thread and their entries are read four at a time
- * (16 index and value loads, then 16 gathers, then the sums).  A gathered
- * vector of at most XS_CAP entries is first copied to LDS, so the gathers of
- * small LPs cost no memory round trip.  The epilogue's per-thread partial
- * sums accumulate over the pass. */
+ * fused epilogue epi(row, sum, s, mx).  Wave w walks rows [cut[w], cut[w+1])
+ * in blocks of at most WB_ROWS rows and WB_NZ entries: it reads the block's
+ * entries coalesced, gathers (from an LDS copy of x when x has at most
+ * XS_CAP entries), stages the products in its own LDS slice, and each lane
+ * then sums one row sequentially in storage order (the oracle's order) and
+ * applies the epilogue, whose operands it loaded with the row pointers.  A
+ * row longer than WB_NZ is summed by the whole wave (lane-strided, then a
+ * shuffle tree).  Waves never wait on each other inside the pass; the
+ * epilogue's per-thread partial sums accumulate over the pass. */
 template <class Epi>
 __device__ __forceinline__ void pass(const int *__restrict__ rp, const int *__restrict__ ci,
                                      const double *__restrict__ v, const double *x, int rows,
                                      int cols, const Epi &epi, double *s, double *mx,
-                                     double *xs) {
-  constexpr int R4 = 4, K4 = 4;
+                                     double *xs, double *wprod, const int *cut) {
+  using PF = EpiPre<Epi>;
   const bool lds = cols <= XS_CAP;
   if (lds) {
     for (int i = threadIdx.x; i < cols; i += DPDLP_BS) xs[i] = x[i];
     __syncthreads();
   }
   const double *xg = lds ? xs : x;
-  for (int r0 = threadIdx.x; r0 < rows; r0 += R4 * DPDLP_BS) {
-    int a[R4], e[R4];
-    double acc[R4];
-    int len = 0;
-    typename EpiPre<Epi>::T pre[R4];
+  const int lane = threadIdx.x & (DPDLP_WAVE - 1), wid = threadIdx.x >> 6;
+  double *prod = wprod + wid * WB_NZ;
+  int r = cut[wid];
+  const int rend = cut[wid + 1];
+  while (r < rend) {
+    /* row pointers of up to WB_ROWS rows: lane l holds rp[r + l] and
+     * rp[r + 64 + l], every lane rp[r + 128] */
+    const int ra = r + lane, rb = r + DPDLP_WAVE + lane, rc = r + WB_ROWS;
+    const int pa = ra <= rend ? rp[ra] : 0;
+    const int pb = rb <= rend ? rp[rb] : 0;
+    const int pc = rc <= rend ? rp[rc] : 0;
+    const int p0 = __shfl(pa, 0, DPDLP_WAVE);
+    /* rows that end within WB_NZ entries of p0 form a prefix of the block */
+    const bool fa = lane >= 1 && ra <= rend && pa - p0 <= WB_NZ;
+    const bool fb = rb <= rend && pb - p0 <= WB_NZ;
+    const int k = __popcll(__ballot(fa)) + __popcll(__ballot(fb)) +
+                  ((rc <= rend && pc - p0 <= WB_NZ) ? 1 : 0);
+    if (k == 0) {
+      /* one row longer than WB_NZ: the whole wave */
+      const int e1 = __shfl(pa, 1, DPDLP_WAVE);
+      double acc = 0.0;
+      for (int q = p0 + lane; q < e1; q += DPDLP_WAVE) acc += v[q] * xg[ci[q]];
+      acc = wave_sum(acc);
+      if (lane == 0) epi(r, acc, s, mx);
+      r += 1;
+      continue;
+    }
+    /* every lane takes part in every shuffle (no shuffle under a
+     * lane-dependent branch); the selections follow */
+    const int nxa = __shfl(pa, (lane + 1) & (DPDLP_WAVE - 1), DPDLP_WAVE);
+    const int nxb = __shfl(pb, (lane + 1) & (DPDLP_WAVE - 1), DPDLP_WAVE);
+    const int pka = __shfl(pa, k & (DPDLP_WAVE - 1), DPDLP_WAVE);
+    const int pkb = __shfl(pb, k & (DPDLP_WAVE - 1), DPDLP_WAVE);
+    const int pb0 = __shfl(pb, 0, DPDLP_WAVE);
+    const int p1 = k < DPDLP_WAVE ? pka : k < WB_ROWS ? pkb : pc;
+    /* this lane's rows j = lane and j = 64 + lane: bounds and epilogue operands */
+    const int sa = pa, ea = lane + 1 < DPDLP_WAVE ? nxa : pb0;
+    const int sb = pb, eb = lane + 1 < DPDLP_WAVE ? nxb : pc;
+    const bool ja = lane < k, jb = DPDLP_WAVE + lane < k;
+    typename PF::T qa, qb;
+    if (ja) qa = PF::load(epi, r + lane);
+    if (jb) qb = PF::load(epi, r + DPDLP_WAVE + lane);
+    /* stage the block's products, entries read coalesced */
+    const int nz = p1 - p0;
+    int cc[WB_NZ / DPDLP_WAVE];
+    double vv[WB_NZ / DPDLP_WAVE];
 #pragma unroll
-    for (int u = 0; u < R4; ++u) {
-      const int r = r0 + u * DPDLP_BS;
-      a[u] = r < rows ? rp[r] : 0;
-      e[u] = r < rows ? rp[r + 1] : 0;
-      acc[u] = 0.0;
-      /* the row's epilogue operands that do not depend on its sum */
-      if (r < rows) pre[u] = EpiPre<Epi>::load(epi, r);
+    for (int u = 0; u < WB_NZ / DPDLP_WAVE; ++u) {
+      const int q = lane + u * DPDLP_WAVE;
+      cc[u] = q < nz ? ci[p0 + q] : 0;
+      vv[u] = q < nz ? v[p0 + q] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < R4; ++u) len = max(len, e[u] - a[u]);
-    for (int k = 0; k < len; k += K4) {
-      int c[R4][K4];
-      double w[R4][K4], g[R4][K4];
-#pragma unroll
-      for (int u = 0; u < R4; ++u)
-#pragma unroll
-        for (int j = 0; j < K4; ++j) {
-          const int q = a[u] + k + j;
-          const bool in = q < e[u];
-          c[u][j] = in ? ci[q] : 0;
-          w[u][j] = in ? v[q] : 0.0;
-        }
-#pragma unroll
-      for (int u = 0; u < R4; ++u)
-#pragma unroll
-        for (int j = 0; j < K4; ++j) g[u][j] = a[u] + k + j < e[u] ? xg[c[u][j]] : 0.0;
-#pragma unroll
-      for (int u = 0; u < R4; ++u)
-#pragma unroll
-        for (int j = 0; j < K4; ++j)
-          if (a[u] + k + j < e[u]) acc[u] += w[u][j] * g[u][j];
+    for (int u = 0; u < WB_NZ / DPDLP_WAVE; ++u) {
+      const int q = lane + u * DPDLP_WAVE;
+      if (q < nz) prod[q] = vv[u] * xg[cc[u]];
     }
-#pragma unroll
-    for (int u = 0; u < R4; ++u) {
-      const int r = r0 + u * DPDLP_BS;
-      if (r < rows) EpiPre<Epi>::apply(epi, r, acc[u], pre[u], s, mx);
+    wave_sync();
+    if (ja) {
+      double acc = 0.0;
+      for (int q = sa - p0; q < ea - p0; ++q) acc += prod[q];
+      PF::apply(epi, r + lane, acc, qa, s, mx);
     }
+    if (jb) {
+      double acc = 0.0;
+      for (int q = sb - p0; q < eb - p0; ++q) acc += prod[q];
+      PF::apply(epi, r + DPDLP_WAVE + lane, acc, qb, s, mx);
+    }
+    wave_sync(); /* the slice is refilled by the next block */
+    r += k;
   }
   __syncthreads(); /* the pass's outputs are read by the next pass */
 }
@@ -131,6 +190,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
                                                      double clk_hz, int budget) {
   __shared__ PShared sh;
   __shared__ double xs[XS_CAP];
+  __shared__ double wprod[4 * WB_NZ];
   dpdlp_plp_t *P = lps + blockIdx.x;
   if (P->stage == DPDLP_PS_DONE) return;
   const pdhg_parameters_t &prm = *prmp;
@@ -142,6 +202,8 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
     sh.pi_iters = P->pi_iters;
     sh.do_restart = P->do_restart;
   }
+  wave_ranges(P->L.A_rp, P->L.m, sh.cutA);
+  wave_ranges(P->L.At_rp, P->L.n, sh.cutAt);
   __syncthreads();
   const dpdlp_blp_t &L = P->L;
   const int tid = threadIdx.x;
@@ -156,13 +218,13 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
     double s[4], mx[1];
     {
       EpiStore e{L.pi_dual, 0};
-      pass(L.At_rp, L.At_ci, L.At_v, L.pi_next, n, m, e, s, mx, xs);
+      pass(L.At_rp, L.At_ci, L.At_v, L.pi_next, n, m, e, s, mx, xs, wprod, sh.cutAt);
     }
     zero(s, 4);
     mx[0] = 0.0;
     {
       EpiPow e{L.pi_eig, L.pi_next, 1};
-      pass(L.A_rp, L.A_ci, L.A_v, L.pi_dual, m, n, e, s, mx, xs);
+      pass(L.A_rp, L.A_ci, L.A_v, L.pi_dual, m, n, e, s, mx, xs, wprod, sh.cutA);
     }
     wg_total<4, 0>(s, mx, sh, S, DPDLP_SL_PI);
     /* residual AA'q - sigma^2 q, then q <- AA'q / |AA'q| for the next step */
@@ -223,7 +285,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
         EpiResP e;
         e.pprod = L.primal_product; e.lc = L.lc; e.uc = L.uc; e.D_r = L.D_r; e.y = L.y_pdhg;
         e.y_init = L.y_init; e.dist = dist; e.partials = 1;
-        pass(L.A_rp, L.A_ci, L.A_v, L.x_pdhg, m, n, e, s, mx, xs);
+        pass(L.A_rp, L.A_ci, L.A_v, L.x_pdhg, m, n, e, s, mx, xs, wprod, sh.cutA);
         wg_total<7, 1>(s, mx, sh, S, DPDLP_SL_RESP);
       }
       {
@@ -233,7 +295,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
         EpiResD e;
         e.dprod = L.dual_product; e.c = L.c; e.z = L.dual_slack; e.D_c = L.D_c;
         e.x_obj = L.x_pdhg; e.x_init = L.x_init; e.dist = dist; e.partials = 1;
-        pass(L.At_rp, L.At_ci, L.At_v, L.y_pdhg, n, m, e, s, mx, xs);
+        pass(L.At_rp, L.At_ci, L.At_v, L.y_pdhg, n, m, e, s, mx, xs, wprod, sh.cutAt);
         wg_total<8, 1>(s, mx, sh, S, DPDLP_SL_RESD);
       }
       if (sh.ev_inf) {
@@ -267,7 +329,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
           EpiInfP e;
           e.pprod = L.primal_product; e.pslack = L.primal_slack; e.lc = L.lc; e.uc = L.uc;
           e.D_r = L.D_r; e.dy = L.dy; e.partials = 1;
-          pass(L.A_rp, L.A_ci, L.A_v, L.dx, m, n, e, s, mx, xs);
+          pass(L.A_rp, L.A_ci, L.A_v, L.dx, m, n, e, s, mx, xs, wprod, sh.cutA);
           wg_total<1, 1>(s, mx, sh, S, DPDLP_SL_INF + 2);
         }
         {
@@ -275,7 +337,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
           EpiInfD e;
           e.dprod = L.dual_product; e.dslack = L.dual_slack; e.lv = L.lv; e.uv = L.uv;
           e.D_c = L.D_c; e.c = L.c; e.dx = L.dx; e.partials = 1;
-          pass(L.At_rp, L.At_ci, L.At_v, L.dy, n, m, e, s, mx, xs);
+          pass(L.At_rp, L.At_ci, L.At_v, L.dy, n, m, e, s, mx, xs, wprod, sh.cutAt);
           wg_total<2, 1>(s, mx, sh, S, DPDLP_SL_INF + 4);
         }
       }
@@ -359,7 +421,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
       e.x_ref32 = nullptr; e.aty0 = nullptr; e.delta32 = 0;
       e.tau = sh.tau; e.w = sh.w; e.rho = sh.rho; e.major = sh.major; e.fpe = sh.fpe;
       e.partials = sh.fpe;
-      pass(L.At_rp, L.At_ci, L.At_v, L.y_cur, n, m, e, s, mx, xs);
+      pass(L.At_rp, L.At_ci, L.At_v, L.y_cur, n, m, e, s, mx, xs, wprod, sh.cutAt);
       if (sh.fpe) wg_total<3, 0>(s, mx, sh, S, DPDLP_SL_FPE);
     }
     {
@@ -369,14 +431,14 @@ __global__ __launch_bounds__(DPDLP_BS) void k_persist(dpdlp_plp_t *__restrict__ 
       e.dy = L.dy; e.y_cur32 = nullptr; e.ax0 = nullptr; e.delta32 = 0;
       e.sigma = sh.sigma; e.w = sh.w; e.rho = sh.rho; e.major = sh.major; e.fpe = sh.fpe;
       e.partials = sh.fpe;
-      pass(L.A_rp, L.A_ci, L.A_v, L.x_ref, m, n, e, s, mx, xs);
+      pass(L.A_rp, L.A_ci, L.A_v, L.x_ref, m, n, e, s, mx, xs, wprod, sh.cutA);
       if (sh.fpe) wg_total<3, 0>(s, mx, sh, S, DPDLP_SL_FPE + 3);
     }
     if (sh.fpe) {
       /* fixed-point error finish (pdlp_core_op.cu:666-692): A'dy and <A'dy, dx> */
       double s[1] = {0.0}, mx[1] = {0.0};
       EpiFpe e{L.dual_product, L.dx, 1};
-      pass(L.At_rp, L.At_ci, L.At_v, L.dy, n, m, e, s, mx, xs);
+      pass(L.At_rp, L.At_ci, L.At_v, L.dy, n, m, e, s, mx, xs, wprod, sh.cutAt);
       wg_total<1, 0>(s, mx, sh, S, DPDLP_SL_FPE + 6);
       if (tid < 7) {
         const double v = S[DPDLP_SL_FPE + tid];
