@@ -176,6 +176,10 @@ def bench_batch(args, dpdlp):
         print(f"# batch fp64: {fp64_s:.2f}s {count / fp64_s:.1f} LPs/s", file=sys.stderr,
               flush=True)
     step_s = sum(runs) / len(runs)
+    persist = None
+    if args.persist_max_nnz > 0:
+        persist = persist_legs(dpdlp, batch_res, probs, params, count, cap_at,
+                               args.persist_max_nnz)
 
     def obj_err(a, b):
         if a is None or b is None:
@@ -227,9 +231,41 @@ def bench_batch(args, dpdlp):
                      "traffic": batch_traffic(dom[0]),
                      "algorithmic_bytes_per_launch": dom[3] / n_s},
         "phase_GBps": {"primal": p1, "dual": p2},
+        "persistent_group": persist,
         "cpu_baseline": cpu, "generate_s": gen_s, "upload_s": up_s,
     }
     print(json.dumps(clean(out)))
+
+
+def persist_legs(dpdlp, res, probs, params, count, cap_at, cut):
+    """The mixed batch stepped by hand without and with a persistent group
+    (LPs of at most `cut` nonzeros whole on the device): LPs/s, and the
+    completion latency of the LPs at or below the cut."""
+    out = {"cut_nnz": cut, "lps_at_or_below_cut": sum(1 for p, _ in probs if p.nnz <= cut)}
+    for name, c in (("windowed", 0), ("persistent", cut)):
+        B = dpdlp.Batch(params, max_lps=count, spmv_cap=BATCH_CAP, spmv_cap_At=cap_at,
+                        precision=dpdlp.PREC_MIXED, persist_max_nnz=c)
+        t0 = time.perf_counter()
+        for k, r in enumerate(res):
+            B.add(r, k)
+        lat = {}
+        while B.in_flight() > 0:
+            B.step()
+            for tag, r, l in B.take():
+                lat[tag] = (l, r.status)
+        for tag, r, l in B.take():
+            lat[tag] = (l, r.status)
+        el = time.perf_counter() - t0
+        B.free()
+        small = sorted(lat[k][0] for k in range(count) if probs[k][0].nnz <= cut)
+        out[name] = {"s": el, "LPs_per_s": count / el,
+                     "optimal": sum(1 for v in lat.values() if v[1] == "OPTIMAL"),
+                     "small_p50_s": small[len(small) // 2] if small else None,
+                     "small_max_s": small[-1] if small else None}
+        print(f"# batch {name} (cut {cut}): {el:.2f}s {count / el:.1f} LPs/s, LPs <= cut "
+              f"p50 {out[name]['small_p50_s']:.3f}s max {out[name]['small_max_s']:.3f}s",
+              file=sys.stderr, flush=True)
+    return out
 
 
 def bench_serve(args, dpdlp):
@@ -247,7 +283,8 @@ def bench_serve(args, dpdlp):
             rep, lat, idx, _ = dpdlp.serve(
                 params, rate=rate, duration_sec=args.serve_duration, drain_sec=args.serve_drain,
                 seed=4000, mode=mode, precision=dpdlp.PREC_MIXED if mode == 0 else dpdlp.PREC_FP64,
-                large_frac=w["large_frac"], large_m=w["large_m"], large_nnz=w["large_nnz"],
+                large_frac=w["large_frac"] if args.large_frac < 0 else args.large_frac,
+                large_m=w["large_m"], large_nnz=w["large_nnz"],
                 pool_small=w["pool_small"], pool_large=w["pool_large"],
                 hbm_budget_gb=w["budget_gb"], max_lps=args.serve_max_lps,
                 persist_max_nnz=args.persist_max_nnz if mode == 0 else 0)
@@ -276,7 +313,8 @@ def bench_serve(args, dpdlp):
         "dtype": "f32-matrix/f64", "data": "synthetic (seeded generator, feasible by construction)",
         "config": {"workload": f"serve-poisson: {w['desc']}", "rates": rates,
                    "arrival_window_s": args.serve_duration, "drain_cap_s": args.serve_drain,
-                   "max_lps": args.serve_max_lps, "persist_max_nnz": args.persist_max_nnz},
+                   "max_lps": args.serve_max_lps, "persist_max_nnz": args.persist_max_nnz,
+                   "large_frac": w["large_frac"] if args.large_frac < 0 else args.large_frac},
         "per_rate": per,
         "speedup_vs_plain_loop_at_top_rate": top["batched"]["lps_per_sec"] /
         max(top["plain_loop"]["lps_per_sec"], 1e-9) if "plain_loop" in top else None,
@@ -428,6 +466,8 @@ def main():
     ap.add_argument("--serve-duration", type=float, default=60.0, help="serving: arrival window s")
     ap.add_argument("--serve-drain", type=float, default=30.0, help="serving: drain cap s")
     ap.add_argument("--serve-max-lps", type=int, default=512, help="serving: LPs in flight")
+    ap.add_argument("--large-frac", type=float, default=-1.0,
+                    help="serving: share of m = 1e6 requests (default: the workload's 0.05)")
     ap.add_argument("--persist-max-nnz", type=int, default=0,
                     help="batch and serving: LPs with at most this many nonzeros run whole in "
                          "the persistent kernel (0: none)")

@@ -23,6 +23,18 @@
 #include "dpdlp_dev.h"
 #include "spmv_core.h"
 
+#include <stddef.h>
+
+/* The descriptor and controller state are written by host C (batch.c) and
+ * read here: both sides assert the same layout. */
+static_assert(sizeof(hstate_t) == 224, "hstate_t layout");
+static_assert(sizeof(dpdlp_blp_t) == 296, "dpdlp_blp_t layout");
+static_assert(sizeof(dpdlp_plp_t) == 616, "dpdlp_plp_t layout");
+static_assert(offsetof(dpdlp_plp_t, k) == 320 && offsetof(dpdlp_plp_t, h) == 360 &&
+                  offsetof(dpdlp_plp_t, clk0) == 592 && offsetof(dpdlp_plp_t, stage) == 600 &&
+                  offsetof(dpdlp_plp_t, iters_run) == 612,
+              "dpdlp_plp_t layout");
+
 #define CTL_FN static __device__ __forceinline__
 #define CTL(x) dctl_##x
 #include "control_impl.h"

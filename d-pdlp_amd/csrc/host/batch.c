@@ -60,6 +60,16 @@ enum { S_FREE = 0, S_RESERVED, S_READY, S_PI, S_PDHG, S_ZOMBIE, S_PERSIST };
 
 enum { NLIST = 5 };
 
+/* the persistent kernel reads dpdlp_plp_t as written here (persist.hip
+ * asserts the same layout on the device side) */
+_Static_assert(sizeof(hstate_t) == 224, "hstate_t layout");
+_Static_assert(sizeof(dpdlp_blp_t) == 296, "dpdlp_blp_t layout");
+_Static_assert(sizeof(dpdlp_plp_t) == 616, "dpdlp_plp_t layout");
+_Static_assert(offsetof(dpdlp_plp_t, k) == 320 && offsetof(dpdlp_plp_t, h) == 360 &&
+                   offsetof(dpdlp_plp_t, clk0) == 592 && offsetof(dpdlp_plp_t, stage) == 600 &&
+                   offsetof(dpdlp_plp_t, iters_run) == 612,
+               "dpdlp_plp_t layout");
+
 typedef struct {
   int stage; /* read and written with atomics: set-up threads move it */
   hipEvent_t ev_ready; /* end of the set-up work on the set-up stream */
