@@ -262,9 +262,10 @@ def persist_legs(dpdlp, res, probs, params, count, cap_at, cut):
                      "optimal": sum(1 for v in lat.values() if v[1] == "OPTIMAL"),
                      "small_p50_s": small[len(small) // 2] if small else None,
                      "small_max_s": small[-1] if small else None}
+        lat_txt = (f"p50 {small[len(small) // 2]:.3f}s max {small[-1]:.3f}s" if small
+                   else "none at or below the cut")
         print(f"# batch {name} (cut {cut}): {el:.2f}s {count / el:.1f} LPs/s, LPs <= cut "
-              f"p50 {out[name]['small_p50_s']:.3f}s max {out[name]['small_max_s']:.3f}s",
-              file=sys.stderr, flush=True)
+              f"{lat_txt}", file=sys.stderr, flush=True)
     return out
 
 
