@@ -23,8 +23,10 @@
  * dereferences NULL); lines longer than 4095 bytes are read whole instead of
  * ending the file.
  *
- * A plain input is read whole into memory; a gzipped one is inflated by a
- * producer thread while the parser consumes it (see text_open).  The body of
+ * A plain input is read whole into memory; a block-gzipped one (BGZF, every
+ * member recording its compressed size) is inflated member by member on the
+ * host worker threads; any other gzipped one is inflated by a producer
+ * thread while the parser consumes it (see text_open).  The body of
  * a COLUMNS section (the bulk of a file) is parsed by host worker threads:
  * it is cut at line starts into chunks; each chunk's lines are split into
  * tokens, their numbers converted and their row names looked up (the row map
@@ -243,6 +245,7 @@ typedef struct {
   char *buf;
   size_t cap, len; /* len: published bytes (under mu) */
   int eof, stop;   /* eof: all published, buf[len] = NUL; stop: reader gave up */
+  int err;         /* zlib reported corrupt or truncated data (set with eof) */
   pthread_mutex_t mu;
   pthread_cond_t cv;
   pthread_t th;
@@ -266,6 +269,7 @@ static void *gz_produce(void *arg) {
     pthread_mutex_lock(&S->mu);
     S->len = len;
     if (got == 0) S->eof = 1;
+    if (r < 0) S->err = 1;
     pthread_cond_broadcast(&S->cv);
     pthread_mutex_unlock(&S->mu);
     if (got == 0) return NULL;
@@ -353,6 +357,7 @@ static int gz_stream_open(const char *filename, text_t *t) {
 /* the whole input, NUL-terminated: plain stdio or zlib */
 static char *load_text(const char *filename, size_t *len_out) {
   size_t fl = strlen(filename), len = 0, cap = 4u << 20;
+  int corrupt = 0;
   gzFile gz = NULL;
   FILE *f = NULL;
   if (fl > 3 && strcmp(filename + fl - 3, ".gz") == 0) {
@@ -387,6 +392,13 @@ static char *load_text(const char *filename, size_t *len_out) {
     if (gz) {
       int r = gzread(gz, buf + len, (unsigned)(want > (1u << 30) ? (1u << 30) : want));
       got = r > 0 ? (size_t)r : 0;
+      if (r < 0) { /* corrupt or truncated data: a read failure, not the end */
+        fprintf(stderr, "ERROR: corrupt or truncated gzip data in %s\n", filename);
+        free(buf);
+        buf = NULL;
+        corrupt = 1;
+        break;
+      }
     } else {
       got = fread(buf + len, 1, want, f);
     }
@@ -396,7 +408,7 @@ static char *load_text(const char *filename, size_t *len_out) {
   if (gz) gzclose(gz);
   if (f) fclose(f);
   if (!buf) {
-    fprintf(stderr, "ERROR: Out of memory reading %s\n", filename);
+    if (!corrupt) fprintf(stderr, "ERROR: Out of memory reading %s\n", filename);
     *len_out = 1; /* opened, not read */
     return NULL;
   }
@@ -405,12 +417,169 @@ static char *load_text(const char *filename, size_t *len_out) {
   return buf;
 }
 
+/* ---- block-gzipped input (BGZF): every gzip member (RFC 1952) carries its
+ * own compressed size in a "BC" extra subfield, so the members can be found
+ * without inflating anything and inflated in parallel on the host worker
+ * threads, each into its place in the text (its uncompressed size is the
+ * member's ISIZE field).  Every member's CRC-32 and size are checked.  Any
+ * other gzip layout (one stream, or members without that subfield) is
+ * inflated serially by the streamed reader. ---- */
+typedef struct {
+  size_t in_off, in_len, out_off;
+  uint32_t out_len, crc;
+} gzmember_t;
+
+static uint32_t le32(const unsigned char *p) {
+  return (uint32_t)p[0] | (uint32_t)p[1] << 8 | (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
+}
+
+/* total size of the gzip member starting at z (n bytes of it readable) from
+ * its "BC" subfield, or 0 if it has none (or is not a gzip member); *hdr: its
+ * header length.  The size may run past the n bytes given. */
+static size_t bgzf_member_size(const unsigned char *z, size_t n, size_t *hdr) {
+  if (n < 18 || z[0] != 0x1f || z[1] != 0x8b || z[2] != 8 || z[3] != 4) return 0;
+  const size_t xlen = (size_t)z[10] | (size_t)z[11] << 8;
+  if (12 + xlen > n) return 0;
+  for (size_t q = 12; q + 4 <= 12 + xlen;) {
+    const size_t slen = (size_t)z[q + 2] | (size_t)z[q + 3] << 8;
+    if (q + 4 + slen > 12 + xlen) return 0;
+    if (z[q] == 'B' && z[q + 1] == 'C' && slen == 2) {
+      const size_t size = ((size_t)z[q + 4] | (size_t)z[q + 5] << 8) + 1;
+      *hdr = 12 + xlen;
+      return size >= *hdr + 8 ? size : 0;
+    }
+    q += 4 + slen;
+  }
+  return 0;
+}
+
+typedef struct {
+  const unsigned char *z;
+  const gzmember_t *mem;
+  int nmem, ntask;
+  char *out;
+  int err;
+} bgzf_job_t;
+
+static void bgzf_task(void *arg, int t) {
+  bgzf_job_t *J = (bgzf_job_t *)arg;
+  const int k0 = (int)((long long)J->nmem * t / J->ntask);
+  const int k1 = (int)((long long)J->nmem * (t + 1) / J->ntask);
+  for (int k = k0; k < k1; ++k) {
+    const gzmember_t *M = &J->mem[k];
+    z_stream zs;
+    memset(&zs, 0, sizeof(zs));
+    if (inflateInit2(&zs, -15) != Z_OK) {
+      __atomic_store_n(&J->err, 1, __ATOMIC_RELAXED);
+      return;
+    }
+    zs.next_in = (Bytef *)(J->z + M->in_off);
+    zs.avail_in = (uInt)M->in_len;
+    zs.next_out = (Bytef *)(J->out + M->out_off);
+    zs.avail_out = M->out_len;
+    const int r = M->out_len ? inflate(&zs, Z_FINISH) : Z_STREAM_END;
+    const int ok = (r == Z_STREAM_END || (M->out_len == 0 && r == Z_BUF_ERROR)) &&
+                   zs.total_out == M->out_len &&
+                   crc32(0L, (const Bytef *)(J->out + M->out_off), M->out_len) == M->crc;
+    inflateEnd(&zs);
+    if (!ok) __atomic_store_n(&J->err, 1, __ATOMIC_RELAXED);
+  }
+}
+
+/* 0: the text is whole in t; 1: not block-gzipped (t untouched); -1: the
+ * file cannot be opened; -2: it cannot be read or a member is corrupt */
+static int bgzf_open(const char *filename, text_t *t) {
+  FILE *f = fopen(filename, "rb");
+  if (!f) return -1;
+  unsigned char head[64];
+  const size_t h = fread(head, 1, sizeof(head), f);
+  size_t hdr = 0;
+  if (bgzf_member_size(head, h, &hdr) == 0) {
+    fclose(f);
+    return 1;
+  }
+  struct stat st;
+  if (fstat(fileno(f), &st) != 0 || st.st_size <= 0 || fseek(f, 0, SEEK_SET) != 0) {
+    fclose(f);
+    return -2;
+  }
+  const size_t n = (size_t)st.st_size;
+  unsigned char *z = (unsigned char *)malloc(n);
+  if (!z || fread(z, 1, n, f) != n) {
+    free(z);
+    fclose(f);
+    return -2;
+  }
+  fclose(f);
+  size_t cap = 1024, nmem = 0, total = 0;
+  gzmember_t *mem = (gzmember_t *)malloc(cap * sizeof(gzmember_t));
+  int rc = mem ? 0 : -2;
+  for (size_t pos = 0; rc == 0 && pos < n;) {
+    const size_t size = bgzf_member_size(z + pos, n - pos, &hdr);
+    if (size > n - pos) {
+      fprintf(stderr, "ERROR: truncated gzip member in %s\n", filename);
+      rc = -2;
+      break;
+    }
+    if (size == 0) {
+      rc = 1; /* a member without its size: the serial reader takes the file */
+      break;
+    }
+    if (nmem == cap) {
+      gzmember_t *nm = (gzmember_t *)realloc(mem, 2 * cap * sizeof(gzmember_t));
+      if (!nm) {
+        rc = -2;
+        break;
+      }
+      mem = nm;
+      cap *= 2;
+    }
+    gzmember_t *M = &mem[nmem++];
+    M->in_off = pos + hdr;
+    M->in_len = size - hdr - 8;
+    M->crc = le32(z + pos + size - 8);
+    M->out_len = le32(z + pos + size - 4);
+    M->out_off = total;
+    total += M->out_len;
+    pos += size;
+  }
+  char *out = NULL;
+  if (rc == 0 && !(out = (char *)malloc(total + 1))) rc = -2;
+  if (rc == 0) {
+    int nt = 4 * dpdlp_host_threads();
+    if ((size_t)nt > nmem) nt = (int)nmem;
+    bgzf_job_t J = {z, mem, (int)nmem, nt > 0 ? nt : 1, out, 0};
+    if (nmem > 0) dpdlp_par_for(J.ntask, bgzf_task, &J);
+    if (J.err) {
+      fprintf(stderr, "ERROR: corrupt gzip member in %s\n", filename);
+      rc = -2;
+    }
+  }
+  free(z);
+  free(mem);
+  if (rc != 0) {
+    free(out);
+    return rc;
+  }
+  out[total] = '\0';
+  t->buf = out;
+  t->len = total;
+  t->pos = 0;
+  t->eof = 1;
+  return 0;
+}
+
 /* opens the input: 0, or -1 if it cannot be opened, -2 if it cannot be read */
 static int text_open(const char *filename, text_t *t) {
   memset(t, 0, sizeof(*t));
   size_t fl = strlen(filename);
   const char *env = getenv("DPDLP_MPS_STREAM");
   int stream = !(env && env[0] == '0');
+  if (fl > 3 && strcmp(filename + fl - 3, ".gz") == 0) {
+    const int rc = bgzf_open(filename, t);
+    if (rc == 0) return 0;
+    if (rc < 0) return rc;
+  }
   if (stream && fl > 3 && strcmp(filename + fl - 3, ".gz") == 0) {
     int rc = gz_stream_open(filename, t);
     if (rc == 0) return 0;
@@ -1212,6 +1381,19 @@ lp_problem_t *read_mps_file(const char *filename) {
       break;
     default:
       break;
+    }
+  }
+  if (T.src && !P.err) {
+    /* the inflate of a streamed input runs to the end of the file (the
+     * gzip trailer's CRC-32 and size are checked there); corrupt or
+     * truncated data fails the read */
+    text_wait(&T, (size_t)-1);
+    pthread_mutex_lock(&T.src->mu);
+    const int bad = T.src->err;
+    pthread_mutex_unlock(&T.src->mu);
+    if (bad) {
+      fprintf(stderr, "ERROR: corrupt or truncated gzip data in %s\n", filename);
+      P.err = 1;
     }
   }
   text_close(&T);

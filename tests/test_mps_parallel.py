@@ -326,6 +326,87 @@ def test_streamed_gzip_reader_errors(dp, tmp_path, monkeypatch):
         dp.Problem.from_mps(str(tmp_path / "missing.mps.gz"))
 
 
+def bgzf_bytes(data, block=65280, level=6, with_size=True):
+    """Block gzip (BGZF): members of at most `block` input bytes, each with a
+    "BC" extra subfield holding its total size - 1 (RFC 1952 FEXTRA), then
+    the empty end member.  with_size=False writes the subfield only in the
+    first member (the rest plain gzip members)."""
+    import struct
+    import zlib
+    out = bytearray()
+    pieces = [data[i:i + block] for i in range(0, len(data), block)] + [b""]
+    for k, piece in enumerate(pieces):
+        co = zlib.compressobj(level, zlib.DEFLATED, -15)
+        payload = co.compress(piece) + co.flush()
+        trailer = struct.pack("<II", zlib.crc32(piece) & 0xffffffff, len(piece) & 0xffffffff)
+        if with_size or k == 0:
+            size = 12 + 6 + len(payload) + 8
+            head = (b"\x1f\x8b\x08\x04" + b"\x00" * 4 + b"\x00\xff" + struct.pack("<H", 6) +
+                    b"BC" + struct.pack("<HH", 2, size - 1))
+        else:
+            head = b"\x1f\x8b\x08\x00" + b"\x00" * 4 + b"\x00\xff"
+        out += head + payload + trailer
+    return bytes(out)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_block_gzip_inflated_in_parallel_equals_line_by_line(dp, tmp_path, monkeypatch, seed):
+    """A block-gzipped file's members are located from their size fields and
+    inflated on the worker threads, each into its place: every thread count
+    and block size gives the line-by-line result, as do a file whose later
+    members lack the size field (the serial reader takes it) and the plain
+    gzip of the same text."""
+    text = random_mps(seed + 200, ncol=400, nrow=80).encode()
+    ref = reference_read(text.decode())
+    assert ref is not None and len(ref["vals"]) > 0
+    cases = [(1, 65280, True), (3, 4000, True), (8, 997, True), (5, 1, True), (4, 3000, False)]
+    for threads, block, sized in cases:
+        path = tmp_path / f"b{seed}_{block}_{int(sized)}.mps.gz"
+        path.write_bytes(bgzf_bytes(text, block=block, with_size=sized))
+        monkeypatch.setenv("DPDLP_HOST_THREADS", str(threads))
+        p = dp.Problem.from_mps(str(path))
+        a = p.arrays()
+        assert (p.m, p.n) == (ref["m"], ref["n"]), (threads, block, sized)
+        _same(a, ref)
+        assert a.get("obj_const", 0.0) == ref["obj_const"]
+
+
+def test_block_gzip_corrupt_member_is_refused(dp, tmp_path, monkeypatch):
+    """A member whose CRC-32 or size does not match its inflated bytes makes
+    the read fail (NULL from read_mps_file), on any thread count."""
+    text = random_mps(7, ncol=200, nrow=40).encode()
+    good = bgzf_bytes(text, block=2000)
+    # flip one bit of the second member's CRC-32: the member's total size is
+    # in its header, the CRC the 8 bytes before its end
+    size0 = int.from_bytes(good[16:18], "little") + 1
+    size1 = int.from_bytes(good[size0 + 16:size0 + 18], "little") + 1
+    bad = bytearray(good)
+    bad[size0 + size1 - 8] ^= 1
+    path = tmp_path / "bad.mps.gz"
+    path.write_bytes(bytes(bad))
+    for threads in (1, 4):
+        monkeypatch.setenv("DPDLP_HOST_THREADS", str(threads))
+        with pytest.raises(ValueError):
+            dp.Problem.from_mps(str(path))
+    # a wrong ISIZE in the last data member
+    bad = bytearray(good)
+    last_end = len(good) - 28  # the empty end member is 28 bytes
+    bad[last_end - 4] ^= 1
+    path.write_bytes(bytes(bad))
+    with pytest.raises(ValueError):
+        dp.Problem.from_mps(str(path))
+    # plain gzip (one stream, inflated serially): a wrong CRC-32 is a read
+    # failure on both the streamed and the whole-file path, not a short file
+    import gzip
+    one = bytearray(gzip.compress(text))
+    one[-8] ^= 1
+    path.write_bytes(bytes(one))
+    for stream in ("1", "0"):
+        monkeypatch.setenv("DPDLP_MPS_STREAM", stream)
+        with pytest.raises(ValueError):
+            dp.Problem.from_mps(str(path))
+
+
 def test_chunked_reader_rejects_pair_before_any_column(dp, tmp_path, monkeypatch):
     """A continuation line before any column name fails (mps_parser.c:616-621),
     also when it is the first line of a later chunk."""
