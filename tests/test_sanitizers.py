@@ -61,9 +61,20 @@ def test_host_code_under_sanitizers(tmp_path):
     bad = tmp_path / "pair_without_column.mps"
     bad.write_text("NAME X\nROWS\n N obj\n L r1\nCOLUMNS\n x r1\n")
     missing = str(tmp_path / "does_not_exist.mps")
+    # block-gzipped copies: one whole, one with a flipped CRC-32 (refused)
+    import test_mps_parallel as tmp_mod
+    with open(os.path.join(GOLDEN, "capi_example.mps"), "rb") as fi:
+        blocks = tmp_mod.bgzf_bytes(fi.read(), block=40)
+    bg = tmp_path / "capi_example_bgzf.mps.gz"
+    bg.write_bytes(blocks)
+    size0 = int.from_bytes(blocks[16:18], "little") + 1
+    corrupt = bytearray(blocks)
+    corrupt[size0 - 8] ^= 1
+    bg_bad = tmp_path / "corrupt_bgzf.mps.gz"
+    bg_bad.write_bytes(bytes(corrupt))
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0",
                UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
-    r = subprocess.run([exe] + good + [gz, "--", missing, str(bad)],
+    r = subprocess.run([exe] + good + [gz, str(bg), "--", missing, str(bad), str(bg_bad)],
                        capture_output=True, text=True, env=env, timeout=300)
     report = r.stdout + r.stderr
     assert r.returncode == 0, report[-6000:]
@@ -110,6 +121,10 @@ def test_mps_reader_threads_under_thread_sanitizer(tmp_path):
         with gzip.open(g, "wt") as fh:
             fh.write(text)
         files.append(str(g))
+        # a block-gzipped copy: members inflated in parallel on the workers
+        b = tmp_path / f"rnd{seed}_bgzf.mps.gz"
+        b.write_bytes(tmp_mod.bgzf_bytes(text.encode(), block=1500))
+        files.append(str(b))
     env = dict(os.environ, DPDLP_HOST_THREADS="6", DPDLP_MPS_CHUNK="256",
                DPDLP_MPS_WINDOW="2048", TSAN_OPTIONS="halt_on_error=1")
     r = subprocess.run([exe, "--mps-only"] + files, capture_output=True, text=True, env=env,
