@@ -182,7 +182,8 @@ __device__ __forceinline__ void pass(const int *__restrict__ rp, const int *__re
 }
 
 /* workgroup totals of NS sums and NM maxima into S[slot0 ...] (sums, then
- * maxima), written by thread 0; every thread must call it */
+ * maxima; thread k writes slot k), visible to every thread on return; every
+ * thread must call it */
 template <int NS, int NM>
 __device__ __forceinline__ void wg_total(double (&s)[NS > 0 ? NS : 1],
                                          double (&mx)[NM > 0 ? NM : 1], PShared &sh, double *S,
