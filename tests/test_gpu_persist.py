@@ -145,3 +145,31 @@ def test_served_small_requests_run_persistent(dp, orc):
     for k, p in enumerate(probs):
         if p.nnz <= cut:
             check_oracle(orc, p, ref[k])
+
+
+def test_persistent_state_survives_every_launch_boundary(dp):
+    """The controller state lives in LDS during a launch and in the LP's
+    descriptor between launches: one iteration per launch (budget 1) gives
+    the same bits as 250 or all of them per launch."""
+    lps = uneven_lps(dp, SMALL[:3], seed0=4100)
+    res = residents(dp, [p for p, _ in lps])
+    params = dp.default_params(presolve=False)
+    runs = [dp.solve_batch(res, params, spmv_cap=CAP, persist_max_nnz=BIG, persist_budget=b)
+            for b in (1, 7, 250, 1 << 30)]
+    for k in range(len(res)):
+        assert runs[0][k].status == "OPTIMAL"
+        for r in runs[1:]:
+            assert same(runs[0][k], r[k]), k
+
+
+def test_persistent_time_limit(dp):
+    """The loop's time is read from the device clock: a zero time limit stops
+    every persistent LP at its first evaluation, as the windowed batch does."""
+    lps = uneven_lps(dp, SMALL[:2], seed0=4200)
+    res = residents(dp, [p for p, _ in lps])
+    params = dp.default_params(presolve=False, termination_criteria__time_sec_limit=0.0)
+    got = dp.solve_batch(res, params, spmv_cap=CAP, persist_max_nnz=BIG)
+    win = dp.solve_batch(res, params, spmv_cap=CAP)
+    for g, w in zip(got, win):
+        assert g.status == w.status == "TIME_LIMIT"
+        assert g.iterations == w.iterations == 0
