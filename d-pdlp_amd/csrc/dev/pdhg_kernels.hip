@@ -81,7 +81,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_xp_partial(
     if (tid == 0) out[r0] = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
     return;
   }
-  const unsigned short *ro = rofs + (size_t)P * (r0 + b) + (size_t)k * (R + 1);
+  const unsigned short *ro = rofs + (size_t)P * ((size_t)r0 + b) + (size_t)k * (R + 1);
   for (int t = tid; t <= R; t += DPDLP_BS) srp[t] = ro[t];
   for (int base = tid; base < nz; base += DPDLP_BS * UNR) {
     int cidx[UNR];
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_xp_fused(
     __syncthreads();
     if (tid == 0) out[r0] = (ssum[0] + ssum[1]) + (ssum[2] + ssum[3]);
   } else {
-    const unsigned short *ro = rofs + (size_t)P * (r0 + b) + (size_t)k * (R + 1);
+    const unsigned short *ro = rofs + (size_t)P * ((size_t)r0 + b) + (size_t)k * (R + 1);
     for (int t = tid; t <= R; t += DPDLP_BS) srp[t] = ro[t];
     for (int base = tid; base < nz; base += DPDLP_BS * UNR) {
       int cidx[UNR];
@@ -551,18 +551,19 @@ __global__ void k_unscale(const double *x, const double *y, const double *D_c, c
 template <int G>
 __device__ __forceinline__ double row_abs_reduce(const int *rp, const double *v, long long r,
                                                  int gl, int mode, double p) {
-  const int a = rp[r], e = rp[r + 1];
+  /* unsigned positions: no overflow stepping past a row ending near 2^31 - 1 */
+  const unsigned a = (unsigned)rp[r], e = (unsigned)rp[r + 1];
   double acc = 0.0;
   if (mode == 0) {
-    for (int q = a + gl; q < e; q += G) acc = fmax(acc, fabs(v[q]));
+    for (unsigned q = a + gl; q < e; q += G) acc = fmax(acc, fabs(v[q]));
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) acc = fmax(acc, __shfl_xor(acc, off, G));
   } else {
-    for (int base = a; base < e; base += G) {
-      const int q = base + gl;
+    for (unsigned base = a; base < e; base += G) {
+      const unsigned q = base + gl;
       double t = 0.0;
       if (q < e) t = (p == 1.0) ? fabs(v[q]) : pow(fabs(v[q]), p);
-      const int cnt = (e - base) < G ? (e - base) : G;
+      const int cnt = (e - base) < (unsigned)G ? (int)(e - base) : G;
       for (int j = 0; j < cnt; ++j) acc += __shfl(t, j, G);
     }
   }
@@ -631,13 +632,13 @@ __global__ __launch_bounds__(DPDLP_BS) void k_scale_values2(const int *rpA, cons
   for (long long r = g0; r < (long long)mA + mB; r += ng) {
     if (r < mA) {
       const double fr = rs[r];
-      const int e = rpA[r + 1];
-      for (int q = rpA[r] + gl; q < e; q += G) vA[q] /= fr * cs[ciA[q]];
+      const unsigned e = (unsigned)rpA[r + 1];
+      for (unsigned q = (unsigned)rpA[r] + gl; q < e; q += G) vA[q] /= fr * cs[ciA[q]];
     } else {
       const long long rr = r - mA;
       const double fc = cs[rr];
-      const int e = rpB[rr + 1];
-      for (int q = rpB[rr] + gl; q < e; q += G) vB[q] /= rs[ciB[q]] * fc;
+      const unsigned e = (unsigned)rpB[rr + 1];
+      for (unsigned q = (unsigned)rpB[rr] + gl; q < e; q += G) vB[q] /= rs[ciB[q]] * fc;
     }
   }
 }

@@ -128,7 +128,8 @@ __device__ __forceinline__ void pass(const int *__restrict__ rp, const int *__re
       /* one row longer than WB_NZ: the whole wave */
       const int e1 = __shfl(pa, 1, DPDLP_WAVE);
       double acc = 0.0;
-      for (int q = p0 + lane; q < e1; q += DPDLP_WAVE) acc += v[q] * xg[ci[q]];
+      for (unsigned q = (unsigned)p0 + lane; q < (unsigned)e1; q += DPDLP_WAVE)
+        acc += v[q] * xg[ci[q]];
       acc = wave_sum(acc);
       if (lane == 0) epi(r, acc, s, mx);
       r += 1;

@@ -17,14 +17,17 @@ __device__ __forceinline__ T ntload(const T *p) {
 }
 
 /* One thread's share of a long row [p0, p1): four independent gathers in
- * flight, four accumulators combined in a fixed order. */
+ * flight, four accumulators combined in a fixed order.  Positions are
+ * unsigned so that stepping past a row that ends near 2^31 - 1 (the largest
+ * nonzero count of the int32 CSR interface) cannot overflow. */
 template <typename VT, typename XT>
 __device__ __forceinline__ double row_sum_wg(const int *__restrict__ ci,
                                              const VT *__restrict__ val,
                                              const XT *__restrict__ x, int p0, int p1) {
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  int k = p0 + (int)threadIdx.x;
-  for (; k + 3 * DPDLP_BS < p1; k += 4 * DPDLP_BS) {
+  const unsigned e = (unsigned)p1;
+  unsigned k = (unsigned)p0 + threadIdx.x;
+  for (; k + 3 * DPDLP_BS < e; k += 4 * DPDLP_BS) {
     int c0 = ntload(ci + k), c1 = ntload(ci + k + DPDLP_BS);
     int c2 = ntload(ci + k + 2 * DPDLP_BS), c3 = ntload(ci + k + 3 * DPDLP_BS);
     VT v0 = ntload(val + k), v1 = ntload(val + k + DPDLP_BS);
@@ -35,7 +38,7 @@ __device__ __forceinline__ double row_sum_wg(const int *__restrict__ ci,
     a2 += (double)v2 * (double)x2;
     a3 += (double)v3 * (double)x3;
   }
-  for (; k < p1; k += DPDLP_BS) a0 += (double)ntload(val + k) * (double)x[ntload(ci + k)];
+  for (; k < e; k += DPDLP_BS) a0 += (double)ntload(val + k) * (double)x[ntload(ci + k)];
   return (a0 + a1) + (a2 + a3);
 }
 

@@ -167,7 +167,7 @@ __global__ void k_tr_finish(const int *__restrict__ rp, int m, const double *__r
     const int k = t_ci[p];
     int lo = 0, hi = m; /* rp[lo] <= k < rp[hi] */
     while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
+      const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
       if (rp[mid] <= k)
         lo = mid;
       else

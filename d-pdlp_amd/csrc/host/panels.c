@@ -63,7 +63,7 @@ static void bp_task(void *arg, int t) {
       if (R > 1 && cnt[k] > 65535) __atomic_store_n(&J->bad, 1, __ATOMIC_RELAXED);
       fill[k] = base[k];
     }
-    unsigned short *ro = J->rofs + (size_t)P * (r0 + b);
+    unsigned short *ro = J->rofs + (size_t)P * ((size_t)r0 + b);
     for (int i = r0; i < r1; ++i) {
       for (int k = 0; k < P; ++k) ro[(size_t)k * (R + 1) + (i - r0)] = (unsigned short)(fill[k] - base[k]);
       for (int e = J->rp[i]; e < J->rp[i + 1]; ++e) {
