@@ -342,6 +342,35 @@ static hipError_t launch_spmv32(const dpdlp_csr_t *A, const float *x, const Epi 
 /* Fixed-order finalize: one workgroup per slot                              */
 /* ------------------------------------------------------------------------ */
 
+/* The sums of K partial slots slot0.. (nparts partials each), by one
+ * workgroup, returned to every thread in tot[].  Thread t adds partials
+ * t, t + 256, ... in that order, each wavefront sums its lanes with the
+ * fixed shuffle tree and the four wavefront sums are combined as
+ * (w0 + w1) + (w2 + w3): k_finalize's order, so a kernel that totals its
+ * inputs itself gets the bits a separate k_finalize launch would give.
+ * red: 4 K doubles of LDS. */
+template <int K>
+__device__ __forceinline__ void wg_slot_sums(const double *__restrict__ part, int stride,
+                                             int nparts, int slot0, double *red, double *tot) {
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += DPDLP_BS) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += part[(size_t)(slot0 + k) * stride + i];
+  }
+  __syncthreads(); /* red may still be read by an earlier call */
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double w = wave_sum(acc[k]);
+    if ((threadIdx.x & 63) == 0) red[4 * k + (threadIdx.x >> 6)] = w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    tot[k] = (red[4 * k] + red[4 * k + 1]) + (red[4 * k + 2] + red[4 * k + 3]);
+}
+
 /* maxmask bit k set: slot k is a maximum, else a sum.  Writes
  * scal[dst0 + k] and, when dst1 >= 0 / dst2 >= 0, copies to those too. */
 __global__ __launch_bounds__(DPDLP_BS) void k_finalize(const double *__restrict__ part,
@@ -351,20 +380,19 @@ __global__ __launch_bounds__(DPDLP_BS) void k_finalize(const double *__restrict_
   __shared__ double red[4];
   const int k = blockIdx.x;
   const bool ismax = (maxmask >> k) & 1u;
-  const double *p = part + (size_t)(slot0 + k) * stride;
-  double acc = 0.0;
+  double v;
   if (ismax) {
+    const double *p = part + (size_t)(slot0 + k) * stride;
+    double acc = 0.0;
     for (int i = threadIdx.x; i < nparts; i += DPDLP_BS) acc = fmax(acc, p[i]);
     acc = wave_max(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    v = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
   } else {
-    for (int i = threadIdx.x; i < nparts; i += DPDLP_BS) acc += p[i];
-    acc = wave_sum(acc);
+    wg_slot_sums<1>(part, stride, nparts, slot0 + k, red, &v);
   }
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
-  __syncthreads();
   if (threadIdx.x == 0) {
-    double v = ismax ? fmax(fmax(red[0], red[1]), fmax(red[2], red[3]))
-                     : (red[0] + red[1]) + (red[2] + red[3]);
     scal[dst0 + k] = v;
     if (dst1 >= 0) scal[dst1 + k] = v;
     if (dst2 >= 0) scal[dst2 + k] = v;
@@ -501,16 +529,45 @@ __global__ void k_pow_normalize(const double *eig, double *next, int m, const do
 }
 
 /* residual z - sigma^2 q (utils.cu:145-153), then next = z / |z| for the
- * following step (both read the old next first). */
+ * following step (both read the old next first).  The four totals of the
+ * A-product's slots (DPDLP_SL_PI: <q, z>, Blue sums of |z|) are read from
+ * scal, or, when pow_nparts > 0, summed here from their pow_nparts partials
+ * (every workgroup the same bits; workgroup 0 stores them at DPDLP_SL_PI and
+ * pow_hist).  When prev_slot >= 0 the previous step's residual partials
+ * (one per workgroup of that launch, same grid) are totalled the same way
+ * into scal[prev_slot] and scal[prev_hist].  This step's residual partials
+ * go to res_slot, which must differ from prev_slot. */
 __global__ __launch_bounds__(DPDLP_BS) void k_pow_update(const double *eig, double *next, int m,
-                                                        const double *scal, int dot_slot,
-                                                        int nrm_slot, double *part, int stride,
-                                                        int slot0) {
-  __shared__ double red[3 * 4];
+                                                        double *scal, double *part, int stride,
+                                                        int pow_nparts, int pow_hist,
+                                                        int res_slot, int prev_slot,
+                                                        int prev_hist) {
+  __shared__ double red[4 * 4];
   double s[3] = {0.0, 0.0, 0.0};
   double mx[1] = {0.0};
-  double neg = -scal[dot_slot];
-  double nrm = blue_combine(scal[nrm_slot], scal[nrm_slot + 1], scal[nrm_slot + 2]);
+  double t[4];
+  if (pow_nparts > 0) {
+    wg_slot_sums<4>(part, stride, pow_nparts, DPDLP_SL_PI, red, t);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int k = 0; k < 4; ++k) {
+        scal[DPDLP_SL_PI + k] = t[k];
+        scal[pow_hist + k] = t[k];
+      }
+  } else {
+    for (int k = 0; k < 4; ++k) t[k] = scal[DPDLP_SL_PI + k];
+  }
+  if (prev_slot >= 0) {
+    double r[3];
+    wg_slot_sums<3>(part, stride, (int)gridDim.x, prev_slot, red, r);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int k = 0; k < 3; ++k) {
+        scal[prev_slot + k] = r[k];
+        scal[prev_hist + k] = r[k];
+      }
+  }
+  __syncthreads(); /* red is reused by block_partials */
+  double neg = -t[0];
+  double nrm = blue_combine(t[1], t[2], t[3]);
   double inv = 1.0 / nrm;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m;
        i += (long long)gridDim.x * blockDim.x) {
@@ -519,7 +576,7 @@ __global__ __launch_bounds__(DPDLP_BS) void k_pow_update(const double *eig, doub
     blue_acc(r, s[0], s[1], s[2]);
     next[i] = e * inv;
   }
-  block_partials<3, 0>(s, mx, red, part, stride, slot0);
+  block_partials<3, 0>(s, mx, red, part, stride, res_slot);
 }
 
 __global__ void k_copy_scal(double *scal, int src, int dst, int count) {
@@ -1097,24 +1154,50 @@ int dpdlp_dev_power_init(const dpdlp_state_t *st, double *eig, double *next, voi
   return (int)hipGetLastError();
 }
 
+/* DPDLP_PI_FUSE=0 keeps the five-launch power step (for comparison) */
+static bool pi_fuse_enabled() {
+  static int enabled = -1;
+  if (enabled < 0) {
+    const char *e = getenv("DPDLP_PI_FUSE");
+    enabled = (e && e[0] == '0') ? 0 : 1;
+  }
+  return enabled == 1;
+}
+
 int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next, double *dual,
                           int iters, void *stream) {
   hipStream_t s = ST(stream);
   int g = ew_grid(st->m);
-  for (int it = 0; it < iters && it < DPDLP_PI_BATCH; ++it) {
+  /* A step's totals are summed inside the next kernel that needs them when
+   * that costs each of its workgroups only a few loads per thread (small
+   * LPs, where every launch's fixed cost counts): three launches per step
+   * instead of five, same bits (wg_slot_sums keeps k_finalize's order). */
+  const int np = spmv_nparts(&st->A);
+  const bool fuse = 4LL * np + 3LL * g <= 8LL * DPDLP_BS && pi_fuse_enabled();
+  const int nit = iters < DPDLP_PI_BATCH ? iters : DPDLP_PI_BATCH;
+  for (int it = 0; it < nit; ++it) {
+    const int hist = DPDLP_SC_PI_HIST + 4 * it;
     EpiStore e1{dual, 0};
     RET(launch_spmv(&st->At, next, e1, nullptr, 0, 0, s));
     EpiPow e2{eig, next, 1};
     RET(launch_spmv(&st->A, dual, e2, st->part, st->part_stride, DPDLP_SL_PI, s));
-    /* sigma^2 also recorded in the history (slot 0 of the 4) */
-    RET(finalize(st, spmv_nparts(&st->A), DPDLP_SL_PI, 4, 0u, DPDLP_SL_PI, DPDLP_SC_PI_HIST + 4 * it,
-                 -1, s));
-    k_pow_update<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, DPDLP_SL_PI,
-                                        DPDLP_SL_PI + 1, st->part, st->part_stride,
-                                        DPDLP_SL_PI + 4);
-    RET(hipGetLastError());
-    RET(finalize(st, g, DPDLP_SL_PI + 4, 3, 0u, DPDLP_SL_PI + 4,
-                 DPDLP_SC_PI_HIST + 4 * it + 1, -1, s));
+    if (fuse) {
+      /* residual partials alternate between two slot sets */
+      const int res = (it & 1) ? DPDLP_SL_PI2 : DPDLP_SL_PI + 4;
+      const int prev = it == 0 ? -1 : ((it & 1) ? DPDLP_SL_PI + 4 : DPDLP_SL_PI2);
+      k_pow_update<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, st->part,
+                                          st->part_stride, np, hist, res, prev,
+                                          hist - 4 + 1);
+      RET(hipGetLastError());
+      if (it == nit - 1) RET(finalize(st, g, res, 3, 0u, res, hist + 1, -1, s));
+    } else {
+      /* sigma^2 also recorded in the history (slot 0 of the 4) */
+      RET(finalize(st, np, DPDLP_SL_PI, 4, 0u, DPDLP_SL_PI, hist, -1, s));
+      k_pow_update<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, st->part,
+                                          st->part_stride, 0, hist, DPDLP_SL_PI + 4, -1, -1);
+      RET(hipGetLastError());
+      RET(finalize(st, g, DPDLP_SL_PI + 4, 3, 0u, DPDLP_SL_PI + 4, hist + 1, -1, s));
+    }
   }
   return 0;
 }

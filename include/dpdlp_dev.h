@@ -223,7 +223,8 @@ enum {
   DPDLP_SL_INF = 24,   /* max|dx|, max|dy|, psum, pinf, dsum, c.dx, dinf   (7) */
   DPDLP_SL_PI = 31,    /* <q,AA'q>, |AA'q| blue[3], residual blue[3]       (7) */
   DPDLP_SL_SCAL = 38,  /* scaling: bound blue[3], obj blue[3]              (6) */
-  DPDLP_SLOTS = 44,
+  DPDLP_SL_PI2 = 44,   /* second set of residual blue[3] (alternate steps) (3) */
+  DPDLP_SLOTS = 47,
   DPDLP_SC_FPE_RESTART = 48, /* copy of the 7 FPE slots from a restart step */
   DPDLP_SC_FPE_MAJOR = 56,   /* copy of the 7 FPE slots from a major step */
   DPDLP_SC_PI_HIST = 64,     /* power iteration history, 4 per iteration */

@@ -454,6 +454,39 @@ def test_power_iteration_step_size(dp, orc, case):
     assert abs(o.step_size - 0.998 / s_orc) <= 1e-12 * o.step_size
 
 
+FUSE_CHILD = """
+import sys, numpy as np
+sys.path.insert(0, sys.argv[2])
+import dpdlp as dp
+prob, _ = dp.generate(2000, 5000, 50000, seed=0)
+params = dp.default_params(presolve=False, termination_criteria__iteration_limit=300)
+res, o, _ = dp.solve(prob, params)
+np.savez(sys.argv[1], x=res.x, y=res.y, step=o.step_size, pi=o.power_iterations)
+"""
+
+
+def test_fused_power_step_gives_the_same_bits(dp, tmp_path):
+    """Small LPs total each power step's partial sums inside the update kernel
+    (three launches per step instead of five); a process that keeps the
+    five-launch step (DPDLP_PI_FUSE=0) gets the same power-iteration count,
+    step size and iterates bit for bit."""
+    import os
+    import subprocess
+    import sys
+    out = tmp_path / "unfused.npz"
+    pkg = os.path.dirname(os.path.dirname(dp.__file__))
+    env = dict(os.environ, DPDLP_PI_FUSE="0")
+    subprocess.run([sys.executable, "-c", FUSE_CHILD, str(out), pkg], env=env, check=True,
+                   timeout=300)
+    ref = np.load(out)
+    prob, _ = dp.generate(2000, 5000, 50000, seed=0)
+    params = dp.default_params(presolve=False, termination_criteria__iteration_limit=300)
+    res, o, _ = dp.solve(prob, params)
+    assert o.power_iterations == int(ref["pi"]) > 100
+    assert o.step_size == float(ref["step"])
+    assert np.array_equal(res.x, ref["x"]) and np.array_equal(res.y, ref["y"])
+
+
 def test_residuals_vanish_at_known_optimum(dp, orc):
     """Residual closed form (utils.cu:639-719): started at the C API example's
     optimum x* = (1, 2), y* = (1, -1, 0), the evaluation at iteration 0 finds
