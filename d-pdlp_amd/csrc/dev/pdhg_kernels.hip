@@ -579,6 +579,71 @@ __global__ __launch_bounds__(DPDLP_BS) void k_pow_update(const double *eig, doub
   block_partials<3, 0>(s, mx, red, part, stride, res_slot);
 }
 
+/* A power-iteration step's A' product with the previous step's update
+ * folded in (utils.cu:124-153).  Every workgroup first totals the previous
+ * A product's partials (<q, z> = sigma^2 and the Blue sums of |z|), or reads
+ * them from scal when pow_nparts == 0; workgroup 0 stores them at
+ * DPDLP_SL_PI and pow_hist.  When prev_slot >= 0 the residual partials of the
+ * update before (prev_nparts of them) are totalled the same way into
+ * scal[prev_slot] and scal[prev_hist].  The workgroups then do that update's
+ * elementwise work with k_pow_update's decomposition: virtual workgroup w of
+ * g = ew_grid(m) covers i = w*256 + tid + k*g*256 and workgroup b takes
+ * w = b, b + gridDim.x, ...; residual z - sigma^2 q into Blue partials at
+ * res_slot (index w), q <- z / |z|.  Last, each runs its row block of the
+ * untiled A' on the new q, gathering z[j] * (1 / |z|): the value q[j] gets,
+ * bit for bit, so nothing reads the q written in the same launch. */
+template <int CAP>
+__global__ __launch_bounds__(DPDLP_BS) void k_pow_at(
+    const int *__restrict__ blk, const int *__restrict__ rp, const int *__restrict__ ci,
+    const double *__restrict__ val, const double *__restrict__ eig, double *__restrict__ next,
+    double *__restrict__ dual, int m, int g, double *scal, double *part, int stride,
+    int pow_nparts, int pow_hist, int res_slot, int prev_slot, int prev_nparts, int prev_hist) {
+  __shared__ double prod[CAP];
+  __shared__ int srp[DPDLP_BS + 1];
+  __shared__ double ssum[DPDLP_BS];
+  __shared__ double red[4 * 4];
+  double t[4];
+  if (pow_nparts > 0) {
+    wg_slot_sums<4>(part, stride, pow_nparts, DPDLP_SL_PI, red, t);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int k = 0; k < 4; ++k) {
+        scal[DPDLP_SL_PI + k] = t[k];
+        scal[pow_hist + k] = t[k];
+      }
+  } else {
+    for (int k = 0; k < 4; ++k) t[k] = scal[DPDLP_SL_PI + k];
+  }
+  if (prev_slot >= 0) {
+    double r[3];
+    wg_slot_sums<3>(part, stride, prev_nparts, prev_slot, red, r);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+      for (int k = 0; k < 3; ++k) {
+        scal[prev_slot + k] = r[k];
+        scal[prev_hist + k] = r[k];
+      }
+  }
+  const double neg = -t[0];
+  const double inv = 1.0 / blue_combine(t[1], t[2], t[3]);
+  for (int w = blockIdx.x; w < g; w += gridDim.x) {
+    double s[3] = {0.0, 0.0, 0.0};
+    double mx[1] = {0.0};
+    for (long long i = (long long)w * DPDLP_BS + threadIdx.x; i < m;
+         i += (long long)g * DPDLP_BS) {
+      double e = eig[i];
+      double r = next[i] * neg + e;
+      blue_acc(r, s[0], s[1], s[2]);
+      next[i] = e * inv;
+    }
+    __syncthreads(); /* red is still read by the totals above / the last w */
+    block_partials_at<3, 0>(s, mx, red, part, stride, res_slot, w);
+  }
+  EpiStore st{dual, 0};
+  double s0[1] = {0.0}, m0[1] = {0.0};
+  spmv_block<double, double, CAP, EpiStore, true>(rp, ci, val, eig, blk[blockIdx.x],
+                                                  blk[blockIdx.x + 1], st, s0, m0, prod, srp,
+                                                  ssum, CAP, inv);
+}
+
 __global__ void k_copy_scal(double *scal, int src, int dst, int count) {
   if (threadIdx.x < count) scal[dst + threadIdx.x] = scal[src + threadIdx.x];
 }
@@ -1154,6 +1219,34 @@ int dpdlp_dev_power_init(const dpdlp_state_t *st, double *eig, double *next, voi
   return (int)hipGetLastError();
 }
 
+static hipError_t launch_pow_at(const dpdlp_state_t *st, const double *eig, double *next,
+                                double *dual, int g, int pow_nparts, int pow_hist, int res_slot,
+                                int prev_slot, int prev_nparts, int prev_hist, hipStream_t s) {
+  const dpdlp_csr_t *A = &st->At;
+  if (A->rows <= 0) return hipSuccess;
+  switch (A->cap) {
+  case 1024:
+    k_pow_at<1024><<<A->nblk, DPDLP_BS, 0, s>>>(A->blk, A->rp, A->ci, A->v, eig, next, dual,
+                                               st->m, g, st->scal, st->part, st->part_stride,
+                                               pow_nparts, pow_hist, res_slot, prev_slot,
+                                               prev_nparts, prev_hist);
+    break;
+  case 4096:
+    k_pow_at<4096><<<A->nblk, DPDLP_BS, 0, s>>>(A->blk, A->rp, A->ci, A->v, eig, next, dual,
+                                               st->m, g, st->scal, st->part, st->part_stride,
+                                               pow_nparts, pow_hist, res_slot, prev_slot,
+                                               prev_nparts, prev_hist);
+    break;
+  default:
+    k_pow_at<2048><<<A->nblk, DPDLP_BS, 0, s>>>(A->blk, A->rp, A->ci, A->v, eig, next, dual,
+                                               st->m, g, st->scal, st->part, st->part_stride,
+                                               pow_nparts, pow_hist, res_slot, prev_slot,
+                                               prev_nparts, prev_hist);
+    break;
+  }
+  return hipGetLastError();
+}
+
 /* DPDLP_PI_FUSE=0 keeps the five-launch power step (for comparison) */
 static bool pi_fuse_enabled() {
   static int enabled = -1;
@@ -1168,26 +1261,48 @@ int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next, do
                           int iters, void *stream) {
   hipStream_t s = ST(stream);
   int g = ew_grid(st->m);
-  /* A step's totals are summed inside the next kernel that needs them when
-   * that costs each of its workgroups only a few loads per thread (small
-   * LPs, where every launch's fixed cost counts): three launches per step
-   * instead of five, same bits (wg_slot_sums keeps k_finalize's order). */
+  /* Step t's update (sigma^2, residual, q <- z / |z|) runs inside step
+   * t + 1's A' product (k_pow_at) when A' is untiled; the last step of the
+   * batch gets a k_pow_update of its own.  Totals of partial sums are formed
+   * inside the kernel that needs them when that costs each workgroup only a
+   * few loads per thread (small LPs, where every launch's fixed cost
+   * counts), else by k_finalize launches.  Every variant gives the same bits
+   * (wg_slot_sums keeps k_finalize's order, the decomposition of the update
+   * is k_pow_update's).  Launches per step: 2 (small LP, A' untiled), 4
+   * (large LP, A' untiled), 3 or 5 (A' tiled). */
   const int np = spmv_nparts(&st->A);
-  const bool fuse = 4LL * np + 3LL * g <= 8LL * DPDLP_BS && pi_fuse_enabled();
+  const bool on = pi_fuse_enabled();
+  const bool small = on && 4LL * np + 3LL * g <= 8LL * DPDLP_BS;
+  const bool at = on && st->At.P <= 1;
   const int nit = iters < DPDLP_PI_BATCH ? iters : DPDLP_PI_BATCH;
+  /* residual partials of step j go to set j & 1 */
+  auto rset = [](int j) { return (j & 1) ? (int)DPDLP_SL_PI2 : (int)DPDLP_SL_PI + 4; };
   for (int it = 0; it < nit; ++it) {
     const int hist = DPDLP_SC_PI_HIST + 4 * it;
-    EpiStore e1{dual, 0};
-    RET(launch_spmv(&st->At, next, e1, nullptr, 0, 0, s));
+    if (it > 0 && at) {
+      /* step it - 1's update folded into this A' product */
+      const int ph = hist - 4;
+      if (small) {
+        RET(launch_pow_at(st, eig, next, dual, g, np, ph, rset(it - 1),
+                          it >= 2 ? rset(it - 2) : -1, g, ph - 4 + 1, s));
+      } else {
+        RET(finalize(st, np, DPDLP_SL_PI, 4, 0u, DPDLP_SL_PI, ph, -1, s));
+        RET(launch_pow_at(st, eig, next, dual, g, 0, ph, rset(it - 1), -1, 0, 0, s));
+        RET(finalize(st, g, rset(it - 1), 3, 0u, rset(it - 1), ph + 1, -1, s));
+      }
+    } else {
+      EpiStore e1{dual, 0};
+      RET(launch_spmv(&st->At, next, e1, nullptr, 0, 0, s));
+    }
     EpiPow e2{eig, next, 1};
     RET(launch_spmv(&st->A, dual, e2, st->part, st->part_stride, DPDLP_SL_PI, s));
-    if (fuse) {
-      /* residual partials alternate between two slot sets */
-      const int res = (it & 1) ? DPDLP_SL_PI2 : DPDLP_SL_PI + 4;
-      const int prev = it == 0 ? -1 : ((it & 1) ? DPDLP_SL_PI + 4 : DPDLP_SL_PI2);
+    if (at && it < nit - 1) continue; /* the next A' product updates */
+    if (small) {
+      /* step it - 1's residual partials (from the last update, folded or
+       * not) are still pending: this launch totals them */
+      const int res = rset(it), prev = it >= 1 ? rset(it - 1) : -1;
       k_pow_update<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, st->part,
-                                          st->part_stride, np, hist, res, prev,
-                                          hist - 4 + 1);
+                                          st->part_stride, np, hist, res, prev, hist - 4 + 1);
       RET(hipGetLastError());
       if (it == nit - 1) RET(finalize(st, g, res, 3, 0u, res, hist + 1, -1, s));
     } else {

@@ -16,14 +16,22 @@ __device__ __forceinline__ T ntload(const T *p) {
   return __builtin_nontemporal_load(p);
 }
 
+/* A gathered entry as a double; with SX, the entry times xs (the gathered
+ * vector is then xs times the stored one, formed as that product). */
+template <bool SX, typename XT>
+__device__ __forceinline__ double gx(XT v, double xs) {
+  return SX ? (double)v * xs : (double)v;
+}
+
 /* One thread's share of a long row [p0, p1): four independent gathers in
  * flight, four accumulators combined in a fixed order.  Positions are
  * unsigned so that stepping past a row that ends near 2^31 - 1 (the largest
  * nonzero count of the int32 CSR interface) cannot overflow. */
-template <typename VT, typename XT>
+template <typename VT, typename XT, bool SX = false>
 __device__ __forceinline__ double row_sum_wg(const int *__restrict__ ci,
                                              const VT *__restrict__ val,
-                                             const XT *__restrict__ x, int p0, int p1) {
+                                             const XT *__restrict__ x, int p0, int p1,
+                                             double xs = 1.0) {
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
   const unsigned e = (unsigned)p1;
   unsigned k = (unsigned)p0 + threadIdx.x;
@@ -33,12 +41,12 @@ __device__ __forceinline__ double row_sum_wg(const int *__restrict__ ci,
     VT v0 = ntload(val + k), v1 = ntload(val + k + DPDLP_BS);
     VT v2 = ntload(val + k + 2 * DPDLP_BS), v3 = ntload(val + k + 3 * DPDLP_BS);
     XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-    a0 += (double)v0 * (double)x0;
-    a1 += (double)v1 * (double)x1;
-    a2 += (double)v2 * (double)x2;
-    a3 += (double)v3 * (double)x3;
+    a0 += (double)v0 * gx<SX>(x0, xs);
+    a1 += (double)v1 * gx<SX>(x1, xs);
+    a2 += (double)v2 * gx<SX>(x2, xs);
+    a3 += (double)v3 * gx<SX>(x3, xs);
   }
-  for (; k < e; k += DPDLP_BS) a0 += (double)ntload(val + k) * (double)x[ntload(ci + k)];
+  for (; k < e; k += DPDLP_BS) a0 += (double)ntload(val + k) * gx<SX>(x[ntload(ci + k)], xs);
   return (a0 + a1) + (a2 + a3);
 }
 
@@ -131,11 +139,12 @@ template <class Epi> struct EpiPre<Epi, decltype((void)sizeof(typename Epi::Pre)
  * capacity its matrix was planned with, at most CAP: then it is one long row)
  * is summed by the whole workgroup without staging.  Every thread of the
  * workgroup must call it. */
-template <typename VT, typename XT, int CAP, class Epi>
+template <typename VT, typename XT, int CAP, class Epi, bool SX = false>
 __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int *__restrict__ ci,
                                            const VT *__restrict__ val, const XT *__restrict__ x,
                                            int r0, int r1, const Epi &epi, double *s, double *mx,
-                                           double *prod, int *srp, double *ssum, int cap = CAP) {
+                                           double *prod, int *srp, double *ssum, int cap = CAP,
+                                           double xs = 1.0) {
   constexpr int UNR = 8;
   using PF = EpiPre<Epi>;
   const int tid = threadIdx.x;
@@ -146,7 +155,7 @@ __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int
   const int nz = p1 - p0;
   if (nz > cap || nz > CAP) {
     /* one long row, whole workgroup */
-    double acc = row_sum_wg<VT, XT>(ci, val, x, p0, p1);
+    double acc = row_sum_wg<VT, XT, SX>(ci, val, x, p0, p1, xs);
     acc = wave_sum(acc);
     if ((tid & 63) == 0) ssum[tid >> 6] = acc;
     __syncthreads();
@@ -177,7 +186,7 @@ __device__ __forceinline__ void spmv_block(const int *__restrict__ rp, const int
 #pragma unroll
     for (int u = 0; u < UNR; ++u) {
       int k = base + u * DPDLP_BS;
-      if (k < nz) prod[k] = (double)vv[u] * (double)xx[u];
+      if (k < nz) prod[k] = (double)vv[u] * gx<SX>(xx[u], xs);
     }
   }
   __syncthreads();

@@ -454,37 +454,57 @@ def test_power_iteration_step_size(dp, orc, case):
     assert abs(o.step_size - 0.998 / s_orc) <= 1e-12 * o.step_size
 
 
+FUSE_CASES = [(2000, 5000, 50000, None), (2000, 5000, 50000, (1, 2)),
+              (100_000, 200_000, 1_000_000, (1, 1))]
 FUSE_CHILD = """
 import sys, numpy as np
 sys.path.insert(0, sys.argv[2])
 import dpdlp as dp
-prob, _ = dp.generate(2000, 5000, 50000, seed=0)
-params = dp.default_params(presolve=False, termination_criteria__iteration_limit=300)
-res, o, _ = dp.solve(prob, params)
-np.savez(sys.argv[1], x=res.x, y=res.y, step=o.step_size, pi=o.power_iterations)
+sys.path.insert(0, sys.argv[3])
+from test_gpu_core import FUSE_CASES, fuse_case
+out = {}
+for k, case in enumerate(FUSE_CASES):
+    res, o = fuse_case(dp, *case)
+    out.update({f"x{k}": res.x, f"y{k}": res.y, f"step{k}": o.step_size,
+                f"pi{k}": o.power_iterations})
+np.savez(sys.argv[1], **out)
 """
 
 
+def fuse_case(dp, m, n, nnz, panels):
+    prob, _ = dp.generate(m, n, nnz, seed=0)
+    params = dp.default_params(presolve=False, termination_criteria__iteration_limit=100)
+    if panels is None:
+        res, o, _ = dp.solve(prob, params)
+        return res, o
+    r = dp.Resident(prob, spmv_cap=1024, panels=panels)
+    assert r.panels == panels
+    return r.solve(params)
+
+
 def test_fused_power_step_gives_the_same_bits(dp, tmp_path):
-    """Small LPs total each power step's partial sums inside the update kernel
-    (three launches per step instead of five); a process that keeps the
-    five-launch step (DPDLP_PI_FUSE=0) gets the same power-iteration count,
-    step size and iterates bit for bit."""
+    """The power iteration folds each step's update into the next A'
+    product and totals partial sums inside the kernels that need them (two
+    launches per step on a small LP, four on a larger one, three with A' on
+    column panels); a process that keeps the five-launch step
+    (DPDLP_PI_FUSE=0) gets the same power-iteration count, step size and
+    iterates bit for bit in each case."""
     import os
     import subprocess
     import sys
     out = tmp_path / "unfused.npz"
     pkg = os.path.dirname(os.path.dirname(dp.__file__))
     env = dict(os.environ, DPDLP_PI_FUSE="0")
-    subprocess.run([sys.executable, "-c", FUSE_CHILD, str(out), pkg], env=env, check=True,
+    subprocess.run([sys.executable, "-c", FUSE_CHILD, str(out), pkg,
+                    os.path.dirname(os.path.abspath(__file__))], env=env, check=True,
                    timeout=300)
     ref = np.load(out)
-    prob, _ = dp.generate(2000, 5000, 50000, seed=0)
-    params = dp.default_params(presolve=False, termination_criteria__iteration_limit=300)
-    res, o, _ = dp.solve(prob, params)
-    assert o.power_iterations == int(ref["pi"]) > 100
-    assert o.step_size == float(ref["step"])
-    assert np.array_equal(res.x, ref["x"]) and np.array_equal(res.y, ref["y"])
+    for k, case in enumerate(FUSE_CASES):
+        res, o = fuse_case(dp, *case)
+        assert o.power_iterations == int(ref[f"pi{k}"]) > 20, case
+        assert o.step_size == float(ref[f"step{k}"]), case
+        assert np.array_equal(res.x, ref[f"x{k}"]), case
+        assert np.array_equal(res.y, ref[f"y{k}"]), case
 
 
 def test_residuals_vanish_at_known_optimum(dp, orc):
