@@ -399,6 +399,26 @@ __global__ __launch_bounds__(DPDLP_BS) void k_finalize(const double *__restrict_
   }
 }
 
+/* Two groups of sums in one launch: workgroups [0, cnt1) total slots
+ * slot1.. over n1 partials into scal[slot1 + k] and scal[hist1 + k], the
+ * next cnt2 total slots slot2.. over n2 partials into scal[slot2 + k] and
+ * scal[hist2 + k]; each in k_finalize's order. */
+__global__ __launch_bounds__(DPDLP_BS) void k_finalize_pair(const double *__restrict__ part,
+                                                           int stride, double *scal, int n1,
+                                                           int slot1, int cnt1, int hist1, int n2,
+                                                           int slot2, int hist2) {
+  __shared__ double red[4];
+  const bool first = (int)blockIdx.x < cnt1;
+  const int k = first ? (int)blockIdx.x : (int)blockIdx.x - cnt1;
+  const int slot = (first ? slot1 : slot2) + k;
+  double v;
+  wg_slot_sums<1>(part, stride, first ? n1 : n2, slot, red, &v);
+  if (threadIdx.x == 0) {
+    scal[slot] = v;
+    scal[(first ? hist1 : hist2) + k] = v;
+  }
+}
+
 static hipError_t finalize(const dpdlp_state_t *st, int nparts, int slot0, int count,
                            unsigned maxmask, int dst0, int dst1, int dst2,
                            hipStream_t s) {
@@ -1266,10 +1286,11 @@ int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next, do
    * batch gets a k_pow_update of its own.  Totals of partial sums are formed
    * inside the kernel that needs them when that costs each workgroup only a
    * few loads per thread (small LPs, where every launch's fixed cost
-   * counts), else by k_finalize launches.  Every variant gives the same bits
-   * (wg_slot_sums keeps k_finalize's order, the decomposition of the update
-   * is k_pow_update's).  Launches per step: 2 (small LP, A' untiled), 4
-   * (large LP, A' untiled), 3 or 5 (A' tiled). */
+   * counts), else by one finalize launch after each A product.  Every
+   * variant gives the same bits (wg_slot_sums keeps k_finalize's order, the
+   * decomposition of the update is k_pow_update's).  Launches per step
+   * besides the A product's: 1 (small LP, A' untiled), 2 (large LP, A'
+   * untiled), 2 or 4 (A' tiled). */
   const int np = spmv_nparts(&st->A);
   const bool on = pi_fuse_enabled();
   const bool small = on && 4LL * np + 3LL * g <= 8LL * DPDLP_BS;
@@ -1286,9 +1307,8 @@ int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next, do
         RET(launch_pow_at(st, eig, next, dual, g, np, ph, rset(it - 1),
                           it >= 2 ? rset(it - 2) : -1, g, ph - 4 + 1, s));
       } else {
-        RET(finalize(st, np, DPDLP_SL_PI, 4, 0u, DPDLP_SL_PI, ph, -1, s));
+        /* sigma^2 and |z| of step it - 1 were totalled after its A product */
         RET(launch_pow_at(st, eig, next, dual, g, 0, ph, rset(it - 1), -1, 0, 0, s));
-        RET(finalize(st, g, rset(it - 1), 3, 0u, rset(it - 1), ph + 1, -1, s));
       }
     } else {
       EpiStore e1{dual, 0};
@@ -1296,6 +1316,24 @@ int dpdlp_dev_power_steps(const dpdlp_state_t *st, double *eig, double *next, do
     }
     EpiPow e2{eig, next, 1};
     RET(launch_spmv(&st->A, dual, e2, st->part, st->part_stride, DPDLP_SL_PI, s));
+    if (at && !small) {
+      /* this step's sigma^2 and |z| and the previous step's residual (from
+       * this step's k_pow_at) in one launch */
+      if (it >= 1)
+        k_finalize_pair<<<7, DPDLP_BS, 0, s>>>(st->part, st->part_stride, st->scal, np,
+                                               DPDLP_SL_PI, 4, hist, g, rset(it - 1),
+                                               hist - 4 + 1);
+      else
+        k_finalize_pair<<<4, DPDLP_BS, 0, s>>>(st->part, st->part_stride, st->scal, np,
+                                               DPDLP_SL_PI, 4, hist, 0, 0, 0);
+      RET(hipGetLastError());
+      if (it < nit - 1) continue; /* the next A' product updates */
+      k_pow_update<<<g, DPDLP_BS, 0, s>>>(eig, next, st->m, st->scal, st->part,
+                                          st->part_stride, 0, hist, rset(it), -1, -1);
+      RET(hipGetLastError());
+      RET(finalize(st, g, rset(it), 3, 0u, rset(it), hist + 1, -1, s));
+      continue;
+    }
     if (at && it < nit - 1) continue; /* the next A' product updates */
     if (small) {
       /* step it - 1's residual partials (from the last update, folded or
