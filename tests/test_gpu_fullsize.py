@@ -139,3 +139,25 @@ def test_zipf4m_mixed_precision(dp):
     ey = np.linalg.norm(mixed.y - fp64.y) / np.linalg.norm(fp64.y)
     print(f"mixed vs fp64: objective {dobj:.2e}, x {ex:.2e}, y {ey:.2e}", flush=True)
     assert dobj <= 1e-4
+
+
+@pytest.mark.timeout(900)
+def test_batch512_every_member_alone_equals_mix(dp):
+    """The whole 512-LP workload (m log-uniform in [500, 2e5], n = m U[1.5, 3],
+    Poisson(8)+1 nonzeros per row, 1.46e8 nonzeros in all) in one
+    mixed-precision batch: every LP OPTIMAL, within 1e-3 of its known
+    optimum, and bit for bit the result of solving it alone."""
+    from test_gpu_batch import same
+    probs = [dp.batch_member(k) for k in range(512)]
+    res = [dp.Resident(p, spmv_cap=4096, panels=(1, 1)) for p, _ in probs]
+    params = dp.default_params(presolve=False)
+    mix = dp.solve_batch(res, params, spmv_cap=4096, precision=dp.PREC_MIXED)
+    print(f"batch of {len(res)} solved", flush=True)
+    for k, r in enumerate(res):
+        assert mix[k].status == "OPTIMAL", k
+        opt = probs[k][1]
+        assert abs(mix[k].primal_obj - opt) <= 1e-3 * max(1.0, abs(opt)), k
+        alone = dp.solve_batch([r], params, spmv_cap=4096, precision=dp.PREC_MIXED)[0]
+        assert same(mix[k], alone), k
+    for r in res:
+        r.free()
