@@ -156,3 +156,35 @@ def test_served_request_matches_oracle(dp, orc):
         assert abs(r.primal_obj - obj) <= 1e-6 * max(1.0, abs(obj))
         assert rel(r.x, ref["x"]) < 1e-8 and rel(r.y, ref["y"]) < 1e-8, j
     assert len(checked) >= 2
+
+
+@pytest.mark.timeout(900)
+def test_serving_workload_short_window(dp):
+    """The serving workload itself for a 4 s arrival window: a pool of the 512
+    batch-workload LPs and 4 m = 1e6, nnz = 1e7 ones, 5 % large requests,
+    mixed precision, 50 arrivals/s.  Every request completes OPTIMAL, and each
+    result equals the same LP solved alone on the path that served it: the
+    batch of that LP alone (4096-nonzero blocks) or, for the large ones, the
+    single-LP solve on the automatic layout."""
+    params = dp.default_params(presolve=False)
+    pool = dict(pool_small=512, pool_large=4, large_frac=0.05, large_m=1_000_000,
+                large_nnz=10_000_000, spmv_cap=4096)
+    rep, lat, idx, res = dp.serve(params, want_results=True, rate=50.0, duration_sec=4.0,
+                                  drain_sec=300.0, seed=4000, precision=dp.PREC_MIXED, **pool)
+    n = rep["arrived"]
+    print(f"served {rep['completed']} of {n}, large {int(np.sum(idx >= 512))}", flush=True)
+    assert n > 100 and rep["completed"] == n and rep["optimal"] == n
+    assert np.any(idx >= 512) and np.any(idx < 512)
+    alone = {}
+    for j in sorted(set(int(i) for i in idx)):
+        prob, opt = dp.serve_pool_member(j, pool_small=512, large_m=1_000_000,
+                                         large_nnz=10_000_000)
+        if j < 512:
+            r = dp.Resident(prob, spmv_cap=4096, spmv_cap_At=4096, panels=(1, 1))
+            alone[j] = dp.solve_batch([r], params, spmv_cap=4096, precision=dp.PREC_MIXED)[0]
+        else:
+            alone[j] = dp.Resident(prob).solve(params, precision=dp.PREC_MIXED)[0]
+        r = None
+        assert abs(alone[j].primal_obj - opt) <= 1e-3 * max(1.0, abs(opt)), j
+    for k in range(n):
+        assert same(res[k], alone[int(idx[k])]), (k, int(idx[k]))
