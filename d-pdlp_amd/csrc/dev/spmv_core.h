@@ -52,12 +52,19 @@ __device__ __forceinline__ double row_sum_wg(const int *__restrict__ ci,
 
 
 /* Deterministic row sums of the staged products prod[srp[r] .. srp[r+1]) for
- * the R rows of a block.  Phase A: rows of at most 16 entries per lane are
- * summed by a group of g lanes (g = 256/R rounded down to a power of two, at
- * most 64; one lane per row when R > 256).  Phase B: every longer row is
- * summed by a whole wavefront (lanes strided, then a fixed shuffle tree), long
- * rows dealt round-robin to the 4 wavefronts.  emit(r, sum) is called once per
- * row by exactly one lane. */
+ * the R rows of a block.  Phase A: rows of at most RR_PER_LANE entries per
+ * lane are summed by a group of g lanes (g = 256/R rounded down to a power of
+ * two, at most 64; one lane per row when R > 256).  Phase B: every longer row
+ * is summed by a whole wavefront (lanes strided, then a fixed shuffle tree),
+ * long rows dealt round-robin to the 4 wavefronts.  emit(r, sum) is called
+ * once per row by exactly one lane.  A wavefront's row costs about 1,400
+ * cycles (each step of the double-precision shuffle tree crosses the LDS
+ * permute twice) and the wavefronts take the long rows one at a time, so the
+ * groups keep every row up to RR_PER_LANE entries per lane: a block of 200
+ * rows of 20 entries is summed in about 1 us by single lanes, against about
+ * 30 us by wavefronts. */
+constexpr int RR_PER_LANE = 64;
+
 template <class Emit>
 __device__ __forceinline__ void rows_reduce(const double *__restrict__ prod,
                                             const int *__restrict__ srp, int R, Emit emit) {
@@ -67,7 +74,7 @@ __device__ __forceinline__ void rows_reduce(const double *__restrict__ prod,
     g = 1 << (31 - __clz(DPDLP_BS / R));
     if (g > DPDLP_WAVE) g = DPDLP_WAVE;
   }
-  const int lim = g * 16;
+  const int lim = g * RR_PER_LANE;
   if (g == 1) {
     for (int r = tid; r < R; r += DPDLP_BS) {
       const int a = srp[r], e = srp[r + 1];
