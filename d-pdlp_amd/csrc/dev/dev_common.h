@@ -58,16 +58,57 @@ __host__ __device__ inline double blue_combine(double asml, double amed, double 
   return scl * sqrt(sumsq);
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, DPDLP_WAVE);
+/* Wavefront reductions through DPP lane permutations (register-to-register
+ * within 16-lane rows) and four lane reads across the rows, instead of a
+ * shuffle tree whose every step of a double crosses the LDS permute twice.
+ * Every lane of the wavefront must be active. */
+enum {
+  DPP_XOR1 = 0xB1,         /* quad_perm [1,0,3,2] */
+  DPP_XOR2 = 0x4E,         /* quad_perm [2,3,0,1] */
+  DPP_HALF_MIRROR = 0x141, /* lane i of an 8-lane half <- lane 7 - i */
+  DPP_MIRROR = 0x140       /* lane i of a 16-lane row <- lane 15 - i */
+};
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+/* Sum over aligned groups of g lanes (g = 1, 2, 4, ..., 64, the same in
+ * every lane), in every lane of the group: pairs, quads, 8-lane halves
+ * (lane i with 7 - i), 16-lane rows (lane i with 15 - i), then the rows'
+ * sums read across.  Both lanes of a pair add the same two values, so every
+ * lane of a group holds the same bits. */
+__device__ __forceinline__ double group_sum(double v, int g) {
+  if (g >= 2) v += dpp_f64<DPP_XOR1>(v);
+  if (g >= 4) v += dpp_f64<DPP_XOR2>(v);
+  if (g >= 8) v += dpp_f64<DPP_HALF_MIRROR>(v);
+  if (g >= 16) v += dpp_f64<DPP_MIRROR>(v);
+  if (g >= 32) {
+    const double r0 = readlane_f64(v, 0), r1 = readlane_f64(v, 16);
+    const double r2 = readlane_f64(v, 32), r3 = readlane_f64(v, 48);
+    v = g == 32 ? ((threadIdx.x & 32) ? r2 + r3 : r0 + r1) : (r0 + r1) + (r2 + r3);
+  }
   return v;
 }
 
+__device__ __forceinline__ double wave_sum(double v) { return group_sum(v, DPDLP_WAVE); }
+
 __device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, DPDLP_WAVE));
-  return v;
+  v = fmax(v, dpp_f64<DPP_XOR1>(v));
+  v = fmax(v, dpp_f64<DPP_XOR2>(v));
+  v = fmax(v, dpp_f64<DPP_HALF_MIRROR>(v));
+  v = fmax(v, dpp_f64<DPP_MIRROR>(v));
+  return fmax(fmax(readlane_f64(v, 0), readlane_f64(v, 16)),
+              fmax(readlane_f64(v, 32), readlane_f64(v, 48)));
 }
 
 /*

@@ -345,7 +345,7 @@ static hipError_t launch_spmv32(const dpdlp_csr_t *A, const float *x, const Epi 
 /* The sums of K partial slots slot0.. (nparts partials each), by one
  * workgroup, returned to every thread in tot[].  Thread t adds partials
  * t, t + 256, ... in that order, each wavefront sums its lanes with the
- * fixed shuffle tree and the four wavefront sums are combined as
+ * fixed reduction (wave_sum) and the four wavefront sums are combined as
  * (w0 + w1) + (w2 + w3): k_finalize's order, so a kernel that totals its
  * inputs itself gets the bits a separate k_finalize launch would give.
  * red: 4 K doubles of LDS. */

@@ -55,11 +55,10 @@ __device__ __forceinline__ double row_sum_wg(const int *__restrict__ ci,
  * the R rows of a block.  Phase A: rows of at most RR_PER_LANE entries per
  * lane are summed by a group of g lanes (g = 256/R rounded down to a power of
  * two, at most 64; one lane per row when R > 256).  Phase B: every longer row
- * is summed by a whole wavefront (lanes strided, then a fixed shuffle tree),
+ * is summed by a whole wavefront (lanes strided, then wave_sum),
  * long rows dealt round-robin to the 4 wavefronts.  emit(r, sum) is called
- * once per row by exactly one lane.  A wavefront's row costs about 1,400
- * cycles (each step of the double-precision shuffle tree crosses the LDS
- * permute twice) and the wavefronts take the long rows one at a time, so the
+ * once per row by exactly one lane.  The wavefronts take the long rows one
+ * at a time (each with its own strided loads and reduction), so the
  * groups keep every row up to RR_PER_LANE entries per lane: a block of 200
  * rows of 20 entries is summed in about 1 us by single lanes, against about
  * 30 us by wavefronts. */
@@ -94,7 +93,7 @@ __device__ __forceinline__ void rows_reduce(const double *__restrict__ prod,
       if (mine)
         for (int q = a + ln; q < e; q += g) acc += prod[q];
     }
-    for (int off = g >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, DPDLP_WAVE);
+    acc = group_sum(acc, g);
     if (mine && ln == 0) emit(grp, acc);
   }
   int cnt = 0;
