@@ -92,8 +92,8 @@ __device__ __forceinline__ void wave_ranges(const int *__restrict__ rp, int rows
  * XS_CAP entries), stages the products in its own LDS slice, and each lane
  * then sums one row sequentially in storage order (the oracle's order) and
  * applies the epilogue, whose operands it loaded with the row pointers.  A
- * row longer than WB_NZ is summed by the whole wave (lane-strided, then a
- * shuffle tree).  Waves never wait on each other inside the pass; the
+ * row longer than WB_NZ is summed by the whole wave (lane-strided, then
+ * wave_sum).  Waves never wait on each other inside the pass; the
  * epilogue's per-thread partial sums accumulate over the pass. */
 template <class Epi>
 __device__ __forceinline__ void pass(const int *__restrict__ rp, const int *__restrict__ ci,
