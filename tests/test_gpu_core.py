@@ -11,13 +11,23 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("cap", [1024, 2048, 4096])
-@pytest.mark.parametrize("shape", ["uniform", "powerlaw", "empty_rows"])
+@pytest.mark.parametrize("shape", ["uniform", "powerlaw", "empty_rows", "rows17to23",
+                                   "rows33to64"])
 def test_spmv_matches_brute_force(dp, cap, shape):
+    """Row lengths that put every row of a block in one lane (rows of 17-23
+    entries, about 200 rows per 4096-nonzero block), in lane pairs and
+    groups, or on whole wavefronts (power-law rows, long rows)."""
     rng = np.random.default_rng(5)
     m, n = 3000, 2000
     if shape == "uniform":
         lens = rng.poisson(10, m)
         extra = ()
+    elif shape == "rows17to23":
+        lens = rng.integers(17, 24, m)
+        extra = ()
+    elif shape == "rows33to64":
+        lens = rng.integers(33, 65, m)
+        extra = ((1500, 1900),)
     elif shape == "powerlaw":
         lens = np.minimum((rng.pareto(1.5, m) * 3).astype(int) + 1, 1500)
         extra = ((7, 1999), (2500, 1800), (2999, 4096 if cap < 4096 else 1024))
