@@ -83,6 +83,38 @@ def test_parity_trace_vs_oracle(dp, orc, seed, row_order):
     assert abs(res.primal_obj - opt) <= 1e-3 * max(1.0, abs(opt))
 
 
+def test_banded_lp_matches_oracle(dp, orc):
+    """A banded LP (row i's 20 entries in columns 2i-10 .. 2i+9, so every row
+    of a 4096-nonzero block is summed by one lane), feasible by construction
+    (ranged rows around A x* with x* in the unit box): the trace of the
+    iterates within 1e-8 of the oracle's, the same status and iteration count,
+    objective within 1e-6."""
+    rng = np.random.default_rng(21)
+    m, n, per = 4000, 8000, 20
+    start = np.clip(2 * np.arange(m) - per // 2, 0, n - per)
+    ci = (start[:, None] + np.arange(per)[None, :]).ravel()
+    rp = np.arange(0, m * per + 1, per)
+    v = rng.standard_normal(m * per)
+    xs = rng.random(n)
+    b = np.add.reduceat(v * xs[ci], rp[:-1])
+    prob = dp.Problem.from_csr(m, n, rp, ci, v, c=rng.standard_normal(n),
+                               con_lb=b - rng.random(m), con_ub=b + rng.random(m),
+                               var_lb=np.zeros(n), var_ub=np.ones(n))
+    iters = [0, 10, 100, 300, 600]
+    kw = dict(presolve=False, termination_criteria__iteration_limit=20000)
+    res, o, (tx, ty) = dp.solve(prob, dp.default_params(**kw), trace_iters=iters, spmv_cap=4096)
+    ref, (ox, oy, _) = orc.solve(prob.ptr, orc.default_params(**kw), trace_iters=iters, n=n, m=m)
+    print(f"banded LP: {res.status} at {res.iterations} (oracle {ref['total_count']})", flush=True)
+    assert res.status == ref["status"] == "OPTIMAL"
+    assert res.iterations == ref["total_count"]
+    assert len(tx) == len(ox) == len(iters)
+    for i in range(len(ox)):
+        assert rel(tx[i], ox[i]) < 1e-8, (iters[i], rel(tx[i], ox[i]))
+        assert rel(ty[i], oy[i]) < 1e-8, (iters[i], rel(ty[i], oy[i]))
+    obj = ref["primal_objective_value"]
+    assert abs(res.primal_obj - obj) <= 1e-6 * max(1.0, abs(obj))
+
+
 CAP = dict(presolve=False, termination_criteria__iteration_limit=5000)
 
 
