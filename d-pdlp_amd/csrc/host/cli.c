@@ -111,6 +111,10 @@ static void usage(const char *prog) {
   fprintf(stderr, "      --opt_norm <norm_type>          Norm for optimality criteria: l2 or linf (default: l2).\n");
   fprintf(stderr, "      --no_presolve                   Disable presolve (default: enabled; no presolver here).\n");
   fprintf(stderr, "      --mixed_precision               FP32 matrix / FP64 state iterations (extension).\n");
+  fprintf(stderr, "      --grid_size <r,c>               Process grid; one GPU here, so only 1,1.\n");
+  fprintf(stderr, "      --partition_method <m>          uniform or nnz (accepted; no partition on one GPU).\n");
+  fprintf(stderr, "      --permute_method <m>            none, random or block (accepted; no effect here).\n");
+  fprintf(stderr, "      --permute_block_size <int>      Permutation block size (accepted; no effect here).\n");
 }
 
 int main(int argc, char *argv[]) {
@@ -137,6 +141,10 @@ int main(int argc, char *argv[]) {
                                {"opt_norm", required_argument, 0, 1014},
                                {"no_presolve", no_argument, 0, 1015},
                                {"mixed_precision", no_argument, 0, 1100},
+                               {"grid_size", required_argument, 0, 2001},
+                               {"partition_method", required_argument, 0, 2002},
+                               {"permute_method", required_argument, 0, 2003},
+                               {"permute_block_size", required_argument, 0, 2004},
                                {0, 0, 0, 0}};
   int o;
   while ((o = getopt_long(argc, argv, "hvfp", lo, NULL)) != -1) {
@@ -169,6 +177,58 @@ int main(int argc, char *argv[]) {
       break;
     case 1015: params.presolve = false; break;
     case 1100: opts.precision = DPDLP_PREC_MIXED; break;
+    /* the reference's process-grid options (cli.c:470-547); here one GPU
+     * holds the whole matrix, so the grid must be 1,1 and the partition and
+     * permutation settings are recorded in the parameters without effect */
+    case 2001: {
+      int r = 0, c = 0;
+      if (sscanf(optarg, "%d,%d", &r, &c) != 2) {
+        fprintf(stderr, "Error: Invalid grid_size format. Use --grid_size r,c (e.g., --grid_size 1,1)\n");
+        return 1;
+      }
+      if (r <= 0 || c <= 0) {
+        fprintf(stderr, "Error: Grid dimensions must be positive integers. Got: %s\n", optarg);
+        return 1;
+      }
+      if (r * c != 1) {
+        fprintf(stderr, "Error: --grid_size %s asks for %d GPUs; this solver runs on one (1,1)\n",
+                optarg, r * c);
+        return 1;
+      }
+      params.grid_size.row_dims = r;
+      params.grid_size.col_dims = c;
+      params.grid_size.decided = true;
+      break;
+    }
+    case 2002:
+      if (strcmp(optarg, "uniform") == 0) {
+        params.partition_method = UNIFORM_PARTITION;
+      } else if (strcmp(optarg, "nnz") == 0 || strcmp(optarg, "nnz_balance") == 0) {
+        params.partition_method = NNZ_BALANCE_PARTITION;
+      } else {
+        fprintf(stderr, "Error: partition_method must be 'uniform' or 'nnz'\n");
+        return 1;
+      }
+      break;
+    case 2003:
+      if (strcmp(optarg, "none") == 0 || strcmp(optarg, "no") == 0) {
+        params.permute_method = NO_PERMUTATION;
+      } else if (strcmp(optarg, "random") == 0 || strcmp(optarg, "full") == 0) {
+        params.permute_method = FULL_RANDOM_PERMUTATION;
+      } else if (strcmp(optarg, "block") == 0) {
+        params.permute_method = BLOCK_RANDOM_PERMUTATION;
+      } else {
+        fprintf(stderr, "Error: permute_method must be 'none', 'random', or 'block'\n");
+        return 1;
+      }
+      break;
+    case 2004:
+      params.permute_block_size = atoi(optarg);
+      if (params.permute_block_size <= 0) {
+        fprintf(stderr, "Error: permute_block_size must be a positive integer.\n");
+        return 1;
+      }
+      break;
     case 'p': break; /* accepted and ignored, as in the reference */
     case '?': return 1;
     }

@@ -192,6 +192,28 @@ def test_cli_arguments(dp, tmp_path):
     assert r.returncode == 1
 
 
+def test_cli_process_grid_options(dp, tmp_path):
+    """The reference's process-grid options (cli.c:470-547) are accepted with
+    its validation; one GPU holds the whole matrix, so only the 1,1 grid
+    passes (the run then stops at the missing file, after the options)."""
+    cli, missing = dp.CLI_PATH, str(tmp_path / "missing.mps")
+
+    def run(*args):
+        return subprocess.run([cli, *args, missing, str(tmp_path)], capture_output=True,
+                              text=True)
+    r = run("--grid_size", "1,1", "--partition_method", "nnz", "--permute_method", "block",
+            "--permute_block_size", "64")
+    assert r.returncode == 1 and "Failed to read" in r.stderr
+    for args, msg in ((("--grid_size", "2,2"), "one (1,1)"), (("--grid_size", "0,1"), "positive"),
+                      (("--grid_size", "3"), "Invalid grid_size"),
+                      (("--partition_method", "rows"), "partition_method"),
+                      (("--permute_method", "shuffle"), "permute_method"),
+                      (("--permute_block_size", "0"), "permute_block_size")):
+        r = run(*args)
+        assert r.returncode == 1 and msg in r.stderr, (args, r.stderr)
+        assert "Failed to read" not in r.stderr
+
+
 @pytest.mark.parametrize("cap", [1024, 2048, 4096])
 def test_block_plan_invariants(dp, cap):
     rng = np.random.default_rng(1)
