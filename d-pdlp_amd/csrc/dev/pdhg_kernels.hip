@@ -406,16 +406,31 @@ __global__ __launch_bounds__(DPDLP_BS) void k_finalize(const double *__restrict_
 __global__ __launch_bounds__(DPDLP_BS) void k_finalize_pair(const double *__restrict__ part,
                                                            int stride, double *scal, int n1,
                                                            int slot1, int cnt1, int hist1, int n2,
-                                                           int slot2, int hist2) {
+                                                           int slot2, int hist2,
+                                                           unsigned max1 = 0u,
+                                                           unsigned max2 = 0u) {
   __shared__ double red[4];
   const bool first = (int)blockIdx.x < cnt1;
   const int k = first ? (int)blockIdx.x : (int)blockIdx.x - cnt1;
   const int slot = (first ? slot1 : slot2) + k;
+  const int nparts = first ? n1 : n2;
   double v;
-  wg_slot_sums<1>(part, stride, first ? n1 : n2, slot, red, &v);
+  if (((first ? max1 : max2) >> k) & 1u) {
+    /* a maximum, as k_finalize takes it */
+    const double *p = part + (size_t)slot * stride;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += DPDLP_BS) acc = fmax(acc, p[i]);
+    acc = wave_max(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    v = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  } else {
+    wg_slot_sums<1>(part, stride, nparts, slot, red, &v);
+  }
   if (threadIdx.x == 0) {
     scal[slot] = v;
-    scal[(first ? hist1 : hist2) + k] = v;
+    const int h = first ? hist1 : hist2;
+    if (h >= 0) scal[h + k] = v;
   }
 }
 
@@ -1140,7 +1155,9 @@ int dpdlp_dev_residual(const dpdlp_state_t *st, const dpdlp_state_t *orig, int w
     e.dist = restart_dist;
     e.partials = 1;
     RET(launch_spmv(&st->A, st->x_pdhg, e, st->part, st->part_stride, DPDLP_SL_RESP, s));
-    RET(finalize(st, spmv_nparts(&st->A), DPDLP_SL_RESP, 8, 1u << 7, DPDLP_SL_RESP, -1, -1, s));
+    /* with both halves, the two totals share one launch after the A' product */
+    if (which != 3)
+      RET(finalize(st, spmv_nparts(&st->A), DPDLP_SL_RESP, 8, 1u << 7, DPDLP_SL_RESP, -1, -1, s));
     if (which == 1) {
       /* primal polish: objective uses the original c (utils.cu:1176-1182) */
       int g = ew_grid(st->n);
@@ -1161,7 +1178,16 @@ int dpdlp_dev_residual(const dpdlp_state_t *st, const dpdlp_state_t *orig, int w
     e.dist = restart_dist;
     e.partials = 1;
     RET(launch_spmv(&st->At, st->y_pdhg, e, st->part, st->part_stride, DPDLP_SL_RESD, s));
-    RET(finalize(st, spmv_nparts(&st->At), DPDLP_SL_RESD, 9, 1u << 8, DPDLP_SL_RESD, -1, -1, s));
+    if (which == 3) {
+      k_finalize_pair<<<8 + 9, DPDLP_BS, 0, s>>>(st->part, st->part_stride, st->scal,
+                                                 spmv_nparts(&st->A), DPDLP_SL_RESP, 8, -1,
+                                                 spmv_nparts(&st->At), DPDLP_SL_RESD, -1,
+                                                 1u << 7, 1u << 8);
+      RET(hipGetLastError());
+    } else {
+      RET(finalize(st, spmv_nparts(&st->At), DPDLP_SL_RESD, 9, 1u << 8, DPDLP_SL_RESD, -1, -1,
+                   s));
+    }
     if (which == 2) {
       /* dual polish: contributions use the original finite bounds */
       int g = ew_grid(st->m);
